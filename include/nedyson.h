@@ -1,0 +1,139 @@
+/*
+ * nedyson_amd C ABI: the Kadanoff-Baym GF2 timestep of NEdyson on one MI355X
+ * (gfx950) GPU.
+ *
+ * One context owns one propagated system: the contour functions G and Sigma
+ * in the packing of the reference's green_func (src/greens.cpp:177-195), the
+ * mean field hmf, the Cholesky-factored interaction and all scratch, resident
+ * in HBM, plus one HIP stream.  Calls on different contexts are independent;
+ * calls on one context are serialised by the caller (the reference's solver
+ * objects are likewise not re-entrant, src/mol2bSolver.h:80-88).
+ *
+ * Storage (blocks are row-major nao x nao complex128, es = nao*nao):
+ *   ret  G^R(i, j<=i)   block i(i+1)/2 + j
+ *   les  G^<(i<=j, j)   block j(j+1)/2 + i
+ *   tv   G^rceil(i, m)  block i*(ntau+1) + m
+ *   mat  G^M(m)         block m
+ * Imaginary time runs over the ntau+1 Gauss-Lobatto-Legendre nodes
+ * tau_m = beta (x_m + 1)/2 (the reference's cpsitop collocation,
+ * src/dyson.cpp:40).
+ *
+ * Every entry point returns NED_OK or an error code; ned_last_error() gives the
+ * message.  Nothing here falls back to the CPU.
+ */
+#ifndef NEDYSON_H
+#define NEDYSON_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  NED_OK = 0,
+  NED_ERR_ARG = 1,       /* bad sizes, indices or unsupported input */
+  NED_ERR_NOMEM = 2,     /* device allocation failed */
+  NED_ERR_DEVICE = 3,    /* HIP runtime / kernel error */
+  NED_ERR_NONFINITE = 4, /* a NaN or Inf appeared in the solution */
+  NED_ERR_NOCONV = 5,    /* iteration did not reach its tolerance */
+  NED_ERR_STATE = 6      /* call out of order (e.g. no interaction set) */
+} ned_status;
+
+typedef enum { NED_PREC_F64 = 0, NED_PREC_F32 = 1, NED_PREC_BF16X3 = 2 } ned_prec;
+
+typedef enum { NED_G = 0, NED_SIGMA = 1 } ned_which;
+typedef enum { NED_RET = 0, NED_LES = 1, NED_TV = 2, NED_MAT = 3 } ned_comp;
+
+typedef struct {
+  int nao;   /* orbitals */
+  int naux;  /* Cholesky vectors (nalpha) */
+  int nt;    /* last time step stored */
+  int ntau;  /* imaginary-time nodes are 0..ntau */
+  int k;     /* Gregory / BDF order */
+  int sig;   /* -1 fermions */
+} ned_dims;
+
+typedef struct ned_ctx ned_ctx;
+
+const char *ned_last_error(void);
+
+/* Replaces the GREEN(nt, ntau, nao, -1) allocations of DecompSimulation
+ * (src/molNEsim_decomp_impl.h:33,39) and the dyson object
+ * (src/dyson.cpp:25-43): all device storage of one system. */
+ned_status ned_ctx_create(ned_ctx **out, const ned_dims *dims, int device);
+ned_status ned_ctx_destroy(ned_ctx *ctx);
+ned_status ned_sync(ned_ctx *ctx);
+
+/* Replaces molGF2SolverDecomp(Vija, Viaj) (src/mol2bSolver.h:101-111):
+ * B_Qij is naux x nao x nao, B[Q][i][j] = Vija(i,j,Q); must be symmetric in
+ * (i,j). */
+ned_status ned_set_coulomb(ned_ctx *ctx, const double *B_Qij);
+
+/* Copy nblocks packed blocks starting at first_block between host and device
+ * (green_func::lesptr/retptr/tvptr/matptr addressing, src/greens.cpp:177-195). */
+ned_status ned_green_write(ned_ctx *ctx, int which, int comp, long long first_block, long long nblocks,
+                           const void *host_blocks);
+ned_status ned_green_read(ned_ctx *ctx, int which, int comp, long long first_block, long long nblocks,
+                          void *host_blocks);
+/* mean field hmf(t) for t in [first, first+count), complex128 nao x nao each */
+ned_status ned_hmf_write(ned_ctx *ctx, int first, int count, const void *host);
+ned_status ned_hmf_read(ned_ctx *ctx, int first, int count, void *host);
+
+/* molGF2SolverDecomp::solve(tstp, Sigma, G) (src/mol2bSolver_decomp_impl.h:412-429):
+ * overwrites Sigma^R(T,0..T), Sigma^<(0..T,T) and Sigma^rceil(T,:) from the
+ * same slices of G.  prec selects the arithmetic of the contractions; Sigma is
+ * stored in FP64 in every case. */
+ned_status ned_sigma_gf2_decomp(ned_ctx *ctx, int tstp, int prec);
+
+/* G.get_dm(tstp, rho); hmf[tstp] = h0; molHFSolverDecomp::solve_HF(tstp, hmf, rho)
+ * (src/greens.h:56-65, src/molHFSolver.cpp:81-110).  h0 is real nao x nao. */
+ned_status ned_hf_decomp(ned_ctx *ctx, int tstp, const double *h0);
+
+/* Imaginary-time setup: builds the mixing kernel of G^M (the decomposed
+ * path's GMCT, src/greens.cpp:112-146, initialised here as the full path does,
+ * src/molNEsim_impl.h:65) and the GLL mass matrix for beta.  Call after G^M is
+ * written and whenever G^M or beta change. */
+ned_status ned_set_matsubara(ned_ctx *ctx, double beta);
+
+/* dyson::dyson_step(n, G, Sig, hmf, mu, beta, dt) (src/dyson_step.h:329-382).
+ * err_out (3 doubles, may be NULL) receives the ret, tv and les update norms
+ * (src/dyson_step.h:69,104,164,315). */
+ned_status ned_dyson_step(ned_ctx *ctx, int tstp, double mu, double beta, double dt, double *err_out);
+
+/* dyson::Extrapolate(n, G) (src/dyson.cpp:47-117). */
+ned_status ned_extrapolate(ned_ctx *ctx, int tstp);
+
+/* dyson::dyson_start(G, Sig, hmf, mu, beta, dt) (src/dyson_boot.h:195-221). */
+ned_status ned_dyson_start(ned_ctx *ctx, double mu, double beta, double dt, double *err_out);
+
+/* DecompSimulation::do_boot (src/molNEsim_decomp_impl.h:63-82): iterate
+ * {hmf[0..k] = fock; Sigma(0..k); dyson_start} until err < tol.  *iters_out
+ * receives the passes used; NED_ERR_NOCONV if maxiter passes do not converge
+ * (the reference exits with status 0 there, src/molNEsim.cpp:129-131). */
+ned_status ned_do_boot(ned_ctx *ctx, const double *fock, double mu, double beta, double dt, int prec, int maxiter,
+                       double tol, int *iters_out, double *err_out);
+
+/* DecompSimulation::do_tstp (src/molNEsim_decomp_impl.h:86-104): predictor,
+ * then up to corr_max passes of {rho, HF, Sigma, Dyson}.  corr_tol = 0 runs
+ * exactly corr_max passes (the reference's CorrSteps); corr_tol > 0 stops once
+ * the Dyson update norm is below it.  *iters_out, *err_out may be NULL. */
+ned_status ned_do_tstp(ned_ctx *ctx, int tstp, const double *h0, double mu, double beta, double dt, int prec,
+                       int corr_max, double corr_tol, int *iters_out, double *err_out);
+
+/* energy_conv(tstp, Sigma, G, beta, dt) (src/dyson_integrals.h:24-70) and the
+ * kinetic part of do_energy (src/molNEsim_decomp_impl.h:132-144):
+ * out[0] = Re tr(rho (hmf + h0)), out[1] = 2 * energy_conv, out[2] = 2 tr rho. */
+ned_status ned_energy(ned_ctx *ctx, int tstp, const double *h0, double beta, double dt, double *out3);
+
+/* Gauss-Lobatto-Legendre nodes x_0..x_ntau on [-1,1] (ascending). Host-only. */
+ned_status ned_gll_nodes(int ntau, double *x);
+
+/* Integrator(k) tables (src/integration.h:23-124): poly_interp (k+1)^2,
+ * poly_diff (k+1)^2, poly_integ (k+1)^3, bd_weights k+2, start (k+1)^2,
+ * Omega (k+1)^2, ex_weights k+1.  Host-only. */
+ned_status ned_integrator(int k, double *interp, double *diff, double *integ, double *bd, double *start,
+                          double *Omega, double *ex);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,317 @@
+"""nedyson_amd: NEdyson's Kadanoff-Baym GF2 timestep on one MI355X (gfx950).
+
+Python binding of the C ABI in include/nedyson.h (libnedyson_amd.so, built
+in-tree by ``make -C nedyson_amd/csrc``).  Importing this package does not
+touch the GPU; creating a :class:`Context` does.  There is no CPU fallback:
+if the library is missing or no HIP device is present, the calls raise.
+
+The classes :class:`molGF2SolverDecomp` and :class:`dyson` keep the call
+signatures of the reference (src/mol2bSolver.h:125, src/dyson.h:86-87,114) on
+host :class:`~nedyson_amd.green.GreenFunc` objects; :class:`Context` is the
+device-resident form used by the timestepping driver and the benchmark.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .green import GreenFunc  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libnedyson_amd.so")
+
+NED_PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+NED_G, NED_SIGMA = 0, 1
+NED_RET, NED_LES, NED_TV, NED_MAT = 0, 1, 2, 3
+_STATUS = {0: "NED_OK", 1: "NED_ERR_ARG", 2: "NED_ERR_NOMEM", 3: "NED_ERR_DEVICE", 4: "NED_ERR_NONFINITE",
+           5: "NED_ERR_NOCONV", 6: "NED_ERR_STATE"}
+
+# every symbol declared in include/nedyson.h
+EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "ned_set_coulomb", "ned_green_write",
+           "ned_green_read", "ned_hmf_write", "ned_hmf_read", "ned_sigma_gf2_decomp", "ned_hf_decomp",
+           "ned_set_matsubara", "ned_dyson_step", "ned_extrapolate", "ned_dyson_start", "ned_do_boot", "ned_do_tstp",
+           "ned_energy", "ned_gll_nodes", "ned_integrator"]
+
+
+class NedError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Dims(C.Structure):
+    _fields_ = [("nao", C.c_int), ("naux", C.c_int), ("nt", C.c_int), ("ntau", C.c_int), ("k", C.c_int),
+                ("sig", C.c_int)]
+
+
+_lib = None
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+
+
+def build():
+    """Compile libnedyson_amd.so for gfx950 (hipcc)."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", os.path.join(_HERE, "csrc"), "-j8"])
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ImportError(f"{LIBPATH} is not built; run nedyson_amd.build() (hipcc, gfx950)")
+    L = C.CDLL(LIBPATH)
+    L.ned_last_error.restype = C.c_char_p
+    L.ned_ctx_create.argtypes = [C.POINTER(_vp), C.POINTER(_Dims), C.c_int]
+    L.ned_ctx_destroy.argtypes = [_vp]
+    L.ned_sync.argtypes = [_vp]
+    L.ned_set_coulomb.argtypes = [_vp, _dp]
+    L.ned_green_write.argtypes = [_vp, C.c_int, C.c_int, C.c_longlong, C.c_longlong, _vp]
+    L.ned_green_read.argtypes = [_vp, C.c_int, C.c_int, C.c_longlong, C.c_longlong, _vp]
+    L.ned_hmf_write.argtypes = [_vp, C.c_int, C.c_int, _vp]
+    L.ned_hmf_read.argtypes = [_vp, C.c_int, C.c_int, _vp]
+    L.ned_sigma_gf2_decomp.argtypes = [_vp, C.c_int, C.c_int]
+    L.ned_hf_decomp.argtypes = [_vp, C.c_int, _dp]
+    L.ned_set_matsubara.argtypes = [_vp, C.c_double]
+    L.ned_dyson_step.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+    L.ned_extrapolate.argtypes = [_vp, C.c_int]
+    L.ned_dyson_start.argtypes = [_vp, C.c_double, C.c_double, C.c_double, _dp]
+    L.ned_do_boot.argtypes = [_vp, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
+                              C.POINTER(C.c_int), _dp]
+    L.ned_do_tstp.argtypes = [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
+                              C.POINTER(C.c_int), _dp]
+    L.ned_energy.argtypes = [_vp, C.c_int, _dp, C.c_double, C.c_double, _dp]
+    L.ned_gll_nodes.argtypes = [C.c_int, _dp]
+    L.ned_integrator.argtypes = [C.c_int] + [_dp] * 7
+    for name in EXPORTS[1:]:
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(code):
+    if code != 0:
+        raise NedError(code, lib().ned_last_error().decode())
+
+
+def _cptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c128(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a
+
+
+def prec_code(prec):
+    if isinstance(prec, str):
+        return NED_PREC[prec]
+    return int(prec)
+
+
+def gll_nodes(ntau):
+    x = np.zeros(ntau + 1)
+    _check(lib().ned_gll_nodes(ntau, _dptr(x)))
+    return x
+
+
+def integrator(k):
+    k1 = k + 1
+    arrs = [np.zeros(k1 * k1), np.zeros(k1 * k1), np.zeros(k1 ** 3), np.zeros(k + 2), np.zeros(k1 * k1),
+            np.zeros(k1 * k1), np.zeros(k1)]
+    _check(lib().ned_integrator(k, *[_dptr(a) for a in arrs]))
+    return dict(zip(["poly_interp", "poly_diff", "poly_integ", "bd", "start", "Omega", "ex"], arrs))
+
+
+class Context:
+    """One system resident on one GPU (see include/nedyson.h)."""
+
+    def __init__(self, nao, naux, nt, ntau, k, sig=-1, device=0):
+        self.nao, self.naux, self.nt, self.ntau, self.k, self.sig = nao, naux, nt, ntau, k, sig
+        self.np = ntau + 1
+        self.h = _vp()
+        d = _Dims(nao, naux, nt, ntau, k, sig)
+        _check(lib().ned_ctx_create(C.byref(self.h), C.byref(d), device))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ned_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- data movement
+    def set_coulomb(self, B):
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        assert B.shape == (self.naux, self.nao, self.nao)
+        _check(lib().ned_set_coulomb(self.h, _dptr(B)))
+
+    def write_blocks(self, which, comp, first, blocks):
+        b = _c128(blocks)
+        nb = b.size // (self.nao * self.nao)
+        _check(lib().ned_green_write(self.h, which, comp, first, nb, _cptr(b)))
+
+    def read_blocks(self, which, comp, first, nb):
+        out = np.zeros((nb, self.nao, self.nao), dtype=np.complex128)
+        _check(lib().ned_green_read(self.h, which, comp, first, nb, _cptr(out)))
+        return out
+
+    def upload(self, which, g):
+        self.write_blocks(which, NED_RET, 0, g.ret)
+        self.write_blocks(which, NED_LES, 0, g.les)
+        self.write_blocks(which, NED_TV, 0, g.tv)
+        self.write_blocks(which, NED_MAT, 0, g.mat)
+
+    def download(self, which, g=None):
+        if g is None:
+            g = GreenFunc(self.nt, self.ntau, self.nao, self.sig)
+        ntri = (self.nt + 1) * (self.nt + 2) // 2
+        g.ret[:] = self.read_blocks(which, NED_RET, 0, ntri)
+        g.les[:] = self.read_blocks(which, NED_LES, 0, ntri)
+        g.tv[:] = self.read_blocks(which, NED_TV, 0, (self.nt + 1) * self.np).reshape(g.tv.shape)
+        g.mat[:] = self.read_blocks(which, NED_MAT, 0, self.np)
+        return g
+
+    def write_hmf(self, hmf, first=0):
+        h = _c128(hmf)
+        _check(lib().ned_hmf_write(self.h, first, h.shape[0], _cptr(h)))
+
+    def read_hmf(self, first=0, count=None):
+        count = self.nt + 1 - first if count is None else count
+        out = np.zeros((count, self.nao, self.nao), dtype=np.complex128)
+        _check(lib().ned_hmf_read(self.h, first, count, _cptr(out)))
+        return out
+
+    # ---- hot path
+    def sigma(self, tstp, prec="f64"):
+        _check(lib().ned_sigma_gf2_decomp(self.h, tstp, prec_code(prec)))
+
+    def hf(self, tstp, h0):
+        h0 = np.ascontiguousarray(h0, dtype=np.float64)
+        _check(lib().ned_hf_decomp(self.h, tstp, _dptr(h0)))
+
+    def set_matsubara(self, beta):
+        _check(lib().ned_set_matsubara(self.h, beta))
+
+    def dyson_step(self, tstp, mu, beta, dt):
+        e = np.zeros(3)
+        _check(lib().ned_dyson_step(self.h, tstp, mu, beta, dt, _dptr(e)))
+        return e
+
+    def extrapolate(self, tstp):
+        _check(lib().ned_extrapolate(self.h, tstp))
+
+    def dyson_start(self, mu, beta, dt):
+        e = np.zeros(1)
+        _check(lib().ned_dyson_start(self.h, mu, beta, dt, _dptr(e)))
+        return float(e[0])
+
+    def do_boot(self, fock, mu, beta, dt, prec="f64", maxiter=100, tol=1e-10):
+        fock = np.ascontiguousarray(fock, dtype=np.float64)
+        it = C.c_int(0)
+        e = np.zeros(1)
+        _check(lib().ned_do_boot(self.h, _dptr(fock), mu, beta, dt, prec_code(prec), maxiter, tol, C.byref(it),
+                                 _dptr(e)))
+        return it.value, float(e[0])
+
+    def do_tstp(self, tstp, h0, mu, beta, dt, prec="f64", corr_max=3, corr_tol=0.0):
+        h0 = np.ascontiguousarray(h0, dtype=np.float64)
+        it = C.c_int(0)
+        e = np.zeros(1)
+        _check(lib().ned_do_tstp(self.h, tstp, _dptr(h0), mu, beta, dt, prec_code(prec), corr_max, corr_tol,
+                                 C.byref(it), _dptr(e)))
+        return it.value, float(e[0])
+
+    def energy(self, tstp, h0, beta, dt):
+        h0 = np.ascontiguousarray(h0, dtype=np.float64)
+        out = np.zeros(3)
+        _check(lib().ned_energy(self.h, tstp, _dptr(h0), beta, dt, _dptr(out)))
+        return out
+
+    def sync(self):
+        _check(lib().ned_sync(self.h))
+
+
+class molGF2SolverDecomp:
+    """Reference-signature wrapper: molGF2SolverDecomp(Vija, Viaj).solve(tstp, Sigma, G)
+    (src/mol2bSolver.h:101-125) on host GreenFunc objects, computed on the GPU."""
+
+    def __init__(self, Vija, Viaj=None, prec="f64", device=0):
+        V = np.ascontiguousarray(Vija, dtype=np.float64)
+        self.nao, self.naux = V.shape[0], V.shape[2]
+        self.B = np.ascontiguousarray(np.transpose(V, (2, 0, 1)))
+        self.prec, self.device = prec, device
+        self._ctx = None
+
+    def _context(self, G):
+        c = self._ctx
+        if c is None or (c.nt, c.ntau) != (G.nt, G.ntau):
+            if c is not None:
+                c.close()
+            c = Context(self.nao, self.naux, G.nt, G.ntau, min(5, max(1, G.nt)), G.sig, self.device)
+            c.set_coulomb(self.B)
+            self._ctx = c
+        return c
+
+    def solve(self, tstp, Sigma, G):
+        c = self._context(G)
+        c.upload(NED_G, G)
+        c.sigma(tstp, self.prec)
+        ntri = (G.nt + 1) * (G.nt + 2) // 2
+        s = tstp * (tstp + 1) // 2
+        Sigma.ret[s:s + tstp + 1] = c.read_blocks(NED_SIGMA, NED_RET, s, tstp + 1)
+        Sigma.les[s:s + tstp + 1] = c.read_blocks(NED_SIGMA, NED_LES, s, tstp + 1)
+        Sigma.tv[tstp] = c.read_blocks(NED_SIGMA, NED_TV, tstp * c.np, c.np)
+        del ntri
+
+
+class dyson:
+    """Reference-signature wrapper: dyson(nt, ntau, nao, k).dyson_step(n, G, Sig, hmf, mu,
+    beta, dt) and Extrapolate(n, G) (src/dyson.h:41-114) on host GreenFunc objects."""
+
+    def __init__(self, nt, ntau, nao, k, device=0):
+        self.nt, self.ntau, self.nao, self.k = nt, ntau, nao, k
+        self.ctx = Context(nao, 1, nt, ntau, k, -1, device)
+        self._beta = None
+
+    def _prep(self, G, Sig, hmf, beta):
+        c = self.ctx
+        c.upload(NED_G, G)
+        if Sig is not None:
+            c.upload(NED_SIGMA, Sig)
+        if hmf is not None:
+            c.write_hmf(hmf)
+        if beta is not None:
+            c.set_matsubara(beta)
+
+    def dyson_step(self, n, G, Sig, hmf, mu, beta, dt):
+        self._prep(G, Sig, hmf, beta)
+        e = self.ctx.dyson_step(n, mu, beta, dt)
+        self.ctx.download(NED_G, G)
+        return e
+
+    def Extrapolate(self, n, G):
+        self._prep(G, None, None, None)
+        self.ctx.extrapolate(n)
+        self.ctx.download(NED_G, G)
+
+    def dyson_start(self, G, Sig, hmf, mu, beta, dt):
+        self._prep(G, Sig, hmf, beta)
+        e = self.ctx.dyson_start(mu, beta, dt)
+        self.ctx.download(NED_G, G)
+        return e

@@ -1,0 +1,336 @@
+// C ABI of the nedyson_amd library (declarations: include/nedyson.h).
+#include "../../include/nedyson.h"
+#include "ctx.h"
+#include "dyson.h"
+#include "quadrature.h"
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+struct ned_ctx {
+  ned::Ctx c;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+ned_status guard(ned_ctx *ctx, F &&f) {
+  try {
+    if (ctx) NED_HIP(hipSetDevice(ctx->c.device));
+    f();
+    return NED_OK;
+  } catch (const std::invalid_argument &e) {
+    g_err = e.what();
+    return NED_ERR_ARG;
+  } catch (const std::bad_alloc &e) {
+    g_err = e.what();
+    return NED_ERR_NOMEM;
+  } catch (const std::runtime_error &e) {
+    g_err = e.what();
+    if (g_err.find("out of memory") != std::string::npos) return NED_ERR_NOMEM;
+    if (g_err.find("non-finite") != std::string::npos) return NED_ERR_NONFINITE;
+    if (g_err.find("not converge") != std::string::npos) return NED_ERR_NOCONV;
+    if (g_err.find("state:") != std::string::npos) return NED_ERR_STATE;
+    return NED_ERR_DEVICE;
+  } catch (...) {
+    g_err = "unknown error";
+    return NED_ERR_DEVICE;
+  }
+}
+
+ned::DevBuf &component(ned::Ctx &c, int which, int comp) {
+  ned::Green &g = which == NED_SIGMA ? c.S : c.G;
+  switch (comp) {
+    case NED_RET: return g.ret;
+    case NED_LES: return g.les;
+    case NED_TV: return g.tv;
+    case NED_MAT: return g.mat;
+  }
+  throw std::invalid_argument("component must be NED_RET, NED_LES, NED_TV or NED_MAT");
+}
+
+long long nblocks_of(const ned::Ctx &c, int comp) {
+  if (comp == NED_RET || comp == NED_LES) return (long long)(c.nt + 1) * (c.nt + 2) / 2;
+  if (comp == NED_TV) return (long long)(c.nt + 1) * c.np;
+  return c.np;
+}
+
+void state(bool ok, const char *msg) {
+  if (!ok) throw std::runtime_error(std::string("state: ") + msg);
+}
+}  // namespace
+
+extern "C" {
+
+const char *ned_last_error(void) { return g_err.c_str(); }
+
+ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
+  if (!out || !d) { g_err = "null argument"; return NED_ERR_ARG; }
+  *out = nullptr;
+  ned_ctx *ctx = nullptr;
+  ned_status st = guard(nullptr, [&] {
+    ned::require(d->nao >= 1 && d->nao <= 64, "nao must be in 1..64");
+    ned::require(d->naux >= 1, "naux must be >= 1");
+    ned::require(d->k >= 1 && d->k <= ned::KMAX, "k must be in 1..8");
+    ned::require(d->nt >= d->k, "nt must be >= k");
+    ned::require(d->ntau >= d->k, "ntau must be >= k (src/params.h:56-102)");
+    ned::require(d->sig == -1, "only fermions (sig = -1) are supported");
+    int ndev = 0;
+    NED_HIP(hipGetDeviceCount(&ndev));
+    ned::require(device >= 0 && device < ndev, "no such HIP device");
+    NED_HIP(hipSetDevice(device));
+    ctx = new ned_ctx();
+    ned::Ctx &c = ctx->c;
+    c.nao = d->nao; c.naux = d->naux; c.nt = d->nt; c.ntau = d->ntau; c.k = d->k; c.sig = d->sig;
+    c.device = device;
+    c.es = c.nao * c.nao;
+    c.np = c.ntau + 1;
+    NED_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    const size_t es = c.es, ntri = (size_t)(c.nt + 1) * (c.nt + 2) / 2;
+    for (ned::Green *g : {&c.G, &c.S}) {
+      g->ret.ensure(ntri * es * sizeof(ned::z2));
+      g->les.ensure(ntri * es * sizeof(ned::z2));
+      g->tv.ensure((size_t)(c.nt + 1) * c.np * es * sizeof(ned::z2));
+      g->mat.ensure((size_t)c.np * es * sizeof(ned::z2));
+      NED_HIP(hipMemsetAsync(g->ret.p, 0, g->ret.bytes, c.stream));
+      NED_HIP(hipMemsetAsync(g->les.p, 0, g->les.bytes, c.stream));
+      NED_HIP(hipMemsetAsync(g->tv.p, 0, g->tv.bytes, c.stream));
+      NED_HIP(hipMemsetAsync(g->mat.p, 0, g->mat.bytes, c.stream));
+    }
+    c.hmf.ensure((size_t)(c.nt + 1) * es * sizeof(ned::z2));
+    NED_HIP(hipMemsetAsync(c.hmf.p, 0, c.hmf.bytes, c.stream));
+    c.Minv.ensure((size_t)(c.nt + 1) * es * sizeof(ned::z2));
+    c.minv_valid.assign(c.nt + 1, 0);
+    c.dy_info.ensure(sizeof(int));
+    // items per Sigma chunk (tests and tuning; 0 = sized to ~3 GiB of scratch)
+    if (const char *ch = std::getenv("NED_SIGMA_CHUNK")) c.sigma_chunk = std::atoi(ch);
+    NED_HIP(hipStreamSynchronize(c.stream));
+  });
+  if (st != NED_OK) {
+    if (ctx) {
+      if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+      delete ctx;
+    }
+    if (g_err.find("out of memory") != std::string::npos) st = NED_ERR_NOMEM;
+    return st;
+  }
+  *out = ctx;
+  return NED_OK;
+}
+
+ned_status ned_ctx_destroy(ned_ctx *ctx) {
+  if (!ctx) return NED_OK;
+  ned_status st = guard(ctx, [&] { NED_HIP(hipStreamSynchronize(ctx->c.stream)); });
+  (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+  return st;
+}
+
+ned_status ned_sync(ned_ctx *ctx) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] { NED_HIP(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+ned_status ned_set_coulomb(ned_ctx *ctx, const double *B) {
+  if (!ctx || !B) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] { ned::build_coulomb(ctx->c, B); });
+}
+
+ned_status ned_green_write(ned_ctx *ctx, int which, int comp, long long first, long long nb, const void *host) {
+  if (!ctx || (!host && nb > 0)) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::Ctx &c = ctx->c;
+    ned::require(which == NED_G || which == NED_SIGMA, "which must be NED_G or NED_SIGMA");
+    ned::DevBuf &b = component(c, which, comp);
+    ned::require(first >= 0 && nb >= 0 && first + nb <= nblocks_of(c, comp), "block range out of bounds");
+    const size_t bs = c.es * sizeof(ned::z2);
+    NED_HIP(hipMemcpyAsync((char *)b.p + first * bs, host, nb * bs, hipMemcpyHostToDevice, c.stream));
+    NED_HIP(hipStreamSynchronize(c.stream));
+    if (which == NED_SIGMA || (which == NED_G && comp == NED_MAT)) {
+      ned::invalidate_minv(c);
+      if (comp == NED_MAT) c.have_mix = false;
+    }
+  });
+}
+
+ned_status ned_green_read(ned_ctx *ctx, int which, int comp, long long first, long long nb, void *host) {
+  if (!ctx || (!host && nb > 0)) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::Ctx &c = ctx->c;
+    ned::require(which == NED_G || which == NED_SIGMA, "which must be NED_G or NED_SIGMA");
+    ned::DevBuf &b = component(c, which, comp);
+    ned::require(first >= 0 && nb >= 0 && first + nb <= nblocks_of(c, comp), "block range out of bounds");
+    const size_t bs = c.es * sizeof(ned::z2);
+    NED_HIP(hipMemcpyAsync(host, (char *)b.p + first * bs, nb * bs, hipMemcpyDeviceToHost, c.stream));
+    NED_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+ned_status ned_hmf_write(ned_ctx *ctx, int first, int count, const void *host) {
+  if (!ctx || (!host && count > 0)) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::Ctx &c = ctx->c;
+    ned::require(first >= 0 && count >= 0 && first + count <= c.nt + 1, "hmf range out of bounds");
+    const size_t bs = c.es * sizeof(ned::z2);
+    NED_HIP(hipMemcpyAsync((char *)c.hmf.p + first * bs, host, count * bs, hipMemcpyHostToDevice, c.stream));
+    NED_HIP(hipStreamSynchronize(c.stream));
+    for (int j = first; j < first + count; j++) c.minv_valid[j] = 0;
+  });
+}
+
+ned_status ned_hmf_read(ned_ctx *ctx, int first, int count, void *host) {
+  if (!ctx || (!host && count > 0)) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::Ctx &c = ctx->c;
+    ned::require(first >= 0 && count >= 0 && first + count <= c.nt + 1, "hmf range out of bounds");
+    const size_t bs = c.es * sizeof(ned::z2);
+    NED_HIP(hipMemcpyAsync(host, (char *)c.hmf.p + first * bs, count * bs, hipMemcpyDeviceToHost, c.stream));
+    NED_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+ned_status ned_sigma_gf2_decomp(ned_ctx *ctx, int tstp, int prec) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::require(prec >= 0 && prec <= 2, "prec must be NED_PREC_F64, _F32 or _BF16X3");
+    state(ctx->c.have_coulomb, "interaction not set (ned_set_coulomb)");
+    ned::sigma_gf2(ctx->c, tstp, (ned::Prec)prec);
+    if (tstp < (int)ctx->c.minv_valid.size()) ctx->c.minv_valid[tstp] = 0;
+  });
+}
+
+ned_status ned_hf_decomp(ned_ctx *ctx, int tstp, const double *h0) {
+  if (!ctx || !h0) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    state(ctx->c.have_coulomb, "interaction not set (ned_set_coulomb)");
+    ned::hf_decomp(ctx->c, tstp, h0);
+  });
+}
+
+ned_status ned_set_matsubara(ned_ctx *ctx, double beta) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::require(beta > 0 && std::isfinite(beta), "beta must be positive");
+    ned::set_matsubara(ctx->c, beta);
+  });
+}
+
+ned_status ned_dyson_step(ned_ctx *ctx, int tstp, double mu, double beta, double dt, double *err_out) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::require(dt > 0, "dt must be positive");
+    state(ctx->c.have_mix && ctx->c.mix_beta == beta, "ned_set_matsubara(beta) not called for this beta");
+    double e[3];
+    ned::dyson_step(ctx->c, tstp, mu, beta, dt, e);
+    if (err_out) { err_out[0] = e[0]; err_out[1] = e[1]; err_out[2] = e[2]; }
+    if (!std::isfinite(e[0] + e[1] + e[2])) throw std::runtime_error("non-finite update in dyson_step");
+  });
+}
+
+ned_status ned_extrapolate(ned_ctx *ctx, int tstp) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] { ned::extrapolate(ctx->c, tstp); });
+}
+
+ned_status ned_dyson_start(ned_ctx *ctx, double mu, double beta, double dt, double *err_out) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::require(dt > 0, "dt must be positive");
+    state(ctx->c.have_mix && ctx->c.mix_beta == beta, "ned_set_matsubara(beta) not called for this beta");
+    double e = ned::dyson_start(ctx->c, mu, beta, dt);
+    if (err_out) *err_out = e;
+    if (!std::isfinite(e)) throw std::runtime_error("non-finite update in dyson_start");
+  });
+}
+
+ned_status ned_do_boot(ned_ctx *ctx, const double *fock, double mu, double beta, double dt, int prec, int maxiter,
+                       double tol, int *iters_out, double *err_out) {
+  if (!ctx || !fock) { g_err = "null argument"; return NED_ERR_ARG; }
+  ned_status st = guard(ctx, [&] {
+    ned::Ctx &c = ctx->c;
+    ned::require(prec >= 0 && prec <= 2, "bad prec");
+    state(c.have_coulomb, "interaction not set (ned_set_coulomb)");
+    state(c.have_mix && c.mix_beta == beta, "ned_set_matsubara(beta) not called for this beta");
+    std::vector<ned::z2> h((size_t)(c.k + 1) * c.es);
+    for (int t = 0; t <= c.k; t++)
+      for (int e = 0; e < c.es; e++) h[(size_t)t * c.es + e] = {fock[e], 0.0};
+    NED_HIP(hipMemcpy(c.hmf.p, h.data(), h.size() * sizeof(ned::z2), hipMemcpyHostToDevice));
+    double err = 0;
+    int it = 0;
+    bool conv = false;
+    for (; it <= maxiter; it++) {
+      for (int t = 0; t <= c.k; t++) ned::sigma_gf2(c, t, (ned::Prec)prec);
+      err = ned::dyson_start(c, mu, beta, dt);
+      if (!std::isfinite(err)) throw std::runtime_error("non-finite update in bootstrap");
+      if (err < tol) { conv = true; it++; break; }
+    }
+    if (iters_out) *iters_out = it;
+    if (err_out) *err_out = err;
+    ned::invalidate_minv(c);
+    if (!conv) throw std::runtime_error("bootstrap did not converge");
+  });
+  return st;
+}
+
+ned_status ned_do_tstp(ned_ctx *ctx, int tstp, const double *h0, double mu, double beta, double dt, int prec,
+                       int corr_max, double corr_tol, int *iters_out, double *err_out) {
+  if (!ctx || !h0) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::Ctx &c = ctx->c;
+    ned::require(prec >= 0 && prec <= 2, "bad prec");
+    ned::require(corr_max >= 1, "corr_max must be >= 1");
+    state(c.have_coulomb, "interaction not set (ned_set_coulomb)");
+    state(c.have_mix && c.mix_beta == beta, "ned_set_matsubara(beta) not called for this beta");
+    ned::extrapolate(c, tstp);
+    double e[3] = {0, 0, 0}, err = 0;
+    int it = 0;
+    for (; it < corr_max;) {
+      ned::hf_decomp(c, tstp, h0);
+      ned::sigma_gf2(c, tstp, (ned::Prec)prec);
+      ned::dyson_step(c, tstp, mu, beta, dt, e);
+      err = e[0] + e[1] + e[2];
+      it++;
+      if (!std::isfinite(err)) throw std::runtime_error("non-finite update in corrector");
+      if (corr_tol > 0 && err < corr_tol) break;
+    }
+    if (iters_out) *iters_out = it;
+    if (err_out) *err_out = err;
+  });
+}
+
+ned_status ned_energy(ned_ctx *ctx, int tstp, const double *h0, double beta, double dt, double *out3) {
+  if (!ctx || !h0 || !out3) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::require(tstp >= 0 && tstp <= ctx->c.nt, "tstp out of range");
+    ned::energy(ctx->c, tstp, h0, beta, dt, out3);
+  });
+}
+
+ned_status ned_gll_nodes(int ntau, double *x) {
+  if (!x) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(nullptr, [&] {
+    ned::GLL g(ntau);
+    std::memcpy(x, g.x.data(), sizeof(double) * g.np);
+  });
+}
+
+ned_status ned_integrator(int k, double *interp, double *diff, double *integ, double *bd, double *start,
+                          double *Omega, double *ex) {
+  return guard(nullptr, [&] {
+    ned::Weights W(k);
+    const int k1 = k + 1;
+    if (interp) std::memcpy(interp, W.interp.data(), sizeof(double) * k1 * k1);
+    if (diff) std::memcpy(diff, W.diff.data(), sizeof(double) * k1 * k1);
+    if (integ) std::memcpy(integ, W.integ.data(), sizeof(double) * k1 * k1 * k1);
+    if (bd) std::memcpy(bd, W.bd.data(), sizeof(double) * (k + 2));
+    if (start) std::memcpy(start, W.start.data(), sizeof(double) * k1 * k1);
+    if (Omega) std::memcpy(Omega, W.Omega.data(), sizeof(double) * k1 * k1);
+    if (ex) std::memcpy(ex, W.ex.data(), sizeof(double) * k1);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// Device context: one propagated system (G, Sigma, mean field, interaction)
+// resident in HBM in the packing of the reference's green_func
+// (src/greens.cpp:177-195): ret(i,j<=i) at block i(i+1)/2+j, les(i<=j,j) at
+// block j(j+1)/2+i, tv(i,m) at block i*(ntau+1)+m, mat(m) at block m; every
+// block a row-major nao x nao complex128 matrix.
+#pragma once
+#include "common.h"
+#include <vector>
+#include <memory>
+
+namespace ned {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    release();
+    if (b == 0) return;
+    NED_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <typename T>
+  T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct Green {
+  DevBuf ret, les, tv, mat;
+};
+
+struct Ctx {
+  int nao = 0, naux = 0, nt = 0, ntau = 0, k = 0, sig = -1, device = 0;
+  int es = 0, np = 0;
+  hipStream_t stream = nullptr;
+  Green G, S;
+  DevBuf hmf;      // (nt+1) x es complex
+  DevBuf Bq;       // naux x nao x nao double, B[Q][i][j] = V(i,j,Q)
+  DevBuf Bq32;     // float copy
+  DevBuf U64, U32; // (nao^2 x nao^2): U'[(m,q)][(k,j)] = sum_Q B[Q][k][m] B[Q][q][j]
+  bool have_coulomb = false;
+  // Sigma scratch
+  DevBuf sg_items_A, sg_items_B, sg_W, sg_X, sg_H, sg_Pi, sg_E;
+  int sigma_chunk = 0;  // items per chunk (0 = choose automatically)
+  // Dyson state
+  DevBuf mixK;                // mixing kernel of G^M (np x np blocks)
+  DevBuf mass;                // (np x np) GLL mass matrix
+  DevBuf Minv;                // (nt+1) blocks: M_j^{-1} cache
+  std::vector<char> minv_valid;
+  double minv_mu = 0.0 / 0.0, minv_dt = 0.0 / 0.0;
+  double mix_beta = 0;
+  bool have_mix = false;
+  DevBuf dy_info, dy_js, dy_gA, dy_gB, dy_err, dy_Q, dy_X, dy_M, dy_Qs, dy_part, dy_scr, dy_Y;
+  DevBuf dy_la, dy_lb, dy_mixtmp, dy_h0, dy_xq, dy_bootI, dy_en;
+};
+
+size_t tri(int t);
+void require(bool cond, const char *msg);
+
+// Sigma: molGF2SolverDecomp::solve (src/mol2bSolver_decomp_impl.h:412-429)
+void sigma_gf2(Ctx &c, int tstp, Prec prec);
+void build_coulomb(Ctx &c, const double *Bq_host);
+
+}  // namespace ned

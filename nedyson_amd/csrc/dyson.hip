@@ -1,0 +1,1050 @@
+// Dyson timestep on the GPU in FP64: dyson::dyson_step (src/dyson_step.h:7-382)
+// with its history integrals (src/dyson_integrals.h), the predictor
+// dyson::Extrapolate (src/dyson.cpp:47-117), the HF mean field
+// (src/molHFSolver.cpp:81-110), the bootstrap dyson_start
+// (src/dyson_boot.h:6-221) and the energy (src/dyson_integrals.h:24-70).
+//
+// Volterra rows (ret) and columns (les) are solved right-looking: once the
+// block X_n is known, one launch adds its contribution to the history sum of
+// every later n' in parallel (one workgroup per n'), and the workgroup that
+// owns n+1 completes that block in the same launch.  The per-time matrices
+//   M_j = (mu + i bd_0/dt) 1 - hmf(j) - dt omega_0 Sigma^R(j,j)
+// that the reference LU-factorises inside both recurrences
+// (src/dyson_step.h:95-103, 259-278) depend only on the past time j, so their
+// inverses are computed once, in one batched launch, and cached.
+#include "dyson.h"
+#include "blockops.h"
+#include "quadrature.h"
+#include <cmath>
+#include <vector>
+#include <algorithm>
+
+namespace ned {
+
+WTab make_wtab(int k) {
+  require(k >= 1 && k <= KMAX, "integrator order k out of range (1..8)");
+  Weights W(k);
+  WTab t{};
+  t.k = k;
+  const int k1 = k + 1;
+  for (int i = 0; i < k1 * k1; i++) { t.start[i] = W.start[i]; t.Omega[i] = W.Omega[i]; t.pdiff[i] = W.diff[i]; }
+  for (int i = 0; i < k + 2; i++) t.bd[i] = W.bd[i];
+  for (int i = 0; i < k1; i++) t.ex[i] = W.ex[i];
+  return t;
+}
+
+namespace {
+
+__device__ __forceinline__ size_t dtri(int t) { return (size_t)t * (t + 1) / 2; }
+
+// gregory_weights(n, j), src/integration.h:78-91
+__device__ __forceinline__ double greg(const WTab &w, int n, int j) {
+  const int k = w.k, k1 = k + 1;
+  const double *om = w.Omega + k * k1;
+  if (n <= k && j <= k) return w.start[n * k1 + j];
+  if (n <= 2 * k + 1) return j <= k ? w.Omega[(n - k - 1) * k1 + j] : om[n - j];
+  if (j <= k) return om[j];
+  if (j < n - k) return 1.0;
+  return om[n - j];
+}
+
+struct Pt {
+  z2 *Gret, *Gles, *Gtv;
+  const z2 *Gmat;
+  const z2 *Sret, *Sles, *Stv;
+  const z2 *hmf;
+  int n, np, T, sig;
+  double mu, beta, dt;
+  __device__ size_t es() const { return (size_t)n * n; }
+  __device__ z2 *gr(int i, int j) const { return Gret + (dtri(i) + j) * es(); }
+  __device__ z2 *gl(int i, int j) const { return Gles + (dtri(j) + i) * es(); }
+  __device__ z2 *gtv(int i, int m) const { return Gtv + ((size_t)i * np + m) * es(); }
+  __device__ const z2 *sr(int i, int j) const { return Sret + (dtri(i) + j) * es(); }
+  __device__ const z2 *sl(int i, int j) const { return Sles + (dtri(j) + i) * es(); }
+  __device__ const z2 *stv(int i, int m) const { return Stv + ((size_t)i * np + m) * es(); }
+  __device__ const z2 *h(int t) const { return hmf + (size_t)t * es(); }
+};
+
+// ------------------------------------------------------------ batched solve
+// A_s X = B_s for every system s (N x N, R right-hand sides), Gaussian
+// elimination with partial pivoting, one workgroup per system, in place in
+// global memory (A destroyed, B <- X).  The reference uses Eigen::FullPivLU;
+// on these diagonally dominant systems both agree to rounding.
+__global__ void __launch_bounds__(256) kern_gesv(z2 *A, long long sA, z2 *B, long long sB, int N, int R, int *info) {
+  z2 *a = A + (size_t)blockIdx.x * sA, *b = B + (size_t)blockIdx.x * sB;
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  __shared__ z2 f[320];
+  const int tid = threadIdx.x;
+  for (int c = 0; c < N; c++) {
+    double best = -1; int bi = c;
+    for (int r = c + tid; r < N; r += 256) {
+      z2 v = a[(size_t)r * N + c];
+      double m = fabs(v.x) + fabs(v.y);
+      if (m > best) { best = m; bi = r; }
+    }
+    rv[tid] = best; ri[tid] = bi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w && rv[tid + w] > rv[tid]) { rv[tid] = rv[tid + w]; ri[tid] = ri[tid + w]; }
+      __syncthreads();
+    }
+    const int p = ri[0];
+    if (rv[0] <= 0.0) { if (tid == 0) atomicAdd(info, 1); }
+    if (p != c) {
+      for (int j = tid; j < N; j += 256) { z2 t = a[(size_t)c * N + j]; a[(size_t)c * N + j] = a[(size_t)p * N + j]; a[(size_t)p * N + j] = t; }
+      for (int j = tid; j < R; j += 256) { z2 t = b[(size_t)c * R + j]; b[(size_t)c * R + j] = b[(size_t)p * R + j]; b[(size_t)p * R + j] = t; }
+    }
+    __syncthreads();
+    const z2 piv = a[(size_t)c * N + c];
+    const double den = piv.x * piv.x + piv.y * piv.y;
+    const z2 inv = den > 0 ? z2{piv.x / den, -piv.y / den} : z2{0.0, 0.0};
+    for (int r = c + 1 + tid; r < N; r += 256) f[r] = zmul(a[(size_t)r * N + c], inv);
+    __syncthreads();
+    const int rows = N - c - 1, colsA = N - c - 1;
+    for (long long e = tid; e < (long long)rows * (colsA + R); e += 256) {
+      int r = c + 1 + (int)(e / (colsA + R));
+      int j = (int)(e % (colsA + R));
+      z2 fr = f[r];
+      if (j < colsA) {
+        int jj = c + 1 + j;
+        a[(size_t)r * N + jj] = zsub(a[(size_t)r * N + jj], zmul(fr, a[(size_t)c * N + jj]));
+      } else {
+        int jj = j - colsA;
+        b[(size_t)r * R + jj] = zsub(b[(size_t)r * R + jj], zmul(fr, b[(size_t)c * R + jj]));
+      }
+    }
+    __syncthreads();
+  }
+  for (int c = N - 1; c >= 0; c--) {
+    const z2 piv = a[(size_t)c * N + c];
+    const double den = piv.x * piv.x + piv.y * piv.y;
+    const z2 inv = den > 0 ? z2{piv.x / den, -piv.y / den} : z2{0.0, 0.0};
+    for (int j = tid; j < R; j += 256) b[(size_t)c * R + j] = zmul(b[(size_t)c * R + j], inv);
+    __syncthreads();
+    for (long long e = tid; e < (long long)c * R; e += 256) {
+      int r = (int)(e / R), j = (int)(e % R);
+      b[(size_t)r * R + j] = zsub(b[(size_t)r * R + j], zmul(a[(size_t)r * N + c], b[(size_t)c * R + j]));
+    }
+    __syncthreads();
+  }
+}
+
+// M_j for the listed j into A (and identity into B), for inversion
+__global__ void kern_build_mj(Pt p, const int *js, int cnt, double bd0, double om0, z2 *A, z2 *B) {
+  const size_t es = p.es();
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)cnt * es) return;
+  int s = (int)(idx / es), e = (int)(idx % es), a = e / p.n, b = e % p.n;
+  int j = js[s];
+  z2 hv = p.h(j)[e], sv = p.sr(j, j)[e];
+  z2 v = {-hv.x - p.dt * om0 * sv.x, -hv.y - p.dt * om0 * sv.y};
+  if (a == b) { v.x += p.mu; v.y += bd0 / p.dt; }
+  A[(size_t)s * es + e] = v;
+  B[(size_t)s * es + e] = (a == b) ? z2{1.0, 0.0} : z2{0.0, 0.0};
+}
+
+__global__ void kern_scatter_blocks(const z2 *src, const int *js, int cnt, size_t es, z2 *dst) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)cnt * es) return;
+  int s = (int)(idx / es);
+  size_t e = idx % es;
+  dst[(size_t)js[s] * es + e] = src[idx];
+}
+
+// ------------------------------------------------------------ retarded
+// k x k block start-up system of src/dyson_step.h:28-65 into M (kn x kn) and
+// Q (kn x n); also sets G^R(T,T) = -i.
+__global__ void __launch_bounds__(256) kern_ret_start(Pt p, WTab w, z2 *M, z2 *Q) {
+  const int n = p.n, k = w.k, kn = k * n, T = p.T, k1 = k + 1;
+  const int tid = threadIdx.x;
+  z2 *Gtt = p.gr(T, T);
+  for (int e = tid; e < n * n; e += 256) Gtt[e] = (e / n == e % n) ? z2{0.0, -1.0} : z2{0.0, 0.0};
+  __syncthreads();
+  // M
+  for (long long e = tid; e < (long long)kn * kn; e += 256) {
+    int row = (int)(e / kn), col = (int)(e % kn);
+    int nn = row / n + 1, a = row % n, l = col / n + 1, b = col % n;
+    z2 v = {0.0, 0.0};
+    // -(-i/dt) pd(n,l) I = (i/dt) pd I
+    if (a == b) v.y += w.pdiff[nn * k1 + l] / p.dt;
+    if (l == nn) {
+      z2 hv = p.h(T - nn)[b * n + a];
+      v.x -= hv.x; v.y -= hv.y;
+      if (a == b) v.x += p.mu;
+    }
+    double wt = p.dt * greg(w, nn, l);
+    if (l <= nn) {
+      z2 s = p.sr(T - l, T - nn)[b * n + a];
+      v.x -= wt * s.x; v.y -= wt * s.y;
+    } else {
+      z2 s = p.sr(T - nn, T - l)[a * n + b];
+      v.x += wt * s.x; v.y -= wt * s.y;
+    }
+    M[e] = v;
+  }
+  // Q: (-i/dt) pd(n,0) G(T,T)^T + dt w(n,0) (G(T,T) Sigma^R(T,T-n))^T
+  for (long long e = tid; e < (long long)kn * n; e += 256) {
+    int row = (int)(e / n), b = (int)(e % n);
+    int nn = row / n + 1, a = row % n;
+    z2 g = Gtt[b * n + a];
+    double pd = w.pdiff[nn * k1 + 0] / p.dt;
+    z2 v = {pd * g.y, -pd * g.x};  // -i * pd * g
+    const z2 *S = p.sr(T, T - nn);
+    z2 s = {0.0, 0.0};
+    for (int c = 0; c < n; c++) s = zadd(s, zmul(Gtt[b * n + c], S[c * n + a]));
+    double wt = p.dt * greg(w, nn, 0);
+    v.x += wt * s.x; v.y += wt * s.y;
+    Q[e] = v;
+  }
+}
+
+// G^R(T,T-l-1) = X_l^T, l = 0..k-1  (one workgroup per l)
+__global__ void __launch_bounds__(256) kern_ret_start_store(Pt p, const z2 *X, double *err) {
+  __shared__ double red[256];
+  const int n = p.n, l = blockIdx.x;
+  BlkAcc acc;
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  const z2 *x = X + (size_t)l * n * n;
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      int i = ty + 16 * r, j = tx + 16 * c;
+      acc.v[r][c] = (i < n && j < n) ? x[j * n + i] : z2{0.0, 0.0};
+    }
+  z2 *g = p.gr(p.T, p.T - l - 1);
+  double d2 = blk_dist2(acc, n, g, red);
+  blk_store(acc, n, g);
+  if (tid == 0) atomicAdd(err, sqrt(d2));
+}
+
+// complete G^R(T,T-m) from its finished history sum (in acc)
+__device__ void ret_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s, int m, z2 *Qm, const z2 *Minv,
+                             double *red, double *err) {
+  const int n = p.n, T = p.T;
+  blk_scale(acc, z2{p.dt, 0.0});
+  for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, p.gr(T, T - m + l), 0, z2{0.0, -w.bd[l] / p.dt});
+  __syncthreads();
+  blk_store(acc, n, Qm);
+  __syncthreads();
+  BlkAcc x;
+  x.zero();
+  blk_mac(x, s, n, Qm, 0, Minv + (size_t)(T - m) * n * n, 0, z2{1.0, 0.0});
+  z2 *g = p.gr(T, T - m);
+  double d2 = blk_dist2(x, n, g, red);
+  blk_store(x, n, g);
+  if (threadIdx.x == 0) atomicAdd(err, sqrt(d2));
+}
+
+// history sums from the start-up blocks l = 0..k, for m = k+1..T
+__global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, WTab w, z2 *Q, const z2 *Minv, double *err) {
+  __shared__ BlkLds s;
+  __shared__ double red[256];
+  const int m = w.k + 1 + blockIdx.x, T = p.T, n = p.n;
+  if (m > T) return;
+  BlkAcc acc;
+  acc.zero();
+  for (int l = 0; l <= w.k; l++) blk_mac(acc, s, n, p.gr(T, T - l), 0, p.sr(T - l, T - m), 0, z2{greg(w, m, l), 0.0});
+  z2 *Qm = Q + (size_t)m * n * n;
+  if (m == w.k + 1) ret_finalize(p, w, acc, s, m, Qm, Minv, red, err);
+  else blk_store(acc, n, Qm);
+}
+
+// add block nn's contribution to every later history sum; finish nn+1
+__global__ void __launch_bounds__(256) kern_ret_step(Pt p, WTab w, int nn, z2 *Q, const z2 *Minv, double *err) {
+  __shared__ BlkLds s;
+  __shared__ double red[256];
+  const int m = nn + 1 + blockIdx.x, T = p.T, n = p.n;
+  if (m > T) return;
+  z2 *Qm = Q + (size_t)m * n * n;
+  BlkAcc acc;
+  blk_load(acc, n, Qm);
+  blk_mac(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
+  if (m == nn + 1) ret_finalize(p, w, acc, s, m, Qm, Minv, red, err);
+  else blk_store(acc, n, Qm);
+}
+
+// ------------------------------------------------------------ mixing (tv)
+// src/dyson_step.h:130-169 with Ctv_tstp/CTV1/Conv.mixing
+// (src/dyson_integrals.h:80-116, 254-324): partial sums over chunks of the
+// time history, one workgroup per (node m, chunk s).
+__global__ void __launch_bounds__(256) kern_tv_rhs(Pt p, WTab w, const z2 *mixK, int nchunk, z2 *part) {
+  __shared__ BlkLds s;
+  const int m = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
+  const int nt_terms = T;  // j = 0..T-1 (G^rceil(T,.) is the unknown)
+  const int per = (nt_terms + nchunk - 1) / nchunk;
+  const int j0 = ch * per, j1 = min(nt_terms, j0 + per);
+  BlkAcc acc;
+  acc.zero();
+  for (int j = j0; j < j1; j++)
+    blk_mac(acc, s, n, p.sr(T, j), 0, p.gtv(j, m), 0, z2{p.dt * greg(w, T, j), 0.0});
+  if (ch == 0) {
+    for (int jp = 0; jp < np; jp++)
+      blk_mac(acc, s, n, p.stv(T, jp), 0, mixK + ((size_t)m * np + jp) * p.es(), 0, z2{1.0, 0.0});
+    for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, p.gtv(T - l, m), 0, z2{0.0, -w.bd[l] / p.dt});
+  }
+  blk_store(acc, n, part + ((size_t)ch * np + m) * p.es());
+}
+
+__global__ void kern_tv_matrix(Pt p, WTab w, z2 *A, z2 *B) {
+  const int n = p.n;
+  const size_t es = p.es();
+  for (int e = threadIdx.x; e < (int)es; e += blockDim.x) {
+    int a = e / n, b = e % n;
+    z2 hv = p.h(p.T)[e], sv = p.sr(p.T, p.T)[e];
+    double om0 = w.Omega[w.k * (w.k + 1)];
+    z2 v = {-hv.x - p.dt * om0 * sv.x, -hv.y - p.dt * om0 * sv.y};
+    if (a == b) { v.x += p.mu; v.y += w.bd[0] / p.dt; }
+    A[e] = v;
+    B[e] = (a == b) ? z2{1.0, 0.0} : z2{0.0, 0.0};
+  }
+}
+
+__global__ void __launch_bounds__(256) kern_tv_fin(Pt p, const z2 *part, int nchunk, const z2 *Minv, z2 *scr,
+                                                  double *err) {
+  __shared__ BlkLds s;
+  __shared__ double red[256];
+  const int m = blockIdx.x, n = p.n, np = p.np;
+  BlkAcc acc;
+  acc.zero();
+  for (int ch = 0; ch < nchunk; ch++) blk_axpy(acc, n, part + ((size_t)ch * np + m) * p.es(), 0, z2{1.0, 0.0});
+  z2 *sc = scr + (size_t)m * p.es();
+  blk_store(acc, n, sc);
+  __syncthreads();
+  BlkAcc x;
+  x.zero();
+  blk_mac(x, s, n, Minv, 0, sc, 0, z2{1.0, 0.0});
+  z2 *g = p.gtv(p.T, m);
+  double d2 = blk_dist2(x, n, g, red);
+  blk_store(x, n, g);
+  if (threadIdx.x == 0) atomicAdd(err, sqrt(d2));
+}
+
+// ------------------------------------------------------------ lesser
+// Y_tau = beta/2 sum_tau' mass[tau][tau'] * sig i G^rceil(T, ntau-tau')^H
+// (the right factor of Cles3_tstp, src/dyson_integrals.h:393-427)
+__global__ void kern_les_Y(Pt p, const double *mass, z2 *Y) {
+  const int tau = blockIdx.x, n = p.n, np = p.np;
+  const size_t es = p.es();
+  for (int e = threadIdx.x; e < (int)es; e += blockDim.x) {
+    int a = e / n, b = e % n;
+    z2 acc = {0.0, 0.0};
+    for (int t2 = 0; t2 < np; t2++) {
+      double wgt = 0.5 * p.beta * mass[(size_t)tau * np + t2];
+      z2 g = p.gtv(p.T, np - 1 - t2)[b * n + a];  // conj-transpose element
+      // sig * i * conj(g)
+      z2 v = {p.sig * g.y, p.sig * g.x};
+      acc.x += wgt * v.x; acc.y += wgt * v.y;
+    }
+    Y[(size_t)tau * es + e] = acc;
+  }
+}
+
+// Q_row = Cles3 + Cles2 (src/dyson_integrals.h:336-380, 393-427), partial
+// sums over chunks of the (tau, j) terms.
+__global__ void __launch_bounds__(256) kern_les_rhs(Pt p, WTab w, const z2 *Y, int nchunk, z2 *part) {
+  __shared__ BlkLds s;
+  const int row = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
+  const int nterms = np + T + 1;
+  const int per = (nterms + nchunk - 1) / nchunk;
+  const int t0 = ch * per, t1 = min(nterms, t0 + per);
+  BlkAcc acc;
+  acc.zero();
+  for (int t = t0; t < t1; t++) {
+    if (t < np) {
+      blk_mac(acc, s, n, p.stv(row, t), 0, Y + (size_t)t * p.es(), 0, z2{1.0, 0.0});
+    } else {
+      int j = t - np;
+      double wt = p.dt * greg(w, T, j);
+      // X_j = dt w(T,j) G^R(T,j)^H ; Sigma^<(row,j) for j >= row, -Sigma^<(j,row)^H for j < row
+      if (j >= row) blk_mac(acc, s, n, p.sl(row, j), 0, p.gr(T, j), 1, z2{wt, 0.0});
+      else blk_mac(acc, s, n, p.sl(j, row), 1, p.gr(T, j), 1, z2{-wt, 0.0});
+    }
+  }
+  blk_store(acc, n, part + ((size_t)ch * (T + 1) + row) * p.es());
+}
+
+__global__ void kern_sum_parts(const z2 *part, int nchunk, long long nel, z2 *out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  z2 s = {0.0, 0.0};
+  for (int c = 0; c < nchunk; c++) s = zadd(s, part[(size_t)c * nel + i]);
+  out[i] = s;
+}
+
+// start-up of src/dyson_step.h:178-251: initial condition, kn x kn system
+__global__ void __launch_bounds__(256) kern_les_start(Pt p, WTab w, const z2 *Q, z2 *X, z2 *M, z2 *Qs, double *err) {
+  __shared__ double red[256];
+  const int n = p.n, k = w.k, kn = k * n, T = p.T, k1 = k + 1, tid = threadIdx.x;
+  // G^<(0,T) = -G^rceil(T,0)^H ; X_0 = same
+  {
+    BlkAcc acc;
+    acc.zero();
+    blk_axpy(acc, n, p.gtv(T, 0), 1, z2{-1.0, 0.0});
+    z2 *g = p.gl(0, T);
+    double d2 = blk_dist2(acc, n, g, red);
+    blk_store(acc, n, g);
+    blk_store(acc, n, X);
+    if (tid == 0) atomicAdd(err, sqrt(d2));
+  }
+  __syncthreads();
+  const z2 *X0 = X;
+  for (long long e = tid; e < (long long)kn * kn; e += 256) {
+    int row = (int)(e / kn), col = (int)(e % kn);
+    int m = row / n + 1, a = row % n, l = col / n + 1, b = col % n;
+    z2 v = {0.0, 0.0};
+    if (a == b) v.y += w.pdiff[m * k1 + l] / p.dt;   // (i/dt) pd(m,l)
+    if (m == l) {
+      z2 hv = p.h(l)[a * n + b];
+      v.x -= hv.x; v.y -= hv.y;
+      if (a == b) v.x += p.mu;
+    }
+    double wt = p.dt * greg(w, m, l);
+    if (m >= l) {
+      z2 s = p.sr(m, l)[a * n + b];
+      v.x -= wt * s.x; v.y -= wt * s.y;
+    } else {
+      z2 s = p.sr(l, m)[b * n + a];
+      v.x += wt * s.x; v.y -= wt * s.y;
+    }
+    M[e] = v;
+  }
+  for (long long e = tid; e < (long long)kn * n; e += 256) {
+    int row = (int)(e / n), b = (int)(e % n);
+    int m = row / n + 1, a = row % n;
+    z2 v = Q[(size_t)m * n * n + a * n + b];
+    double pd = w.pdiff[m * k1 + 0] / p.dt;
+    z2 x0 = X0[a * n + b];
+    v.x += pd * x0.y; v.y -= pd * x0.x;   // - i pd x0
+    const z2 *S = p.sr(m, 0);
+    z2 sx = {0.0, 0.0};
+    for (int c = 0; c < n; c++) sx = zadd(sx, zmul(S[a * n + c], X0[c * n + b]));
+    double wt = p.dt * greg(w, m, 0);
+    v.x += wt * sx.x; v.y += wt * sx.y;
+    Qs[e] = v;
+  }
+}
+
+__device__ void les_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s, int m, z2 *Qm, z2 *X,
+                             const z2 *Minv) {
+  const int n = p.n;
+  for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, X + (size_t)(m - l) * p.es(), 0, z2{0.0, -w.bd[l] / p.dt});
+  __syncthreads();
+  blk_store(acc, n, Qm);
+  __syncthreads();
+  BlkAcc x;
+  x.zero();
+  blk_mac(x, s, n, Minv + (size_t)m * p.es(), 0, Qm, 0, z2{1.0, 0.0});
+  blk_store(x, n, X + (size_t)m * p.es());
+}
+
+__global__ void __launch_bounds__(256) kern_les_qinit(Pt p, WTab w, z2 *Q, z2 *X, const z2 *Minv) {
+  __shared__ BlkLds s;
+  const int m = w.k + 1 + blockIdx.x, T = p.T, n = p.n;
+  if (m > T) return;
+  z2 *Qm = Q + (size_t)m * p.es();
+  BlkAcc acc;
+  blk_load(acc, n, Qm);
+  for (int l = 0; l <= w.k; l++)
+    blk_mac(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+  if (m == w.k + 1 && m <= T - 1) les_finalize(p, w, acc, s, m, Qm, X, Minv);
+  else blk_store(acc, n, Qm);
+}
+
+__global__ void __launch_bounds__(256) kern_les_step(Pt p, WTab w, int l, z2 *Q, z2 *X, const z2 *Minv) {
+  __shared__ BlkLds s;
+  const int m = l + 1 + blockIdx.x, T = p.T, n = p.n;
+  if (m > T) return;
+  z2 *Qm = Q + (size_t)m * p.es();
+  BlkAcc acc;
+  blk_load(acc, n, Qm);
+  blk_mac(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+  if (m == l + 1 && m <= T - 1) les_finalize(p, w, acc, s, m, Qm, X, Minv);
+  else blk_store(acc, n, Qm);
+}
+
+// diagonal step, src/dyson_step.h:283-310
+__global__ void __launch_bounds__(256) kern_les_diag(Pt p, WTab w, z2 *Q, z2 *X, z2 *scr) {
+  __shared__ BlkLds s;
+  const int T = p.T, n = p.n;
+  const size_t es = p.es();
+  z2 *XT = X + (size_t)T * es;
+  BlkAcc acc;
+  acc.zero();
+  for (int jj = 0; jj <= w.k; jj++) blk_axpy(acc, n, p.gl(T - jj - 1, T - jj - 1), 0, z2{w.ex[jj], 0.0});
+  blk_store(acc, n, XT);
+  __syncthreads();
+  // Q_T += dt w(T,T) Sigma^R(T,T) X_T + (hmf(T) - mu) X_T
+  blk_load(acc, n, Q + (size_t)T * es);
+  blk_mac(acc, s, n, p.sr(T, T), 0, XT, 0, z2{p.dt * greg(w, T, T), 0.0});
+  blk_mac(acc, s, n, p.h(T), 0, XT, 0, z2{1.0, 0.0});
+  blk_axpy(acc, n, XT, 0, z2{-p.mu, 0.0});
+  __syncthreads();
+  blk_store(acc, n, scr);
+  __syncthreads();
+  // Q0 = -i (Q + Q^H) - 1/dt sum_l bd_l G^<(T-l,T-l);  X_T = Q0 dt / bd_0
+  BlkAcc q0;
+  q0.zero();
+  blk_axpy(q0, n, scr, 0, z2{0.0, -1.0});
+  blk_axpy(q0, n, scr, 1, z2{0.0, -1.0});
+  for (int l = 1; l <= w.k + 1; l++) blk_axpy(q0, n, p.gl(T - l, T - l), 0, z2{-w.bd[l] / p.dt, 0.0});
+  blk_scale(q0, z2{p.dt / w.bd[0], 0.0});
+  __syncthreads();
+  blk_store(q0, n, XT);
+}
+
+__global__ void __launch_bounds__(256) kern_les_store(Pt p, const z2 *X, double *err) {
+  __shared__ double red[256];
+  const int l = blockIdx.x, n = p.n;
+  BlkAcc acc;
+  blk_load(acc, n, X + (size_t)l * p.es());
+  z2 *g = p.gl(l, p.T);
+  double d2 = blk_dist2(acc, n, g, red);
+  blk_store(acc, n, g);
+  if (threadIdx.x == 0) atomicAdd(err, sqrt(d2));
+}
+
+// ------------------------------------------------------------ predictor
+// dyson::Extrapolate (src/dyson.cpp:47-117); one thread per output element
+__global__ void kern_extrap(Pt p, WTab w, int which) {
+  const int n = p.n, k = w.k, T = p.T, np = p.np;
+  const size_t es = p.es();
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nblk = which == 0 ? np : T + 1;
+  if (idx >= (long long)nblk * es) return;
+  const int l = (int)(idx / es), e = (int)(idx % es), a = e / n, b = e % n;
+  const double *ex = w.ex;
+  z2 r = {0.0, 0.0};
+  if (which == 0) {  // tv
+    for (int j = 0; j <= k; j++) r = zadd(r, zscale(p.gtv(T - j - 1, l)[e], ex[j]));
+    p.gtv(T, l)[e] = r;
+  } else if (which == 1) {  // ret column index l
+    if (l > k) {  // l = T - l' with l' in [0, T-k-1]
+      const int lp = T - l;
+      for (int j = 0; j <= k; j++) r = zadd(r, zscale(p.gr(T - j - 1, T - lp - j - 1)[e], ex[j]));
+    } else {
+      const int jcut = (l <= T - k - 1) ? k : (T - l - 1);
+      for (int j = 0; j <= jcut; j++) r = zadd(r, zscale(p.gr(T - j - 1, l)[e], ex[j]));
+      for (int j = jcut + 1; j <= k; j++) r = zsub(r, zscale(zconj(p.gr(l, T - j - 1)[b * n + a]), ex[j]));
+    }
+    p.gr(T, l)[e] = r;
+  } else {  // les row index l
+    if (l == 0) {
+      r = zscale(zconj(p.gtv(T, 0)[b * n + a]), -1.0);
+    } else if (l <= k) {
+      const int jcut = (k > T - l - 1) ? (T - l - 1) : k;
+      for (int j = 0; j <= jcut; j++) r = zadd(r, zscale(p.gl(l, T - 1 - j)[e], ex[j]));
+      for (int j = jcut + 1; j <= k; j++) r = zsub(r, zscale(zconj(p.gl(T - 1 - j, l)[b * n + a]), ex[j]));
+    } else {
+      for (int j = 0; j <= k; j++) r = zadd(r, zscale(p.gl(l - j - 1, T - j - 1)[e], ex[j]));
+    }
+    p.gl(l, T)[e] = r;
+  }
+}
+
+// ------------------------------------------------------------ HF
+// X_Q = sum_kl rho_kl B[Q][k][l]; rho = i sig G^<(T,T)
+__global__ void kern_hf_x(Pt p, const double *Bq, z2 *Xq) {
+  __shared__ double rx[256], ry[256];
+  const int Q = blockIdx.x, n = p.n;
+  const size_t es = p.es();
+  const z2 *gl = p.gl(p.T, p.T);
+  double sx = 0, sy = 0;
+  for (int e = threadIdx.x; e < (int)es; e += 256) {
+    z2 g = gl[e];
+    z2 rho = {-p.sig * g.y, p.sig * g.x};
+    double v = Bq[(size_t)Q * es + e];
+    sx += rho.x * v; sy += rho.y * v;
+  }
+  rx[threadIdx.x] = sx; ry[threadIdx.x] = sy;
+  __syncthreads();
+  for (int w2 = 128; w2 > 0; w2 >>= 1) {
+    if (threadIdx.x < w2) { rx[threadIdx.x] += rx[threadIdx.x + w2]; ry[threadIdx.x] += ry[threadIdx.x + w2]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) Xq[Q] = {rx[0], ry[0]};
+}
+
+// hmf(T)_ij = h0_ij + 2 sum_Q X_Q B[Q][i][j] - sum_(l,k) rho_kl U'[(l,k)][(i,j)]
+__global__ void kern_hf_h(Pt p, const double *h0, const double *Bq, const double *U, const z2 *Xq, int naux, z2 *hmf) {
+  const int n = p.n;
+  const size_t es = p.es();
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)es) return;
+  z2 v = {h0[e], 0.0};
+  for (int Q = 0; Q < naux; Q++) {
+    double b = Bq[(size_t)Q * es + e];
+    v.x += 2 * Xq[Q].x * b; v.y += 2 * Xq[Q].y * b;
+  }
+  const z2 *gl = p.gl(p.T, p.T);
+  for (int l = 0; l < n; l++)
+    for (int k = 0; k < n; k++) {
+      z2 g = gl[k * n + l];
+      z2 rho = {-p.sig * g.y, p.sig * g.x};
+      double u = U[((size_t)l * n + k) * es + e];
+      v.x -= rho.x * u; v.y -= rho.y * u;
+    }
+  hmf[(size_t)p.T * es + e] = v;
+}
+
+// ------------------------------------------------------------ energy
+// partial traces of energy_conv's history terms (src/dyson_integrals.h:37-58)
+__global__ void kern_energy(Pt p, WTab w, int top, const double *mass, double *out) {
+  __shared__ double rx[256], ry[256];
+  const int n = p.n, T = p.T, np = p.np;
+  const size_t es = p.es();
+  double sx = 0, sy = 0;
+  const int i = blockIdx.x;
+  auto trT = [&](const z2 *S, const z2 *G, int e) {  // S_ab G_ba
+    int a = e / n, b = e % n;
+    return zmul(S[e], G[b * n + a]);
+  };
+  if (i <= top) {
+    double wt = greg(w, T, i);
+    for (int e = threadIdx.x; e < (int)es; e += 256) {
+      z2 t1, t2;
+      if (i <= T) {
+        t1 = zscale(trT(p.sr(T, i), p.gl(i, T), e), wt);
+        t2 = zscale(zconj(trT(p.sl(i, T), p.gr(T, i), e)), -wt);
+      } else {
+        t1 = zscale(zconj(trT(p.sr(i, T), p.gl(T, i), e)), wt);
+        t2 = zscale(trT(p.sl(T, i), p.gr(i, T), e), -wt);
+      }
+      sx += (t1.x + t2.x) * p.dt; sy += (t1.y + t2.y) * p.dt;
+    }
+  } else {
+    // Conv.energy row: tau index jt = i - top - 1
+    int jt = i - top - 1;
+    for (int e = threadIdx.x; e < (int)es; e += 256) {
+      z2 S = p.stv(T, jt)[e];
+      for (int kk = 0; kk < np; kk++) {
+        double wq = 0.5 * p.beta * mass[(size_t)jt * np + (np - 1 - kk)];
+        z2 g = zconj(p.gtv(T, kk)[e]);
+        z2 v = zmul(S, g);
+        // res2 = -sig * (...) then times -i
+        double f = -p.sig * wq;
+        sx += f * v.y; sy += -f * v.x;
+      }
+    }
+  }
+  rx[threadIdx.x] = sx; ry[threadIdx.x] = sy;
+  __syncthreads();
+  for (int w2 = 128; w2 > 0; w2 >>= 1) {
+    if (threadIdx.x < w2) { rx[threadIdx.x] += rx[threadIdx.x + w2]; ry[threadIdx.x] += ry[threadIdx.x + w2]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[2 * i] = rx[0]; out[2 * i + 1] = ry[0]; }
+}
+
+// ------------------------------------------------------------ bootstrap
+// src/dyson_boot.h:93-179: boundary condition, then the kn x kn system (the
+// same for every tau node) with one right-hand side per node.
+__global__ void kern_boot_tv0(Pt p, double *err) {
+  __shared__ double red[256];
+  const int m = blockIdx.x, n = p.n;
+  BlkAcc acc;
+  acc.zero();
+  blk_axpy(acc, n, p.Gmat + (size_t)(p.np - 1 - m) * p.es(), 0, z2{0.0, (double)p.sig});
+  z2 *g = p.gtv(0, m);
+  double d2 = blk_dist2(acc, n, g, red);
+  blk_store(acc, n, g);
+  if (threadIdx.x == 0) atomicAdd(err, sqrt(d2));
+}
+
+__global__ void kern_boot_matrix(Pt p, WTab w, z2 *M, z2 *B) {
+  const int n = p.n, k = w.k, kn = k * n, k1 = k + 1;
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)kn * kn) return;
+  int row = (int)(e / kn), col = (int)(e % kn);
+  int nn = row / n + 1, a = row % n, l = col / n + 1, b = col % n;
+  z2 v = {0.0, 0.0};
+  if (a == b) v.y += w.pdiff[nn * k1 + l] / p.dt;
+  if (l == nn) {
+    z2 hv = p.h(l)[a * n + b];
+    v.x -= hv.x; v.y -= hv.y;
+    if (a == b) v.x += p.mu;
+  }
+  double wt = p.dt * greg(w, nn, l);
+  if (nn >= l) {
+    z2 s = p.sr(nn, l)[a * n + b];
+    v.x -= wt * s.x; v.y -= wt * s.y;
+  } else {
+    z2 s = p.sr(l, nn)[b * n + a];
+    v.x += wt * s.x; v.y -= wt * s.y;
+  }
+  M[e] = v;
+  B[e] = (row == col) ? z2{1.0, 0.0} : z2{0.0, 0.0};
+}
+
+// right-hand side of node m, step nn: tv_it_conv + derivative + integral terms
+__global__ void __launch_bounds__(256) kern_boot_rhs(Pt p, WTab w, const z2 *mixK, z2 *R) {
+  __shared__ BlkLds s;
+  const int m = blockIdx.x, nn = blockIdx.y + 1, n = p.n, np = p.np, k1 = w.k + 1;
+  BlkAcc acc;
+  acc.zero();
+  for (int i = 0; i < np; i++) blk_mac(acc, s, n, p.stv(nn, i), 0, mixK + ((size_t)m * np + i) * p.es(), 0, z2{1.0, 0.0});
+  const z2 *g0 = p.gtv(0, m);
+  blk_axpy(acc, n, g0, 0, z2{0.0, -w.pdiff[nn * k1] / p.dt});
+  blk_mac(acc, s, n, p.sr(nn, 0), 0, g0, 0, z2{p.dt * greg(w, nn, 0), 0.0});
+  blk_store(acc, n, R + ((size_t)(nn - 1) * np + m) * p.es());
+}
+
+// G^rceil(l+1, m) = sum_n' Minv[l][n'] R[n'][m]
+__global__ void __launch_bounds__(256) kern_boot_apply(Pt p, int k, const z2 *Minv, const z2 *R, double *err) {
+  __shared__ BlkLds s;
+  __shared__ double red[256];
+  const int m = blockIdx.x, l = blockIdx.y, n = p.n, np = p.np, kn = k * n;
+  // Minv is (kn x kn) row-major; block (l, n') starts at l*n*kn + n'*n with row stride kn.
+  // Stage it as a dense n x n block in the part of R reserved for scratch.
+  BlkAcc acc;
+  acc.zero();
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  for (int np2 = 0; np2 < k; np2++) {
+    const z2 *Rb = R + ((size_t)np2 * np + m) * p.es();
+    for (int kk0 = 0; kk0 < n; kk0 += BO_KT) {
+      __syncthreads();
+      for (int e = tid; e < BO_NMAX * BO_KT; e += 256) {
+        int r = e / BO_KT, kk = e % BO_KT;
+        s.X[r][kk] = (r < n && kk0 + kk < n) ? Minv[(size_t)(l * n + r) * kn + np2 * n + kk0 + kk] : z2{0.0, 0.0};
+      }
+      for (int e = tid; e < BO_KT * BO_NMAX; e += 256) {
+        int kk = e / BO_NMAX, c = e % BO_NMAX;
+        s.Y[kk][c] = (c < n && kk0 + kk < n) ? Rb[(kk0 + kk) * n + c] : z2{0.0, 0.0};
+      }
+      __syncthreads();
+      const int kmax = min(BO_KT, n - kk0);
+      for (int kk = 0; kk < kmax; kk++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            z2 a = s.X[ty + 16 * r][kk], b = s.Y[kk][tx + 16 * c];
+            acc.v[r][c] = zadd(acc.v[r][c], zmul(a, b));
+          }
+    }
+  }
+  z2 *g = p.gtv(l + 1, m);
+  double d2 = blk_dist2(acc, n, g, red);
+  blk_store(acc, n, g);
+  if (tid == 0) atomicAdd(err, sqrt(d2));
+}
+
+// src/dyson_boot.h:78-88 and :181-192
+__global__ void __launch_bounds__(256) kern_boot_retles(Pt p, int k, double *err) {
+  __shared__ double red[256];
+  const int blk = blockIdx.x, n = p.n;
+  // enumerate (nn, l) pairs for ret: nn in 0..k, l in 0..k-nn; then les (l, nn): l in 0..k, nn in 0..l
+  int cnt = 0;
+  int kind = -1, A = 0, Bi = 0;
+  for (int nn = 0; nn <= k && kind < 0; nn++)
+    for (int l = 0; l <= k - nn; l++) {
+      if (cnt == blk) { kind = 0; A = nn; Bi = l; break; }
+      cnt++;
+    }
+  for (int l = 0; l <= k && kind < 0; l++)
+    for (int nn = 0; nn <= l; nn++) {
+      if (cnt == blk) { kind = 1; A = l; Bi = nn; break; }
+      cnt++;
+    }
+  if (kind < 0) return;
+  BlkAcc acc;
+  acc.zero();
+  z2 *g;
+  if (kind == 0) {
+    const int nn = A, l = Bi;
+    if (nn == 0) {
+      const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (ty + 16 * r == tx + 16 * c) acc.v[r][c] = {0.0, -1.0};
+    } else {
+      blk_axpy(acc, n, p.gtv(nn, p.np - 1), 0, z2{(double)p.sig, 0.0});
+      blk_axpy(acc, n, p.gtv(nn, 0), 0, z2{-1.0, 0.0});
+    }
+    g = p.gr(nn + l, l);
+  } else {
+    const int l = A, nn = Bi;
+    blk_axpy(acc, n, p.gtv(l - nn, 0), 1, z2{-1.0, 0.0});
+    g = p.gl(nn, l);
+  }
+  double d2 = blk_dist2(acc, n, g, red);
+  blk_store(acc, n, g);
+  if (threadIdx.x == 0) atomicAdd(err, sqrt(d2));
+}
+
+__global__ void kern_mix_bs(const double *lb, const z2 *GM, int np, int nq, size_t es, int i, z2 *Bs) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)nq * es) return;
+  int q = (int)(idx / es);
+  size_t e = idx % es;
+  const double *row = lb + ((size_t)i * nq + q) * np;
+  z2 s = {0.0, 0.0};
+  for (int k = 0; k < np; k++) s = zadd(s, zscale(GM[(size_t)k * es + e], row[k]));
+  Bs[idx] = s;
+}
+
+__global__ void kern_mix_k(const double *la, const z2 *Bs, int np, int nq, size_t es, int i, z2 *K) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)np * es) return;
+  int j = (int)(idx / es);
+  size_t e = idx % es;
+  z2 s = {0.0, 0.0};
+  for (int q = 0; q < nq; q++) s = zadd(s, zscale(Bs[(size_t)q * es + e], la[((size_t)i * nq + q) * np + j]));
+  K[((size_t)i * np + j) * es + e] = s;
+}
+
+Pt make_pt(Ctx &c, int T, double mu, double beta, double dt) {
+  Pt p;
+  p.Gret = c.G.ret.as<z2>(); p.Gles = c.G.les.as<z2>(); p.Gtv = c.G.tv.as<z2>(); p.Gmat = c.G.mat.as<z2>();
+  p.Sret = c.S.ret.as<z2>(); p.Sles = c.S.les.as<z2>(); p.Stv = c.S.tv.as<z2>();
+  p.hmf = c.hmf.as<z2>();
+  p.n = c.nao; p.np = c.np; p.T = T; p.sig = c.sig;
+  p.mu = mu; p.beta = beta; p.dt = dt;
+  return p;
+}
+
+inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+void gesv(Ctx &c, z2 *A, long long sA, z2 *B, long long sB, int N, int R, int nsys) {
+  require(N <= 320, "linear system larger than 320");
+  int *info = c.dy_info.as<int>();
+  NED_HIP(hipMemsetAsync(info, 0, sizeof(int), c.stream));
+  hipLaunchKernelGGL(kern_gesv, dim3(nsys), dim3(256), 0, c.stream, A, sA, B, sB, N, R, info);
+  check_launch("gesv");
+}
+
+void update_minv(Ctx &c, int T, double mu, double dt, const WTab &w) {
+  const size_t es = c.es;
+  if (c.minv_mu != mu || c.minv_dt != dt) {
+    std::fill(c.minv_valid.begin(), c.minv_valid.end(), 0);
+    c.minv_mu = mu; c.minv_dt = dt;
+  }
+  std::vector<int> js;
+  for (int j = 0; j < T; j++) if (!c.minv_valid[j]) js.push_back(j);
+  if (js.empty()) return;
+  const int cnt = (int)js.size();
+  c.dy_js.ensure(cnt * sizeof(int));
+  NED_HIP(hipMemcpyAsync(c.dy_js.p, js.data(), cnt * sizeof(int), hipMemcpyHostToDevice, c.stream));
+  c.dy_gA.ensure((size_t)cnt * es * sizeof(z2));
+  c.dy_gB.ensure((size_t)cnt * es * sizeof(z2));
+  Pt p = make_pt(c, T, mu, 0.0, dt);
+  hipLaunchKernelGGL(kern_build_mj, dim3(nblk((long long)cnt * es)), dim3(256), 0, c.stream, p, c.dy_js.as<int>(), cnt,
+                     w.bd[0], w.Omega[w.k * (w.k + 1)], c.dy_gA.as<z2>(), c.dy_gB.as<z2>());
+  check_launch("build mj");
+  gesv(c, c.dy_gA.as<z2>(), es, c.dy_gB.as<z2>(), es, c.nao, c.nao, cnt);
+  hipLaunchKernelGGL(kern_scatter_blocks, dim3(nblk((long long)cnt * es)), dim3(256), 0, c.stream, c.dy_gB.as<z2>(),
+                     c.dy_js.as<int>(), cnt, es, c.Minv.as<z2>());
+  check_launch("scatter minv");
+  NED_HIP(hipStreamSynchronize(c.stream));  // js host vector goes out of scope
+  for (int j : js) c.minv_valid[j] = 1;
+}
+
+int chunks_for(int rows, int terms) {
+  // enough workgroups to cover the chip (~4 per CU) without tiny chunks
+  int want = (1024 + rows - 1) / std::max(rows, 1);
+  want = std::min(want, std::max(1, terms / 8));
+  return std::max(1, std::min(want, 64));
+}
+
+}  // namespace
+
+void invalidate_minv(Ctx &c) { std::fill(c.minv_valid.begin(), c.minv_valid.end(), 0); }
+
+void set_matsubara(Ctx &c, double beta) {
+  GLL g(c.ntau);
+  const int np = c.np, nq = 2 * np;
+  const size_t es = c.es;
+  std::vector<double> la, lb;
+  g.mixing_rows(beta, c.sig, la, lb);
+  c.dy_la.ensure(la.size() * sizeof(double));
+  c.dy_lb.ensure(lb.size() * sizeof(double));
+  NED_HIP(hipMemcpyAsync(c.dy_la.p, la.data(), la.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  NED_HIP(hipMemcpyAsync(c.dy_lb.p, lb.data(), lb.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  c.mixK.ensure((size_t)np * np * es * sizeof(z2));
+  c.dy_mixtmp.ensure((size_t)nq * es * sizeof(z2));
+  for (int i = 0; i < np; i++) {
+    hipLaunchKernelGGL(kern_mix_bs, dim3(nblk((long long)nq * es)), dim3(256), 0, c.stream, c.dy_lb.as<double>(),
+                       c.G.mat.as<z2>(), np, nq, es, i, c.dy_mixtmp.as<z2>());
+    hipLaunchKernelGGL(kern_mix_k, dim3(nblk((long long)np * es)), dim3(256), 0, c.stream, c.dy_la.as<double>(),
+                       c.dy_mixtmp.as<z2>(), np, nq, es, i, c.mixK.as<z2>());
+  }
+  check_launch("mixing kernel");
+  c.mass.ensure((size_t)np * np * sizeof(double));
+  NED_HIP(hipMemcpyAsync(c.mass.p, g.mass.data(), g.mass.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  NED_HIP(hipStreamSynchronize(c.stream));
+  c.mix_beta = beta;
+  c.have_mix = true;
+}
+
+void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) {
+  require(c.have_mix && c.mix_beta == beta, "dyson_step: call ned_set_matsubara(beta) first");
+  require(T > c.k && T <= c.nt, "dyson_step: tstp must be in (k, nt]");
+  const WTab w = make_wtab(c.k);
+  const int n = c.nao, k = c.k, kn = k * n, np = c.np;
+  const size_t es = c.es;
+  Pt p = make_pt(c, T, mu, beta, dt);
+  hipStream_t s = c.stream;
+  update_minv(c, T, mu, dt, w);
+  c.dy_err.ensure(4 * sizeof(double));
+  double *err = c.dy_err.as<double>();
+  NED_HIP(hipMemsetAsync(err, 0, 4 * sizeof(double), s));
+  c.dy_Q.ensure((size_t)(T + 1) * es * sizeof(z2));
+  c.dy_X.ensure((size_t)(T + 1) * es * sizeof(z2));
+  c.dy_M.ensure((size_t)kn * kn * sizeof(z2));
+  c.dy_Qs.ensure((size_t)kn * n * sizeof(z2));
+  z2 *Q = c.dy_Q.as<z2>(), *X = c.dy_X.as<z2>(), *M = c.dy_M.as<z2>(), *Qs = c.dy_Qs.as<z2>();
+  const z2 *Minv = c.Minv.as<z2>();
+
+  // ---- retarded (src/dyson_step.h:7-127)
+  hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, w, M, Qs);
+  check_launch("ret start");
+  gesv(c, M, 0, Qs, 0, kn, n, 1);
+  hipLaunchKernelGGL(kern_ret_start_store, dim3(k), dim3(256), 0, s, p, Qs, err + 0);
+  hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, w, Q, Minv, err + 0);
+  for (int nn = k + 1; nn < T; nn++)
+    hipLaunchKernelGGL(kern_ret_step, dim3(T - nn), dim3(256), 0, s, p, w, nn, Q, Minv, err + 0);
+  check_launch("ret history");
+
+  // ---- right mixing (src/dyson_step.h:130-169)
+  {
+    int nch = chunks_for(np, T);
+    c.dy_part.ensure((size_t)std::max(nch * np, 1) * es * sizeof(z2));
+    c.dy_gA.ensure(es * sizeof(z2));
+    c.dy_gB.ensure(es * sizeof(z2));
+    c.dy_scr.ensure((size_t)std::max(np, T + 1) * es * sizeof(z2));
+    hipLaunchKernelGGL(kern_tv_matrix, dim3(1), dim3(256), 0, s, p, w, c.dy_gA.as<z2>(), c.dy_gB.as<z2>());
+    gesv(c, c.dy_gA.as<z2>(), 0, c.dy_gB.as<z2>(), 0, n, n, 1);
+    hipLaunchKernelGGL(kern_tv_rhs, dim3(np, nch), dim3(256), 0, s, p, w, c.mixK.as<z2>(), nch, c.dy_part.as<z2>());
+    hipLaunchKernelGGL(kern_tv_fin, dim3(np), dim3(256), 0, s, p, c.dy_part.as<z2>(), nch, c.dy_gB.as<z2>(),
+                       c.dy_scr.as<z2>(), err + 1);
+    check_launch("tv");
+  }
+
+  // ---- lesser (src/dyson_step.h:171-326)
+  {
+    c.dy_Y.ensure((size_t)np * es * sizeof(z2));
+    hipLaunchKernelGGL(kern_les_Y, dim3(np), dim3(256), 0, s, p, c.mass.as<double>(), c.dy_Y.as<z2>());
+    int nch = chunks_for(T + 1, np + T + 1);
+    c.dy_part.ensure((size_t)nch * (T + 1) * es * sizeof(z2));
+    hipLaunchKernelGGL(kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, w, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
+    long long nel = (long long)(T + 1) * es;
+    hipLaunchKernelGGL(kern_sum_parts, dim3(nblk(nel)), dim3(256), 0, s, c.dy_part.as<z2>(), nch, nel, Q);
+    hipLaunchKernelGGL(kern_les_start, dim3(1), dim3(256), 0, s, p, w, Q, X, M, Qs, err + 2);
+    gesv(c, M, 0, Qs, 0, kn, n, 1);
+    NED_HIP(hipMemcpyAsync(X + es, Qs, (size_t)kn * n * sizeof(z2), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, w, Q, X, Minv);
+    for (int l = k + 1; l < T; l++)
+      hipLaunchKernelGGL(kern_les_step, dim3(T - l), dim3(256), 0, s, p, w, l, Q, X, Minv);
+    hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, w, Q, X, c.dy_scr.as<z2>());
+    hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, X, err + 2);
+    check_launch("les");
+  }
+  if (err3) {
+    double h[4];
+    NED_HIP(hipMemcpyAsync(h, err, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    NED_HIP(hipStreamSynchronize(s));
+    err3[0] = h[0]; err3[1] = h[1]; err3[2] = h[2];
+  }
+}
+
+void extrapolate(Ctx &c, int T) {
+  require(T > c.k && T <= c.nt, "extrapolate: tstp must be in (k, nt]");
+  const WTab w = make_wtab(c.k);
+  Pt p = make_pt(c, T, 0, 0, 0);
+  const size_t es = c.es;
+  hipLaunchKernelGGL(kern_extrap, dim3(nblk((long long)c.np * es)), dim3(256), 0, c.stream, p, w, 0);
+  hipLaunchKernelGGL(kern_extrap, dim3(nblk((long long)(T + 1) * es)), dim3(256), 0, c.stream, p, w, 1);
+  // the les column needs the new G^rceil(T,0) (row 0) -- launched after tv
+  hipLaunchKernelGGL(kern_extrap, dim3(nblk((long long)(T + 1) * es)), dim3(256), 0, c.stream, p, w, 2);
+  check_launch("extrapolate");
+}
+
+void hf_decomp(Ctx &c, int T, const double *h0_host) {
+  require(c.have_coulomb, "hf: Coulomb tensor not set");
+  require(T >= 0 && T <= c.nt, "hf: tstp out of range");
+  const size_t es = c.es;
+  c.dy_h0.ensure(es * sizeof(double));
+  c.dy_xq.ensure((size_t)c.naux * sizeof(z2));
+  NED_HIP(hipMemcpyAsync(c.dy_h0.p, h0_host, es * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  Pt p = make_pt(c, T, 0, 0, 0);
+  hipLaunchKernelGGL(kern_hf_x, dim3(c.naux), dim3(256), 0, c.stream, p, c.Bq.as<double>(), c.dy_xq.as<z2>());
+  hipLaunchKernelGGL(kern_hf_h, dim3(nblk((long long)es)), dim3(256), 0, c.stream, p, c.dy_h0.as<double>(),
+                     c.Bq.as<double>(), c.U64.as<double>(), c.dy_xq.as<z2>(), c.naux, c.hmf.as<z2>());
+  check_launch("hf");
+  NED_HIP(hipStreamSynchronize(c.stream));  // h0_host may be released by the caller
+  if (T < (int)c.minv_valid.size()) c.minv_valid[T] = 0;
+}
+
+double dyson_start(Ctx &c, double mu, double beta, double dt) {
+  require(c.have_mix && c.mix_beta == beta, "dyson_start: call ned_set_matsubara(beta) first");
+  require(c.nt >= c.k, "dyson_start: nt must be >= k");
+  const WTab w = make_wtab(c.k);
+  const int n = c.nao, k = c.k, kn = k * n, np = c.np;
+  const size_t es = c.es;
+  Pt p = make_pt(c, 0, mu, beta, dt);
+  hipStream_t s = c.stream;
+  c.dy_err.ensure(4 * sizeof(double));
+  double *err = c.dy_err.as<double>();
+  NED_HIP(hipMemsetAsync(err, 0, 4 * sizeof(double), s));
+  hipLaunchKernelGGL(kern_boot_tv0, dim3(np), dim3(256), 0, s, p, err);
+  c.dy_M.ensure((size_t)kn * kn * sizeof(z2));
+  c.dy_bootI.ensure((size_t)kn * kn * sizeof(z2));
+  hipLaunchKernelGGL(kern_boot_matrix, dim3(nblk((long long)kn * kn)), dim3(256), 0, s, p, w, c.dy_M.as<z2>(),
+                     c.dy_bootI.as<z2>());
+  gesv(c, c.dy_M.as<z2>(), 0, c.dy_bootI.as<z2>(), 0, kn, kn, 1);
+  c.dy_part.ensure((size_t)k * np * es * sizeof(z2));
+  hipLaunchKernelGGL(kern_boot_rhs, dim3(np, k), dim3(256), 0, s, p, w, c.mixK.as<z2>(), c.dy_part.as<z2>());
+  hipLaunchKernelGGL(kern_boot_apply, dim3(np, k), dim3(256), 0, s, p, k, c.dy_bootI.as<z2>(), c.dy_part.as<z2>(), err);
+  int npairs = (k + 1) * (k + 2);  // ret pairs + les pairs
+  hipLaunchKernelGGL(kern_boot_retles, dim3(npairs), dim3(256), 0, s, p, k, err);
+  check_launch("dyson_start");
+  double h;
+  NED_HIP(hipMemcpyAsync(&h, err, sizeof(double), hipMemcpyDeviceToHost, s));
+  NED_HIP(hipStreamSynchronize(s));
+  invalidate_minv(c);
+  return h;
+}
+
+void energy(Ctx &c, int T, const double *h0_host, double beta, double dt, double *out3) {
+  require(c.have_mix && c.mix_beta == beta, "energy: call ned_set_matsubara(beta) first");
+  const WTab w = make_wtab(c.k);
+  const int top = T >= c.k ? T : c.k, np = c.np;
+  const int nb = top + 1 + np;
+  c.dy_en.ensure((size_t)2 * nb * sizeof(double));
+  Pt p = make_pt(c, T, 0, beta, dt);
+  hipLaunchKernelGGL(kern_energy, dim3(nb), dim3(256), 0, c.stream, p, w, top, c.mass.as<double>(), c.dy_en.as<double>());
+  check_launch("energy");
+  std::vector<double> h(2 * nb);
+  std::vector<z2> gl(c.es), hm(c.es);
+  NED_HIP(hipMemcpyAsync(h.data(), c.dy_en.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  NED_HIP(hipMemcpyAsync(gl.data(), c.G.les.as<z2>() + (tri(T) + T) * c.es, c.es * sizeof(z2), hipMemcpyDeviceToHost,
+                         c.stream));
+  NED_HIP(hipMemcpyAsync(hm.data(), c.hmf.as<z2>() + (size_t)T * c.es, c.es * sizeof(z2), hipMemcpyDeviceToHost,
+                         c.stream));
+  NED_HIP(hipStreamSynchronize(c.stream));
+  double ry = 0;
+  for (int i = 0; i < nb; i++) ry += h[2 * i + 1];
+  // E = Re(i sig/2 (res1 + res2))
+  const double e_corr = -0.5 * c.sig * ry;
+  const int n = c.nao;
+  double ekin = 0, ntr = 0;
+  for (int a = 0; a < n; a++) {
+    for (int b = 0; b < n; b++) {
+      z2 g = gl[a * n + b];
+      z2 rho = {-c.sig * g.y, c.sig * g.x};           // rho_ab = i sig G^<_ab
+      z2 hb = hm[b * n + a];                            // (hmf + h0)^T_ab
+      hb.x += h0_host[b * n + a];
+      ekin += rho.x * hb.x - rho.y * hb.y;
+    }
+    z2 g = gl[a * n + a];
+    ntr += -c.sig * g.y;
+  }
+  out3[0] = ekin;
+  out3[1] = 2 * e_corr;
+  out3[2] = 2 * ntr;
+}
+
+}  // namespace ned

@@ -1,0 +1,328 @@
+// Strided batched GEMM on the CDNA4 matrix cores (gfx950), real or planar
+// complex operands, in FP64, FP32 or split-bf16 (bf16x3) arithmetic.
+//
+// Work decomposition: one 256-thread workgroup (4 wave64) per 64x64 output
+// tile; each wave owns a 32x32 quadrant = 2x2 MFMA 16x16 blocks.  Operand
+// tiles are staged in LDS; the loader follows whichever operand dimension is
+// unit-stride so that consecutive lanes read consecutive addresses.
+#include "common.h"
+
+namespace ned {
+
+namespace {
+
+constexpr int BM = 64, BN = 64, NT = 256;
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ long long koff(const GOp &o, int k) {
+  if (o.kdiv) return (long long)(k / o.kdiv) * o.ks_hi + (long long)(k % o.kdiv) * o.ks_lo;
+  return (long long)k * o.ks_lo;
+}
+__device__ __forceinline__ long long boff(const GOp &o, int b) {
+  return (long long)(b / o.bdiv) * o.bs_hi + (long long)(b % o.bdiv) * o.bs_lo;
+}
+
+// round-to-nearest-even float -> bf16 bits (finite inputs)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+template <typename T>
+__device__ __forceinline__ T ldg(const void *p, long long off) {
+  return reinterpret_cast<const T *>(p)[off];
+}
+
+// ---------------------------------------------------------------- FP64 / FP32
+template <typename T, int NPA, int NPB>
+__global__ void __launch_bounds__(NT) kern_gemm_fp(GemmDesc d) {
+  constexpr int BK = 16;
+  constexpr int NPC = (NPA > 1 || NPB > 1) ? 2 : 1;
+  __shared__ T As[NPA][BM][BK + 1];
+  __shared__ T Bs[NPB][BK][BN + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tilesN = (d.N + BN - 1) / BN;
+  const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
+  const bool a_kfast = d.A.ks_lo == 1 && d.A.kdiv == 0;
+  const bool b_nfast = d.B.ms == 1;
+
+  for (int b = blockIdx.y; b < d.batch; b += gridDim.y) {
+    const long long ba = boff(d.A, b), bb = boff(d.B, b);
+    using V = typename std::conditional<sizeof(T) == 8, v4d, v4f>::type;
+    V acc[2][2][NPC];
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++)
+        for (int p = 0; p < NPC; p++) acc[i][j][p] = V{0, 0, 0, 0};
+
+    for (int k0 = 0; k0 < d.K; k0 += BK) {
+      // ---- stage A (BM x BK) and B (BK x BN)
+#pragma unroll
+      for (int it = 0; it < (BM * BK) / NT; it++) {
+        int e = tid + it * NT;
+        int r, kk;
+        if (a_kfast) { r = e / BK; kk = e % BK; } else { r = e % BM; kk = e / BM; }
+        int gm = m0 + r, gk = k0 + kk;
+        bool ok = gm < d.M && gk < d.K;
+        long long off = ba + (long long)gm * d.A.ms + koff(d.A, gk);
+#pragma unroll
+        for (int p = 0; p < NPA; p++) As[p][r][kk] = ok ? ldg<T>(d.A.ptr, off + p * d.A.ps) : T(0);
+      }
+#pragma unroll
+      for (int it = 0; it < (BK * BN) / NT; it++) {
+        int e = tid + it * NT;
+        int c, kk;
+        if (b_nfast) { c = e % BN; kk = e / BN; } else { c = e / BK; kk = e % BK; }
+        int gn = n0 + c, gk = k0 + kk;
+        bool ok = gn < d.N && gk < d.K;
+        long long off = bb + (long long)gn * d.B.ms + koff(d.B, gk);
+#pragma unroll
+        for (int p = 0; p < NPB; p++) Bs[p][kk][c] = ok ? ldg<T>(d.B.ptr, off + p * d.B.ps) : T(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 4) {
+        T a[2][NPA], bv[2][NPB];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int p = 0; p < NPA; p++) a[i][p] = As[p][wm * 32 + i * 16 + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+          for (int p = 0; p < NPB; p++) bv[j][p] = Bs[p][ks + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            if constexpr (sizeof(T) == 8) {
+              if constexpr (NPA == 2 && NPB == 2) {
+                acc[i][j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][0], bv[j][0], acc[i][j][0], 0, 0, 0);
+                acc[i][j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[i][1], bv[j][1], acc[i][j][0], 0, 0, 0);
+                acc[i][j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][0], bv[j][1], acc[i][j][1], 0, 0, 0);
+                acc[i][j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][1], bv[j][0], acc[i][j][1], 0, 0, 0);
+              } else if constexpr (NPA == 2) {
+                acc[i][j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][0], bv[j][0], acc[i][j][0], 0, 0, 0);
+                acc[i][j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][1], bv[j][0], acc[i][j][1], 0, 0, 0);
+              } else if constexpr (NPB == 2) {
+                acc[i][j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][0], bv[j][0], acc[i][j][0], 0, 0, 0);
+                acc[i][j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][0], bv[j][1], acc[i][j][1], 0, 0, 0);
+              } else {
+                acc[i][j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][0], bv[j][0], acc[i][j][0], 0, 0, 0);
+              }
+            } else {
+              if constexpr (NPA == 2 && NPB == 2) {
+                acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][0], bv[j][0], acc[i][j][0], 0, 0, 0);
+                acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[i][1], bv[j][1], acc[i][j][0], 0, 0, 0);
+                acc[i][j][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][0], bv[j][1], acc[i][j][1], 0, 0, 0);
+                acc[i][j][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][1], bv[j][0], acc[i][j][1], 0, 0, 0);
+              } else if constexpr (NPA == 2) {
+                acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][0], bv[j][0], acc[i][j][0], 0, 0, 0);
+                acc[i][j][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][1], bv[j][0], acc[i][j][1], 0, 0, 0);
+              } else if constexpr (NPB == 2) {
+                acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][0], bv[j][0], acc[i][j][0], 0, 0, 0);
+                acc[i][j][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][0], bv[j][1], acc[i][j][1], 0, 0, 0);
+              } else {
+                acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][0], bv[j][0], acc[i][j][0], 0, 0, 0);
+              }
+            }
+          }
+      }
+      __syncthreads();
+    }
+    // ---- epilogue: lane holds D[4*(lane/16)+q][lane%16] of each 16x16 block
+    const long long bc = (long long)(b / d.C.bdiv) * d.C.bs_hi + (long long)(b % d.C.bdiv) * d.C.bs_lo;
+    T *C = reinterpret_cast<T *>(d.C.ptr);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + q;
+          int col = n0 + wn * 32 + j * 16 + (lane & 15);
+          if (row < d.M && col < d.N) {
+            long long off = bc + (long long)row * d.C.rs + (long long)col * d.C.cs;
+#pragma unroll
+            for (int p = 0; p < NPC; p++) {
+              T v = (T)d.alpha * acc[i][j][p][q];
+              if (d.beta) v += C[off + p * d.C.ps];
+              C[off + p * d.C.ps] = v;
+            }
+          }
+        }
+  }
+}
+
+// ---------------------------------------------------------------- bf16x3
+// LDS holds hi and lo bf16 parts; A as [row][k], B transposed as [col][k],
+// so that each lane's 8 consecutive K values are one 16-byte read.
+template <int NPA, int NPB>
+__global__ void __launch_bounds__(NT) kern_gemm_bf16x3(GemmDesc d) {
+  constexpr int BK = 32, KP = BK + 8;  // padded row (80 bytes) keeps 16-byte alignment
+  constexpr int NPC = (NPA > 1 || NPB > 1) ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) uint16_t Ah[NPA][BM][KP];
+  __shared__ __attribute__((aligned(16))) uint16_t Al[NPA][BM][KP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bh[NPB][BN][KP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bl[NPB][BN][KP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tilesN = (d.N + BN - 1) / BN;
+  const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
+  const bool a_kfast = d.A.ks_lo == 1 && d.A.kdiv == 0;
+  const bool b_nfast = d.B.ms == 1;
+
+  for (int b = blockIdx.y; b < d.batch; b += gridDim.y) {
+    const long long ba = boff(d.A, b), bb = boff(d.B, b);
+    v4f acc[2][2][NPC];
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++)
+        for (int p = 0; p < NPC; p++) acc[i][j][p] = v4f{0, 0, 0, 0};
+
+    for (int k0 = 0; k0 < d.K; k0 += BK) {
+#pragma unroll
+      for (int it = 0; it < (BM * BK) / NT; it++) {
+        int e = tid + it * NT;
+        int r, kk;
+        if (a_kfast) { r = e / BK; kk = e % BK; } else { r = e % BM; kk = e / BM; }
+        int gm = m0 + r, gk = k0 + kk;
+        bool ok = gm < d.M && gk < d.K;
+        long long off = ba + (long long)gm * d.A.ms + koff(d.A, gk);
+#pragma unroll
+        for (int p = 0; p < NPA; p++) {
+          float v = ok ? ldg<float>(d.A.ptr, off + p * d.A.ps) : 0.f;
+          uint16_t h = f2bf(v);
+          Ah[p][r][kk] = h;
+          Al[p][r][kk] = f2bf(v - bf2f(h));
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < (BK * BN) / NT; it++) {
+        int e = tid + it * NT;
+        int c, kk;
+        if (b_nfast) { c = e % BN; kk = e / BN; } else { c = e / BK; kk = e % BK; }
+        int gn = n0 + c, gk = k0 + kk;
+        bool ok = gn < d.N && gk < d.K;
+        long long off = bb + (long long)gn * d.B.ms + koff(d.B, gk);
+#pragma unroll
+        for (int p = 0; p < NPB; p++) {
+          float v = ok ? ldg<float>(d.B.ptr, off + p * d.B.ps) : 0.f;
+          uint16_t h = f2bf(v);
+          Bh[p][c][kk] = h;
+          Bl[p][c][kk] = f2bf(v - bf2f(h));
+        }
+      }
+      __syncthreads();
+      const int kq = 8 * (lane >> 4);
+      uint4 ah[2][NPA], al[2][NPA], bh[2][NPB], bl[2][NPB];
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int p = 0; p < NPA; p++) {
+          int r = wm * 32 + i * 16 + (lane & 15);
+          ah[i][p] = *reinterpret_cast<const uint4 *>(&Ah[p][r][kq]);
+          al[i][p] = *reinterpret_cast<const uint4 *>(&Al[p][r][kq]);
+        }
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int p = 0; p < NPB; p++) {
+          int c = wn * 32 + j * 16 + (lane & 15);
+          bh[j][p] = *reinterpret_cast<const uint4 *>(&Bh[p][c][kq]);
+          bl[j][p] = *reinterpret_cast<const uint4 *>(&Bl[p][c][kq]);
+        }
+      auto neg = [](uint4 v) { return uint4{v.x ^ 0x80008000u, v.y ^ 0x80008000u, v.z ^ 0x80008000u, v.w ^ 0x80008000u}; };
+      auto mma3 = [](uint4 xh, uint4 xl, uint4 yh, uint4 yl, v4f c) {
+        v8bf a_h = __builtin_bit_cast(v8bf, xh), a_l = __builtin_bit_cast(v8bf, xl);
+        v8bf b_h = __builtin_bit_cast(v8bf, yh), b_l = __builtin_bit_cast(v8bf, yl);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, b_h, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_l, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_h, c, 0, 0, 0);
+        return c;
+      };
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          if constexpr (NPA == 2 && NPB == 2) {
+            acc[i][j][0] = mma3(ah[i][0], al[i][0], bh[j][0], bl[j][0], acc[i][j][0]);
+            acc[i][j][0] = mma3(neg(ah[i][1]), neg(al[i][1]), bh[j][1], bl[j][1], acc[i][j][0]);
+            acc[i][j][1] = mma3(ah[i][0], al[i][0], bh[j][1], bl[j][1], acc[i][j][1]);
+            acc[i][j][1] = mma3(ah[i][1], al[i][1], bh[j][0], bl[j][0], acc[i][j][1]);
+          } else if constexpr (NPA == 2) {
+            acc[i][j][0] = mma3(ah[i][0], al[i][0], bh[j][0], bl[j][0], acc[i][j][0]);
+            acc[i][j][1] = mma3(ah[i][1], al[i][1], bh[j][0], bl[j][0], acc[i][j][1]);
+          } else if constexpr (NPB == 2) {
+            acc[i][j][0] = mma3(ah[i][0], al[i][0], bh[j][0], bl[j][0], acc[i][j][0]);
+            acc[i][j][1] = mma3(ah[i][0], al[i][0], bh[j][1], bl[j][1], acc[i][j][1]);
+          } else {
+            acc[i][j][0] = mma3(ah[i][0], al[i][0], bh[j][0], bl[j][0], acc[i][j][0]);
+          }
+        }
+      __syncthreads();
+    }
+    const long long bc = (long long)(b / d.C.bdiv) * d.C.bs_hi + (long long)(b % d.C.bdiv) * d.C.bs_lo;
+    float *C = reinterpret_cast<float *>(d.C.ptr);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + q;
+          int col = n0 + wn * 32 + j * 16 + (lane & 15);
+          if (row < d.M && col < d.N) {
+            long long off = bc + (long long)row * d.C.rs + (long long)col * d.C.cs;
+#pragma unroll
+            for (int p = 0; p < NPC; p++) {
+              float v = (float)d.alpha * acc[i][j][p][q];
+              if (d.beta) v += C[off + p * d.C.ps];
+              C[off + p * d.C.ps] = v;
+            }
+          }
+        }
+  }
+}
+
+template <typename K>
+void launch(K kern, const GemmDesc &d, hipStream_t s) {
+  int tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
+  int by = d.batch < 65535 ? d.batch : 65535;
+  hipLaunchKernelGGL(kern, dim3(tiles, by), dim3(NT), 0, s, d);
+}
+
+}  // namespace
+
+void gemm(const GemmDesc &d, Prec prec, hipStream_t s) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  if (d.K <= 0) throw std::runtime_error("gemm: K must be positive");
+  const int key = (d.ca ? 2 : 0) + (d.cb ? 1 : 0);
+  switch (prec) {
+    case PREC_F64:
+      if (key == 0) launch(kern_gemm_fp<double, 1, 1>, d, s);
+      if (key == 1) launch(kern_gemm_fp<double, 1, 2>, d, s);
+      if (key == 2) launch(kern_gemm_fp<double, 2, 1>, d, s);
+      if (key == 3) launch(kern_gemm_fp<double, 2, 2>, d, s);
+      break;
+    case PREC_F32:
+      if (key == 0) launch(kern_gemm_fp<float, 1, 1>, d, s);
+      if (key == 1) launch(kern_gemm_fp<float, 1, 2>, d, s);
+      if (key == 2) launch(kern_gemm_fp<float, 2, 1>, d, s);
+      if (key == 3) launch(kern_gemm_fp<float, 2, 2>, d, s);
+      break;
+    case PREC_BF16X3:
+      if (key == 0) launch(kern_gemm_bf16x3<1, 1>, d, s);
+      if (key == 1) launch(kern_gemm_bf16x3<1, 2>, d, s);
+      if (key == 2) launch(kern_gemm_bf16x3<2, 1>, d, s);
+      if (key == 3) launch(kern_gemm_bf16x3<2, 2>, d, s);
+      break;
+  }
+  check_launch("gemm");
+}
+
+}  // namespace ned

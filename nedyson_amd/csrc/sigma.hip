@@ -1,0 +1,261 @@
+// Second-Born self-energy from the Cholesky-decomposed interaction, one time
+// slice T at a time: the contract of molGF2SolverDecomp::solve
+// (src/mol2bSolver_decomp_impl.h:412-429), restructured for the matrix cores.
+//
+// Every component is an instance of one functional of two nao x nao matrices,
+//   F[A,B]_ij = sum_{lnpkqm} (2 U_{il,np} U_{kj,qm} - U_{il,np} U_{qj,km}) A_lk B_mn A_pq
+// with U_{ij,kl} = sum_Q V^Q_ij V^Q_kl (the index pattern of solve_loop,
+// src/mol2bSolver_decomp_impl.h:459-467, 585-593).  Writing W^a = V^a A,
+//   bubble   2 sum_a W^a (sum_b Pi_ab V^b),  Pi_ab = tr(V^a A V^b B)
+//   exchange  -sum_a W^a D^a,  D^a_kj = sum_{mq} U_{km,qj} (B W^a)_mq
+// The exchange is reassociated through U, so its n^4 naux work is a single
+// real GEMM [(item, plane, a) x (m,q)] x U'[(m,q) x (k,j)] that streams the
+// whole slice through the MFMA units at once (the reference does two n^4 naux
+// GEMMs per time point, lines 206-226).
+//
+// Items of slice T (L = G^<(t,T), R = G^R(T,t)):
+//   "<" t:  A = G^<(T,t) = -L^H,      B = G^>(t,T) = L - R^H   -> Sigma^<(T,t)
+//   ">" t:  A = G^>(T,t) = R - L^H,   B = G^<(t,T) = L         -> Sigma^>(T,t)
+//   tv tau: A = G^rceil(T,tau),       B = G^lceil(tau,T) = -sig G^rceil(T,beta-tau)^H
+// and then Sigma^R(T,t) = Sigma^>(T,t) - Sigma^<(T,t) (Langreth; equal to the
+// four-term expression of lines 620-624) and Sigma^<(t,T) = -Sigma^<(T,t)^H,
+// which holds for symmetric V^Q (real orbitals; checked at upload).
+#include "ctx.h"
+#include <cmath>
+#include <algorithm>
+
+namespace ned {
+
+size_t tri(int t) { return (size_t)t * (t + 1) / 2; }
+__device__ __forceinline__ size_t dtri_s(int t) { return (size_t)t * (t + 1) / 2; }
+void require(bool cond, const char *msg) {
+  if (!cond) throw std::invalid_argument(msg);
+}
+
+namespace {
+
+__global__ void kern_build_U(const double *Bq, int n, int a, double *U64, float *U32) {
+  // U'[(m,q)][(k,j)] = sum_Q B[Q][k][m] B[Q][q][j]
+  long long n2 = (long long)n * n;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n2 * n2) return;
+  int row = (int)(idx / n2), col = (int)(idx % n2);
+  int m = row / n, q = row % n, k = col / n, j = col % n;
+  double s = 0;
+  for (int Q = 0; Q < a; Q++) s += Bq[(long long)Q * n2 + k * n + m] * Bq[(long long)Q * n2 + q * n + j];
+  U64[idx] = s;
+  U32[idx] = (float)s;
+}
+
+__global__ void kern_d2f(const double *x, float *y, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
+template <typename T>
+__global__ void kern_items(const z2 *Gret, const z2 *Gles, const z2 *Gtv, int T_, int n, int np, int sig, int p0,
+                           int P, T *Ap, T *Bp) {
+  const int n2 = n * n;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)P * n2) return;
+  int p = (int)(idx / n2), e = (int)(idx % n2), i = e / n, k = e % n;
+  int g = p0 + p;
+  z2 A, B;
+  const size_t rowT = dtri_s(T_);
+  if (g <= 2 * T_ + 1) {
+    int t = g <= T_ ? g : g - T_ - 1;
+    const z2 *L = Gles + (rowT + t) * n2, *R = Gret + (rowT + t) * n2;
+    z2 Lik = L[i * n + k], Lki = L[k * n + i], Rik = R[i * n + k], Rki = R[k * n + i];
+    if (g <= T_) {  // "<"
+      A = {-Lki.x, Lki.y};                         // -conj(L_ki)
+      B = {Lik.x - Rki.x, Lik.y + Rki.y};          // L_ik - conj(R_ki)
+    } else {        // ">"
+      A = {Rik.x - Lki.x, Rik.y + Lki.y};          // R_ik - conj(L_ki)
+      B = Lik;
+    }
+  } else {
+    int tau = g - 2 * T_ - 2;
+    const z2 *Ga = Gtv + ((size_t)T_ * np + tau) * n2;
+    const z2 *Gb = Gtv + ((size_t)T_ * np + (np - 1 - tau)) * n2;
+    A = Ga[i * n + k];
+    z2 bk = Gb[k * n + i];
+    B = {-(double)sig * bk.x, (double)sig * bk.y};  // -sig conj
+  }
+  T *a = Ap + (size_t)p * 2 * n2, *b = Bp + (size_t)p * 2 * n2;
+  a[e] = (T)A.x; a[n2 + e] = (T)A.y;
+  b[e] = (T)B.x; b[n2 + e] = (T)B.y;
+}
+
+template <typename T>
+__global__ void kern_scatter(const T *E, int T_, int n, int np, z2 *Sret, z2 *Sles, z2 *Stv) {
+  const int n2 = n * n;
+  const int nles = T_ + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)(nles + np) * n2;
+  if (idx >= total) return;
+  int g = (int)(idx / n2), e = (int)(idx % n2), i = e / n, k = e % n;
+  const size_t rowT = dtri_s(T_);
+  if (g < nles) {
+    int t = g;
+    const T *El = E + (size_t)t * 2 * n2, *Eg = E + (size_t)(t + nles) * 2 * n2;
+    z2 lt = {(double)El[e], (double)El[n2 + e]};
+    z2 gt = {(double)Eg[e], (double)Eg[n2 + e]};
+    Sret[(rowT + t) * n2 + e] = {gt.x - lt.x, gt.y - lt.y};
+    // Sigma^<(t,T)_ik = -conj(Sigma^<(T,t)_ki)
+    z2 lki = {(double)El[k * n + i], (double)El[n2 + k * n + i]};
+    Sles[(rowT + t) * n2 + e] = {-lki.x, lki.y};
+  } else {
+    int tau = g - nles;
+    const T *Et = E + (size_t)(2 * nles + tau) * 2 * n2;
+    Stv[((size_t)T_ * np + tau) * n2 + e] = {(double)Et[e], (double)Et[n2 + e]};
+  }
+}
+
+template <typename T>
+struct Bufs {
+  T *A, *B, *W, *X, *H, *Pi, *E;
+};
+
+// One chunk of P items starting at item p0.
+template <typename T>
+void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq, const T *U) {
+  const int n = c.nao, a = c.naux;
+  const long long n2 = (long long)n * n, an2 = (long long)a * n2;
+  hipStream_t s = c.stream;
+  // W_p = Vstack A_p : (a n x n)(n x n), Vstack[(a,i)][l] = B[a][i][l]
+  {
+    GemmDesc d;
+    d.M = a * n; d.N = n; d.K = n; d.batch = P; d.ca = false; d.cb = true;
+    d.A.ptr = Vq; d.A.ms = n; d.A.ks_lo = 1;
+    d.B.ptr = bf.A; d.B.ms = 1; d.B.ks_lo = n; d.B.ps = n2; d.B.bs_lo = 2 * n2;
+    d.C.ptr = bf.W; d.C.rs = n; d.C.cs = 1; d.C.ps = an2; d.C.bs_lo = 2 * an2;
+    gemm(d, prec, s);
+  }
+  // Z^b = B_p^T (V^b)^T  (into X):  Z^b_ik = sum_l B_li V^b_kl
+  {
+    GemmDesc d;
+    d.M = n; d.N = n; d.K = n; d.batch = P * a; d.ca = true; d.cb = false;
+    d.A.ptr = bf.B; d.A.ms = 1; d.A.ks_lo = n; d.A.ps = n2; d.A.bdiv = a; d.A.bs_hi = 2 * n2; d.A.bs_lo = 0;
+    d.B.ptr = Vq; d.B.ms = n; d.B.ks_lo = 1; d.B.bdiv = a; d.B.bs_hi = 0; d.B.bs_lo = n2;
+    d.C.ptr = bf.X; d.C.rs = n; d.C.cs = 1; d.C.ps = an2; d.C.bdiv = a; d.C.bs_hi = 2 * an2; d.C.bs_lo = n2;
+    gemm(d, prec, s);
+  }
+  // Pi_p[a][b] = sum_(ik) W^a_ik Z^b_ik
+  {
+    GemmDesc d;
+    d.M = a; d.N = a; d.K = (int)n2; d.batch = P; d.ca = true; d.cb = true;
+    d.A.ptr = bf.W; d.A.ms = n2; d.A.ks_lo = 1; d.A.ps = an2; d.A.bs_lo = 2 * an2;
+    d.B.ptr = bf.X; d.B.ms = n2; d.B.ks_lo = 1; d.B.ps = an2; d.B.bs_lo = 2 * an2;
+    d.C.ptr = bf.Pi; d.C.rs = a; d.C.cs = 1; d.C.ps = (long long)a * a; d.C.bs_lo = 2LL * a * a;
+    gemm(d, prec, s);
+  }
+  // C^a = B_p W^a  (into X)
+  {
+    GemmDesc d;
+    d.M = n; d.N = n; d.K = n; d.batch = P * a; d.ca = true; d.cb = true;
+    d.A.ptr = bf.B; d.A.ms = n; d.A.ks_lo = 1; d.A.ps = n2; d.A.bdiv = a; d.A.bs_hi = 2 * n2; d.A.bs_lo = 0;
+    d.B.ptr = bf.W; d.B.ms = 1; d.B.ks_lo = n; d.B.ps = an2; d.B.bdiv = a; d.B.bs_hi = 2 * an2; d.B.bs_lo = n2;
+    d.C.ptr = bf.X; d.C.rs = n; d.C.cs = 1; d.C.ps = an2; d.C.bdiv = a; d.C.bs_hi = 2 * an2; d.C.bs_lo = n2;
+    gemm(d, prec, s);
+  }
+  // H = -C U'   (rows (p, plane, a) of X, all at once; the n^4 naux GEMM)
+  {
+    GemmDesc d;
+    d.M = 2 * a * P; d.N = (int)n2; d.K = (int)n2; d.batch = 1;
+    d.A.ptr = bf.X; d.A.ms = n2; d.A.ks_lo = 1;
+    d.B.ptr = U; d.B.ms = 1; d.B.ks_lo = n2;
+    d.C.ptr = bf.H; d.C.rs = n2; d.C.cs = 1;
+    d.alpha = -1.0;
+    gemm(d, prec, s);
+  }
+  // H += 2 Pi Vmat   (Vmat[b][(k,j)] = B[b][k][j])
+  {
+    GemmDesc d;
+    d.M = a; d.N = (int)n2; d.K = a; d.batch = P; d.ca = true; d.cb = false;
+    d.A.ptr = bf.Pi; d.A.ms = a; d.A.ks_lo = 1; d.A.ps = (long long)a * a; d.A.bs_lo = 2LL * a * a;
+    d.B.ptr = Vq; d.B.ms = 1; d.B.ks_lo = n2;
+    d.C.ptr = bf.H; d.C.rs = n2; d.C.cs = 1; d.C.ps = an2; d.C.bs_lo = 2 * an2;
+    d.alpha = 2.0; d.beta = 1;
+    gemm(d, prec, s);
+  }
+  // E_p = sum_a W^a H^a  (K index (a,k): W offset a*n2 + k)
+  {
+    GemmDesc d;
+    d.M = n; d.N = n; d.K = a * n; d.batch = P; d.ca = true; d.cb = true;
+    d.A.ptr = bf.W; d.A.ms = n; d.A.kdiv = n; d.A.ks_hi = n2; d.A.ks_lo = 1; d.A.ps = an2; d.A.bs_lo = 2 * an2;
+    d.B.ptr = bf.H; d.B.ms = 1; d.B.ks_lo = n; d.B.ps = an2; d.B.bs_lo = 2 * an2;
+    d.C.ptr = bf.E + (size_t)p0 * 2 * n2; d.C.rs = n; d.C.cs = 1; d.C.ps = n2; d.C.bs_lo = 2 * n2;
+    gemm(d, prec, s);
+  }
+}
+
+template <typename T>
+void sigma_typed(Ctx &c, int tstp, Prec prec) {
+  const int n = c.nao, a = c.naux, np = c.np;
+  const long long n2 = (long long)n * n, an2 = (long long)a * n2;
+  const int Ptot = 2 * (tstp + 1) + np;
+  const size_t per_item = (size_t)(3 * 2 * an2 + 2LL * a * a + 4 * n2) * sizeof(T);
+  int P = c.sigma_chunk > 0 ? c.sigma_chunk : (int)std::max<size_t>(1, ((size_t)3 << 30) / std::max<size_t>(per_item, 1));
+  P = std::min(P, Ptot);
+  c.sg_items_A.ensure((size_t)P * 2 * n2 * sizeof(T));
+  c.sg_items_B.ensure((size_t)P * 2 * n2 * sizeof(T));
+  c.sg_W.ensure((size_t)P * 2 * an2 * sizeof(T));
+  c.sg_X.ensure((size_t)P * 2 * an2 * sizeof(T));
+  c.sg_H.ensure((size_t)P * 2 * an2 * sizeof(T));
+  c.sg_Pi.ensure((size_t)P * 2 * a * a * sizeof(T));
+  c.sg_E.ensure((size_t)Ptot * 2 * n2 * sizeof(T));
+  Bufs<T> bf{c.sg_items_A.as<T>(), c.sg_items_B.as<T>(), c.sg_W.as<T>(), c.sg_X.as<T>(), c.sg_H.as<T>(),
+             c.sg_Pi.as<T>(), c.sg_E.as<T>()};
+  const T *Vq = sizeof(T) == 8 ? (const T *)c.Bq.p : (const T *)c.Bq32.p;
+  const T *U = sizeof(T) == 8 ? (const T *)c.U64.p : (const T *)c.U32.p;
+  for (int p0 = 0; p0 < Ptot; p0 += P) {
+    int Pc = std::min(P, Ptot - p0);
+    long long tot = (long long)Pc * n2;
+    hipLaunchKernelGGL(kern_items<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, c.G.ret.as<z2>(),
+                       c.G.les.as<z2>(), c.G.tv.as<z2>(), tstp, n, np, c.sig, p0, Pc, bf.A, bf.B);
+    check_launch("sigma items");
+    run_chunk<T>(c, prec, p0, Pc, bf, Vq, U);
+  }
+  long long tot = (long long)(tstp + 1 + np) * n2;
+  hipLaunchKernelGGL(kern_scatter<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, bf.E, tstp, n, np,
+                     c.S.ret.as<z2>(), c.S.les.as<z2>(), c.S.tv.as<z2>());
+  check_launch("sigma scatter");
+}
+
+}  // namespace
+
+void build_coulomb(Ctx &c, const double *Bq_host) {
+  const int n = c.nao, a = c.naux;
+  const size_t n2 = (size_t)n * n;
+  for (int Q = 0; Q < a; Q++)
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < i; j++) {
+        double x = Bq_host[Q * n2 + i * n + j], y = Bq_host[Q * n2 + j * n + i];
+        if (std::fabs(x - y) > 1e-12 * (1.0 + std::fabs(x)))
+          throw std::invalid_argument("Coulomb factor B[Q,i,j] must be symmetric in (i,j)");
+      }
+  c.Bq.ensure(a * n2 * sizeof(double));
+  c.Bq32.ensure(a * n2 * sizeof(float));
+  c.U64.ensure(n2 * n2 * sizeof(double));
+  c.U32.ensure(n2 * n2 * sizeof(float));
+  NED_HIP(hipMemcpyAsync(c.Bq.p, Bq_host, a * n2 * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  long long nb = (long long)a * n2;
+  hipLaunchKernelGGL(kern_d2f, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c.stream, c.Bq.as<double>(),
+                     c.Bq32.as<float>(), nb);
+  check_launch("coulomb d2f");
+  long long nu = (long long)n2 * n2;
+  hipLaunchKernelGGL(kern_build_U, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, c.stream, c.Bq.as<double>(), n, a,
+                     c.U64.as<double>(), c.U32.as<float>());
+  check_launch("coulomb U");
+  NED_HIP(hipStreamSynchronize(c.stream));
+  c.have_coulomb = true;
+}
+
+void sigma_gf2(Ctx &c, int tstp, Prec prec) {
+  require(c.have_coulomb, "sigma: Coulomb tensor not set");
+  require(tstp >= 0 && tstp <= c.nt, "sigma: tstp out of range");
+  if (prec == PREC_F64) sigma_typed<double>(c, tstp, prec);
+  else sigma_typed<float>(c, tstp, prec);
+}
+
+}  // namespace ned

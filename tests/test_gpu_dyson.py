@@ -1,0 +1,196 @@
+"""GPU parity of the Dyson step, predictor, mean field, bootstrap, energy and
+whole timestepping runs with the CPU oracle."""
+import numpy as np
+import pytest
+
+import nedyson_amd as ned
+from helpers import free_green, poly_green, sym_V
+from nedyson_amd import problem
+from nedyson_amd.green import GreenFunc
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_problem(nt=14, ntau=15, nao=4, naux=12, seed=0):
+    return problem.Problem(nao=nao, naux=naux, nelec=2, beta=10.0, ntau=ntau, nt=nt, dt=0.02, seed=seed,
+                           spacing=1.0, b_sd=0.1, quench="h00", quench_norm=0.5)
+
+
+def _ctx_from(p, G=None, S=None, hmf=None):
+    c = ned.Context(p.nao, p.naux, p.nt, p.ntau, p.k)
+    c.set_coulomb(p.B)
+    if G is not None:
+        c.upload(ned.NED_G, G)
+    if S is not None:
+        c.upload(ned.NED_SIGMA, S)
+    if hmf is not None:
+        c.write_hmf(hmf)
+    c.set_matsubara(p.beta)
+    return c
+
+
+def _oracle_run(orc, p, upto, corr=3):
+    G = p.initial_green()
+    S = GreenFunc(p.nt, p.ntau, p.nao)
+    hmf = np.zeros((p.nt + 1, p.nao, p.nao), dtype=complex)
+    D = orc.Dyson(p.nt, p.ntau, p.nao, p.k)
+    it, err = D.do_boot(p.V, G, S, hmf, p.fock, p.mu, p.beta, p.dt, 50, 1e-12)
+    assert it > 0, err
+    for T in range(p.k + 1, upto + 1):
+        D.do_tstp(p.V, T, G, S, hmf, p.h0, p.mu, p.beta, p.dt, corr)
+    return D, G, S, hmf
+
+
+def _relerr(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def test_dyson_step_free_limit_gpu(orc):
+    """Sigma = 0: the GPU Dyson step reproduces the closed-form free G with
+    sixth-order accuracy, and the oracle's step to rounding."""
+    rng = np.random.default_rng(2)
+    h = rng.normal(0, 0.5, (3, 3))
+    h = h + h.T
+    mu, beta, dt, nt, ntau, k = 0.1, 4.0, 0.01, 60, 24, 5
+    x, _ = orc.gll(ntau)
+    ex = free_green(h, mu, beta, dt, nt, x)
+    G = ex.copy()
+    for n in range(k + 1, nt + 1):
+        G.ret_row(n)[:] = 0
+        G.les_col(n)[:] = 0
+        G.tv[n] = 0
+    Go = G.copy()
+    S = GreenFunc(nt, ntau, 3)
+    hmf = np.repeat(h[None].astype(complex), nt + 1, axis=0)
+    with ned.Context(3, 1, nt, ntau, k) as c:
+        c.set_coulomb(np.zeros((1, 3, 3)))
+        c.upload(ned.NED_G, G)
+        c.upload(ned.NED_SIGMA, S)
+        c.write_hmf(hmf)
+        c.set_matsubara(beta)
+        for n in range(k + 1, nt + 1):
+            c.extrapolate(n)
+            c.dyson_step(n, mu, beta, dt)
+        c.download(ned.NED_G, G)
+    D = orc.Dyson(nt, ntau, 3, k)
+    for n in range(k + 1, nt + 1):
+        D.extrapolate(n, Go)
+        D.dyson_step(n, Go, S, hmf, mu, beta, dt)
+    for comp in ("ret", "les", "tv"):
+        assert np.max(np.abs(getattr(G, comp) - getattr(ex, comp))) < 1e-9, comp
+        assert np.max(np.abs(getattr(G, comp) - getattr(Go, comp))) < 1e-12, comp
+
+
+def test_extrapolate_parity(orc):
+    rng = np.random.default_rng(5)
+    nt, ntau, nao, k, dt = 20, 6, 3, 5, 0.05
+    g = poly_green(nt, ntau, nao, dt, k, rng)
+    D = orc.Dyson(nt, ntau, nao, k)
+    with ned.Context(nao, 1, nt, ntau, k) as c:
+        c.upload(ned.NED_G, g)
+        for n in (k + 1, k + 2, 13, 20):
+            h = g.copy()
+            D.extrapolate(n, h)
+            c.extrapolate(n)
+            d = c.download(ned.NED_G)
+            for a, b in ((d.ret_row(n), h.ret_row(n)), (d.les_col(n), h.les_col(n)), (d.tv[n], h.tv[n])):
+                assert np.max(np.abs(a - b)) < 1e-12
+                # and exact on polynomial data
+            assert np.max(np.abs(d.ret_row(n) - g.ret_row(n))) < 1e-9 * np.max(np.abs(g.ret_row(n)))
+
+
+def test_hf_parity(orc):
+    rng = np.random.default_rng(3)
+    nao, naux, nt = 5, 9, 6
+    V, B = sym_V(rng, nao, naux)
+    G = GreenFunc(nt, 5, nao)
+    for t in range(nt + 1):
+        a = rng.normal(size=(nao, nao)) + 1j * rng.normal(size=(nao, nao))
+        G.lesptr(t, t)[:] = 1j * (a + a.conj().T)
+    h0 = rng.normal(size=(nao, nao))
+    h0 = h0 + h0.T
+    hmf = np.zeros((nt + 1, nao, nao), dtype=complex)
+    with ned.Context(nao, naux, nt, 5, 5) as c:
+        c.set_coulomb(B)
+        c.upload(ned.NED_G, G)
+        for t in range(nt + 1):
+            c.hf(t, h0)
+            rho = 1j * (-1) * G.lesptr(t, t)
+            hmf[t] = h0
+            orc.hf(V, t, hmf, rho)
+        out = c.read_hmf()
+    assert np.max(np.abs(out - hmf)) < 1e-12
+
+
+def test_dyson_step_parity_gf2_state(orc):
+    """From a GF2 state produced by the oracle, one GPU dyson_step equals the
+    oracle's dyson_step on every stored G block of slice T."""
+    p = _small_problem(nt=16)
+    D, G, S, hmf = _oracle_run(orc, p, 12)
+    T = 13
+    D.extrapolate(T, G)
+    rho = -1j * G.lesptr(T, T)
+    hmf[T] = p.h0
+    orc.hf(p.V, T, hmf, rho)
+    orc.sigma_solve(p.V, T, S, G)
+    Gg = G.copy()
+    e_ref, errs_ref = D.dyson_step(T, G, S, hmf, p.mu, p.beta, p.dt)
+    with _ctx_from(p, Gg, S, hmf) as c:
+        errs = c.dyson_step(T, p.mu, p.beta, p.dt)
+        d = c.download(ned.NED_G)
+    assert _relerr(d.ret_row(T), G.ret_row(T)) < 1e-12
+    assert _relerr(d.les_col(T), G.les_col(T)) < 1e-12
+    assert _relerr(d.tv[T], G.tv[T]) < 1e-12
+    assert np.max(np.abs(errs - errs_ref)) < 1e-9 * max(1.0, np.max(errs_ref))
+
+
+def test_bootstrap_parity(orc):
+    p = _small_problem(nt=8)
+    G = p.initial_green()
+    S = GreenFunc(p.nt, p.ntau, p.nao)
+    hmf = np.zeros((p.nt + 1, p.nao, p.nao), dtype=complex)
+    D = orc.Dyson(p.nt, p.ntau, p.nao, p.k)
+    it_o, err_o = D.do_boot(p.V, G, S, hmf, p.fock, p.mu, p.beta, p.dt, 50, 1e-12)
+    with _ctx_from(p, p.initial_green()) as c:
+        it_g, err_g = c.do_boot(p.fock, p.mu, p.beta, p.dt, "f64", 50, 1e-12)
+        d = c.download(ned.NED_G)
+    assert it_g == it_o
+    for t in range(p.k + 1):
+        assert _relerr(d.tv[t], G.tv[t]) < 1e-11
+        assert _relerr(d.ret_row(t), G.ret_row(t)) < 1e-11
+        assert _relerr(d.les_col(t), G.les_col(t)) < 1e-11
+
+
+def test_full_run_parity_f64(orc):
+    """Bootstrap + every timestep on the GPU (FP64) against the oracle."""
+    p = _small_problem(nt=24)
+    D, G, S, hmf = _oracle_run(orc, p, p.nt)
+    with _ctx_from(p, p.initial_green()) as c:
+        c.do_boot(p.fock, p.mu, p.beta, p.dt, "f64", 50, 1e-12)
+        for T in range(p.k + 1, p.nt + 1):
+            c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, "f64", 3)
+        d = c.download(ned.NED_G)
+        ds = c.download(ned.NED_SIGMA)
+        for T in (p.k + 1, 15, p.nt):
+            e = c.energy(T, p.h0, p.beta, p.dt)
+            assert abs(e[1] - 2 * D.energy_conv(T, S, G, p.beta, p.dt)) < 1e-10
+    for comp in ("ret", "les", "tv"):
+        assert _relerr(getattr(d, comp), getattr(G, comp)) < 1e-10, comp
+        assert _relerr(getattr(ds, comp), getattr(S, comp)) < 1e-10, comp
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16x3", 1e-4)])
+def test_mixed_precision_run_close_to_f64(prec, tol):
+    """Sigma contractions in FP32 / bf16x3, Dyson in FP64: G stays within tol
+    of the all-FP64 run (max-abs over all stored blocks)."""
+    p = _small_problem(nt=24)
+    outs = {}
+    for pr in ("f64", prec):
+        with _ctx_from(p, p.initial_green()) as c:
+            c.do_boot(p.fock, p.mu, p.beta, p.dt, pr, 50, 1e-12)
+            for T in range(p.k + 1, p.nt + 1):
+                c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, pr, 3)
+            outs[pr] = c.download(ned.NED_G)
+    for comp in ("ret", "les", "tv"):
+        err = np.max(np.abs(getattr(outs[prec], comp) - getattr(outs["f64"], comp)))
+        assert err < tol, (comp, err)

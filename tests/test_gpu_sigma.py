@@ -1,0 +1,89 @@
+"""GPU parity of the second-Born self-energy (ned_sigma_gf2_decomp) with the
+CPU oracle's restatement of molGF2SolverDecomp::solve."""
+import numpy as np
+import pytest
+
+import nedyson_amd as ned
+from helpers import random_green, sym_V
+from nedyson_amd.green import GreenFunc
+
+pytestmark = pytest.mark.gpu
+
+# max-abs error relative to max |Sigma| (FP64 is checked absolutely below)
+TOL = {"f64": 1e-12, "f32": 2e-5, "bf16x3": 2e-4}
+
+
+def _sigma_gpu(V, B, G, tstp, prec, chunk=None):
+    nao, naux = V.shape[0], V.shape[2]
+    with ned.Context(nao, naux, G.nt, G.ntau, 1) as c:
+        c.set_coulomb(B)
+        c.upload(ned.NED_G, G)
+        c.sigma(tstp, prec)
+        return c.download(ned.NED_SIGMA)
+
+
+@pytest.mark.parametrize("nao,naux", [(2, 3), (4, 12), (8, 32), (13, 20)])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3"])
+def test_sigma_parity(orc, nao, naux, prec):
+    print(f"[sigma parity] nao={nao} naux={naux} prec={prec}", flush=True)
+    rng = np.random.default_rng(100 * nao + naux)
+    V, B = sym_V(rng, nao, naux)
+    nt, ntau = 6, 7
+    G = random_green(rng, nt, ntau, nao, scale=0.5)
+    for tstp in (0, 3, nt):
+        S_ref = GreenFunc(nt, ntau, nao)
+        orc.sigma_solve(V, tstp, S_ref, G)
+        S = _sigma_gpu(V, B, G, tstp, prec)
+        s = tstp * (tstp + 1) // 2
+        for name, a, b in (("ret", S.ret_row(tstp), S_ref.ret_row(tstp)),
+                           ("les", S.les_col(tstp), S_ref.les_col(tstp)),
+                           ("tv", S.tv[tstp], S_ref.tv[tstp])):
+            scale = max(np.max(np.abs(b)), 1e-30)
+            err = np.max(np.abs(a - b))
+            assert err <= TOL[prec] * scale, (name, tstp, err, scale)
+        del s
+
+
+def test_sigma_chunking_invariant(orc):
+    """Splitting the slice into item chunks must not change the result."""
+    rng = np.random.default_rng(5)
+    nao, naux, nt, ntau = 6, 10, 9, 11
+    V, B = sym_V(rng, nao, naux)
+    G = random_green(rng, nt, ntau, nao, scale=0.5)
+    outs = []
+    import os
+    for chunk in (0, 1, 5):
+        os.environ["NED_SIGMA_CHUNK"] = str(chunk)
+        try:
+            with ned.Context(nao, naux, nt, ntau, 1) as c:
+                c.set_coulomb(B)
+                c.upload(ned.NED_G, G)
+                c.sigma(nt, "f64")
+                outs.append(c.download(ned.NED_SIGMA))
+        finally:
+            del os.environ["NED_SIGMA_CHUNK"]
+    for o in outs[1:]:
+        assert np.max(np.abs(o.ret - outs[0].ret)) == 0
+
+
+def test_sigma_rejects_nonsymmetric_V():
+    rng = np.random.default_rng(1)
+    B = rng.normal(size=(3, 2, 2))
+    with ned.Context(2, 3, 6, 5, 1) as c:
+        with pytest.raises(ned.NedError):
+            c.set_coulomb(B)
+
+
+def test_sigma_wrapper_reference_signature(orc):
+    """molGF2SolverDecomp(Vija, Viaj).solve(tstp, Sigma, G) on host objects."""
+    rng = np.random.default_rng(2)
+    nao, naux, nt, ntau = 3, 5, 5, 6
+    V, B = sym_V(rng, nao, naux)
+    G = random_green(rng, nt, ntau, nao, scale=0.5)
+    solver = ned.molGF2SolverDecomp(V)
+    S, S_ref = GreenFunc(nt, ntau, nao), GreenFunc(nt, ntau, nao)
+    for t in range(nt + 1):
+        solver.solve(t, S, G)
+        orc.sigma_solve(V, t, S_ref, G)
+    for c in ("ret", "les", "tv"):
+        assert np.max(np.abs(getattr(S, c) - getattr(S_ref, c))) < 1e-12
