@@ -124,6 +124,24 @@ ned_status ned_do_tstp(ned_ctx *ctx, int tstp, const double *h0, double mu, doub
  * out[0] = Re tr(rho (hmf + h0)), out[1] = 2 * energy_conv, out[2] = 2 tr rho. */
 ned_status ned_energy(ned_ctx *ctx, int tstp, const double *h0, double beta, double dt, double *out3);
 
+/* dyson::G0_from_h0(G, mu, H0, beta, dt) (src/dyson_free.h:226-297) for
+ * time steps 0..tmax (tmax = k is the reference's bootstrap initial guess):
+ * G^R, G^<, G^rceil of the constant mean field fock (real nao x nao).  G^M is
+ * not written, as in the reference. */
+ned_status ned_g0_from_h0(ned_ctx *ctx, int tmax, const double *fock, double mu, double beta, double dt);
+
+/* Time-translation-invariant continuation of slice tsrc to all t <= tsrc:
+ * ret(t,t') = ret(tsrc, tsrc-(t-t')), les(t',t) = les(tsrc-(t-t'), tsrc),
+ * tv(t,:) = tv(tsrc,:) -- the relation the bootstrap imposes on G^R
+ * (src/dyson_boot.h:78-88).  Used to seed long equilibrium histories. */
+ned_status ned_tti_extend(ned_ctx *ctx, int which, int tsrc);
+
+/* Event timing of the hot phases on the context's stream.  ned_profile_read
+ * returns {ms, count} for the Sigma exchange GEMM, the whole Sigma and the
+ * whole Dyson step accumulated since the last read (6 doubles). */
+ned_status ned_profile(ned_ctx *ctx, int enable);
+ned_status ned_profile_read(ned_ctx *ctx, double *out6);
+
 /* Gauss-Lobatto-Legendre nodes x_0..x_ntau on [-1,1] (ascending). Host-only. */
 ned_status ned_gll_nodes(int ntau, double *x);
 

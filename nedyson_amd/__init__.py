@@ -30,7 +30,8 @@ _STATUS = {0: "NED_OK", 1: "NED_ERR_ARG", 2: "NED_ERR_NOMEM", 3: "NED_ERR_DEVICE
 EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "ned_set_coulomb", "ned_green_write",
            "ned_green_read", "ned_hmf_write", "ned_hmf_read", "ned_sigma_gf2_decomp", "ned_hf_decomp",
            "ned_set_matsubara", "ned_dyson_step", "ned_extrapolate", "ned_dyson_start", "ned_do_boot", "ned_do_tstp",
-           "ned_energy", "ned_gll_nodes", "ned_integrator"]
+           "ned_energy", "ned_g0_from_h0", "ned_tti_extend", "ned_profile", "ned_profile_read", "ned_gll_nodes",
+           "ned_integrator"]
 
 
 class NedError(RuntimeError):
@@ -82,6 +83,10 @@ def lib():
     L.ned_do_tstp.argtypes = [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
                               C.POINTER(C.c_int), _dp]
     L.ned_energy.argtypes = [_vp, C.c_int, _dp, C.c_double, C.c_double, _dp]
+    L.ned_g0_from_h0.argtypes = [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double]
+    L.ned_tti_extend.argtypes = [_vp, C.c_int, C.c_int]
+    L.ned_profile.argtypes = [_vp, C.c_int]
+    L.ned_profile_read.argtypes = [_vp, _dp]
     L.ned_gll_nodes.argtypes = [C.c_int, _dp]
     L.ned_integrator.argtypes = [C.c_int] + [_dp] * 7
     for name in EXPORTS[1:]:
@@ -242,6 +247,22 @@ class Context:
         out = np.zeros(3)
         _check(lib().ned_energy(self.h, tstp, _dptr(h0), beta, dt, _dptr(out)))
         return out
+
+    def g0_from_h0(self, tmax, fock, mu, beta, dt):
+        fock = np.ascontiguousarray(fock, dtype=np.float64)
+        _check(lib().ned_g0_from_h0(self.h, tmax, _dptr(fock), mu, beta, dt))
+
+    def tti_extend(self, which, tsrc):
+        _check(lib().ned_tti_extend(self.h, which, tsrc))
+
+    def profile(self, enable=True):
+        _check(lib().ned_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        out = np.zeros(6)
+        _check(lib().ned_profile_read(self.h, _dptr(out)))
+        return {"ugemm_ms": out[0], "ugemm_n": int(out[1]), "sigma_ms": out[2], "sigma_n": int(out[3]),
+                "dyson_ms": out[4], "dyson_n": int(out[5])}
 
     def sync(self):
         _check(lib().ned_sync(self.h))

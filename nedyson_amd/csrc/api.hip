@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <new>
 #include <string>
 #include <vector>
@@ -307,6 +308,48 @@ ned_status ned_energy(ned_ctx *ctx, int tstp, const double *h0, double beta, dou
   return guard(ctx, [&] {
     ned::require(tstp >= 0 && tstp <= ctx->c.nt, "tstp out of range");
     ned::energy(ctx->c, tstp, h0, beta, dt, out3);
+  });
+}
+
+ned_status ned_g0_from_h0(ned_ctx *ctx, int tmax, const double *fock, double mu, double beta, double dt) {
+  if (!ctx || !fock) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] { ned::g0_from_h0(ctx->c, tmax, fock, mu, beta, dt); });
+}
+
+ned_status ned_tti_extend(ned_ctx *ctx, int which, int tsrc) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::require(which == NED_G || which == NED_SIGMA, "which must be NED_G or NED_SIGMA");
+    ned::tti_extend(ctx->c, which, tsrc);
+  });
+}
+
+ned_status ned_profile(ned_ctx *ctx, int enable) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] { ctx->c.prof = enable != 0; });
+}
+
+ned_status ned_profile_read(ned_ctx *ctx, double *out6) {
+  if (!ctx || !out6) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::Ctx &c = ctx->c;
+    NED_HIP(hipStreamSynchronize(c.stream));
+    for (int ph = 0; ph < 3; ph++) {
+      auto &a = c.ev[2 * ph], &b = c.ev[2 * ph + 1];
+      double tot = 0;
+      size_t cnt = std::min(a.size(), b.size());
+      for (size_t i = 0; i < cnt; i++) {
+        float ms = 0;
+        NED_HIP(hipEventElapsedTime(&ms, a[i], b[i]));
+        tot += ms;
+      }
+      for (auto e : a) (void)hipEventDestroy(e);
+      for (auto e : b) (void)hipEventDestroy(e);
+      a.clear();
+      b.clear();
+      out6[2 * ph] = tot;
+      out6[2 * ph + 1] = (double)cnt;
+    }
   });
 }
 

@@ -50,14 +50,14 @@ struct GOp {
   long long ks_hi = 0;     // stride of the high level of the K index
   int kdiv = 0;            // 0: single-level K
   long long ps = 0;        // plane stride (complex)
-  int bdiv = 1;            // batch b -> (b / bdiv) * bs_hi + (b % bdiv) * bs_lo
-  long long bs_hi = 0, bs_lo = 0;
+  int bdiv = 1 << 30;      // batch b -> (b / bdiv) * bs_hi + (b % bdiv) * bs_lo
+  long long bs_hi = 0, bs_lo = 0;  // (single-level batches use bs_lo)
 };
 
 struct GOut {
   void *ptr = nullptr;
   long long rs = 0, cs = 1, ps = 0;
-  int bdiv = 1;
+  int bdiv = 1 << 30;
   long long bs_hi = 0, bs_lo = 0;
 };
 

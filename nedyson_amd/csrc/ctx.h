@@ -60,6 +60,16 @@ struct Ctx {
   bool have_mix = false;
   DevBuf dy_info, dy_js, dy_gA, dy_gB, dy_err, dy_Q, dy_X, dy_M, dy_Qs, dy_part, dy_scr, dy_Y;
   DevBuf dy_la, dy_lb, dy_mixtmp, dy_h0, dy_xq, dy_bootI, dy_en;
+  // event timing of the hot phases (enabled by ned_profile)
+  bool prof = false;
+  std::vector<hipEvent_t> ev[6];  // U-GEMM start/end, Sigma start/end, Dyson start/end
+  void mark(int slot) {
+    if (!prof) return;
+    hipEvent_t e;
+    NED_HIP(hipEventCreate(&e));
+    NED_HIP(hipEventRecord(e, stream));
+    ev[slot].push_back(e);
+  }
 };
 
 size_t tri(int t);
@@ -68,5 +78,7 @@ void require(bool cond, const char *msg);
 // Sigma: molGF2SolverDecomp::solve (src/mol2bSolver_decomp_impl.h:412-429)
 void sigma_gf2(Ctx &c, int tstp, Prec prec);
 void build_coulomb(Ctx &c, const double *Bq_host);
+void g0_from_h0(Ctx &c, int tmax, const double *fock, double mu, double beta, double dt);
+void tti_extend(Ctx &c, int which, int tsrc);
 
 }  // namespace ned

@@ -887,6 +887,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   const size_t es = c.es;
   Pt p = make_pt(c, T, mu, beta, dt);
   hipStream_t s = c.stream;
+  c.mark(4);
   update_minv(c, T, mu, dt, w);
   c.dy_err.ensure(4 * sizeof(double));
   double *err = c.dy_err.as<double>();
@@ -942,6 +943,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, X, err + 2);
     check_launch("les");
   }
+  c.mark(5);
   if (err3) {
     double h[4];
     NED_HIP(hipMemcpyAsync(h, err, 4 * sizeof(double), hipMemcpyDeviceToHost, s));

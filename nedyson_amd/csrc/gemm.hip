@@ -135,7 +135,9 @@ __global__ void __launch_bounds__(NT) kern_gemm_fp(GemmDesc d) {
       }
       __syncthreads();
     }
-    // ---- epilogue: lane holds D[4*(lane/16)+q][lane%16] of each 16x16 block
+    // ---- epilogue.  Register q of a lane holds, in each 16x16 block,
+    //   f32 16x16x4:  D[4*(lane/16) + q][lane%16]   (groups of 4 rows)
+    //   f64 16x16x4:  D[4*q + lane/16][lane%16]     (4 groups of 1 row)
     const long long bc = (long long)(b / d.C.bdiv) * d.C.bs_hi + (long long)(b % d.C.bdiv) * d.C.bs_lo;
     T *C = reinterpret_cast<T *>(d.C.ptr);
 #pragma unroll
@@ -144,7 +146,8 @@ __global__ void __launch_bounds__(NT) kern_gemm_fp(GemmDesc d) {
       for (int j = 0; j < 2; j++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + q;
+          const int rin = sizeof(T) == 8 ? 4 * q + (lane >> 4) : 4 * (lane >> 4) + q;
+          int row = m0 + wm * 32 + i * 16 + rin;
           int col = n0 + wn * 32 + j * 16 + (lane & 15);
           if (row < d.M && col < d.N) {
             long long off = bc + (long long)row * d.C.rs + (long long)col * d.C.cs;

@@ -29,3 +29,9 @@ struct GLL {
 };
 
 }  // namespace ned
+
+namespace ned {
+// Real symmetric eigendecomposition (cyclic Jacobi): A = V diag(w) V^T,
+// eigenvalues ascending, eigenvectors in the columns of V (row-major n x n).
+void sym_eig(int n, const double *A, double *w, double *V);
+}  // namespace ned

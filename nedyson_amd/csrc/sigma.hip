@@ -159,6 +159,7 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     gemm(d, prec, s);
   }
   // H = -C U'   (rows (p, plane, a) of X, all at once; the n^4 naux GEMM)
+  c.mark(0);
   {
     GemmDesc d;
     d.M = 2 * a * P; d.N = (int)n2; d.K = (int)n2; d.batch = 1;
@@ -168,6 +169,7 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     d.alpha = -1.0;
     gemm(d, prec, s);
   }
+  c.mark(1);
   // H += 2 Pi Vmat   (Vmat[b][(k,j)] = B[b][k][j])
   {
     GemmDesc d;
@@ -254,8 +256,10 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
 void sigma_gf2(Ctx &c, int tstp, Prec prec) {
   require(c.have_coulomb, "sigma: Coulomb tensor not set");
   require(tstp >= 0 && tstp <= c.nt, "sigma: tstp out of range");
+  c.mark(2);
   if (prec == PREC_F64) sigma_typed<double>(c, tstp, prec);
   else sigma_typed<float>(c, tstp, prec);
+  c.mark(3);
 }
 
 }  // namespace ned

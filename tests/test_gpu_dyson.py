@@ -194,3 +194,42 @@ def test_mixed_precision_run_close_to_f64(prec, tol):
     for comp in ("ret", "les", "tv"):
         err = np.max(np.abs(getattr(outs[prec], comp) - getattr(outs["f64"], comp)))
         assert err < tol, (comp, err)
+
+
+def test_g0_from_h0_gpu(orc):
+    """ned_g0_from_h0 == oracle G0_from_h0 for t <= k, and == the closed-form
+    free propagator for every later t."""
+    rng = np.random.default_rng(8)
+    h = rng.normal(0, 0.5, (4, 4))
+    h = h + h.T
+    mu, beta, dt, nt, ntau, k = 0.2, 3.0, 0.05, 12, 10, 5
+    x, _ = orc.gll(ntau)
+    ex = free_green(h, mu, beta, dt, nt, x)
+    Go = GreenFunc(nt, ntau, 4)
+    orc.Dyson(nt, ntau, 4, k).g0_from_h0(Go, mu, h, beta, dt)
+    with ned.Context(4, 1, nt, ntau, k) as c:
+        c.g0_from_h0(nt, h, mu, beta, dt)
+        d = c.download(ned.NED_G)
+    for t in range(k + 1):
+        assert np.max(np.abs(d.ret_row(t) - Go.ret_row(t))) < 1e-13
+        assert np.max(np.abs(d.les_col(t) - Go.les_col(t))) < 1e-13
+        assert np.max(np.abs(d.tv[t] - Go.tv[t])) < 1e-13
+    for comp in ("ret", "les", "tv"):
+        assert np.max(np.abs(getattr(d, comp) - getattr(ex, comp))) < 1e-12
+
+
+def test_tti_extend_gpu():
+    rng = np.random.default_rng(9)
+    nt, ntau, nao = 7, 5, 2
+    from helpers import random_green
+    g = random_green(rng, nt, ntau, nao)
+    with ned.Context(nao, 1, nt, ntau, 5) as c:
+        c.upload(ned.NED_G, g)
+        c.tti_extend(ned.NED_G, 6)
+        d = c.download(ned.NED_G)
+    for t in range(7):
+        for s in range(t + 1):
+            assert np.array_equal(d.retptr(t, s), g.retptr(6, 6 - (t - s)))
+            assert np.array_equal(d.lesptr(s, t), g.lesptr(6 - (t - s), 6))
+        assert np.array_equal(d.tv[t], g.tv[6])
+    assert np.array_equal(d.tv[7], g.tv[7])
