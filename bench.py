@@ -1,0 +1,219 @@
+"""Benchmark: KBE-GF2 timesteps/s at nao=64 (one MI355X).
+
+Workload (default): nao=64, naux=256, 128 GLL nodes, dt=0.01, k=5, GF2,
+spin-restricted -- the nao=64 configuration -- timed at mid-run, T = 1000 of
+nt = 2000, where the per-step cost equals the whole-run average (the Sigma cost
+grows linearly with T).  The history t <= T0 is the non-interacting solution
+of the equilibrium Fock matrix (ned_g0_from_h0) with its self-energy slice
+continued time-translation invariantly (ned_tti_extend); the timed steps
+T0+W+1 .. T0+W+K then run the full timestep -- predictor, and `corr` corrector
+passes of {HF, Sigma, Dyson} -- on the quenched Hamiltonian.
+
+The same steps are re-run with FP64 Sigma contractions from the same history;
+the line reports the mixed-precision rate as `value`, the FP64 rate, their
+ratio, and the max-abs difference of every G block written by the steps.
+
+`roofline` is for the dominant kernel, the exchange GEMM of Sigma
+(kern_gemm_*<1,1>: rows (item, plane, aux) x U'), timed with HIP events on
+the library's stream.  `cpu_baseline` times the CPU oracle (the restated
+reference algorithm, one thread) on one lesser, one retarded and one mixing
+time point of Sigma at the same sizes and scales to a whole timestep with the
+reference's flop model.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "KBE timesteps/s at nao=64 GF2 vs plain FP64 at ≤1e-6 G error; % MFMA peak"
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0 / 3}
+
+
+def sigma_flops_reassoc(n, a, items):
+    """Flops of the executed Sigma algorithm per slice (U-reassociated exchange):
+    per item 4 n^4 a (exchange GEMM) + bubble and V-products."""
+    per = 4 * n ** 4 * a + 8 * n ** 3 * a * 4 + 12 * n ** 2 * a ** 2
+    return per * items
+
+
+def ref_sigma_flops(n, a, T, ntau1):
+    """Reference decomposition (mol2bSolver_decomp_impl.h), 4/8 flops per
+    real/complex MAC: (T+1)(36 n^4 a + 128 n^3 a + 44 n^2 a^2) + Ntau(12 n^4 a + 36 n^3 a + 12 n^2 a^2)."""
+    return (T + 1) * (36 * n ** 4 * a + 128 * n ** 3 * a + 44 * n ** 2 * a ** 2) + \
+        ntau1 * (12 * n ** 4 * a + 36 * n ** 3 * a + 12 * n ** 2 * a ** 2)
+
+
+def dyson_flops(n, T, ntau1):
+    return 16 * T ** 2 * n ** 3 + 16 * T * ntau1 * n ** 3 + 4 * ntau1 ** 2 * n ** 3
+
+
+def cpu_baseline(p, T, corr, log):
+    """Oracle timing on one les, ret and tv point of Sigma (one thread)."""
+    from oracle import oracle as O
+    from nedyson_amd.green import GreenFunc
+    n, a = p.nao, p.naux
+    rng = np.random.default_rng(1)
+    G = GreenFunc(2, 3, n)
+    for arr in (G.ret, G.les, G.tv):
+        arr[:] = 0.1 * (rng.uniform(-1, 1, arr.shape) + 1j * rng.uniform(-1, 1, arr.shape))
+    S = GreenFunc(2, 3, n)
+    times = {}
+    for comp, name in ((0, "les"), (1, "ret"), (2, "tv")):
+        t0 = time.perf_counter()
+        O.sigma_point(p.V, comp, 2, 1, S, G)
+        times[name] = time.perf_counter() - t0
+        log(f"[cpu] Sigma {name} point: {times[name]:.2f} s")
+    les_flops = 12 * n ** 4 * a + 36 * n ** 3 * a + 12 * n ** 2 * a ** 2
+    rate = les_flops / times["les"]
+    t_sigma = (T + 1) * (times["les"] + times["ret"]) + (p.ntau + 1) * times["tv"]
+    t_dyson = dyson_flops(n, T, p.ntau + 1) / rate
+    t_step = corr * (t_sigma + t_dyson)
+    return {"value": 1.0 / t_step, "unit": "timesteps/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle (reference decomposition restated, 1 thread): one lesser, one retarded and one "
+                      f"mixing Sigma time point at nao={n}, naux={a} ({sum(times.values()):.1f} s), scaled to a "
+                      f"timestep at T={T} with {corr} corrector passes as (T+1)(t_les+t_ret)+Ntau*t_tv plus the "
+                      f"Dyson flop model at the measured {rate / 1e9:.1f} GFLOP/s",
+            "sec_per_step": t_step, "gflops_measured": rate / 1e9}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--T0", type=int, default=None, help="last history step (default nt/2)")
+    ap.add_argument("--prec", default="bf16x3", choices=["f32", "bf16x3"])
+    ap.add_argument("--corr", type=int, default=3, help="corrector passes per step")
+    ap.add_argument("--f64-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--b-sd", type=float, default=None, help="override sigma_B of the Coulomb factor")
+    ap.add_argument("--dt", type=float, default=None)
+    ap.add_argument("--energy", action="store_true", help="log particle number and energy per step")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    import nedyson_amd as ned
+    from nedyson_amd import problem
+
+    t_setup = time.perf_counter()
+    over = {}
+    if args.b_sd is not None:
+        over["b_sd"] = args.b_sd
+    if args.dt is not None:
+        over["dt"] = args.dt
+    p = problem.config(args.config, **over)
+    log(f"[setup] problem c{args.config}: nao={p.nao} naux={p.naux} ntau+1={p.ntau + 1} mu={p.mu:.6f} "
+        f"({time.perf_counter() - t_setup:.1f} s)")
+    T0 = p.nt // 2 if args.T0 is None else args.T0
+    W, K = args.warmup, args.steps
+    nt = T0 + W + K
+    n, a = p.nao, p.naux
+    c = ned.Context(n, a, nt, p.ntau, p.k)
+    c.set_coulomb(p.B)
+    c.write_blocks(ned.NED_G, ned.NED_MAT, 0, p.Gmat)
+    c.set_matsubara(p.beta)
+    c.g0_from_h0(T0, p.fock, p.mu, p.beta, p.dt)
+    c.write_hmf(np.repeat(p.fock[None].astype(complex), T0 + 1, axis=0))
+    c.sigma(T0, "f64")
+    c.tti_extend(ned.NED_SIGMA, T0)
+    c.sync()
+    log(f"[setup] history t<=T0={T0} ready ({time.perf_counter() - t_setup:.1f} s)")
+
+    def run(prec, steps, profile):
+        t_list, iters = [], []
+        for i, T in enumerate(range(T0 + 1, nt + 1)):
+            timed = i >= W
+            if timed and profile:
+                c.profile(True)
+            c.sync()
+            t0 = time.perf_counter()
+            it, err = c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, prec, args.corr, 0.0)
+            c.sync()
+            dt_ = time.perf_counter() - t0
+            e = c.energy(T, p.h0, p.beta, p.dt) if args.energy else (np.nan,) * 3
+            log(f"[{prec}] step T={T} iters={it} err={err:.3e} {dt_ * 1e3:.1f} ms  N={e[2]:.10f} E={e[0] + e[1]:.10f}")
+            if timed:
+                t_list.append(dt_)
+                iters.append(it)
+        prof = c.profile_read() if profile else None
+        c.profile(False)
+        return t_list, iters, prof
+
+    def slices():
+        out = []
+        for T in range(T0 + 1, nt + 1):
+            s = T * (T + 1) // 2
+            out.append(c.read_blocks(ned.NED_G, ned.NED_RET, s, T + 1))
+            out.append(c.read_blocks(ned.NED_G, ned.NED_LES, s, T + 1))
+            out.append(c.read_blocks(ned.NED_G, ned.NED_TV, T * (p.ntau + 1), p.ntau + 1))
+        return out
+
+    t_mixed, iters, prof = run(args.prec, K, True)
+    g_mixed = slices()
+    elapsed = sum(t_mixed)
+    value = K / elapsed
+    items = 2 * (T0 + W + 1) + p.ntau + 1  # items of a slice near the timed steps
+    f64_value, max_err = None, None
+    if not args.no_f64:
+        t_f64, _, _ = run("f64", K, False)
+        g_f64 = slices()
+        f64_value = K / sum(t_f64)
+        max_err = float(max(np.max(np.abs(x - y)) for x, y in zip(g_mixed, g_f64)))
+    c.close()
+
+    ugemm_ms = prof["ugemm_ms"] / max(prof["ugemm_n"], 1)
+    n_sigma = max(prof["sigma_n"], 1)
+    # algorithmic flops of one exchange-GEMM launch, 2 M N K with M = 2 naux P
+    # rows (P time points of the slice), N = K = nao^2, averaged over launches
+    flops_per_launch = prof["ugemm_flops"] / max(prof["ugemm_n"], 1)
+    achieved = flops_per_launch / (ugemm_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[args.prec]
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * elapsed / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{args.prec} Sigma contractions, f64 Dyson",
+        "data": "synthetic (seeded h1, B[Q,i,j]; HF equilibrium; history = free G of the Fock matrix)",
+        "config": {"workload": f"GF2 KBE timestep at T={T0 + W + 1}..{nt}, nao={n}, naux={a}, "
+                               f"ntau+1={p.ntau + 1}, dt={p.dt}, k={p.k}, {args.corr} corrector passes",
+                   "nao": n, "naux": a, "nL": p.ntau + 1, "T": T0 + W + 1, "nt_run": p.nt,
+                   "corrector_passes": args.corr, "sigma_prec": args.prec},
+        "f64_value": f64_value,
+        "speedup_vs_f64": (value / f64_value) if f64_value else None,
+        "max_abs_G_err_vs_f64": max_err,
+        "corrector_iters_per_step": iters,
+        "sigma_ms_per_eval": prof["sigma_ms"] / n_sigma,
+        "dyson_ms_per_step": prof["dyson_ms"] / max(prof["dyson_n"], 1),
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": f"Sigma exchange GEMM ({args.prec}), {flops_per_launch:.3e} algorithmic flop "
+                               f"per launch, avg {ugemm_ms:.3f} ms over {prof['ugemm_n']} launches"},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(p, T0 + W + 1, args.corr, log)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

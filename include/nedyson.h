@@ -137,10 +137,11 @@ ned_status ned_g0_from_h0(ned_ctx *ctx, int tmax, const double *fock, double mu,
 ned_status ned_tti_extend(ned_ctx *ctx, int which, int tsrc);
 
 /* Event timing of the hot phases on the context's stream.  ned_profile_read
- * returns {ms, count} for the Sigma exchange GEMM, the whole Sigma and the
- * whole Dyson step accumulated since the last read (6 doubles). */
+ * fills 8 doubles: {ms, count} for the Sigma exchange GEMM, the whole Sigma
+ * and the whole Dyson step, then the algorithmic flops of the timed exchange
+ * GEMM launches and a reserved 0, accumulated since the last read. */
 ned_status ned_profile(ned_ctx *ctx, int enable);
-ned_status ned_profile_read(ned_ctx *ctx, double *out6);
+ned_status ned_profile_read(ned_ctx *ctx, double *out8);
 
 /* Gauss-Lobatto-Legendre nodes x_0..x_ntau on [-1,1] (ascending). Host-only. */
 ned_status ned_gll_nodes(int ntau, double *x);

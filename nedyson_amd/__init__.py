@@ -259,10 +259,10 @@ class Context:
         _check(lib().ned_profile(self.h, int(enable)))
 
     def profile_read(self):
-        out = np.zeros(6)
+        out = np.zeros(8)
         _check(lib().ned_profile_read(self.h, _dptr(out)))
         return {"ugemm_ms": out[0], "ugemm_n": int(out[1]), "sigma_ms": out[2], "sigma_n": int(out[3]),
-                "dyson_ms": out[4], "dyson_n": int(out[5])}
+                "dyson_ms": out[4], "dyson_n": int(out[5]), "ugemm_flops": out[6]}
 
     def sync(self):
         _check(lib().ned_sync(self.h))

@@ -179,7 +179,7 @@ ned_status ned_hmf_write(ned_ctx *ctx, int first, int count, const void *host) {
     const size_t bs = c.es * sizeof(ned::z2);
     NED_HIP(hipMemcpyAsync((char *)c.hmf.p + first * bs, host, count * bs, hipMemcpyHostToDevice, c.stream));
     NED_HIP(hipStreamSynchronize(c.stream));
-    for (int j = first; j < first + count; j++) c.minv_valid[j] = 0;
+    for (int j = first; j < first + count; j++) ned::touch_time(c, j);
   });
 }
 
@@ -200,7 +200,7 @@ ned_status ned_sigma_gf2_decomp(ned_ctx *ctx, int tstp, int prec) {
     ned::require(prec >= 0 && prec <= 2, "prec must be NED_PREC_F64, _F32 or _BF16X3");
     state(ctx->c.have_coulomb, "interaction not set (ned_set_coulomb)");
     ned::sigma_gf2(ctx->c, tstp, (ned::Prec)prec);
-    if (tstp < (int)ctx->c.minv_valid.size()) ctx->c.minv_valid[tstp] = 0;
+    ned::touch_time(ctx->c, tstp);
   });
 }
 
@@ -292,6 +292,7 @@ ned_status ned_do_tstp(ned_ctx *ctx, int tstp, const double *h0, double mu, doub
     for (; it < corr_max;) {
       ned::hf_decomp(c, tstp, h0);
       ned::sigma_gf2(c, tstp, (ned::Prec)prec);
+      ned::touch_time(c, tstp);
       ned::dyson_step(c, tstp, mu, beta, dt, e);
       err = e[0] + e[1] + e[2];
       it++;
@@ -329,8 +330,9 @@ ned_status ned_profile(ned_ctx *ctx, int enable) {
   return guard(ctx, [&] { ctx->c.prof = enable != 0; });
 }
 
-ned_status ned_profile_read(ned_ctx *ctx, double *out6) {
-  if (!ctx || !out6) { g_err = "null argument"; return NED_ERR_ARG; }
+ned_status ned_profile_read(ned_ctx *ctx, double *out8) {
+  if (!ctx || !out8) { g_err = "null argument"; return NED_ERR_ARG; }
+  double *out6 = out8;
   return guard(ctx, [&] {
     ned::Ctx &c = ctx->c;
     NED_HIP(hipStreamSynchronize(c.stream));
@@ -350,6 +352,9 @@ ned_status ned_profile_read(ned_ctx *ctx, double *out6) {
       out6[2 * ph] = tot;
       out6[2 * ph + 1] = (double)cnt;
     }
+    out8[6] = c.prof_flops;
+    out8[7] = 0;
+    c.prof_flops = 0;
   });
 }
 

@@ -78,4 +78,18 @@ struct GemmDesc {
 //                v_mfma_f32_16x16x32_bf16 with FP32 accumulation
 void gemm(const GemmDesc &d, Prec prec, hipStream_t s);
 
+// Split-bf16 NT GEMM for the self-energy exchange contraction:
+// C (M x N, row stride ldc) = alpha * A B^T (+ C if beta), A = Ahi + Alo
+// (M x K), B = Bhi + Blo (N x K), K-contiguous bf16 planes.
+// Operand: hi/lo planes, row stride ld (elements), batch stride.
+struct NtOp {
+  const uint16_t *hi, *lo;
+  long long ld, stride;
+};
+bool nt_bf16x3_ok(int M, int N, int K);
+void gemm_nt_bf16x3(const NtOp &A, const NtOp &B, float *C, long long ldc, long long sC, int M, int N, int K,
+                    int batch, float alpha, int beta, hipStream_t s);
+// x -> hi + lo with hi = bf16(x), lo = bf16(x - hi), round to nearest even
+void split_bf16(const float *x, uint16_t *hi, uint16_t *lo, long long n, hipStream_t s);
+
 }  // namespace ned

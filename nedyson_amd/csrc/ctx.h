@@ -46,9 +46,10 @@ struct Ctx {
   DevBuf Bq;       // naux x nao x nao double, B[Q][i][j] = V(i,j,Q)
   DevBuf Bq32;     // float copy
   DevBuf U64, U32; // (nao^2 x nao^2): U'[(m,q)][(k,j)] = sum_Q B[Q][k][m] B[Q][q][j]
+  DevBuf Uth, Utl; // U'^T split into bf16 hi / lo planes (operand B of the exchange GEMM)
   bool have_coulomb = false;
   // Sigma scratch
-  DevBuf sg_items_A, sg_items_B, sg_W, sg_X, sg_H, sg_Pi, sg_E;
+  DevBuf sg_items_A, sg_items_B, sg_W, sg_X, sg_H, sg_Pi, sg_E, sg_Xh, sg_Xl;
   int sigma_chunk = 0;  // items per chunk (0 = choose automatically)
   // Dyson state
   DevBuf mixK;                // mixing kernel of G^M (np x np blocks)
@@ -60,8 +61,15 @@ struct Ctx {
   bool have_mix = false;
   DevBuf dy_info, dy_js, dy_gA, dy_gB, dy_err, dy_Q, dy_X, dy_M, dy_Qs, dy_part, dy_scr, dy_Y;
   DevBuf dy_la, dy_lb, dy_mixtmp, dy_h0, dy_xq, dy_bootI, dy_en;
+  // inverses of the k x k block start-up systems: the retarded one depends on
+  // times T-k..T-1 only (valid for every corrector pass of step retinv_T), the
+  // lesser one on times 0..k only (valid until the early history changes)
+  DevBuf dy_retinv, dy_lesinv, dy_Xs;
+  int retinv_T = -1;
+  bool lesinv_valid = false;
   // event timing of the hot phases (enabled by ned_profile)
   bool prof = false;
+  double prof_flops = 0;  // algorithmic flops of the timed exchange-GEMM launches
   std::vector<hipEvent_t> ev[6];  // U-GEMM start/end, Sigma start/end, Dyson start/end
   void mark(int slot) {
     if (!prof) return;

@@ -25,5 +25,7 @@ void hf_decomp(Ctx &c, int tstp, const double *h0_host);
 double dyson_start(Ctx &c, double mu, double beta, double dt);
 void energy(Ctx &c, int tstp, const double *h0_host, double beta, double dt, double *out3);
 void invalidate_minv(Ctx &c);
+// mark hmf(t) / Sigma slice t as changed: drops the cached inverses that use it
+void touch_time(Ctx &c, int t);
 
 }  // namespace ned

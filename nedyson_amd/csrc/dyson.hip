@@ -153,36 +153,40 @@ __global__ void kern_scatter_blocks(const z2 *src, const int *js, int cnt, size_
 }
 
 // ------------------------------------------------------------ retarded
-// k x k block start-up system of src/dyson_step.h:28-65 into M (kn x kn) and
-// Q (kn x n); also sets G^R(T,T) = -i.
-__global__ void __launch_bounds__(256) kern_ret_start(Pt p, WTab w, z2 *M, z2 *Q) {
+// k x k block start-up system of src/dyson_step.h:28-65: build_M writes M
+// (kn x kn, depends on times T-k..T-1 only) and sets B to the identity;
+// otherwise the kernel writes Q (kn x n) and sets G^R(T,T) = -i.
+__global__ void __launch_bounds__(256) kern_ret_start(Pt p, WTab w, int build_M, z2 *M, z2 *Q) {
   const int n = p.n, k = w.k, kn = k * n, T = p.T, k1 = k + 1;
   const int tid = threadIdx.x;
+  if (build_M) {
+    for (long long e = tid; e < (long long)kn * kn; e += 256) {
+      int row = (int)(e / kn), col = (int)(e % kn);
+      int nn = row / n + 1, a = row % n, l = col / n + 1, b = col % n;
+      z2 v = {0.0, 0.0};
+      // -(-i/dt) pd(n,l) I = (i/dt) pd I
+      if (a == b) v.y += w.pdiff[nn * k1 + l] / p.dt;
+      if (l == nn) {
+        z2 hv = p.h(T - nn)[b * n + a];
+        v.x -= hv.x; v.y -= hv.y;
+        if (a == b) v.x += p.mu;
+      }
+      double wt = p.dt * greg(w, nn, l);
+      if (l <= nn) {
+        z2 s = p.sr(T - l, T - nn)[b * n + a];
+        v.x -= wt * s.x; v.y -= wt * s.y;
+      } else {
+        z2 s = p.sr(T - nn, T - l)[a * n + b];
+        v.x += wt * s.x; v.y -= wt * s.y;
+      }
+      M[e] = v;
+      Q[e] = (row == col) ? z2{1.0, 0.0} : z2{0.0, 0.0};
+    }
+    return;
+  }
   z2 *Gtt = p.gr(T, T);
   for (int e = tid; e < n * n; e += 256) Gtt[e] = (e / n == e % n) ? z2{0.0, -1.0} : z2{0.0, 0.0};
   __syncthreads();
-  // M
-  for (long long e = tid; e < (long long)kn * kn; e += 256) {
-    int row = (int)(e / kn), col = (int)(e % kn);
-    int nn = row / n + 1, a = row % n, l = col / n + 1, b = col % n;
-    z2 v = {0.0, 0.0};
-    // -(-i/dt) pd(n,l) I = (i/dt) pd I
-    if (a == b) v.y += w.pdiff[nn * k1 + l] / p.dt;
-    if (l == nn) {
-      z2 hv = p.h(T - nn)[b * n + a];
-      v.x -= hv.x; v.y -= hv.y;
-      if (a == b) v.x += p.mu;
-    }
-    double wt = p.dt * greg(w, nn, l);
-    if (l <= nn) {
-      z2 s = p.sr(T - l, T - nn)[b * n + a];
-      v.x -= wt * s.x; v.y -= wt * s.y;
-    } else {
-      z2 s = p.sr(T - nn, T - l)[a * n + b];
-      v.x += wt * s.x; v.y -= wt * s.y;
-    }
-    M[e] = v;
-  }
   // Q: (-i/dt) pd(n,0) G(T,T)^T + dt w(n,0) (G(T,T) Sigma^R(T,T-n))^T
   for (long long e = tid; e < (long long)kn * n; e += 256) {
     int row = (int)(e / n), b = (int)(e % n);
@@ -197,6 +201,16 @@ __global__ void __launch_bounds__(256) kern_ret_start(Pt p, WTab w, z2 *M, z2 *Q
     v.x += wt * s.x; v.y += wt * s.y;
     Q[e] = v;
   }
+}
+
+// C (R x Cc) = A (R x R) B (R x Cc), z2 row-major; one thread per output
+__global__ void kern_zmm(const z2 *A, const z2 *B, int R, int Cc, z2 *C) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)R * Cc) return;
+  int r = (int)(idx / Cc), c = (int)(idx % Cc);
+  z2 s = {0.0, 0.0};
+  for (int q = 0; q < R; q++) s = zadd(s, zmul(A[(size_t)r * R + q], B[(size_t)q * Cc + c]));
+  C[idx] = s;
 }
 
 // G^R(T,T-l-1) = X_l^T, l = 0..k-1  (one workgroup per l)
@@ -373,8 +387,11 @@ __global__ void kern_sum_parts(const z2 *part, int nchunk, long long nel, z2 *ou
   out[i] = s;
 }
 
-// start-up of src/dyson_step.h:178-251: initial condition, kn x kn system
-__global__ void __launch_bounds__(256) kern_les_start(Pt p, WTab w, const z2 *Q, z2 *X, z2 *M, z2 *Qs, double *err) {
+// start-up of src/dyson_step.h:178-251: initial condition, kn x kn system.
+// The matrix (times 0..k only) is built only when build_M is set, with the
+// identity in MI for its inversion.
+__global__ void __launch_bounds__(256) kern_les_start(Pt p, WTab w, int build_M, const z2 *Q, z2 *X, z2 *M, z2 *MI,
+                                                      z2 *Qs, double *err) {
   __shared__ double red[256];
   const int n = p.n, k = w.k, kn = k * n, T = p.T, k1 = k + 1, tid = threadIdx.x;
   // G^<(0,T) = -G^rceil(T,0)^H ; X_0 = same
@@ -390,26 +407,28 @@ __global__ void __launch_bounds__(256) kern_les_start(Pt p, WTab w, const z2 *Q,
   }
   __syncthreads();
   const z2 *X0 = X;
-  for (long long e = tid; e < (long long)kn * kn; e += 256) {
-    int row = (int)(e / kn), col = (int)(e % kn);
-    int m = row / n + 1, a = row % n, l = col / n + 1, b = col % n;
-    z2 v = {0.0, 0.0};
-    if (a == b) v.y += w.pdiff[m * k1 + l] / p.dt;   // (i/dt) pd(m,l)
-    if (m == l) {
-      z2 hv = p.h(l)[a * n + b];
-      v.x -= hv.x; v.y -= hv.y;
-      if (a == b) v.x += p.mu;
+  if (build_M)
+    for (long long e = tid; e < (long long)kn * kn; e += 256) {
+      int row = (int)(e / kn), col = (int)(e % kn);
+      int m = row / n + 1, a = row % n, l = col / n + 1, b = col % n;
+      z2 v = {0.0, 0.0};
+      if (a == b) v.y += w.pdiff[m * k1 + l] / p.dt;   // (i/dt) pd(m,l)
+      if (m == l) {
+        z2 hv = p.h(l)[a * n + b];
+        v.x -= hv.x; v.y -= hv.y;
+        if (a == b) v.x += p.mu;
+      }
+      double wt = p.dt * greg(w, m, l);
+      if (m >= l) {
+        z2 s = p.sr(m, l)[a * n + b];
+        v.x -= wt * s.x; v.y -= wt * s.y;
+      } else {
+        z2 s = p.sr(l, m)[b * n + a];
+        v.x += wt * s.x; v.y -= wt * s.y;
+      }
+      M[e] = v;
+      MI[e] = (row == col) ? z2{1.0, 0.0} : z2{0.0, 0.0};
     }
-    double wt = p.dt * greg(w, m, l);
-    if (m >= l) {
-      z2 s = p.sr(m, l)[a * n + b];
-      v.x -= wt * s.x; v.y -= wt * s.y;
-    } else {
-      z2 s = p.sr(l, m)[b * n + a];
-      v.x += wt * s.x; v.y -= wt * s.y;
-    }
-    M[e] = v;
-  }
   for (long long e = tid; e < (long long)kn * n; e += 256) {
     int row = (int)(e / n), b = (int)(e % n);
     int m = row / n + 1, a = row % n;
@@ -819,7 +838,7 @@ void gesv(Ctx &c, z2 *A, long long sA, z2 *B, long long sB, int N, int R, int ns
 void update_minv(Ctx &c, int T, double mu, double dt, const WTab &w) {
   const size_t es = c.es;
   if (c.minv_mu != mu || c.minv_dt != dt) {
-    std::fill(c.minv_valid.begin(), c.minv_valid.end(), 0);
+    invalidate_minv(c);
     c.minv_mu = mu; c.minv_dt = dt;
   }
   std::vector<int> js;
@@ -851,7 +870,17 @@ int chunks_for(int rows, int terms) {
 
 }  // namespace
 
-void invalidate_minv(Ctx &c) { std::fill(c.minv_valid.begin(), c.minv_valid.end(), 0); }
+void touch_time(Ctx &c, int t) {
+  if (t >= 0 && t < (int)c.minv_valid.size()) c.minv_valid[t] = 0;
+  if (t <= c.k) c.lesinv_valid = false;
+  if (t < c.retinv_T) c.retinv_T = -1;
+}
+
+void invalidate_minv(Ctx &c) {
+  std::fill(c.minv_valid.begin(), c.minv_valid.end(), 0);
+  c.retinv_T = -1;
+  c.lesinv_valid = false;
+}
 
 void set_matsubara(Ctx &c, double beta) {
   GLL g(c.ntau);
@@ -900,10 +929,19 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   const z2 *Minv = c.Minv.as<z2>();
 
   // ---- retarded (src/dyson_step.h:7-127)
-  hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, w, M, Qs);
+  c.dy_retinv.ensure((size_t)kn * kn * sizeof(z2));
+  c.dy_lesinv.ensure((size_t)kn * kn * sizeof(z2));
+  c.dy_Xs.ensure((size_t)kn * n * sizeof(z2));
+  z2 *Xs = c.dy_Xs.as<z2>();
+  if (c.retinv_T != T) {
+    hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, w, 1, M, c.dy_retinv.as<z2>());
+    gesv(c, M, 0, c.dy_retinv.as<z2>(), 0, kn, kn, 1);
+    c.retinv_T = T;
+  }
+  hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, w, 0, M, Qs);
+  hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_retinv.as<z2>(), Qs, kn, n, Xs);
   check_launch("ret start");
-  gesv(c, M, 0, Qs, 0, kn, n, 1);
-  hipLaunchKernelGGL(kern_ret_start_store, dim3(k), dim3(256), 0, s, p, Qs, err + 0);
+  hipLaunchKernelGGL(kern_ret_start_store, dim3(k), dim3(256), 0, s, p, Xs, err + 0);
   hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, w, Q, Minv, err + 0);
   for (int nn = k + 1; nn < T; nn++)
     hipLaunchKernelGGL(kern_ret_step, dim3(T - nn), dim3(256), 0, s, p, w, nn, Q, Minv, err + 0);
@@ -933,9 +971,15 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     hipLaunchKernelGGL(kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, w, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
     long long nel = (long long)(T + 1) * es;
     hipLaunchKernelGGL(kern_sum_parts, dim3(nblk(nel)), dim3(256), 0, s, c.dy_part.as<z2>(), nch, nel, Q);
-    hipLaunchKernelGGL(kern_les_start, dim3(1), dim3(256), 0, s, p, w, Q, X, M, Qs, err + 2);
-    gesv(c, M, 0, Qs, 0, kn, n, 1);
-    NED_HIP(hipMemcpyAsync(X + es, Qs, (size_t)kn * n * sizeof(z2), hipMemcpyDeviceToDevice, s));
+    const int build = c.lesinv_valid ? 0 : 1;
+    hipLaunchKernelGGL(kern_les_start, dim3(1), dim3(256), 0, s, p, w, build, Q, X, M, c.dy_lesinv.as<z2>(), Qs,
+                       err + 2);
+    if (build) {
+      gesv(c, M, 0, c.dy_lesinv.as<z2>(), 0, kn, kn, 1);
+      c.lesinv_valid = true;
+    }
+    hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_lesinv.as<z2>(), Qs, kn, n,
+                       X + es);
     hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, w, Q, X, Minv);
     for (int l = k + 1; l < T; l++)
       hipLaunchKernelGGL(kern_les_step, dim3(T - l), dim3(256), 0, s, p, w, l, Q, X, Minv);
@@ -977,7 +1021,7 @@ void hf_decomp(Ctx &c, int T, const double *h0_host) {
                      c.Bq.as<double>(), c.U64.as<double>(), c.dy_xq.as<z2>(), c.naux, c.hmf.as<z2>());
   check_launch("hf");
   NED_HIP(hipStreamSynchronize(c.stream));  // h0_host may be released by the caller
-  if (T < (int)c.minv_valid.size()) c.minv_valid[T] = 0;
+  touch_time(c, T);
 }
 
 double dyson_start(Ctx &c, double mu, double beta, double dt) {

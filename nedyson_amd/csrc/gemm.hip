@@ -329,3 +329,164 @@ void gemm(const GemmDesc &d, Prec prec, hipStream_t s) {
 }
 
 }  // namespace ned
+
+// ====================================================================
+// Exchange GEMM of the self-energy in split-bf16: C = alpha A B^T (+ C)
+// with A (M x K) and B (N x K) both K-contiguous and pre-split into hi/lo
+// bf16 planes.  128 x 128 output tile per 256-thread workgroup, each wave a
+// 64 x 64 quadrant = 4 x 4 blocks of v_mfma_f32_16x16x32_bf16 (3 products
+// per block: lo*hi + hi*lo + hi*hi, FP32 accumulation).  The next K slice is
+// fetched into registers (16-byte loads) while the current one is consumed.
+namespace ned {
+namespace {
+
+constexpr int NT_TM = 128, NT_TN = 128, NT_TK = 64, NT_LDK = NT_TK + 8;
+
+__global__ void __launch_bounds__(256) kern_nt_bf16x3(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo,
+                                                      long long lda, long long strA,
+                                                      const uint16_t *__restrict__ Bhi, const uint16_t *__restrict__ Blo,
+                                                      long long ldb, long long strB, float *__restrict__ C,
+                                                      long long ldc, long long strC, int M, int N, int K,
+                                                      float alpha, int beta) {
+  Ahi += (size_t)blockIdx.y * strA;
+  Alo += (size_t)blockIdx.y * strA;
+  Bhi += (size_t)blockIdx.y * strB;
+  Blo += (size_t)blockIdx.y * strB;
+  C += (size_t)blockIdx.y * strC;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][NT_TM][NT_LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][NT_TN][NT_LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 1, wn = wave & 1;
+  const int tilesN = N / NT_TN;
+  // group 8 row-tiles together so that consecutive workgroups share B tiles in L2
+  const int tilesM = M / NT_TM;
+  const int group = 8;
+  const int bid = blockIdx.x;
+  const int gsize = group * tilesN;
+  const int g = bid / gsize, r = bid % gsize;
+  const int gm = min(group, tilesM - g * group);
+  const int tm = g * group + r % gm, tn = r / gm;
+  const int m0 = tm * NT_TM, n0 = tn * NT_TN;
+
+  v4f acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = v4f{0, 0, 0, 0};
+
+  uint4 ra[2][4], rb[2][4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int idx = tid + 256 * i, row = idx >> 3, seg = idx & 7;
+      const size_t oa = (size_t)(m0 + row) * lda + k0 + seg * 8;
+      const size_t ob = (size_t)(n0 + row) * ldb + k0 + seg * 8;
+      ra[0][i] = *reinterpret_cast<const uint4 *>(Ahi + oa);
+      ra[1][i] = *reinterpret_cast<const uint4 *>(Alo + oa);
+      rb[0][i] = *reinterpret_cast<const uint4 *>(Bhi + ob);
+      rb[1][i] = *reinterpret_cast<const uint4 *>(Blo + ob);
+    }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int idx = tid + 256 * i, row = idx >> 3, seg = idx & 7;
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+        *reinterpret_cast<uint4 *>(&sA[p][row][seg * 8]) = ra[p][i];
+        *reinterpret_cast<uint4 *>(&sB[p][row][seg * 8]) = rb[p][i];
+      }
+    }
+  };
+  gload(0);
+  for (int k0 = 0; k0 < K; k0 += NT_TK) {
+    __syncthreads();
+    lstore();
+    __syncthreads();
+    if (k0 + NT_TK < K) gload(k0 + NT_TK);
+#pragma unroll
+    for (int kk = 0; kk < NT_TK; kk += 32) {
+      const int kq = kk + 8 * (lane >> 4);
+      uint4 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int row = wm * 64 + i * 16 + (lane & 15);
+        ah[i] = *reinterpret_cast<const uint4 *>(&sA[0][row][kq]);
+        al[i] = *reinterpret_cast<const uint4 *>(&sA[1][row][kq]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int col = wn * 64 + j * 16 + (lane & 15);
+        bh[j] = *reinterpret_cast<const uint4 *>(&sB[0][col][kq]);
+        bl[j] = *reinterpret_cast<const uint4 *>(&sB[1][col][kq]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          v8bf a_h = __builtin_bit_cast(v8bf, ah[i]), a_l = __builtin_bit_cast(v8bf, al[i]);
+          v8bf b_h = __builtin_bit_cast(v8bf, bh[j]), b_l = __builtin_bit_cast(v8bf, bl[j]);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, b_h, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_l, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_h, acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + q;
+        const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+        float *cp = C + (size_t)row * ldc + col;
+        float v = alpha * acc[i][j][q];
+        if (beta) v += *cp;
+        *cp = v;
+      }
+}
+
+__global__ void kern_split_bf16(const float *__restrict__ x, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
+                                long long n) {
+  long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i + 3 < n) {
+    float4 v = *reinterpret_cast<const float4 *>(x + i);
+    float f[4] = {v.x, v.y, v.z, v.w};
+    uint16_t h[4], l[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      h[q] = f2bf(f[q]);
+      l[q] = f2bf(f[q] - bf2f(h[q]));
+    }
+    *reinterpret_cast<uint2 *>(hi + i) = uint2{(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
+    *reinterpret_cast<uint2 *>(lo + i) = uint2{(uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16)};
+  } else {
+    for (; i < n; i++) {
+      uint16_t h = f2bf(x[i]);
+      hi[i] = h;
+      lo[i] = f2bf(x[i] - bf2f(h));
+    }
+  }
+}
+
+}  // namespace
+
+bool nt_bf16x3_ok(int M, int N, int K) { return M % NT_TM == 0 && N % NT_TN == 0 && K % NT_TK == 0; }
+
+void gemm_nt_bf16x3(const NtOp &A, const NtOp &B, float *C, long long ldc, long long strC, int M, int N, int K,
+                    int batch, float alpha, int beta, hipStream_t s) {
+  if (!nt_bf16x3_ok(M, N, K) || A.ld % 8 || B.ld % 8 || A.stride % 8 || B.stride % 8)
+    throw std::invalid_argument("gemm_nt_bf16x3: M, N must be multiples of 128, K of 64, strides of 8");
+  const int tiles = (M / NT_TM) * (N / NT_TN);
+  hipLaunchKernelGGL(kern_nt_bf16x3, dim3(tiles, batch), dim3(256), 0, s, A.hi, A.lo, A.ld, A.stride, B.hi, B.lo, B.ld,
+                     B.stride, C, ldc, strC, M, N, K, alpha, beta);
+  check_launch("gemm_nt_bf16x3");
+}
+
+void split_bf16(const float *x, uint16_t *hi, uint16_t *lo, long long n, hipStream_t s) {
+  long long th = (n + 3) / 4;
+  hipLaunchKernelGGL(kern_split_bf16, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, x, hi, lo, n);
+  check_launch("split_bf16");
+}
+
+}  // namespace ned

@@ -47,6 +47,19 @@ __global__ void kern_build_U(const double *Bq, int n, int a, double *U64, float 
   U32[idx] = (float)s;
 }
 
+__global__ void kern_split_Ut(const float *U, int n2, uint16_t *hi, uint16_t *lo) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n2 * n2) return;
+  long long r = idx / n2, c = idx % n2;
+  float v = U[c * n2 + r];
+  uint32_t u = __float_as_uint(v);
+  uint16_t h = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  float rem = v - __uint_as_float(((uint32_t)h) << 16);
+  uint32_t w = __float_as_uint(rem);
+  hi[idx] = h;
+  lo[idx] = (uint16_t)((w + 0x7fffu + ((w >> 16) & 1u)) >> 16);
+}
+
 __global__ void kern_d2f(const double *x, float *y, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = (float)x[i];
@@ -159,17 +172,28 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     gemm(d, prec, s);
   }
   // H = -C U'   (rows (p, plane, a) of X, all at once; the n^4 naux GEMM)
-  c.mark(0);
   {
-    GemmDesc d;
-    d.M = 2 * a * P; d.N = (int)n2; d.K = (int)n2; d.batch = 1;
-    d.A.ptr = bf.X; d.A.ms = n2; d.A.ks_lo = 1;
-    d.B.ptr = U; d.B.ms = 1; d.B.ks_lo = n2;
-    d.C.ptr = bf.H; d.C.rs = n2; d.C.cs = 1;
-    d.alpha = -1.0;
-    gemm(d, prec, s);
+    const int M = 2 * a * P, NN = (int)n2;
+    if (prec == PREC_BF16X3 && nt_bf16x3_ok(M, NN, NN)) {
+      uint16_t *xh = c.sg_Xh.as<uint16_t>(), *xl = c.sg_Xl.as<uint16_t>();
+      split_bf16((const float *)bf.X, xh, xl, (long long)M * NN, s);
+      c.mark(0);
+      gemm_nt_bf16x3(NtOp{xh, xl, NN, 0}, NtOp{c.Uth.as<uint16_t>(), c.Utl.as<uint16_t>(), NN, 0}, (float *)bf.H, NN,
+                     0, M, NN, NN, 1, -1.0f, 0, s);
+      c.mark(1);
+    } else {
+      GemmDesc d;
+      d.M = M; d.N = NN; d.K = NN; d.batch = 1;
+      d.A.ptr = bf.X; d.A.ms = n2; d.A.ks_lo = 1;
+      d.B.ptr = U; d.B.ms = 1; d.B.ks_lo = n2;
+      d.C.ptr = bf.H; d.C.rs = n2; d.C.cs = 1;
+      d.alpha = -1.0;
+      c.mark(0);
+      gemm(d, prec, s);
+      c.mark(1);
+    }
+    if (c.prof) c.prof_flops += 2.0 * M * (double)NN * NN;
   }
-  c.mark(1);
   // H += 2 Pi Vmat   (Vmat[b][(k,j)] = B[b][k][j])
   {
     GemmDesc d;
@@ -196,7 +220,7 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   const int n = c.nao, a = c.naux, np = c.np;
   const long long n2 = (long long)n * n, an2 = (long long)a * n2;
   const int Ptot = 2 * (tstp + 1) + np;
-  const size_t per_item = (size_t)(3 * 2 * an2 + 2LL * a * a + 4 * n2) * sizeof(T);
+  const size_t per_item = (size_t)(3 * 2 * an2 + 2LL * a * a + 4 * n2) * sizeof(T) + (prec == PREC_BF16X3 ? 2 * 2 * an2 * 2 : 0);
   int P = c.sigma_chunk > 0 ? c.sigma_chunk : (int)std::max<size_t>(1, ((size_t)3 << 30) / std::max<size_t>(per_item, 1));
   P = std::min(P, Ptot);
   c.sg_items_A.ensure((size_t)P * 2 * n2 * sizeof(T));
@@ -206,6 +230,10 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   c.sg_H.ensure((size_t)P * 2 * an2 * sizeof(T));
   c.sg_Pi.ensure((size_t)P * 2 * a * a * sizeof(T));
   c.sg_E.ensure((size_t)Ptot * 2 * n2 * sizeof(T));
+  if (prec == PREC_BF16X3) {
+    c.sg_Xh.ensure((size_t)P * 2 * an2 * sizeof(uint16_t));
+    c.sg_Xl.ensure((size_t)P * 2 * an2 * sizeof(uint16_t));
+  }
   Bufs<T> bf{c.sg_items_A.as<T>(), c.sg_items_B.as<T>(), c.sg_W.as<T>(), c.sg_X.as<T>(), c.sg_H.as<T>(),
              c.sg_Pi.as<T>(), c.sg_E.as<T>()};
   const T *Vq = sizeof(T) == 8 ? (const T *)c.Bq.p : (const T *)c.Bq32.p;
@@ -240,6 +268,8 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
   c.Bq32.ensure(a * n2 * sizeof(float));
   c.U64.ensure(n2 * n2 * sizeof(double));
   c.U32.ensure(n2 * n2 * sizeof(float));
+  c.Uth.ensure(n2 * n2 * sizeof(uint16_t));
+  c.Utl.ensure(n2 * n2 * sizeof(uint16_t));
   NED_HIP(hipMemcpyAsync(c.Bq.p, Bq_host, a * n2 * sizeof(double), hipMemcpyHostToDevice, c.stream));
   long long nb = (long long)a * n2;
   hipLaunchKernelGGL(kern_d2f, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c.stream, c.Bq.as<double>(),
@@ -249,6 +279,9 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
   hipLaunchKernelGGL(kern_build_U, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, c.stream, c.Bq.as<double>(), n, a,
                      c.U64.as<double>(), c.U32.as<float>());
   check_launch("coulomb U");
+  hipLaunchKernelGGL(kern_split_Ut, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, c.stream, c.U32.as<float>(),
+                     (int)n2, c.Uth.as<uint16_t>(), c.Utl.as<uint16_t>());
+  check_launch("coulomb U split");
   NED_HIP(hipStreamSynchronize(c.stream));
   c.have_coulomb = true;
 }
