@@ -16,9 +16,9 @@ ratio, and the max-abs difference of every G block written by the steps.
 `roofline` is for the dominant kernel, the exchange GEMM of Sigma
 (kern_gemm_*<1,1>: rows (item, plane, aux) x U'), timed with HIP events on
 the library's stream.  `cpu_baseline` times the CPU oracle (the restated
-reference algorithm, one thread) on one lesser, one retarded and one mixing
-time point of Sigma at the same sizes and scales to a whole timestep with the
-reference's flop model.
+reference algorithm) on one lesser, one retarded and one mixing time point of
+Sigma at the same sizes on one thread, and on a small slice with OpenMP over
+time points on all host threads, and scales to a whole timestep.
 """
 import argparse
 import json
@@ -33,6 +33,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "KBE timesteps/s at nao=64 GF2 vs plain FP64 at ≤1e-6 G error; % MFMA peak"
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0 / 3}
+# Coulomb-factor spread used for the nao=64 workload.  With B[Q,i,j] ~ N(0, 0.1)
+# the GF2 propagation started from the Hartree-Fock state (the equilibrium
+# GF2 Matsubara solver is not part of this build) leaves the stable range
+# within a few steps, and on the 1000-step synthetic history 0.02 still grows
+# the corrector update to O(100); at 0.01 the window stays smooth (corrector
+# update ~1e-3, particle number constant to 1e-7).  The work per step does
+# not depend on the value.
+STABLE_B_SD = {2: 0.01}
 
 
 def sigma_flops_reassoc(n, a, items):
@@ -54,32 +62,54 @@ def dyson_flops(n, T, ntau1):
 
 
 def cpu_baseline(p, T, corr, log):
-    """Oracle timing on one les, ret and tv point of Sigma (one thread)."""
+    """CPU oracle timed on this host: one lesser, one retarded and one mixing
+    Sigma time point single-threaded, then one small slice with OpenMP over its
+    time points on every host thread; both scaled to a timestep at T."""
     from oracle import oracle as O
     from nedyson_amd.green import GreenFunc
     n, a = p.nao, p.naux
     rng = np.random.default_rng(1)
-    G = GreenFunc(2, 3, n)
-    for arr in (G.ret, G.les, G.tv):
-        arr[:] = 0.1 * (rng.uniform(-1, 1, arr.shape) + 1j * rng.uniform(-1, 1, arr.shape))
+
+    def rand_green(nt, ntau):
+        g = GreenFunc(nt, ntau, n)
+        for arr in (g.ret, g.les, g.tv):
+            arr[:] = 0.1 * (rng.uniform(-1, 1, arr.shape) + 1j * rng.uniform(-1, 1, arr.shape))
+        return g
+
+    G = rand_green(2, 3)
     S = GreenFunc(2, 3, n)
     times = {}
     for comp, name in ((0, "les"), (1, "ret"), (2, "tv")):
         t0 = time.perf_counter()
         O.sigma_point(p.V, comp, 2, 1, S, G)
         times[name] = time.perf_counter() - t0
-        log(f"[cpu] Sigma {name} point: {times[name]:.2f} s")
+        log(f"[cpu] Sigma {name} point, 1 thread: {times[name]:.2f} s")
     les_flops = 12 * n ** 4 * a + 36 * n ** 3 * a + 12 * n ** 2 * a ** 2
     rate = les_flops / times["les"]
     t_sigma = (T + 1) * (times["les"] + times["ret"]) + (p.ntau + 1) * times["tv"]
     t_dyson = dyson_flops(n, T, p.ntau + 1) / rate
-    t_step = corr * (t_sigma + t_dyson)
-    return {"value": 1.0 / t_step, "unit": "timesteps/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle (reference decomposition restated, 1 thread): one lesser, one retarded and one "
-                      f"mixing Sigma time point at nao={n}, naux={a} ({sum(times.values()):.1f} s), scaled to a "
-                      f"timestep at T={T} with {corr} corrector passes as (T+1)(t_les+t_ret)+Ntau*t_tv plus the "
-                      f"Dyson flop model at the measured {rate / 1e9:.1f} GFLOP/s",
-            "sec_per_step": t_step, "gflops_measured": rate / 1e9}
+    t_step1 = corr * (t_sigma + t_dyson)
+
+    # OpenMP over the time points of a slice with about one point per thread
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    Ts, ntau_s = max(1, threads // 4 - 1), max(1, threads // 2 - 1)
+    Gs, Ss = rand_green(Ts, ntau_s), GreenFunc(Ts, ntau_s, n)
+    t0 = time.perf_counter()
+    O.sigma_solve(p.V, Ts, Ss, Gs, nthreads=threads)
+    t_omp = time.perf_counter() - t0
+    serial = (Ts + 1) * (times["les"] + times["ret"]) + (ntau_s + 1) * times["tv"]
+    speedup = serial / t_omp
+    log(f"[cpu] Sigma slice T={Ts}, ntau+1={ntau_s + 1} ({2 * Ts + 2 + ntau_s + 1} points), {threads} threads: "
+        f"{t_omp:.2f} s (x{speedup:.2f} over 1 thread)")
+    t_stepN = t_step1 / speedup
+    return {"value": 1.0 / t_stepN, "unit": "timesteps/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (reference decomposition restated, C++): one lesser, one retarded and one mixing "
+                      f"Sigma time point at nao={n}, naux={a} on 1 thread ({sum(times.values()):.1f} s), then a "
+                      f"slice of {2 * Ts + 2 + ntau_s + 1} time points with OpenMP over points on {threads} threads "
+                      f"({t_omp:.1f} s, x{speedup:.2f}); scaled to a timestep at T={T} with {corr} corrector passes "
+                      f"as (T+1)(t_les+t_ret)+Ntau*t_tv plus the Dyson flop model at the measured "
+                      f"{rate / 1e9:.1f} GFLOP/s per thread",
+            "value_1thread": 1.0 / t_step1, "sec_per_step": t_stepN, "gflops_1thread": rate / 1e9}
 
 
 def main():
@@ -94,9 +124,9 @@ def main():
     ap.add_argument("--f64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
-    ap.add_argument("--b-sd", type=float, default=None, help="override sigma_B of the Coulomb factor")
+    ap.add_argument("--b-sd", type=float, default=None,
+                    help="sigma_B of the Coulomb factor (default: %s for the nao=64 workload)" % STABLE_B_SD)
     ap.add_argument("--dt", type=float, default=None)
-    ap.add_argument("--energy", action="store_true", help="log particle number and energy per step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -113,6 +143,8 @@ def main():
     over = {}
     if args.b_sd is not None:
         over["b_sd"] = args.b_sd
+    elif args.config in STABLE_B_SD:
+        over["b_sd"] = STABLE_B_SD[args.config]
     if args.dt is not None:
         over["dt"] = args.dt
     p = problem.config(args.config, **over)
@@ -134,7 +166,7 @@ def main():
     log(f"[setup] history t<=T0={T0} ready ({time.perf_counter() - t_setup:.1f} s)")
 
     def run(prec, steps, profile):
-        t_list, iters = [], []
+        t_list, iters, obs = [], [], []
         for i, T in enumerate(range(T0 + 1, nt + 1)):
             timed = i >= W
             if timed and profile:
@@ -144,14 +176,15 @@ def main():
             it, err = c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, prec, args.corr, 0.0)
             c.sync()
             dt_ = time.perf_counter() - t0
-            e = c.energy(T, p.h0, p.beta, p.dt) if args.energy else (np.nan,) * 3
+            c.profile(False)
+            e = c.energy(T, p.h0, p.beta, p.dt)  # outside the timed region
             log(f"[{prec}] step T={T} iters={it} err={err:.3e} {dt_ * 1e3:.1f} ms  N={e[2]:.10f} E={e[0] + e[1]:.10f}")
             if timed:
                 t_list.append(dt_)
                 iters.append(it)
+                obs.append((e[2], e[0] + e[1]))
         prof = c.profile_read() if profile else None
-        c.profile(False)
-        return t_list, iters, prof
+        return t_list, iters, prof, np.array(obs)
 
     def slices():
         out = []
@@ -162,17 +195,19 @@ def main():
             out.append(c.read_blocks(ned.NED_G, ned.NED_TV, T * (p.ntau + 1), p.ntau + 1))
         return out
 
-    t_mixed, iters, prof = run(args.prec, K, True)
+    t_mixed, iters, prof, obs_mixed = run(args.prec, K, True)
     g_mixed = slices()
     elapsed = sum(t_mixed)
     value = K / elapsed
     items = 2 * (T0 + W + 1) + p.ntau + 1  # items of a slice near the timed steps
-    f64_value, max_err = None, None
+    f64_value, max_err, dN, dE, iters_f64 = None, None, None, None, None
     if not args.no_f64:
-        t_f64, _, _ = run("f64", K, False)
+        t_f64, iters_f64, _, obs_f64 = run("f64", K, False)
         g_f64 = slices()
         f64_value = K / sum(t_f64)
         max_err = float(max(np.max(np.abs(x - y)) for x, y in zip(g_mixed, g_f64)))
+        dN = float(np.max(np.abs(obs_mixed[:, 0] - obs_f64[:, 0])))
+        dE = float(np.max(np.abs(obs_mixed[:, 1] - obs_f64[:, 1])))
     c.close()
 
     ugemm_ms = prof["ugemm_ms"] / max(prof["ugemm_n"], 1)
@@ -202,7 +237,10 @@ def main():
         "f64_value": f64_value,
         "speedup_vs_f64": (value / f64_value) if f64_value else None,
         "max_abs_G_err_vs_f64": max_err,
+        "max_abs_N_diff_vs_f64": dN,
+        "max_abs_E_diff_vs_f64": dE,
         "corrector_iters_per_step": iters,
+        "corrector_iters_per_step_f64": iters_f64,
         "sigma_ms_per_eval": prof["sigma_ms"] / n_sigma,
         "dyson_ms_per_step": prof["dyson_ms"] / max(prof["dyson_n"], 1),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",

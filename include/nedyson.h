@@ -105,9 +105,10 @@ ned_status ned_extrapolate(ned_ctx *ctx, int tstp);
 /* dyson::dyson_start(G, Sig, hmf, mu, beta, dt) (src/dyson_boot.h:195-221). */
 ned_status ned_dyson_start(ned_ctx *ctx, double mu, double beta, double dt, double *err_out);
 
-/* DecompSimulation::do_boot (src/molNEsim_decomp_impl.h:63-82): iterate
- * {hmf[0..k] = fock; Sigma(0..k); dyson_start} until err < tol.  *iters_out
- * receives the passes used; NED_ERR_NOCONV if maxiter passes do not converge
+/* DecompSimulation::do_boot (src/molNEsim_decomp_impl.h:63-82): passes
+ * iter = 0..maxiter (maxiter is the reference's BootMax) of
+ * {hmf[0..k] = fock; Sigma(0..k); dyson_start}, stopping once err < tol.
+ * *iters_out receives the passes used; NED_ERR_NOCONV if none reached tol
  * (the reference exits with status 0 there, src/molNEsim.cpp:129-131). */
 ned_status ned_do_boot(ned_ctx *ctx, const double *fock, double mu, double beta, double dt, int prec, int maxiter,
                        double tol, int *iters_out, double *err_out);

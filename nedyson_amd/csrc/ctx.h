@@ -66,6 +66,8 @@ struct Ctx {
   // lesser one on times 0..k only (valid until the early history changes)
   DevBuf dy_retinv, dy_lesinv, dy_Xs;
   int retinv_T = -1;
+  DevBuf wtab_d;  // integrator tables of order wtab_k
+  int wtab_k = -1;
   bool lesinv_valid = false;
   // event timing of the hot phases (enabled by ned_profile)
   bool prof = false;

@@ -33,6 +33,20 @@ WTab make_wtab(int k) {
   return t;
 }
 
+// the integrator tables in device memory (kernels index them at run time, so
+// they are read through a pointer rather than copied per thread from the
+// kernel arguments)
+const WTab *dev_wtab(Ctx &c) {
+  if (c.wtab_k != c.k) {
+    const WTab w = make_wtab(c.k);
+    c.wtab_d.ensure(sizeof(WTab));
+    NED_HIP(hipMemcpyAsync(c.wtab_d.p, &w, sizeof(WTab), hipMemcpyHostToDevice, c.stream));
+    NED_HIP(hipStreamSynchronize(c.stream));
+    c.wtab_k = c.k;
+  }
+  return (const WTab *)c.wtab_d.p;
+}
+
 namespace {
 
 __device__ __forceinline__ size_t dtri(int t) { return (size_t)t * (t + 1) / 2; }
@@ -156,7 +170,8 @@ __global__ void kern_scatter_blocks(const z2 *src, const int *js, int cnt, size_
 // k x k block start-up system of src/dyson_step.h:28-65: build_M writes M
 // (kn x kn, depends on times T-k..T-1 only) and sets B to the identity;
 // otherwise the kernel writes Q (kn x n) and sets G^R(T,T) = -i.
-__global__ void __launch_bounds__(256) kern_ret_start(Pt p, WTab w, int build_M, z2 *M, z2 *Q) {
+__global__ void __launch_bounds__(256) kern_ret_start(Pt p, const WTab *__restrict__ wtab, int build_M, z2 *M, z2 *Q) {
+  const WTab &w = *wtab;
   const int n = p.n, k = w.k, kn = k * n, T = p.T, k1 = k + 1;
   const int tid = threadIdx.x;
   if (build_M) {
@@ -252,7 +267,8 @@ __device__ void ret_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s,
 }
 
 // history sums from the start-up blocks l = 0..k, for m = k+1..T
-__global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, WTab w, z2 *Q, const z2 *Minv, double *err) {
+__global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, const WTab *__restrict__ wtab, z2 *Q, const z2 *Minv, double *err) {
+  const WTab &w = *wtab;
   __shared__ BlkLds s;
   __shared__ double red[256];
   const int m = w.k + 1 + blockIdx.x, T = p.T, n = p.n;
@@ -266,7 +282,8 @@ __global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, WTab w, z2 *Q, const
 }
 
 // add block nn's contribution to every later history sum; finish nn+1
-__global__ void __launch_bounds__(256) kern_ret_step(Pt p, WTab w, int nn, z2 *Q, const z2 *Minv, double *err) {
+__global__ void __launch_bounds__(256) kern_ret_step(Pt p, const WTab *__restrict__ wtab, int nn, z2 *Q, const z2 *Minv, double *err) {
+  const WTab &w = *wtab;
   __shared__ BlkLds s;
   __shared__ double red[256];
   const int m = nn + 1 + blockIdx.x, T = p.T, n = p.n;
@@ -283,7 +300,8 @@ __global__ void __launch_bounds__(256) kern_ret_step(Pt p, WTab w, int nn, z2 *Q
 // src/dyson_step.h:130-169 with Ctv_tstp/CTV1/Conv.mixing
 // (src/dyson_integrals.h:80-116, 254-324): partial sums over chunks of the
 // time history, one workgroup per (node m, chunk s).
-__global__ void __launch_bounds__(256) kern_tv_rhs(Pt p, WTab w, const z2 *mixK, int nchunk, z2 *part) {
+__global__ void __launch_bounds__(256) kern_tv_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *mixK, int nchunk, z2 *part) {
+  const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
   const int nt_terms = T;  // j = 0..T-1 (G^rceil(T,.) is the unknown)
@@ -301,7 +319,8 @@ __global__ void __launch_bounds__(256) kern_tv_rhs(Pt p, WTab w, const z2 *mixK,
   blk_store(acc, n, part + ((size_t)ch * np + m) * p.es());
 }
 
-__global__ void kern_tv_matrix(Pt p, WTab w, z2 *A, z2 *B) {
+__global__ void kern_tv_matrix(Pt p, const WTab *__restrict__ wtab, z2 *A, z2 *B) {
+  const WTab &w = *wtab;
   const int n = p.n;
   const size_t es = p.es();
   for (int e = threadIdx.x; e < (int)es; e += blockDim.x) {
@@ -357,7 +376,8 @@ __global__ void kern_les_Y(Pt p, const double *mass, z2 *Y) {
 
 // Q_row = Cles3 + Cles2 (src/dyson_integrals.h:336-380, 393-427), partial
 // sums over chunks of the (tau, j) terms.
-__global__ void __launch_bounds__(256) kern_les_rhs(Pt p, WTab w, const z2 *Y, int nchunk, z2 *part) {
+__global__ void __launch_bounds__(256) kern_les_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *Y, int nchunk, z2 *part) {
+  const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int row = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
   const int nterms = np + T + 1;
@@ -390,8 +410,9 @@ __global__ void kern_sum_parts(const z2 *part, int nchunk, long long nel, z2 *ou
 // start-up of src/dyson_step.h:178-251: initial condition, kn x kn system.
 // The matrix (times 0..k only) is built only when build_M is set, with the
 // identity in MI for its inversion.
-__global__ void __launch_bounds__(256) kern_les_start(Pt p, WTab w, int build_M, const z2 *Q, z2 *X, z2 *M, z2 *MI,
+__global__ void __launch_bounds__(256) kern_les_start(Pt p, const WTab *__restrict__ wtab, int build_M, const z2 *Q, z2 *X, z2 *M, z2 *MI,
                                                       z2 *Qs, double *err) {
+  const WTab &w = *wtab;
   __shared__ double red[256];
   const int n = p.n, k = w.k, kn = k * n, T = p.T, k1 = k + 1, tid = threadIdx.x;
   // G^<(0,T) = -G^rceil(T,0)^H ; X_0 = same
@@ -458,7 +479,8 @@ __device__ void les_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s,
   blk_store(x, n, X + (size_t)m * p.es());
 }
 
-__global__ void __launch_bounds__(256) kern_les_qinit(Pt p, WTab w, z2 *Q, z2 *X, const z2 *Minv) {
+__global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restrict__ wtab, z2 *Q, z2 *X, const z2 *Minv) {
+  const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = w.k + 1 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
@@ -471,7 +493,8 @@ __global__ void __launch_bounds__(256) kern_les_qinit(Pt p, WTab w, z2 *Q, z2 *X
   else blk_store(acc, n, Qm);
 }
 
-__global__ void __launch_bounds__(256) kern_les_step(Pt p, WTab w, int l, z2 *Q, z2 *X, const z2 *Minv) {
+__global__ void __launch_bounds__(256) kern_les_step(Pt p, const WTab *__restrict__ wtab, int l, z2 *Q, z2 *X, const z2 *Minv) {
+  const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = l + 1 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
@@ -484,7 +507,8 @@ __global__ void __launch_bounds__(256) kern_les_step(Pt p, WTab w, int l, z2 *Q,
 }
 
 // diagonal step, src/dyson_step.h:283-310
-__global__ void __launch_bounds__(256) kern_les_diag(Pt p, WTab w, z2 *Q, z2 *X, z2 *scr) {
+__global__ void __launch_bounds__(256) kern_les_diag(Pt p, const WTab *__restrict__ wtab, z2 *Q, z2 *X, z2 *scr) {
+  const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int T = p.T, n = p.n;
   const size_t es = p.es();
@@ -526,7 +550,8 @@ __global__ void __launch_bounds__(256) kern_les_store(Pt p, const z2 *X, double 
 
 // ------------------------------------------------------------ predictor
 // dyson::Extrapolate (src/dyson.cpp:47-117); one thread per output element
-__global__ void kern_extrap(Pt p, WTab w, int which) {
+__global__ void kern_extrap(Pt p, const WTab *__restrict__ wtab, int which) {
+  const WTab &w = *wtab;
   const int n = p.n, k = w.k, T = p.T, np = p.np;
   const size_t es = p.es();
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -609,7 +634,8 @@ __global__ void kern_hf_h(Pt p, const double *h0, const double *Bq, const double
 
 // ------------------------------------------------------------ energy
 // partial traces of energy_conv's history terms (src/dyson_integrals.h:37-58)
-__global__ void kern_energy(Pt p, WTab w, int top, const double *mass, double *out) {
+__global__ void kern_energy(Pt p, const WTab *__restrict__ wtab, int top, const double *mass, double *out) {
+  const WTab &w = *wtab;
   __shared__ double rx[256], ry[256];
   const int n = p.n, T = p.T, np = p.np;
   const size_t es = p.es();
@@ -671,7 +697,8 @@ __global__ void kern_boot_tv0(Pt p, double *err) {
   if (threadIdx.x == 0) atomicAdd(err, sqrt(d2));
 }
 
-__global__ void kern_boot_matrix(Pt p, WTab w, z2 *M, z2 *B) {
+__global__ void kern_boot_matrix(Pt p, const WTab *__restrict__ wtab, z2 *M, z2 *B) {
+  const WTab &w = *wtab;
   const int n = p.n, k = w.k, kn = k * n, k1 = k + 1;
   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)kn * kn) return;
@@ -697,7 +724,8 @@ __global__ void kern_boot_matrix(Pt p, WTab w, z2 *M, z2 *B) {
 }
 
 // right-hand side of node m, step nn: tv_it_conv + derivative + integral terms
-__global__ void __launch_bounds__(256) kern_boot_rhs(Pt p, WTab w, const z2 *mixK, z2 *R) {
+__global__ void __launch_bounds__(256) kern_boot_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *mixK, z2 *R) {
+  const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = blockIdx.x, nn = blockIdx.y + 1, n = p.n, np = p.np, k1 = w.k + 1;
   BlkAcc acc;
@@ -912,6 +940,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   require(c.have_mix && c.mix_beta == beta, "dyson_step: call ned_set_matsubara(beta) first");
   require(T > c.k && T <= c.nt, "dyson_step: tstp must be in (k, nt]");
   const WTab w = make_wtab(c.k);
+  const WTab *wd = dev_wtab(c);
   const int n = c.nao, k = c.k, kn = k * n, np = c.np;
   const size_t es = c.es;
   Pt p = make_pt(c, T, mu, beta, dt);
@@ -934,17 +963,17 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   c.dy_Xs.ensure((size_t)kn * n * sizeof(z2));
   z2 *Xs = c.dy_Xs.as<z2>();
   if (c.retinv_T != T) {
-    hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, w, 1, M, c.dy_retinv.as<z2>());
+    hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, wd, 1, M, c.dy_retinv.as<z2>());
     gesv(c, M, 0, c.dy_retinv.as<z2>(), 0, kn, kn, 1);
     c.retinv_T = T;
   }
-  hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, w, 0, M, Qs);
+  hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, wd, 0, M, Qs);
   hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_retinv.as<z2>(), Qs, kn, n, Xs);
   check_launch("ret start");
   hipLaunchKernelGGL(kern_ret_start_store, dim3(k), dim3(256), 0, s, p, Xs, err + 0);
-  hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, w, Q, Minv, err + 0);
+  hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, Minv, err + 0);
   for (int nn = k + 1; nn < T; nn++)
-    hipLaunchKernelGGL(kern_ret_step, dim3(T - nn), dim3(256), 0, s, p, w, nn, Q, Minv, err + 0);
+    hipLaunchKernelGGL(kern_ret_step, dim3(T - nn), dim3(256), 0, s, p, wd, nn, Q, Minv, err + 0);
   check_launch("ret history");
 
   // ---- right mixing (src/dyson_step.h:130-169)
@@ -954,9 +983,9 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     c.dy_gA.ensure(es * sizeof(z2));
     c.dy_gB.ensure(es * sizeof(z2));
     c.dy_scr.ensure((size_t)std::max(np, T + 1) * es * sizeof(z2));
-    hipLaunchKernelGGL(kern_tv_matrix, dim3(1), dim3(256), 0, s, p, w, c.dy_gA.as<z2>(), c.dy_gB.as<z2>());
+    hipLaunchKernelGGL(kern_tv_matrix, dim3(1), dim3(256), 0, s, p, wd, c.dy_gA.as<z2>(), c.dy_gB.as<z2>());
     gesv(c, c.dy_gA.as<z2>(), 0, c.dy_gB.as<z2>(), 0, n, n, 1);
-    hipLaunchKernelGGL(kern_tv_rhs, dim3(np, nch), dim3(256), 0, s, p, w, c.mixK.as<z2>(), nch, c.dy_part.as<z2>());
+    hipLaunchKernelGGL(kern_tv_rhs, dim3(np, nch), dim3(256), 0, s, p, wd, c.mixK.as<z2>(), nch, c.dy_part.as<z2>());
     hipLaunchKernelGGL(kern_tv_fin, dim3(np), dim3(256), 0, s, p, c.dy_part.as<z2>(), nch, c.dy_gB.as<z2>(),
                        c.dy_scr.as<z2>(), err + 1);
     check_launch("tv");
@@ -968,11 +997,11 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     hipLaunchKernelGGL(kern_les_Y, dim3(np), dim3(256), 0, s, p, c.mass.as<double>(), c.dy_Y.as<z2>());
     int nch = chunks_for(T + 1, np + T + 1);
     c.dy_part.ensure((size_t)nch * (T + 1) * es * sizeof(z2));
-    hipLaunchKernelGGL(kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, w, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
+    hipLaunchKernelGGL(kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, wd, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
     long long nel = (long long)(T + 1) * es;
     hipLaunchKernelGGL(kern_sum_parts, dim3(nblk(nel)), dim3(256), 0, s, c.dy_part.as<z2>(), nch, nel, Q);
     const int build = c.lesinv_valid ? 0 : 1;
-    hipLaunchKernelGGL(kern_les_start, dim3(1), dim3(256), 0, s, p, w, build, Q, X, M, c.dy_lesinv.as<z2>(), Qs,
+    hipLaunchKernelGGL(kern_les_start, dim3(1), dim3(256), 0, s, p, wd, build, Q, X, M, c.dy_lesinv.as<z2>(), Qs,
                        err + 2);
     if (build) {
       gesv(c, M, 0, c.dy_lesinv.as<z2>(), 0, kn, kn, 1);
@@ -980,10 +1009,10 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     }
     hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_lesinv.as<z2>(), Qs, kn, n,
                        X + es);
-    hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, w, Q, X, Minv);
+    hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, X, Minv);
     for (int l = k + 1; l < T; l++)
-      hipLaunchKernelGGL(kern_les_step, dim3(T - l), dim3(256), 0, s, p, w, l, Q, X, Minv);
-    hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, w, Q, X, c.dy_scr.as<z2>());
+      hipLaunchKernelGGL(kern_les_step, dim3(T - l), dim3(256), 0, s, p, wd, l, Q, X, Minv);
+    hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, Q, X, c.dy_scr.as<z2>());
     hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, X, err + 2);
     check_launch("les");
   }
@@ -999,12 +1028,13 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
 void extrapolate(Ctx &c, int T) {
   require(T > c.k && T <= c.nt, "extrapolate: tstp must be in (k, nt]");
   const WTab w = make_wtab(c.k);
+  const WTab *wd = dev_wtab(c);
   Pt p = make_pt(c, T, 0, 0, 0);
   const size_t es = c.es;
-  hipLaunchKernelGGL(kern_extrap, dim3(nblk((long long)c.np * es)), dim3(256), 0, c.stream, p, w, 0);
-  hipLaunchKernelGGL(kern_extrap, dim3(nblk((long long)(T + 1) * es)), dim3(256), 0, c.stream, p, w, 1);
+  hipLaunchKernelGGL(kern_extrap, dim3(nblk((long long)c.np * es)), dim3(256), 0, c.stream, p, wd, 0);
+  hipLaunchKernelGGL(kern_extrap, dim3(nblk((long long)(T + 1) * es)), dim3(256), 0, c.stream, p, wd, 1);
   // the les column needs the new G^rceil(T,0) (row 0) -- launched after tv
-  hipLaunchKernelGGL(kern_extrap, dim3(nblk((long long)(T + 1) * es)), dim3(256), 0, c.stream, p, w, 2);
+  hipLaunchKernelGGL(kern_extrap, dim3(nblk((long long)(T + 1) * es)), dim3(256), 0, c.stream, p, wd, 2);
   check_launch("extrapolate");
 }
 
@@ -1028,6 +1058,7 @@ double dyson_start(Ctx &c, double mu, double beta, double dt) {
   require(c.have_mix && c.mix_beta == beta, "dyson_start: call ned_set_matsubara(beta) first");
   require(c.nt >= c.k, "dyson_start: nt must be >= k");
   const WTab w = make_wtab(c.k);
+  const WTab *wd = dev_wtab(c);
   const int n = c.nao, k = c.k, kn = k * n, np = c.np;
   const size_t es = c.es;
   Pt p = make_pt(c, 0, mu, beta, dt);
@@ -1038,11 +1069,11 @@ double dyson_start(Ctx &c, double mu, double beta, double dt) {
   hipLaunchKernelGGL(kern_boot_tv0, dim3(np), dim3(256), 0, s, p, err);
   c.dy_M.ensure((size_t)kn * kn * sizeof(z2));
   c.dy_bootI.ensure((size_t)kn * kn * sizeof(z2));
-  hipLaunchKernelGGL(kern_boot_matrix, dim3(nblk((long long)kn * kn)), dim3(256), 0, s, p, w, c.dy_M.as<z2>(),
+  hipLaunchKernelGGL(kern_boot_matrix, dim3(nblk((long long)kn * kn)), dim3(256), 0, s, p, wd, c.dy_M.as<z2>(),
                      c.dy_bootI.as<z2>());
   gesv(c, c.dy_M.as<z2>(), 0, c.dy_bootI.as<z2>(), 0, kn, kn, 1);
   c.dy_part.ensure((size_t)k * np * es * sizeof(z2));
-  hipLaunchKernelGGL(kern_boot_rhs, dim3(np, k), dim3(256), 0, s, p, w, c.mixK.as<z2>(), c.dy_part.as<z2>());
+  hipLaunchKernelGGL(kern_boot_rhs, dim3(np, k), dim3(256), 0, s, p, wd, c.mixK.as<z2>(), c.dy_part.as<z2>());
   hipLaunchKernelGGL(kern_boot_apply, dim3(np, k), dim3(256), 0, s, p, k, c.dy_bootI.as<z2>(), c.dy_part.as<z2>(), err);
   int npairs = (k + 1) * (k + 2);  // ret pairs + les pairs
   hipLaunchKernelGGL(kern_boot_retles, dim3(npairs), dim3(256), 0, s, p, k, err);
@@ -1057,11 +1088,12 @@ double dyson_start(Ctx &c, double mu, double beta, double dt) {
 void energy(Ctx &c, int T, const double *h0_host, double beta, double dt, double *out3) {
   require(c.have_mix && c.mix_beta == beta, "energy: call ned_set_matsubara(beta) first");
   const WTab w = make_wtab(c.k);
+  const WTab *wd = dev_wtab(c);
   const int top = T >= c.k ? T : c.k, np = c.np;
   const int nb = top + 1 + np;
   c.dy_en.ensure((size_t)2 * nb * sizeof(double));
   Pt p = make_pt(c, T, 0, beta, dt);
-  hipLaunchKernelGGL(kern_energy, dim3(nb), dim3(256), 0, c.stream, p, w, top, c.mass.as<double>(), c.dy_en.as<double>());
+  hipLaunchKernelGGL(kern_energy, dim3(nb), dim3(256), 0, c.stream, p, wd, top, c.mass.as<double>(), c.dy_en.as<double>());
   check_launch("energy");
   std::vector<double> h(2 * nb);
   std::vector<z2> gl(c.es), hm(c.es);

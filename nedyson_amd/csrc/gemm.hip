@@ -340,6 +340,9 @@ void gemm(const GemmDesc &d, Prec prec, hipStream_t s) {
 namespace ned {
 namespace {
 
+// native 4 x 32-bit vector: keeps the prefetch registers out of scratch
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 constexpr int NT_TM = 128, NT_TN = 128, NT_TK = 64, NT_LDK = NT_TK + 8;
 
 __global__ void __launch_bounds__(256) kern_nt_bf16x3(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo,
@@ -373,51 +376,48 @@ __global__ void __launch_bounds__(256) kern_nt_bf16x3(const uint16_t *__restrict
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = v4f{0, 0, 0, 0};
 
-  uint4 ra[2][4], rb[2][4];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int idx = tid + 256 * i, row = idx >> 3, seg = idx & 7;
-      const size_t oa = (size_t)(m0 + row) * lda + k0 + seg * 8;
-      const size_t ob = (size_t)(n0 + row) * ldb + k0 + seg * 8;
-      ra[0][i] = *reinterpret_cast<const uint4 *>(Ahi + oa);
-      ra[1][i] = *reinterpret_cast<const uint4 *>(Alo + oa);
-      rb[0][i] = *reinterpret_cast<const uint4 *>(Bhi + ob);
-      rb[1][i] = *reinterpret_cast<const uint4 *>(Blo + ob);
-    }
-  };
-  auto lstore = [&]() {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int idx = tid + 256 * i, row = idx >> 3, seg = idx & 7;
-#pragma unroll
-      for (int p = 0; p < 2; p++) {
-        *reinterpret_cast<uint4 *>(&sA[p][row][seg * 8]) = ra[p][i];
-        *reinterpret_cast<uint4 *>(&sB[p][row][seg * 8]) = rb[p][i];
-      }
-    }
-  };
-  gload(0);
+  // next K slice, held in registers while the current one is consumed
+  u32x4 ra0[4], ra1[4], rb0[4], rb1[4];
+#define NT_GLOAD(k0_)                                                        \
+  _Pragma("unroll") for (int i = 0; i < 4; i++) {                            \
+    const int idx = tid + 256 * i, row = idx >> 3, seg = idx & 7;            \
+    const size_t oa = (size_t)(m0 + row) * lda + (k0_) + seg * 8;            \
+    const size_t ob = (size_t)(n0 + row) * ldb + (k0_) + seg * 8;            \
+    ra0[i] = *reinterpret_cast<const u32x4 *>(Ahi + oa);                     \
+    ra1[i] = *reinterpret_cast<const u32x4 *>(Alo + oa);                     \
+    rb0[i] = *reinterpret_cast<const u32x4 *>(Bhi + ob);                     \
+    rb1[i] = *reinterpret_cast<const u32x4 *>(Blo + ob);                     \
+  }
+  NT_GLOAD(0)
   for (int k0 = 0; k0 < K; k0 += NT_TK) {
     __syncthreads();
-    lstore();
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int idx = tid + 256 * i, row = idx >> 3, seg = idx & 7;
+      *reinterpret_cast<u32x4 *>(&sA[0][row][seg * 8]) = ra0[i];
+      *reinterpret_cast<u32x4 *>(&sA[1][row][seg * 8]) = ra1[i];
+      *reinterpret_cast<u32x4 *>(&sB[0][row][seg * 8]) = rb0[i];
+      *reinterpret_cast<u32x4 *>(&sB[1][row][seg * 8]) = rb1[i];
+    }
     __syncthreads();
-    if (k0 + NT_TK < K) gload(k0 + NT_TK);
+    if (k0 + NT_TK < K) {
+      NT_GLOAD(k0 + NT_TK)
+    }
 #pragma unroll
     for (int kk = 0; kk < NT_TK; kk += 32) {
       const int kq = kk + 8 * (lane >> 4);
-      uint4 ah[4], al[4], bh[4], bl[4];
+      u32x4 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int row = wm * 64 + i * 16 + (lane & 15);
-        ah[i] = *reinterpret_cast<const uint4 *>(&sA[0][row][kq]);
-        al[i] = *reinterpret_cast<const uint4 *>(&sA[1][row][kq]);
+        ah[i] = *reinterpret_cast<const u32x4 *>(&sA[0][row][kq]);
+        al[i] = *reinterpret_cast<const u32x4 *>(&sA[1][row][kq]);
       }
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int col = wn * 64 + j * 16 + (lane & 15);
-        bh[j] = *reinterpret_cast<const uint4 *>(&sB[0][col][kq]);
-        bl[j] = *reinterpret_cast<const uint4 *>(&sB[1][col][kq]);
+        bh[j] = *reinterpret_cast<const u32x4 *>(&sB[0][col][kq]);
+        bl[j] = *reinterpret_cast<const u32x4 *>(&sB[1][col][kq]);
       }
 #pragma unroll
       for (int i = 0; i < 4; i++)
@@ -431,6 +431,7 @@ __global__ void __launch_bounds__(256) kern_nt_bf16x3(const uint16_t *__restrict
         }
     }
   }
+#undef NT_GLOAD
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll

@@ -1,0 +1,71 @@
+"""The C++ mirror of the reference interface (include/nedyson.hpp) driving a
+whole GF2 propagation, checked against the CPU oracle.
+
+tests/cpp/gf2_run is built by __graft_entry__.build(); it runs
+G0_from_h0 -> do_boot -> do_tstp(k+1..nt) -> energies through the mirror
+classes (molGF2SolverDecomp, dyson, DecompSimulation) and writes G.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from nedyson_amd import problem
+from nedyson_amd.green import GreenFunc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "gf2_run")
+
+
+def _write_inputs(d, p, prec, corr, bootmax, boottol):
+    n, a = p.nao, p.naux
+    B = p.B  # (naux, nao, nao)
+    vija = np.ascontiguousarray(np.transpose(B, (1, 2, 0)))  # Vija(i,j,Q)
+    viaj = np.ascontiguousarray(np.transpose(B, (1, 0, 2)))  # Viaj(i,Q,j)
+    vija.astype(np.float64).tofile(os.path.join(d, "Vija.bin"))
+    viaj.astype(np.float64).tofile(os.path.join(d, "Viaj.bin"))
+    np.ascontiguousarray(p.h0, dtype=np.float64).tofile(os.path.join(d, "h0.bin"))
+    np.ascontiguousarray(p.fock, dtype=np.float64).tofile(os.path.join(d, "fock.bin"))
+    np.ascontiguousarray(p.Gmat, dtype=np.complex128).tofile(os.path.join(d, "Gmat.bin"))
+    with open(os.path.join(d, "params.txt"), "w") as f:
+        f.write(f"{n} {a} {p.nt} {p.ntau} {p.k} {float(p.mu):.17g} {float(p.beta):.17g} {float(p.dt):.17g} "
+                f"{corr} {bootmax} {float(boottol):.17g} {prec} 0\n")
+
+
+def test_cpp_driver_usage():
+    """The binary links against the in-tree library and rejects bad usage
+    without touching a GPU."""
+    if not os.path.exists(EXE):
+        pytest.fail("tests/cpp/gf2_run not built (run __graft_entry__.build())")
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_gf2_run_vs_oracle(tmp_path, orc):
+    p = problem.Problem(nao=4, naux=12, nelec=2, beta=10.0, ntau=15, nt=12, dt=0.02, seed=0, spacing=1.0,
+                        b_sd=0.1, quench="h00", quench_norm=0.5)
+    corr, bootmax, boottol = 3, 50, 1e-12
+    d = str(tmp_path)
+    _write_inputs(d, p, "f64", corr, bootmax, boottol)
+    print(f"[gf2_run] nao={p.nao} nt={p.nt} dir={d}", flush=True)
+    r = subprocess.run([EXE, d], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # CPU oracle, same sequence
+    G = p.initial_green()
+    S = GreenFunc(p.nt, p.ntau, p.nao)
+    hmf = np.zeros((p.nt + 1, p.nao, p.nao), dtype=complex)
+    D = orc.Dyson(p.nt, p.ntau, p.nao, p.k)
+    D.do_boot(p.V, G, S, hmf, p.fock, p.mu, p.beta, p.dt, bootmax, boottol)
+    for T in range(p.k + 1, p.nt + 1):
+        D.do_tstp(p.V, T, G, S, hmf, p.h0, p.mu, p.beta, p.dt, corr)
+    ret = np.fromfile(os.path.join(d, "Gret.bin"), dtype=np.complex128).reshape(G.ret.shape)
+    les = np.fromfile(os.path.join(d, "Gles.bin"), dtype=np.complex128).reshape(G.les.shape)
+    tv = np.fromfile(os.path.join(d, "Gtv.bin"), dtype=np.complex128).reshape(G.tv.shape)
+    err = max(np.max(np.abs(ret - G.ret)), np.max(np.abs(les - G.les)), np.max(np.abs(tv - G.tv)))
+    assert err < 1e-10, err
+    e = np.loadtxt(os.path.join(d, "energy.txt"))
+    assert e.shape == (p.nt + 1, 4)
+    # particle number 2 tr rho stays at the filling after the quench (GF2 conserves it)
+    assert np.max(np.abs(e[p.k + 1:, 3] - e[p.k + 1, 3])) < 1e-6

@@ -233,3 +233,52 @@ def test_tti_extend_gpu():
             assert np.array_equal(d.lesptr(s, t), g.lesptr(6 - (t - s), 6))
         assert np.array_equal(d.tv[t], g.tv[6])
     assert np.array_equal(d.tv[7], g.tv[7])
+
+
+def _tti_extend_host(S, T0):
+    """Host restatement of ned_tti_extend: slice T0 continued to every t <= T0."""
+    for t in range(T0 + 1):
+        for tp in range(t + 1):
+            if t == T0:
+                continue
+            S.retptr(t, tp)[:] = S.retptr(T0, T0 - (t - tp))
+            S.lesptr(tp, t)[:] = S.lesptr(T0 - (t - tp), T0)
+        if t != T0:
+            S.tv[t] = S.tv[T0]
+
+
+@pytest.mark.parametrize("nao,naux,T0", [(4, 12, 300), (13, 20, 150)])
+def test_long_history_window_parity(orc, nao, naux, T0):
+    """The benchmark's window at long history: free G of the Fock matrix for
+    t <= T0 (ned_g0_from_h0), Sigma(T0) continued time-translation
+    invariantly (ned_tti_extend), then three full timesteps on the quenched
+    Hamiltonian -- GPU (FP64) against the oracle on the same history.  Covers
+    the multi-chunk history reductions and long Volterra recurrences."""
+    K = 3
+    p = problem.Problem(nao=nao, naux=naux, nelec=2, beta=10.0, ntau=15, nt=T0 + K, dt=0.01, seed=3,
+                        spacing=1.0, b_sd=0.1, quench="h00", quench_norm=0.5)
+    print(f"[window] nao={nao} naux={naux} T0={T0}", flush=True)
+    G = problem.free_gf(p.fock, p.mu, p.beta, p.dt, p.nt, p.ntau, T0, p.Gmat)
+    S = GreenFunc(p.nt, p.ntau, p.nao)
+    hmf = np.zeros((p.nt + 1, nao, nao), dtype=complex)
+    hmf[:T0 + 1] = p.fock
+    orc.sigma_solve(p.V, T0, S, G)
+    _tti_extend_host(S, T0)
+    D = orc.Dyson(p.nt, p.ntau, nao, p.k)
+    for T in range(T0 + 1, T0 + K + 1):
+        D.do_tstp(p.V, T, G, S, hmf, p.h0, p.mu, p.beta, p.dt, 3)
+    with ned.Context(nao, naux, p.nt, p.ntau, p.k) as c:
+        c.set_coulomb(p.B)
+        c.write_blocks(ned.NED_G, ned.NED_MAT, 0, p.Gmat)
+        c.set_matsubara(p.beta)
+        c.g0_from_h0(T0, p.fock, p.mu, p.beta, p.dt)
+        c.write_hmf(np.repeat(p.fock[None].astype(complex), T0 + 1, axis=0))
+        c.sigma(T0, "f64")
+        c.tti_extend(ned.NED_SIGMA, T0)
+        for T in range(T0 + 1, T0 + K + 1):
+            c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, "f64", 3)
+        d = c.download(ned.NED_G)
+    for T in range(T0 + 1, T0 + K + 1):
+        assert _relerr(d.ret_row(T), G.ret_row(T)) < 1e-10, ("ret", T)
+        assert _relerr(d.les_col(T), G.les_col(T)) < 1e-10, ("les", T)
+        assert _relerr(d.tv[T], G.tv[T]) < 1e-10, ("tv", T)
