@@ -36,6 +36,14 @@ __host__ __device__ inline z2 zsub(z2 a, z2 b) { return {a.x - b.x, a.y - b.y}; 
 __host__ __device__ inline z2 zconj(z2 a) { return {a.x, -a.y}; }
 __host__ __device__ inline z2 zscale(z2 a, double s) { return {a.x * s, a.y * s}; }
 
+// bf16 round-to-nearest-even of an FP32 value (finite inputs), and back
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
 // ------------------------------------------------------------------ GEMM
 // One operand of the strided batched GEMM.  The logical matrix is A (M x K)
 // or B (K x N); element (r, c) of batch b sits at
@@ -89,7 +97,9 @@ struct NtOp {
 bool nt_bf16x3_ok(int M, int N, int K);
 void gemm_nt_bf16x3(const NtOp &A, const NtOp &B, float *C, long long ldc, long long sC, int M, int N, int K,
                     int batch, float alpha, int beta, hipStream_t s);
-// x -> hi + lo with hi = bf16(x), lo = bf16(x - hi), round to nearest even
-void split_bf16(const float *x, uint16_t *hi, uint16_t *lo, long long n, hipStream_t s);
+// x -> hi + lo with hi = bf16(x), lo = bf16(x - hi), round to nearest even;
+// rows x cols (cols a multiple of 4) from row stride ldx to row stride ldh
+void split_bf16(const float *x, long long ldx, uint16_t *hi, uint16_t *lo, long long ldh, long long rows, int cols,
+                hipStream_t s);
 
 }  // namespace ned

@@ -46,10 +46,14 @@ struct Ctx {
   DevBuf Bq;       // naux x nao x nao double, B[Q][i][j] = V(i,j,Q)
   DevBuf Bq32;     // float copy
   DevBuf U64, U32; // (nao^2 x nao^2): U'[(m,q)][(k,j)] = sum_Q B[Q][k][m] B[Q][q][j]
-  DevBuf Uth, Utl; // U'^T split into bf16 hi / lo planes (operand B of the exchange GEMM)
+  // operand B of the split-bf16 exchange GEMM, hi / lo planes of
+  // Baug[(k,j)][c] = -U'[c][(k,j)] (c < nao^2), 2 B[c-nao^2][k][j] (c >= nao^2):
+  // the bubble's Pi V product rides along the exchange as nao^2 + naux extra K
+  DevBuf Uah, Ual;
+  bool uaug = false;  // Baug built (nao^2 and 2 naux multiples of 128, naux of 64)
   bool have_coulomb = false;
   // Sigma scratch
-  DevBuf sg_items_A, sg_items_B, sg_W, sg_X, sg_H, sg_Pi, sg_E, sg_Xh, sg_Xl;
+  DevBuf sg_items_A, sg_items_B, sg_W, sg_X, sg_H, sg_Pi, sg_E, sg_Ep, sg_S1h, sg_S1l, sg_Pf;
   int sigma_chunk = 0;  // items per chunk (0 = choose automatically)
   // Dyson state
   DevBuf mixK;                // mixing kernel of G^M (np x np blocks)

@@ -26,12 +26,6 @@ __device__ __forceinline__ long long boff(const GOp &o, int b) {
 }
 
 // round-to-nearest-even float -> bf16 bits (finite inputs)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
 template <typename T>
 __device__ __forceinline__ T ldg(const void *p, long long off) {
@@ -447,27 +441,25 @@ __global__ void __launch_bounds__(256) kern_nt_bf16x3(const uint16_t *__restrict
       }
 }
 
-__global__ void kern_split_bf16(const float *__restrict__ x, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo,
-                                long long n) {
-  long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-  if (i + 3 < n) {
-    float4 v = *reinterpret_cast<const float4 *>(x + i);
-    float f[4] = {v.x, v.y, v.z, v.w};
-    uint16_t h[4], l[4];
+__global__ void kern_split_bf16(const float *__restrict__ x, long long ldx, uint16_t *__restrict__ hi,
+                                uint16_t *__restrict__ lo, long long ldh, long long rows, int cols) {
+  const int c4 = cols >> 2;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * c4) return;
+  const long long r = i / c4;
+  const int c = (int)(i % c4) * 4;
+  const float4 v = *reinterpret_cast<const float4 *>(x + r * ldx + c);
+  const float f[4] = {v.x, v.y, v.z, v.w};
+  uint16_t h[4], l[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      h[q] = f2bf(f[q]);
-      l[q] = f2bf(f[q] - bf2f(h[q]));
-    }
-    *reinterpret_cast<uint2 *>(hi + i) = uint2{(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
-    *reinterpret_cast<uint2 *>(lo + i) = uint2{(uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16)};
-  } else {
-    for (; i < n; i++) {
-      uint16_t h = f2bf(x[i]);
-      hi[i] = h;
-      lo[i] = f2bf(x[i] - bf2f(h));
-    }
+  for (int q = 0; q < 4; q++) {
+    h[q] = f2bf(f[q]);
+    l[q] = f2bf(f[q] - bf2f(h[q]));
   }
+  *reinterpret_cast<uint2 *>(hi + r * ldh + c) =
+      uint2{(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
+  *reinterpret_cast<uint2 *>(lo + r * ldh + c) =
+      uint2{(uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16)};
 }
 
 }  // namespace
@@ -484,9 +476,13 @@ void gemm_nt_bf16x3(const NtOp &A, const NtOp &B, float *C, long long ldc, long 
   check_launch("gemm_nt_bf16x3");
 }
 
-void split_bf16(const float *x, uint16_t *hi, uint16_t *lo, long long n, hipStream_t s) {
-  long long th = (n + 3) / 4;
-  hipLaunchKernelGGL(kern_split_bf16, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, x, hi, lo, n);
+void split_bf16(const float *x, long long ldx, uint16_t *hi, uint16_t *lo, long long ldh, long long rows, int cols,
+                hipStream_t s) {
+  if (cols % 4 || ldx % 4 || ldh % 4) throw std::invalid_argument("split_bf16: cols and strides must be multiples of 4");
+  const long long th = rows * (cols / 4);
+  if (th == 0) return;
+  hipLaunchKernelGGL(kern_split_bf16, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, x, ldx, hi, lo, ldh, rows,
+                     cols);
   check_launch("split_bf16");
 }
 

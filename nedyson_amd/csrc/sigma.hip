@@ -47,17 +47,49 @@ __global__ void kern_build_U(const double *Bq, int n, int a, double *U64, float 
   U32[idx] = (float)s;
 }
 
-__global__ void kern_split_Ut(const float *U, int n2, uint16_t *hi, uint16_t *lo) {
+// Baug[(k,j)][c] = -U'[c][(k,j)] for c < n2, 2 B[c-n2][k][j] for c >= n2,
+// split into bf16 hi / lo planes (row stride n2 + a)
+__global__ void kern_build_Uaug(const float *U, const float *Bq, int n2, int a, uint16_t *hi, uint16_t *lo) {
+  const long long ka = n2 + a;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)n2 * n2) return;
-  long long r = idx / n2, c = idx % n2;
-  float v = U[c * n2 + r];
-  uint32_t u = __float_as_uint(v);
-  uint16_t h = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-  float rem = v - __uint_as_float(((uint32_t)h) << 16);
-  uint32_t w = __float_as_uint(rem);
+  if (idx >= (long long)n2 * ka) return;
+  const long long r = idx / ka, col = idx % ka;
+  const float v = col < n2 ? -U[col * n2 + r] : 2.0f * Bq[(col - n2) * n2 + r];
+  const uint16_t h = f2bf(v);
   hi[idx] = h;
-  lo[idx] = (uint16_t)((w + 0x7fffu + ((w >> 16) & 1u)) >> 16);
+  lo[idx] = f2bf(v - bf2f(h));
+}
+
+// Pi of every item from the real product Pf = [Wr; Wi] [Zr; Zi]^T (2a x 2a):
+// Pi_r = Pf_rr - Pf_ii, Pi_i = Pf_ri + Pf_ir, split into bf16 and written to
+// rows (item, plane, a), columns col0 + b of the exchange GEMM's operand A
+__global__ void kern_pi_combine(const float *Pf, int a, int P, uint16_t *hi, uint16_t *lo, long long ldh, int col0) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long a2 = (long long)a * a;
+  if (idx >= (long long)P * 2 * a2) return;
+  const int p = (int)(idx / (2 * a2));
+  const long long rem = idx % (2 * a2);
+  const int pl = (int)(rem / a2), x = (int)((rem % a2) / a), b = (int)(rem % a);
+  const float *F = Pf + (size_t)p * 4 * a2;
+  const long long w2 = 2LL * a;
+  float v;
+  if (pl == 0) v = F[x * w2 + b] - F[(a + x) * w2 + a + b];
+  else v = F[x * w2 + a + b] + F[(a + x) * w2 + b];
+  const size_t o = ((size_t)p * 2 * a + (size_t)pl * a + x) * ldh + col0 + b;
+  const uint16_t h = f2bf(v);
+  hi[o] = h;
+  lo[o] = f2bf(v - bf2f(h));
+}
+
+// E[p0 + p] = sum_s part[p S + s]   (2 n2 values per item)
+template <typename T>
+__global__ void kern_sum_splits(const T *part, int S, int P, long long per, T *E) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)P * per) return;
+  const long long p = idx / per, e = idx % per;
+  T acc = 0;
+  for (int q = 0; q < S; q++) acc += part[(p * S + q) * per + e];
+  E[idx] = acc;
 }
 
 __global__ void kern_d2f(const double *x, float *y, long long n) {
@@ -126,8 +158,19 @@ __global__ void kern_scatter(const T *E, int T_, int n, int np, z2 *Sret, z2 *Sl
 
 template <typename T>
 struct Bufs {
-  T *A, *B, *W, *X, *H, *Pi, *E;
+  T *A, *B, *W, *X, *H, *Pi, *E, *Ep;
+  uint16_t *S1h, *S1l;  // split-bf16 operands (bf16x3 path)
+  float *Pf;            // [Wr; Wi][Zr; Zi]^T per item (bf16x3 path)
 };
+
+// aux-index splits of the E reduction (K = naux nao is long, M = N = nao small)
+inline int e_splits(int a) {
+  for (int S : {16, 8, 4, 2})
+    if (a % S == 0 && a / S >= 8) return S;
+  return 1;
+}
+
+inline bool fast_bf16(const Ctx &c, Prec prec) { return prec == PREC_BF16X3 && c.uaug; }
 
 // One chunk of P items starting at item p0.
 template <typename T>
@@ -153,8 +196,18 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     d.C.ptr = bf.X; d.C.rs = n; d.C.cs = 1; d.C.ps = an2; d.C.bdiv = a; d.C.bs_hi = 2 * an2; d.C.bs_lo = n2;
     gemm(d, prec, s);
   }
-  // Pi_p[a][b] = sum_(ik) W^a_ik Z^b_ik
-  {
+  const bool fast = fast_bf16(c, prec);
+  const int M = 2 * a * P, NN = (int)n2;
+  if (fast) {
+    // split-bf16 operands: W into the (still unused) H buffer, Z into S1
+    uint16_t *wh = reinterpret_cast<uint16_t *>(bf.H), *wl = wh + (size_t)P * 2 * an2;
+    split_bf16((const float *)bf.W, n2, wh, wl, n2, (long long)M, NN, s);
+    split_bf16((const float *)bf.X, n2, bf.S1h, bf.S1l, n2, (long long)M, NN, s);
+    // Pf_p = [Wr; Wi] [Zr; Zi]^T: M = N = 2a, K = n^2, one launch for the chunk
+    gemm_nt_bf16x3(NtOp{wh, wl, n2, 2 * an2}, NtOp{bf.S1h, bf.S1l, n2, 2 * an2}, bf.Pf, 2LL * a, 4LL * a * a, 2 * a,
+                   2 * a, NN, P, 1.0f, 0, s);
+  } else {
+    // Pi_p[a][b] = sum_(ik) W^a_ik Z^b_ik
     GemmDesc d;
     d.M = a; d.N = a; d.K = (int)n2; d.batch = P; d.ca = true; d.cb = true;
     d.A.ptr = bf.W; d.A.ms = n2; d.A.ks_lo = 1; d.A.ps = an2; d.A.bs_lo = 2 * an2;
@@ -171,17 +224,23 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     d.C.ptr = bf.X; d.C.rs = n; d.C.cs = 1; d.C.ps = an2; d.C.bdiv = a; d.C.bs_hi = 2 * an2; d.C.bs_lo = n2;
     gemm(d, prec, s);
   }
-  // H = -C U'   (rows (p, plane, a) of X, all at once; the n^4 naux GEMM)
-  {
-    const int M = 2 * a * P, NN = (int)n2;
-    if (prec == PREC_BF16X3 && nt_bf16x3_ok(M, NN, NN)) {
-      uint16_t *xh = c.sg_Xh.as<uint16_t>(), *xl = c.sg_Xl.as<uint16_t>();
-      split_bf16((const float *)bf.X, xh, xl, (long long)M * NN, s);
-      c.mark(0);
-      gemm_nt_bf16x3(NtOp{xh, xl, NN, 0}, NtOp{c.Uth.as<uint16_t>(), c.Utl.as<uint16_t>(), NN, 0}, (float *)bf.H, NN,
-                     0, M, NN, NN, 1, -1.0f, 0, s);
-      c.mark(1);
-    } else {
+  if (fast) {
+    // H = [C | Pi] [-U'^T | 2 Vmat^T]^T = -C U' + 2 Pi Vmat: the exchange and the
+    // bubble's Pi V in one GEMM, rows (item, plane, a), K = n^2 + a
+    const long long ka = n2 + a;
+    long long tot = (long long)P * 2 * a * a;
+    hipLaunchKernelGGL(kern_pi_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Pf, a, P, bf.S1h,
+                       bf.S1l, ka, NN);
+    check_launch("sigma pi combine");
+    split_bf16((const float *)bf.X, n2, bf.S1h, bf.S1l, ka, (long long)M, NN, s);
+    c.mark(0);
+    gemm_nt_bf16x3(NtOp{bf.S1h, bf.S1l, ka, 0}, NtOp{c.Uah.as<uint16_t>(), c.Ual.as<uint16_t>(), ka, 0},
+                   (float *)bf.H, n2, 0, M, NN, (int)ka, 1, 1.0f, 0, s);
+    c.mark(1);
+    if (c.prof) c.prof_flops += 2.0 * M * (double)NN * (double)ka;
+  } else {
+    // H = -C U'   (rows (p, plane, a) of X, all at once; the n^4 naux GEMM)
+    {
       GemmDesc d;
       d.M = M; d.N = NN; d.K = NN; d.batch = 1;
       d.A.ptr = bf.X; d.A.ms = n2; d.A.ks_lo = 1;
@@ -191,27 +250,35 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
       c.mark(0);
       gemm(d, prec, s);
       c.mark(1);
+      if (c.prof) c.prof_flops += 2.0 * M * (double)NN * NN;
     }
-    if (c.prof) c.prof_flops += 2.0 * M * (double)NN * NN;
+    // H += 2 Pi Vmat   (Vmat[b][(k,j)] = B[b][k][j])
+    {
+      GemmDesc d;
+      d.M = a; d.N = (int)n2; d.K = a; d.batch = P; d.ca = true; d.cb = false;
+      d.A.ptr = bf.Pi; d.A.ms = a; d.A.ks_lo = 1; d.A.ps = (long long)a * a; d.A.bs_lo = 2LL * a * a;
+      d.B.ptr = Vq; d.B.ms = 1; d.B.ks_lo = n2;
+      d.C.ptr = bf.H; d.C.rs = n2; d.C.cs = 1; d.C.ps = an2; d.C.bs_lo = 2 * an2;
+      d.alpha = 2.0; d.beta = 1;
+      gemm(d, prec, s);
+    }
   }
-  // H += 2 Pi Vmat   (Vmat[b][(k,j)] = B[b][k][j])
+  // E_p = sum_a W^a H^a  (K index (a,k): W offset a*n2 + k), split over S
+  // ranges of a so that the launch has P*S workgroups, then summed
   {
+    const int S = e_splits(a), as = a / S;
     GemmDesc d;
-    d.M = a; d.N = (int)n2; d.K = a; d.batch = P; d.ca = true; d.cb = false;
-    d.A.ptr = bf.Pi; d.A.ms = a; d.A.ks_lo = 1; d.A.ps = (long long)a * a; d.A.bs_lo = 2LL * a * a;
-    d.B.ptr = Vq; d.B.ms = 1; d.B.ks_lo = n2;
-    d.C.ptr = bf.H; d.C.rs = n2; d.C.cs = 1; d.C.ps = an2; d.C.bs_lo = 2 * an2;
-    d.alpha = 2.0; d.beta = 1;
+    d.M = n; d.N = n; d.K = as * n; d.batch = P * S; d.ca = true; d.cb = true;
+    d.A.ptr = bf.W; d.A.ms = n; d.A.kdiv = n; d.A.ks_hi = n2; d.A.ks_lo = 1; d.A.ps = an2;
+    d.A.bdiv = S; d.A.bs_hi = 2 * an2; d.A.bs_lo = (long long)as * n2;
+    d.B.ptr = bf.H; d.B.ms = 1; d.B.ks_lo = n; d.B.ps = an2;
+    d.B.bdiv = S; d.B.bs_hi = 2 * an2; d.B.bs_lo = (long long)as * n2;
+    d.C.ptr = bf.Ep; d.C.rs = n; d.C.cs = 1; d.C.ps = n2; d.C.bs_lo = 2 * n2;
     gemm(d, prec, s);
-  }
-  // E_p = sum_a W^a H^a  (K index (a,k): W offset a*n2 + k)
-  {
-    GemmDesc d;
-    d.M = n; d.N = n; d.K = a * n; d.batch = P; d.ca = true; d.cb = true;
-    d.A.ptr = bf.W; d.A.ms = n; d.A.kdiv = n; d.A.ks_hi = n2; d.A.ks_lo = 1; d.A.ps = an2; d.A.bs_lo = 2 * an2;
-    d.B.ptr = bf.H; d.B.ms = 1; d.B.ks_lo = n; d.B.ps = an2; d.B.bs_lo = 2 * an2;
-    d.C.ptr = bf.E + (size_t)p0 * 2 * n2; d.C.rs = n; d.C.cs = 1; d.C.ps = n2; d.C.bs_lo = 2 * n2;
-    gemm(d, prec, s);
+    long long tot = (long long)P * 2 * n2;
+    hipLaunchKernelGGL(kern_sum_splits<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, P, 2 * n2,
+                       bf.E + (size_t)p0 * 2 * n2);
+    check_launch("sigma E sum");
   }
 }
 
@@ -220,22 +287,34 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   const int n = c.nao, a = c.naux, np = c.np;
   const long long n2 = (long long)n * n, an2 = (long long)a * n2;
   const int Ptot = 2 * (tstp + 1) + np;
-  const size_t per_item = (size_t)(3 * 2 * an2 + 2LL * a * a + 4 * n2) * sizeof(T) + (prec == PREC_BF16X3 ? 2 * 2 * an2 * 2 : 0);
-  int P = c.sigma_chunk > 0 ? c.sigma_chunk : (int)std::max<size_t>(1, ((size_t)3 << 30) / std::max<size_t>(per_item, 1));
+  const bool fast = fast_bf16(c, prec);
+  const int S = e_splits(a);
+  const long long ka = n2 + a;
+  // device scratch per item: items, W, X, H, Pi (generic path), E partials,
+  // and on the bf16x3 path the split operand S1 and the Pi product Pf
+  size_t per_item = (size_t)(3 * 2 * an2 + 4 * n2 + (long long)S * 2 * n2) * sizeof(T);
+  if (fast) per_item += (size_t)2 * a * ka * 2 * sizeof(uint16_t) + (size_t)4 * a * a * sizeof(float);
+  else per_item += (size_t)2 * a * a * sizeof(T);
+  const size_t budget = (size_t)4 << 30;
+  int P = c.sigma_chunk > 0 ? c.sigma_chunk : (int)std::max<size_t>(1, budget / std::max<size_t>(per_item, 1));
   P = std::min(P, Ptot);
   c.sg_items_A.ensure((size_t)P * 2 * n2 * sizeof(T));
   c.sg_items_B.ensure((size_t)P * 2 * n2 * sizeof(T));
   c.sg_W.ensure((size_t)P * 2 * an2 * sizeof(T));
   c.sg_X.ensure((size_t)P * 2 * an2 * sizeof(T));
   c.sg_H.ensure((size_t)P * 2 * an2 * sizeof(T));
-  c.sg_Pi.ensure((size_t)P * 2 * a * a * sizeof(T));
   c.sg_E.ensure((size_t)Ptot * 2 * n2 * sizeof(T));
-  if (prec == PREC_BF16X3) {
-    c.sg_Xh.ensure((size_t)P * 2 * an2 * sizeof(uint16_t));
-    c.sg_Xl.ensure((size_t)P * 2 * an2 * sizeof(uint16_t));
+  c.sg_Ep.ensure((size_t)P * S * 2 * n2 * sizeof(T));
+  if (fast) {
+    c.sg_S1h.ensure((size_t)P * 2 * a * ka * sizeof(uint16_t));
+    c.sg_S1l.ensure((size_t)P * 2 * a * ka * sizeof(uint16_t));
+    c.sg_Pf.ensure((size_t)P * 4 * a * a * sizeof(float));
+  } else {
+    c.sg_Pi.ensure((size_t)P * 2 * a * a * sizeof(T));
   }
   Bufs<T> bf{c.sg_items_A.as<T>(), c.sg_items_B.as<T>(), c.sg_W.as<T>(), c.sg_X.as<T>(), c.sg_H.as<T>(),
-             c.sg_Pi.as<T>(), c.sg_E.as<T>()};
+             c.sg_Pi.as<T>(), c.sg_E.as<T>(), c.sg_Ep.as<T>(), c.sg_S1h.as<uint16_t>(), c.sg_S1l.as<uint16_t>(),
+             c.sg_Pf.as<float>()};
   const T *Vq = sizeof(T) == 8 ? (const T *)c.Bq.p : (const T *)c.Bq32.p;
   const T *U = sizeof(T) == 8 ? (const T *)c.U64.p : (const T *)c.U32.p;
   for (int p0 = 0; p0 < Ptot; p0 += P) {
@@ -268,8 +347,6 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
   c.Bq32.ensure(a * n2 * sizeof(float));
   c.U64.ensure(n2 * n2 * sizeof(double));
   c.U32.ensure(n2 * n2 * sizeof(float));
-  c.Uth.ensure(n2 * n2 * sizeof(uint16_t));
-  c.Utl.ensure(n2 * n2 * sizeof(uint16_t));
   NED_HIP(hipMemcpyAsync(c.Bq.p, Bq_host, a * n2 * sizeof(double), hipMemcpyHostToDevice, c.stream));
   long long nb = (long long)a * n2;
   hipLaunchKernelGGL(kern_d2f, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c.stream, c.Bq.as<double>(),
@@ -279,9 +356,16 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
   hipLaunchKernelGGL(kern_build_U, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, c.stream, c.Bq.as<double>(), n, a,
                      c.U64.as<double>(), c.U32.as<float>());
   check_launch("coulomb U");
-  hipLaunchKernelGGL(kern_split_Ut, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, c.stream, c.U32.as<float>(),
-                     (int)n2, c.Uth.as<uint16_t>(), c.Utl.as<uint16_t>());
-  check_launch("coulomb U split");
+  c.uaug = n2 % 128 == 0 && (2 * a) % 128 == 0 && a % 64 == 0;
+  if (c.uaug) {
+    const size_t ka = n2 + a;
+    c.Uah.ensure(n2 * ka * sizeof(uint16_t));
+    c.Ual.ensure(n2 * ka * sizeof(uint16_t));
+    long long nt_ = (long long)(n2 * ka);
+    hipLaunchKernelGGL(kern_build_Uaug, dim3((unsigned)((nt_ + 255) / 256)), dim3(256), 0, c.stream, c.U32.as<float>(),
+                       c.Bq32.as<float>(), (int)n2, a, c.Uah.as<uint16_t>(), c.Ual.as<uint16_t>());
+    check_launch("coulomb Baug");
+  }
   NED_HIP(hipStreamSynchronize(c.stream));
   c.have_coulomb = true;
 }

@@ -22,7 +22,9 @@ def _sigma_gpu(V, B, G, tstp, prec, chunk=None):
         return c.download(ned.NED_SIGMA)
 
 
-@pytest.mark.parametrize("nao,naux", [(2, 3), (4, 12), (8, 32), (13, 20)])
+# (16, 64) and (32, 64) take the fused split-bf16 path in bf16x3 (nao^2 and
+# 2 naux multiples of 128): Pi on the NT kernel, Pi V folded into the exchange
+@pytest.mark.parametrize("nao,naux", [(2, 3), (4, 12), (8, 32), (13, 20), (16, 64), (32, 64)])
 @pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3"])
 def test_sigma_parity(orc, nao, naux, prec):
     print(f"[sigma parity] nao={nao} naux={naux} prec={prec}", flush=True)
@@ -44,10 +46,11 @@ def test_sigma_parity(orc, nao, naux, prec):
         del s
 
 
-def test_sigma_chunking_invariant(orc):
+@pytest.mark.parametrize("nao,naux,prec", [(6, 10, "f64"), (16, 64, "bf16x3")])
+def test_sigma_chunking_invariant(orc, nao, naux, prec):
     """Splitting the slice into item chunks must not change the result."""
     rng = np.random.default_rng(5)
-    nao, naux, nt, ntau = 6, 10, 9, 11
+    nt, ntau = 9, 11
     V, B = sym_V(rng, nao, naux)
     G = random_green(rng, nt, ntau, nao, scale=0.5)
     outs = []
@@ -58,12 +61,13 @@ def test_sigma_chunking_invariant(orc):
             with ned.Context(nao, naux, nt, ntau, 1) as c:
                 c.set_coulomb(B)
                 c.upload(ned.NED_G, G)
-                c.sigma(nt, "f64")
+                c.sigma(nt, prec)
                 outs.append(c.download(ned.NED_SIGMA))
         finally:
             del os.environ["NED_SIGMA_CHUNK"]
     for o in outs[1:]:
-        assert np.max(np.abs(o.ret - outs[0].ret)) == 0
+        for comp in ("ret", "les", "tv"):
+            assert np.max(np.abs(getattr(o, comp) - getattr(outs[0], comp))) == 0
 
 
 def test_sigma_rejects_nonsymmetric_V():
