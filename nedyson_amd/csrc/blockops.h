@@ -1,8 +1,13 @@
 // Workgroup-level operations on nao x nao complex FP64 blocks (nao <= 64):
-// the building block of the Dyson-step history sums.  256 threads form a
-// 16 x 16 grid; thread (ty, tx) owns rows ty+16r and columns tx+16c,
-// r, c = 0..3, of the output block, accumulated in registers.  Operand
-// K-slices of 16 are staged through LDS.
+// the building block of the Dyson-step history sums.
+//
+// The output block lives in registers in the layout of the FP64 matrix core
+// (v_mfma_f64_16x16x4f64): wave w owns rows 16w..16w+15, and lane l of it
+// holds, for each 16-column tile c and each q = 0..3, the element
+//   row 16w + 4q + l/16, column 16c + l%16
+// (real and imaginary parts in separate 4-vectors).  Products run on the
+// matrix core with operand K-slices of 16 staged through LDS; tiles that lie
+// wholly outside nao x nao are skipped.
 #pragma once
 #include "common.h"
 
@@ -12,13 +17,21 @@ constexpr int BO_T = 256;   // threads per workgroup
 constexpr int BO_KT = 16;   // K slice
 constexpr int BO_NMAX = 64; // largest supported nao
 
+typedef double blk_v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int blk_i(int q) { return 16 * (threadIdx.x >> 6) + 4 * q + ((threadIdx.x & 63) >> 4); }
+__device__ __forceinline__ int blk_j(int c) { return 16 * c + (threadIdx.x & 15); }
+
 struct BlkAcc {
-  z2 v[4][4];
+  blk_v4d re[4], im[4];  // [column tile][q]
   __device__ void zero() {
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-      for (int c = 0; c < 4; c++) v[r][c] = {0.0, 0.0};
+    for (int c = 0; c < 4; c++) re[c] = im[c] = blk_v4d{0.0, 0.0, 0.0, 0.0};
+  }
+  __device__ z2 get(int q, int c) const { return z2{re[c][q], im[c][q]}; }
+  __device__ void set(int q, int c, z2 v) {
+    re[c][q] = v.x;
+    im[c][q] = v.y;
   }
 };
 
@@ -27,103 +40,117 @@ struct BlkLds {
   z2 Y[BO_KT][BO_NMAX + 1];
 };
 
-// op: 0 = as stored, 1 = adjoint (conjugate transpose), 2 = transpose
-__device__ __forceinline__ z2 blk_get(const z2 *M, int n, int op, int r, int c) {
-  if (op == 0) return M[r * n + c];
-  z2 v = M[c * n + r];
+// op: 0 = as stored, 1 = adjoint (conjugate transpose), 2 = transpose;
+// ld = row stride of the stored matrix
+__device__ __forceinline__ z2 blk_get(const z2 *M, int ld, int op, int r, int c) {
+  if (op == 0) return M[r * ld + c];
+  z2 v = M[c * ld + r];
   return op == 1 ? zconj(v) : v;
 }
 
-// acc += coef * op(X) * op(Y)   (all threads of the workgroup call this)
+// acc += coef * op(X) * op(Y)   (all threads of the workgroup call this);
+// ldX, ldY are the row strides of X and Y as stored (n for a dense block)
 __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 *X, int opX, const z2 *Y, int opY,
-                                        z2 coef) {
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+                                        z2 coef, int ldX = 0, int ldY = 0) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (ldX == 0) ldX = n;
+  if (ldY == 0) ldY = n;
+  const bool wave_on = 16 * w < n;
   for (int k0 = 0; k0 < n; k0 += BO_KT) {
     __syncthreads();
     for (int e = tid; e < BO_NMAX * BO_KT; e += BO_T) {
       int r = e / BO_KT, kk = e % BO_KT;
       z2 v = {0.0, 0.0};
-      if (r < n && k0 + kk < n) v = zmul(coef, blk_get(X, n, opX, r, k0 + kk));
+      if (r < n && k0 + kk < n) v = zmul(coef, blk_get(X, ldX, opX, r, k0 + kk));
       s.X[r][kk] = v;
     }
     for (int e = tid; e < BO_KT * BO_NMAX; e += BO_T) {
       int kk = e / BO_NMAX, c = e % BO_NMAX;
       z2 v = {0.0, 0.0};
-      if (c < n && k0 + kk < n) v = blk_get(Y, n, opY, k0 + kk, c);
+      if (c < n && k0 + kk < n) v = blk_get(Y, ldY, opY, k0 + kk, c);
       s.Y[kk][c] = v;
     }
     __syncthreads();
-    const int kmax = (n - k0) < BO_KT ? (n - k0) : BO_KT;
-    for (int kk = 0; kk < kmax; kk++) {
-      z2 a[4], b[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) a[r] = s.X[ty + 16 * r][kk];
-#pragma unroll
-      for (int c = 0; c < 4; c++) b[c] = s.Y[kk][tx + 16 * c];
-#pragma unroll
-      for (int r = 0; r < 4; r++)
+    if (wave_on) {
+      const int ksteps = min(BO_KT, n - k0);
+      for (int ks = 0; ks < ksteps; ks += 4) {
+        const int kk = ks + (lane >> 4);
+        const z2 a = s.X[16 * w + (lane & 15)][kk];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-          acc.v[r][c].x = fma(a[r].x, b[c].x, fma(-a[r].y, b[c].y, acc.v[r][c].x));
-          acc.v[r][c].y = fma(a[r].x, b[c].y, fma(a[r].y, b[c].x, acc.v[r][c].y));
+          if (16 * c < n) {
+            const z2 b = s.Y[kk][16 * c + (lane & 15)];
+            acc.re[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, acc.re[c], 0, 0, 0);
+            acc.re[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, acc.re[c], 0, 0, 0);
+            acc.im[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, acc.im[c], 0, 0, 0);
+            acc.im[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, acc.im[c], 0, 0, 0);
+          }
         }
+      }
     }
   }
 }
 
 // acc += coef * op(X)   (elementwise, no product)
 __device__ __forceinline__ void blk_axpy(BlkAcc &acc, int n, const z2 *X, int opX, z2 coef) {
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int q = 0; q < 4; q++)
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      int i = ty + 16 * r, j = tx + 16 * c;
-      if (i < n && j < n) acc.v[r][c] = zadd(acc.v[r][c], zmul(coef, blk_get(X, n, opX, i, j)));
+      int i = blk_i(q), j = blk_j(c);
+      if (i < n && j < n) acc.set(q, c, zadd(acc.get(q, c), zmul(coef, blk_get(X, n, opX, i, j))));
     }
 }
 
 __device__ __forceinline__ void blk_load(BlkAcc &acc, int n, const z2 *X) {
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int q = 0; q < 4; q++)
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      int i = ty + 16 * r, j = tx + 16 * c;
-      acc.v[r][c] = (i < n && j < n) ? X[i * n + j] : z2{0.0, 0.0};
+      int i = blk_i(q), j = blk_j(c);
+      acc.set(q, c, (i < n && j < n) ? X[i * n + j] : z2{0.0, 0.0});
     }
 }
 
 __device__ __forceinline__ void blk_store(const BlkAcc &acc, int n, z2 *X) {
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int q = 0; q < 4; q++)
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      int i = ty + 16 * r, j = tx + 16 * c;
-      if (i < n && j < n) X[i * n + j] = acc.v[r][c];
+      int i = blk_i(q), j = blk_j(c);
+      if (i < n && j < n) X[i * n + j] = acc.get(q, c);
     }
 }
 
 __device__ __forceinline__ void blk_scale(BlkAcc &acc, z2 f) {
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int q = 0; q < 4; q++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) acc.v[r][c] = zmul(acc.v[r][c], f);
+    for (int c = 0; c < 4; c++) acc.set(q, c, zmul(acc.get(q, c), f));
+}
+
+// acc = diag(d) (the identity times d)
+__device__ __forceinline__ void blk_diag(BlkAcc &acc, int n, z2 d) {
+  acc.zero();
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      if (blk_i(q) == blk_j(c) && blk_i(q) < n) acc.set(q, c, d);
 }
 
 // squared Frobenius distance between acc and X, reduced over the workgroup
 // (result valid in thread 0)
 __device__ __forceinline__ double blk_dist2(const BlkAcc &acc, int n, const z2 *X, double *red) {
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  const int tid = threadIdx.x;
   double s = 0;
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int q = 0; q < 4; q++)
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      int i = ty + 16 * r, j = tx + 16 * c;
+      int i = blk_i(q), j = blk_j(c);
       if (i < n && j < n) {
-        z2 d = zsub(acc.v[r][c], X[i * n + j]);
+        z2 d = zsub(acc.get(q, c), X[i * n + j]);
         s += d.x * d.x + d.y * d.y;
       }
     }

@@ -1,0 +1,17 @@
+// Batched 64 x 64 complex products of the self-energy chain (split-bf16,
+// nao = 64); see cgemm64.hip for the operand layouts.  G = auxiliary indices
+// per workgroup (S = number of partial sums for c64_e).
+#pragma once
+#include "common.h"
+
+namespace ned {
+
+void c64_w(const uint16_t *Vh, const uint16_t *Vl, const float *At, int P, int a, int G, float *W, uint16_t *Wth,
+           uint16_t *Wtl, hipStream_t s);
+void c64_z(const uint16_t *Vh, const uint16_t *Vl, const float *Bt, int P, int a, int G, uint16_t *Zth,
+           uint16_t *Ztl, hipStream_t s);
+void c64_c(const float *Bm, const uint16_t *Wth, const uint16_t *Wtl, int P, int a, int G, uint16_t *Sh,
+           uint16_t *Sl, long long ld, hipStream_t s);
+void c64_e(const float *W, const float *Ht, int P, int a, int S, float *Ep, hipStream_t s);
+
+}  // namespace ned

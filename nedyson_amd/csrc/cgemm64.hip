@@ -1,0 +1,298 @@
+// Batched 64 x 64 complex products of the self-energy chain in split-bf16
+// (three v_mfma_f32_16x16x32_bf16 per real product, FP32 accumulation), for
+// nao = 64.  One workgroup takes one time point p and a run of G auxiliary
+// indices; the operand that all G products share is split and staged in LDS
+// once, the per-index operand is streamed, and results leave either as FP32
+// or already split into the bf16 planes that the next GEMM reads, in the
+// layout it wants (transposed through LDS where needed).  These products move
+// ~16 bytes per 64 flops, so they are HBM-bound; the point of the kernel is to
+// touch every byte once.
+//
+// Operands are K-contiguous: A as [row][k], B as [col][k], so that a lane's
+// eight consecutive K values are one 16-byte LDS read.
+//
+//   W^a = V^a A_p     -> W (FP32, [p][pl][a][i][k]) and Wt (split, [p][pl][a][k][i])
+//   Z^b = B_p^T V^b   -> Zt (split, [p][pl][b][k][i])          (V^b symmetric)
+//   C^a = B_p W^a     -> rows (p, pl, a) of the exchange operand (split, [i][k])
+//   E_p = sum_a W^a H^a over the run of a -> FP32 partial sums [p][s][pl][i][j]
+#include "common.h"
+#include "cgemm64.h"
+
+namespace ned {
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int N64 = 64, KS = 72;  // padded K stride (144 bytes: 16-byte aligned, conflict-free columns)
+
+// LDS image of one 64 x 64 operand: [plane][row][k], hi and lo bf16 parts
+struct Img {
+  uint16_t h[2][N64][KS];
+  uint16_t l[2][N64][KS];
+};
+
+__device__ __forceinline__ f32x4 mma3(u32x4_t ah, u32x4_t al, u32x4_t bh, u32x4_t bl, f32x4 c) {
+  const bf16x8 a_h = __builtin_bit_cast(bf16x8, ah), a_l = __builtin_bit_cast(bf16x8, al);
+  const bf16x8 b_h = __builtin_bit_cast(bf16x8, bh), b_l = __builtin_bit_cast(bf16x8, bl);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, b_h, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_l, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_h, c, 0, 0, 0);
+  return c;
+}
+
+__device__ __forceinline__ u32x4_t negbf(u32x4_t v) { return v ^ 0x80008000u; }
+
+// FP32 [row][k] planes (plane stride ps) -> split image
+__device__ __forceinline__ void load_f32(Img &im, const float *src, long long ps, int planes) {
+  const int tid = threadIdx.x;
+  for (int pl = 0; pl < planes; pl++)
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      const int e = tid + 256 * it, row = e >> 4, c4 = (e & 15) * 4;
+      const float4 v = *reinterpret_cast<const float4 *>(src + pl * ps + row * N64 + c4);
+      const float f[4] = {v.x, v.y, v.z, v.w};
+      uint16_t hh[4], ll[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        hh[q] = f2bf(f[q]);
+        ll[q] = f2bf(f[q] - bf2f(hh[q]));
+      }
+      *reinterpret_cast<uint2 *>(&im.h[pl][row][c4]) =
+          uint2{(uint32_t)hh[0] | ((uint32_t)hh[1] << 16), (uint32_t)hh[2] | ((uint32_t)hh[3] << 16)};
+      *reinterpret_cast<uint2 *>(&im.l[pl][row][c4]) =
+          uint2{(uint32_t)ll[0] | ((uint32_t)ll[1] << 16), (uint32_t)ll[2] | ((uint32_t)ll[3] << 16)};
+    }
+}
+
+// pre-split [row][k] planes -> image
+__device__ __forceinline__ void load_bf(Img &im, const uint16_t *hi, const uint16_t *lo, long long ps, int planes) {
+  const int tid = threadIdx.x;
+  for (int pl = 0; pl < planes; pl++)
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      const int e = tid + 256 * it, row = e >> 3, c8 = (e & 7) * 8;
+      *reinterpret_cast<uint4 *>(&im.h[pl][row][c8]) = *reinterpret_cast<const uint4 *>(hi + pl * ps + row * N64 + c8);
+      *reinterpret_cast<uint4 *>(&im.l[pl][row][c8]) = *reinterpret_cast<const uint4 *>(lo + pl * ps + row * N64 + c8);
+    }
+}
+
+// acc[j][pl] += A(rows 16w..16w+15) * B(cols 16j..16j+15), complex or real
+// operands by plane count
+template <int PA, int PB>
+__device__ __forceinline__ void mac64(f32x4 (&acc)[4][2], const Img &A, const Img &B) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ar = 16 * w + (lane & 15);
+#pragma unroll
+  for (int ks = 0; ks < N64; ks += 32) {
+    const int kq = ks + 8 * (lane >> 4);
+    u32x4_t ah[PA], al[PA];
+#pragma unroll
+    for (int pl = 0; pl < PA; pl++) {
+      ah[pl] = *reinterpret_cast<const u32x4_t *>(&A.h[pl][ar][kq]);
+      al[pl] = *reinterpret_cast<const u32x4_t *>(&A.l[pl][ar][kq]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int bc = 16 * j + (lane & 15);
+      u32x4_t bh[PB], bl[PB];
+#pragma unroll
+      for (int pl = 0; pl < PB; pl++) {
+        bh[pl] = *reinterpret_cast<const u32x4_t *>(&B.h[pl][bc][kq]);
+        bl[pl] = *reinterpret_cast<const u32x4_t *>(&B.l[pl][bc][kq]);
+      }
+      if constexpr (PA == 2 && PB == 2) {
+        acc[j][0] = mma3(ah[0], al[0], bh[0], bl[0], acc[j][0]);
+        acc[j][0] = mma3(negbf(ah[1]), negbf(al[1]), bh[1], bl[1], acc[j][0]);
+        acc[j][1] = mma3(ah[0], al[0], bh[1], bl[1], acc[j][1]);
+        acc[j][1] = mma3(ah[1], al[1], bh[0], bl[0], acc[j][1]);
+      } else if constexpr (PA == 1) {
+        acc[j][0] = mma3(ah[0], al[0], bh[0], bl[0], acc[j][0]);
+        acc[j][1] = mma3(ah[0], al[0], bh[1], bl[1], acc[j][1]);
+      } else {
+        acc[j][0] = mma3(ah[0], al[0], bh[0], bl[0], acc[j][0]);
+        acc[j][1] = mma3(ah[1], al[1], bh[0], bl[0], acc[j][1]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void zero(f32x4 (&acc)[4][2]) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) acc[j][0] = acc[j][1] = f32x4{0, 0, 0, 0};
+}
+
+// split the accumulators into an image, natural ([i][k]) or transposed ([k][i])
+__device__ __forceinline__ void stage(Img &st, const f32x4 (&acc)[4][2], bool transposed) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = 16 * w + 4 * (lane >> 4) + q, k = 16 * j + (lane & 15);
+        const float v = acc[j][pl][q];
+        const uint16_t h = f2bf(v), l = f2bf(v - bf2f(h));
+        if (transposed) { st.h[pl][k][i] = h; st.l[pl][k][i] = l; }
+        else { st.h[pl][i][k] = h; st.l[pl][i][k] = l; }
+      }
+}
+
+// image rows -> global [row][k] planes with row stride ld and plane stride ps
+__device__ __forceinline__ void store_bf(const Img &st, uint16_t *hi, uint16_t *lo, long long ld, long long ps) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      const int e = tid + 256 * it, row = e >> 3, c8 = (e & 7) * 8;
+      *reinterpret_cast<uint4 *>(hi + pl * ps + row * ld + c8) = *reinterpret_cast<const uint4 *>(&st.h[pl][row][c8]);
+      *reinterpret_cast<uint4 *>(lo + pl * ps + row * ld + c8) = *reinterpret_cast<const uint4 *>(&st.l[pl][row][c8]);
+    }
+}
+
+__device__ __forceinline__ void store_f32(float *dst, long long ps, const f32x4 (&acc)[4][2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = 16 * w + 4 * (lane >> 4) + q, k = 16 * j + (lane & 15);
+        dst[pl * ps + i * N64 + k] = acc[j][pl][q];
+      }
+}
+
+constexpr long long E2 = (long long)N64 * N64;  // elements of one plane
+
+// W^a = V^a A_p: A = V^a (real, pre-split [a][i][l]), B = A_p^T ([k][l], FP32)
+__global__ void __launch_bounds__(256) kern_c64_w(const uint16_t *__restrict__ Vh, const uint16_t *__restrict__ Vl,
+                                                  const float *__restrict__ At, int a, int G, float *__restrict__ W,
+                                                  uint16_t *__restrict__ Wth, uint16_t *__restrict__ Wtl) {
+  __shared__ __attribute__((aligned(16))) Img S, X;
+  const int p = blockIdx.y, a0 = blockIdx.x * G;
+  const long long an2 = (long long)a * E2;
+  load_f32(S, At + (size_t)p * 2 * E2, E2, 2);
+  for (int x = a0; x < a0 + G && x < a; x++) {
+    __syncthreads();
+    load_bf(X, Vh + (size_t)x * E2, Vl + (size_t)x * E2, 0, 1);
+    __syncthreads();
+    f32x4 acc[4][2];
+    zero(acc);
+    mac64<1, 2>(acc, X, S);
+    const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
+    store_f32(W + o, an2, acc);
+    __syncthreads();
+    stage(X, acc, true);
+    __syncthreads();
+    store_bf(X, Wth + o, Wtl + o, N64, an2);
+  }
+}
+
+// Z^b = B_p^T V^b: A = B_p^T ([i][l], FP32), B = V^b ([k][l] = V^b[l][k], pre-split)
+__global__ void __launch_bounds__(256) kern_c64_z(const uint16_t *__restrict__ Vh, const uint16_t *__restrict__ Vl,
+                                                  const float *__restrict__ Bt, int a, int G,
+                                                  uint16_t *__restrict__ Zth, uint16_t *__restrict__ Ztl) {
+  __shared__ __attribute__((aligned(16))) Img S, X;
+  const int p = blockIdx.y, a0 = blockIdx.x * G;
+  const long long an2 = (long long)a * E2;
+  load_f32(S, Bt + (size_t)p * 2 * E2, E2, 2);
+  for (int x = a0; x < a0 + G && x < a; x++) {
+    __syncthreads();
+    load_bf(X, Vh + (size_t)x * E2, Vl + (size_t)x * E2, 0, 1);
+    __syncthreads();
+    f32x4 acc[4][2];
+    zero(acc);
+    mac64<2, 1>(acc, S, X);
+    __syncthreads();
+    stage(X, acc, true);
+    __syncthreads();
+    const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
+    store_bf(X, Zth + o, Ztl + o, N64, an2);
+  }
+}
+
+// C^a = B_p W^a: A = B_p ([i][l], FP32), B = Wt^a ([k][l], pre-split); rows
+// (p, pl, a) of the exchange operand, row stride ld
+__global__ void __launch_bounds__(256) kern_c64_c(const float *__restrict__ Bm, const uint16_t *__restrict__ Wth,
+                                                  const uint16_t *__restrict__ Wtl, int a, int G,
+                                                  uint16_t *__restrict__ Sh, uint16_t *__restrict__ Sl, long long ld) {
+  __shared__ __attribute__((aligned(16))) Img S, X;
+  const int p = blockIdx.y, a0 = blockIdx.x * G;
+  const long long an2 = (long long)a * E2;
+  load_f32(S, Bm + (size_t)p * 2 * E2, E2, 2);
+  for (int x = a0; x < a0 + G && x < a; x++) {
+    __syncthreads();
+    const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
+    load_bf(X, Wth + o, Wtl + o, an2, 2);
+    __syncthreads();
+    f32x4 acc[4][2];
+    zero(acc);
+    mac64<2, 2>(acc, S, X);
+    __syncthreads();
+    stage(X, acc, false);
+    __syncthreads();
+    // the 64 x 64 block is one row of the operand: image row i -> columns i*64..
+    const size_t r0 = ((size_t)p * 2 * a + x) * ld;  // plane 0 row; plane 1 is a rows further
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+      for (int it = 0; it < 2; it++) {
+        const int e = tid + 256 * it, row = e >> 3, c8 = (e & 7) * 8;
+        const size_t go = r0 + (size_t)pl * a * ld + row * N64 + c8;
+        *reinterpret_cast<uint4 *>(Sh + go) = *reinterpret_cast<const uint4 *>(&X.h[pl][row][c8]);
+        *reinterpret_cast<uint4 *>(Sl + go) = *reinterpret_cast<const uint4 *>(&X.l[pl][row][c8]);
+      }
+  }
+}
+
+// partial E over a in [a0, a0+G): A = W^a ([i][k], FP32), B = Ht^a ([j][k], FP32)
+__global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, const float *__restrict__ Ht, int a,
+                                                  int G, float *__restrict__ Ep) {
+  __shared__ __attribute__((aligned(16))) Img S, X;
+  const int p = blockIdx.y, s = blockIdx.x, a0 = s * G;
+  const long long an2 = (long long)a * E2;
+  f32x4 acc[4][2];
+  zero(acc);
+  for (int x = a0; x < a0 + G && x < a; x++) {
+    __syncthreads();
+    const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
+    load_f32(S, W + o, an2, 2);
+    load_f32(X, Ht + o, an2, 2);
+    __syncthreads();
+    mac64<2, 2>(acc, S, X);
+  }
+  store_f32(Ep + ((size_t)p * gridDim.x + s) * 2 * E2, E2, acc);
+}
+
+}  // namespace
+
+void c64_w(const uint16_t *Vh, const uint16_t *Vl, const float *At, int P, int a, int G, float *W, uint16_t *Wth,
+           uint16_t *Wtl, hipStream_t s) {
+  hipLaunchKernelGGL(kern_c64_w, dim3((a + G - 1) / G, P), dim3(256), 0, s, Vh, Vl, At, a, G, W, Wth, Wtl);
+  check_launch("c64_w");
+}
+
+void c64_z(const uint16_t *Vh, const uint16_t *Vl, const float *Bt, int P, int a, int G, uint16_t *Zth,
+           uint16_t *Ztl, hipStream_t s) {
+  hipLaunchKernelGGL(kern_c64_z, dim3((a + G - 1) / G, P), dim3(256), 0, s, Vh, Vl, Bt, a, G, Zth, Ztl);
+  check_launch("c64_z");
+}
+
+void c64_c(const float *Bm, const uint16_t *Wth, const uint16_t *Wtl, int P, int a, int G, uint16_t *Sh,
+           uint16_t *Sl, long long ld, hipStream_t s) {
+  hipLaunchKernelGGL(kern_c64_c, dim3((a + G - 1) / G, P), dim3(256), 0, s, Bm, Wth, Wtl, a, G, Sh, Sl, ld);
+  check_launch("c64_c");
+}
+
+void c64_e(const float *W, const float *Ht, int P, int a, int S, float *Ep, hipStream_t s) {
+  if (a % S) throw std::invalid_argument("c64_e: naux must be a multiple of the split count");
+  hipLaunchKernelGGL(kern_c64_e, dim3(S, P), dim3(256), 0, s, W, Ht, a, a / S, Ep);
+  check_launch("c64_e");
+}
+
+}  // namespace ned

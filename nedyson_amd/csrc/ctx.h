@@ -51,9 +51,11 @@ struct Ctx {
   // the bubble's Pi V product rides along the exchange as nao^2 + naux extra K
   DevBuf Uah, Ual;
   bool uaug = false;  // Baug built (nao^2 and 2 naux multiples of 128, naux of 64)
+  bool fast64 = false;  // nao = 64 as well: the batched 64 x 64 kernels carry the chain
+  DevBuf Vh, Vl;        // B[Q][i][l] split into bf16 hi / lo (fast64)
   bool have_coulomb = false;
   // Sigma scratch
-  DevBuf sg_items_A, sg_items_B, sg_W, sg_X, sg_H, sg_Pi, sg_E, sg_Ep, sg_S1h, sg_S1l, sg_Pf;
+  DevBuf sg_items_A, sg_items_B, sg_items_Bt, sg_W, sg_X, sg_H, sg_Pi, sg_E, sg_Ep, sg_S1h, sg_S1l, sg_Pf;
   int sigma_chunk = 0;  // items per chunk (0 = choose automatically)
   // Dyson state
   DevBuf mixK;                // mixing kernel of G^M (np x np blocks)
@@ -68,7 +70,7 @@ struct Ctx {
   // inverses of the k x k block start-up systems: the retarded one depends on
   // times T-k..T-1 only (valid for every corrector pass of step retinv_T), the
   // lesser one on times 0..k only (valid until the early history changes)
-  DevBuf dy_retinv, dy_lesinv, dy_Xs;
+  DevBuf dy_retinv, dy_lesinv, dy_Xs, dy_pinv;
   int retinv_T = -1;
   DevBuf wtab_d;  // integrator tables of order wtab_k
   int wtab_k = -1;

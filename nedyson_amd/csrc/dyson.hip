@@ -49,6 +49,9 @@ const WTab *dev_wtab(Ctx &c) {
 
 namespace {
 
+// blocks per panel of the blocked Volterra recurrences
+constexpr int RPANEL = 16;
+
 __device__ __forceinline__ size_t dtri(int t) { return (size_t)t * (t + 1) / 2; }
 
 // gregory_weights(n, j), src/integration.h:78-91
@@ -144,6 +147,129 @@ __global__ void __launch_bounds__(256) kern_gesv(z2 *A, long long sA, z2 *B, lon
   }
 }
 
+// The same elimination for N, R <= 64 with both matrices resident in LDS
+// (one workgroup per system).
+constexpr int LDS_N = 64;
+__global__ void __launch_bounds__(256) kern_gesv_lds(z2 *A, long long sA, z2 *B, long long sB, int N, int R, int *info) {
+  __shared__ z2 a[LDS_N][LDS_N + 1];
+  __shared__ z2 b[LDS_N][LDS_N + 1];
+  __shared__ double rv[64];
+  __shared__ int ri[64];
+  __shared__ z2 f[LDS_N];
+  z2 *ga = A + (size_t)blockIdx.x * sA, *gb = B + (size_t)blockIdx.x * sB;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < N * N; e += 256) a[e / N][e % N] = ga[e];
+  for (int e = tid; e < N * R; e += 256) b[e / R][e % R] = gb[e];
+  __syncthreads();
+  for (int c = 0; c < N; c++) {
+    if (tid < 64) {
+      double best = -1; int bi = c;
+      const int r = c + tid;
+      if (r < N) { best = fabs(a[r][c].x) + fabs(a[r][c].y); bi = r; }
+      rv[tid] = best; ri[tid] = bi;
+    }
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+      if (tid < w && rv[tid + w] > rv[tid]) { rv[tid] = rv[tid + w]; ri[tid] = ri[tid + w]; }
+      __syncthreads();
+    }
+    const int p = ri[0];
+    if (rv[0] <= 0.0 && tid == 0) atomicAdd(info, 1);
+    if (p != c) {
+      for (int j = tid; j < N; j += 256) { z2 t = a[c][j]; a[c][j] = a[p][j]; a[p][j] = t; }
+      for (int j = tid; j < R; j += 256) { z2 t = b[c][j]; b[c][j] = b[p][j]; b[p][j] = t; }
+    }
+    __syncthreads();
+    const z2 piv = a[c][c];
+    const double den = piv.x * piv.x + piv.y * piv.y;
+    const z2 inv = den > 0 ? z2{piv.x / den, -piv.y / den} : z2{0.0, 0.0};
+    for (int r = c + 1 + tid; r < N; r += 256) f[r] = zmul(a[r][c], inv);
+    __syncthreads();
+    const int rows = N - c - 1, colsA = N - c - 1, w = colsA + R;
+    for (int e = tid; e < rows * w; e += 256) {
+      const int r = c + 1 + e / w, j = e % w;
+      const z2 fr = f[r];
+      if (j < colsA) a[r][c + 1 + j] = zsub(a[r][c + 1 + j], zmul(fr, a[c][c + 1 + j]));
+      else b[r][j - colsA] = zsub(b[r][j - colsA], zmul(fr, b[c][j - colsA]));
+    }
+    __syncthreads();
+  }
+  for (int c = N - 1; c >= 0; c--) {
+    const z2 piv = a[c][c];
+    const double den = piv.x * piv.x + piv.y * piv.y;
+    const z2 inv = den > 0 ? z2{piv.x / den, -piv.y / den} : z2{0.0, 0.0};
+    for (int j = tid; j < R; j += 256) b[c][j] = zmul(b[c][j], inv);
+    __syncthreads();
+    for (int e = tid; e < c * R; e += 256) {
+      const int r = e / R, j = e % R;
+      b[r][j] = zsub(b[r][j], zmul(a[r][c], b[c][j]));
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < N * R; e += 256) gb[e] = b[e / R][e % R];
+}
+
+// Gauss-Jordan with partial pivoting for one large system, one column per
+// pair of launches: the pivot search and row swap (one workgroup), then the
+// elimination of that column from every other row across the whole chip.
+// Columns left of the pivot are never read again and are not updated.
+__global__ void __launch_bounds__(256) kern_gj_pivot(z2 *A, z2 *B, int N, int R, int c, z2 *pinv, int *info) {
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  const int tid = threadIdx.x;
+  double best = -1; int bi = c;
+  for (int r = c + tid; r < N; r += 256) {
+    const z2 v = A[(size_t)r * N + c];
+    const double m = fabs(v.x) + fabs(v.y);
+    if (m > best) { best = m; bi = r; }
+  }
+  rv[tid] = best; ri[tid] = bi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w && rv[tid + w] > rv[tid]) { rv[tid] = rv[tid + w]; ri[tid] = ri[tid + w]; }
+    __syncthreads();
+  }
+  const int p = ri[0];
+  if (rv[0] <= 0.0 && tid == 0) atomicAdd(info, 1);
+  if (p != c) {
+    for (int j = c + tid; j < N; j += 256) {
+      z2 t = A[(size_t)c * N + j]; A[(size_t)c * N + j] = A[(size_t)p * N + j]; A[(size_t)p * N + j] = t;
+    }
+    for (int j = tid; j < R; j += 256) {
+      z2 t = B[(size_t)c * R + j]; B[(size_t)c * R + j] = B[(size_t)p * R + j]; B[(size_t)p * R + j] = t;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const z2 piv = A[(size_t)c * N + c];
+    const double den = piv.x * piv.x + piv.y * piv.y;
+    pinv[c] = den > 0 ? z2{piv.x / den, -piv.y / den} : z2{0.0, 0.0};
+  }
+}
+
+__global__ void kern_gj_elim(z2 *A, z2 *B, int N, int R, int c, const z2 *pinv) {
+  const int wA = N - c - 1, w = wA + R;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)(N - 1) * w) return;
+  int r = (int)(e / w);
+  if (r >= c) r++;  // every row but the pivot row
+  const int j = (int)(e % w);
+  const z2 f = zmul(A[(size_t)r * N + c], pinv[c]);
+  if (j < wA) {
+    const int jj = c + 1 + j;
+    A[(size_t)r * N + jj] = zsub(A[(size_t)r * N + jj], zmul(f, A[(size_t)c * N + jj]));
+  } else {
+    const int jj = j - wA;
+    B[(size_t)r * R + jj] = zsub(B[(size_t)r * R + jj], zmul(f, B[(size_t)c * R + jj]));
+  }
+}
+
+__global__ void kern_gj_scale(z2 *B, int N, int R, const z2 *pinv) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)N * R) return;
+  B[e] = zmul(B[e], pinv[e / R]);
+}
+
 // M_j for the listed j into A (and identity into B), for inversion
 __global__ void kern_build_mj(Pt p, const int *js, int cnt, double bd0, double om0, z2 *A, z2 *B) {
   const size_t es = p.es();
@@ -233,15 +359,10 @@ __global__ void __launch_bounds__(256) kern_ret_start_store(Pt p, const z2 *X, d
   __shared__ double red[256];
   const int n = p.n, l = blockIdx.x;
   BlkAcc acc;
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  const int tid = threadIdx.x;
   const z2 *x = X + (size_t)l * n * n;
-#pragma unroll
-  for (int r = 0; r < 4; r++)
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      int i = ty + 16 * r, j = tx + 16 * c;
-      acc.v[r][c] = (i < n && j < n) ? x[j * n + i] : z2{0.0, 0.0};
-    }
+  acc.zero();
+  blk_axpy(acc, n, x, 2, z2{1.0, 0.0});
   z2 *g = p.gr(p.T, p.T - l - 1);
   double d2 = blk_dist2(acc, n, g, red);
   blk_store(acc, n, g);
@@ -294,6 +415,22 @@ __global__ void __launch_bounds__(256) kern_ret_step(Pt p, const WTab *__restric
   blk_mac(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
   if (m == nn + 1) ret_finalize(p, w, acc, s, m, Qm, Minv, red, err);
   else blk_store(acc, n, Qm);
+}
+
+// contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
+// the history sums stay in registers across the whole panel
+__global__ void __launch_bounds__(256) kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q) {
+  const WTab &w = *wtab;
+  __shared__ BlkLds s;
+  const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
+  if (m > T) return;
+  z2 *Qm = Q + (size_t)m * n * n;
+  BlkAcc acc;
+  blk_load(acc, n, Qm);
+  for (int nn = q0; nn < q1; nn++)
+    blk_mac(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
+  __syncthreads();
+  blk_store(acc, n, Qm);
 }
 
 // ------------------------------------------------------------ mixing (tv)
@@ -504,6 +641,22 @@ __global__ void __launch_bounds__(256) kern_les_step(Pt p, const WTab *__restric
   blk_mac(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
   if (m == l + 1 && m <= T - 1) les_finalize(p, w, acc, s, m, Qm, X, Minv);
   else blk_store(acc, n, Qm);
+}
+
+// contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
+__global__ void __launch_bounds__(256) kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q,
+                                                      const z2 *X) {
+  const WTab &w = *wtab;
+  __shared__ BlkLds s;
+  const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
+  if (m > T) return;
+  z2 *Qm = Q + (size_t)m * p.es();
+  BlkAcc acc;
+  blk_load(acc, n, Qm);
+  for (int l = q0; l < q1; l++)
+    blk_mac(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+  __syncthreads();
+  blk_store(acc, n, Qm);
 }
 
 // diagonal step, src/dyson_step.h:283-310
@@ -746,30 +899,11 @@ __global__ void __launch_bounds__(256) kern_boot_apply(Pt p, int k, const z2 *Mi
   // Stage it as a dense n x n block in the part of R reserved for scratch.
   BlkAcc acc;
   acc.zero();
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  const int tid = threadIdx.x;
   for (int np2 = 0; np2 < k; np2++) {
     const z2 *Rb = R + ((size_t)np2 * np + m) * p.es();
-    for (int kk0 = 0; kk0 < n; kk0 += BO_KT) {
-      __syncthreads();
-      for (int e = tid; e < BO_NMAX * BO_KT; e += 256) {
-        int r = e / BO_KT, kk = e % BO_KT;
-        s.X[r][kk] = (r < n && kk0 + kk < n) ? Minv[(size_t)(l * n + r) * kn + np2 * n + kk0 + kk] : z2{0.0, 0.0};
-      }
-      for (int e = tid; e < BO_KT * BO_NMAX; e += 256) {
-        int kk = e / BO_NMAX, c = e % BO_NMAX;
-        s.Y[kk][c] = (c < n && kk0 + kk < n) ? Rb[(kk0 + kk) * n + c] : z2{0.0, 0.0};
-      }
-      __syncthreads();
-      const int kmax = min(BO_KT, n - kk0);
-      for (int kk = 0; kk < kmax; kk++)
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-          for (int c = 0; c < 4; c++) {
-            z2 a = s.X[ty + 16 * r][kk], b = s.Y[kk][tx + 16 * c];
-            acc.v[r][c] = zadd(acc.v[r][c], zmul(a, b));
-          }
-    }
+    // block (l, np2) of Minv: rows l*n.., columns np2*n.., row stride kn
+    blk_mac(acc, s, n, Minv + (size_t)l * n * kn + (size_t)np2 * n, 0, Rb, 0, z2{1.0, 0.0}, kn, n);
   }
   z2 *g = p.gtv(l + 1, m);
   double d2 = blk_dist2(acc, n, g, red);
@@ -801,12 +935,7 @@ __global__ void __launch_bounds__(256) kern_boot_retles(Pt p, int k, double *err
   if (kind == 0) {
     const int nn = A, l = Bi;
     if (nn == 0) {
-      const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-          if (ty + 16 * r == tx + 16 * c) acc.v[r][c] = {0.0, -1.0};
+      blk_diag(acc, n, z2{0.0, -1.0});
     } else {
       blk_axpy(acc, n, p.gtv(nn, p.np - 1), 0, z2{(double)p.sig, 0.0});
       blk_axpy(acc, n, p.gtv(nn, 0), 0, z2{-1.0, 0.0});
@@ -859,7 +988,20 @@ void gesv(Ctx &c, z2 *A, long long sA, z2 *B, long long sB, int N, int R, int ns
   require(N <= 320, "linear system larger than 320");
   int *info = c.dy_info.as<int>();
   NED_HIP(hipMemsetAsync(info, 0, sizeof(int), c.stream));
-  hipLaunchKernelGGL(kern_gesv, dim3(nsys), dim3(256), 0, c.stream, A, sA, B, sB, N, R, info);
+  if (N <= LDS_N && R <= LDS_N) {
+    hipLaunchKernelGGL(kern_gesv_lds, dim3(nsys), dim3(256), 0, c.stream, A, sA, B, sB, N, R, info);
+  } else if (nsys == 1) {
+    c.dy_pinv.ensure((size_t)N * sizeof(z2));
+    z2 *pinv = c.dy_pinv.as<z2>();
+    for (int col = 0; col < N; col++) {
+      hipLaunchKernelGGL(kern_gj_pivot, dim3(1), dim3(256), 0, c.stream, A, B, N, R, col, pinv, info);
+      const long long work = (long long)(N - 1) * (N - col - 1 + R);
+      hipLaunchKernelGGL(kern_gj_elim, dim3(nblk(work)), dim3(256), 0, c.stream, A, B, N, R, col, pinv);
+    }
+    hipLaunchKernelGGL(kern_gj_scale, dim3(nblk((long long)N * R)), dim3(256), 0, c.stream, B, N, R, pinv);
+  } else {
+    hipLaunchKernelGGL(kern_gesv, dim3(nsys), dim3(256), 0, c.stream, A, sA, B, sB, N, R, info);
+  }
   check_launch("gesv");
 }
 
@@ -972,8 +1114,16 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   check_launch("ret start");
   hipLaunchKernelGGL(kern_ret_start_store, dim3(k), dim3(256), 0, s, p, Xs, err + 0);
   hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, Minv, err + 0);
-  for (int nn = k + 1; nn < T; nn++)
-    hipLaunchKernelGGL(kern_ret_step, dim3(T - nn), dim3(256), 0, s, p, wd, nn, Q, Minv, err + 0);
+  // blocked right-looking recurrence: inside a panel of RPANEL blocks each
+  // finished block updates only the panel's own later blocks (small launches
+  // along the dependency chain); then one launch applies the whole panel to
+  // every later block
+  for (int q0 = k + 1; q0 < T; q0 += RPANEL) {
+    const int q1 = std::min(q0 + RPANEL, T);
+    for (int nn = q0; nn < q1; nn++)
+      hipLaunchKernelGGL(kern_ret_step, dim3(q1 - nn), dim3(256), 0, s, p, wd, nn, Q, Minv, err + 0);
+    if (q1 < T) hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q);
+  }
   check_launch("ret history");
 
   // ---- right mixing (src/dyson_step.h:130-169)
@@ -1010,8 +1160,12 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_lesinv.as<z2>(), Qs, kn, n,
                        X + es);
     hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, X, Minv);
-    for (int l = k + 1; l < T; l++)
-      hipLaunchKernelGGL(kern_les_step, dim3(T - l), dim3(256), 0, s, p, wd, l, Q, X, Minv);
+    for (int q0 = k + 1; q0 < T; q0 += RPANEL) {
+      const int q1 = std::min(q0 + RPANEL, T);
+      for (int l = q0; l < q1; l++)
+        hipLaunchKernelGGL(kern_les_step, dim3(q1 - l), dim3(256), 0, s, p, wd, l, Q, X, Minv);
+      if (q1 < T) hipLaunchKernelGGL(kern_les_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q, X);
+    }
     hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, Q, X, c.dy_scr.as<z2>());
     hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, X, err + 2);
     check_launch("les");

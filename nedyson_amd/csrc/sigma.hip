@@ -21,6 +21,7 @@
 // four-term expression of lines 620-624) and Sigma^<(t,T) = -Sigma^<(T,t)^H,
 // which holds for symmetric V^Q (real orbitals; checked at upload).
 #include "ctx.h"
+#include "cgemm64.h"
 #include <cmath>
 #include <algorithm>
 
@@ -48,12 +49,16 @@ __global__ void kern_build_U(const double *Bq, int n, int a, double *U64, float 
 }
 
 // Baug[(k,j)][c] = -U'[c][(k,j)] for c < n2, 2 B[c-n2][k][j] for c >= n2,
-// split into bf16 hi / lo planes (row stride n2 + a)
-__global__ void kern_build_Uaug(const float *U, const float *Bq, int n2, int a, uint16_t *hi, uint16_t *lo) {
+// split into bf16 hi / lo planes (row stride n2 + a).  With swap_rows the
+// rows are ordered (j,k) instead, so that the GEMM writes each H^a transposed.
+__global__ void kern_build_Uaug(const float *U, const float *Bq, int n, int a, int swap_rows, uint16_t *hi,
+                                uint16_t *lo) {
+  const int n2 = n * n;
   const long long ka = n2 + a;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)n2 * ka) return;
-  const long long r = idx / ka, col = idx % ka;
+  const long long rr = idx / ka, col = idx % ka;
+  const long long r = swap_rows ? (rr % n) * n + rr / n : rr;
   const float v = col < n2 ? -U[col * n2 + r] : 2.0f * Bq[(col - n2) * n2 + r];
   const uint16_t h = f2bf(v);
   hi[idx] = h;
@@ -99,7 +104,7 @@ __global__ void kern_d2f(const double *x, float *y, long long n) {
 
 template <typename T>
 __global__ void kern_items(const z2 *Gret, const z2 *Gles, const z2 *Gtv, int T_, int n, int np, int sig, int p0,
-                           int P, T *Ap, T *Bp) {
+                           int P, T *Ap, T *Bp, int transA, T *Btp) {
   const int n2 = n * n;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)P * n2) return;
@@ -126,9 +131,16 @@ __global__ void kern_items(const z2 *Gret, const z2 *Gles, const z2 *Gtv, int T_
     z2 bk = Gb[k * n + i];
     B = {-(double)sig * bk.x, (double)sig * bk.y};  // -sig conj
   }
+  // A as stored, or transposed (A^T[k][i] = A[i][k]); B as stored, and
+  // optionally also transposed
   T *a = Ap + (size_t)p * 2 * n2, *b = Bp + (size_t)p * 2 * n2;
-  a[e] = (T)A.x; a[n2 + e] = (T)A.y;
+  const int ea = transA ? k * n + i : e;
+  a[ea] = (T)A.x; a[n2 + ea] = (T)A.y;
   b[e] = (T)B.x; b[n2 + e] = (T)B.y;
+  if (Btp) {
+    T *bt = Btp + (size_t)p * 2 * n2;
+    bt[k * n + i] = (T)B.x; bt[n2 + k * n + i] = (T)B.y;
+  }
 }
 
 template <typename T>
@@ -158,7 +170,7 @@ __global__ void kern_scatter(const T *E, int T_, int n, int np, z2 *Sret, z2 *Sl
 
 template <typename T>
 struct Bufs {
-  T *A, *B, *W, *X, *H, *Pi, *E, *Ep;
+  T *A, *B, *Bt, *W, *X, *H, *Pi, *E, *Ep;
   uint16_t *S1h, *S1l;  // split-bf16 operands (bf16x3 path)
   float *Pf;            // [Wr; Wi][Zr; Zi]^T per item (bf16x3 path)
 };
@@ -178,6 +190,39 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
   const int n = c.nao, a = c.naux;
   const long long n2 = (long long)n * n, an2 = (long long)a * n2;
   hipStream_t s = c.stream;
+  if (c.fast64 && prec == PREC_BF16X3) {
+    // nao = 64: every product of the chain on split-bf16 MFMA, each operand
+    // split once, intermediates handed over in the layout the next step reads.
+    // kern_items left A^T in A, B in B and B^T in Bt.
+    constexpr int G = 16;  // aux indices per workgroup
+    const long long ka = n2 + a;
+    const int M = 2 * a * P, NN = (int)n2;
+    float *W = (float *)bf.W, *H = (float *)bf.H;
+    uint16_t *wth = reinterpret_cast<uint16_t *>(H), *wtl = wth + (size_t)P * 2 * an2;  // Wt lives in H until H is formed
+    c64_w(c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (const float *)bf.A, P, a, G, W, wth, wtl, s);
+    c64_z(c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (const float *)bf.Bt, P, a, G, bf.S1h, bf.S1l, s);
+    // Pf_p = [Wt_r; Wt_i] [Zt_r; Zt_i]^T (K index (k,i) in both)
+    gemm_nt_bf16x3(NtOp{wth, wtl, n2, 2 * an2}, NtOp{bf.S1h, bf.S1l, n2, 2 * an2}, bf.Pf, 2LL * a, 4LL * a * a, 2 * a,
+                   2 * a, NN, P, 1.0f, 0, s);
+    long long tot = (long long)P * 2 * a * a;
+    hipLaunchKernelGGL(kern_pi_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Pf, a, P, bf.S1h,
+                       bf.S1l, ka, NN);
+    check_launch("sigma pi combine");
+    c64_c((const float *)bf.B, wth, wtl, P, a, G, bf.S1h, bf.S1l, ka, s);
+    // Ht^a = ([C | Pi] Baug^T) with Baug rows ordered (j,k): H^a transposed
+    c.mark(0);
+    gemm_nt_bf16x3(NtOp{bf.S1h, bf.S1l, ka, 0}, NtOp{c.Uah.as<uint16_t>(), c.Ual.as<uint16_t>(), ka, 0}, H, n2, 0, M,
+                   NN, (int)ka, 1, 1.0f, 0, s);
+    c.mark(1);
+    if (c.prof) c.prof_flops += 2.0 * M * (double)NN * (double)ka;
+    const int S = e_splits(a);
+    c64_e(W, H, P, a, S, (float *)bf.Ep, s);
+    tot = (long long)P * 2 * n2;
+    hipLaunchKernelGGL(kern_sum_splits<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, P, 2 * n2,
+                       bf.E + (size_t)p0 * 2 * n2);
+    check_launch("sigma E sum");
+    return;
+  }
   // W_p = Vstack A_p : (a n x n)(n x n), Vstack[(a,i)][l] = B[a][i][l]
   {
     GemmDesc d;
@@ -288,11 +333,13 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   const long long n2 = (long long)n * n, an2 = (long long)a * n2;
   const int Ptot = 2 * (tstp + 1) + np;
   const bool fast = fast_bf16(c, prec);
+  const bool f64w = fast && c.fast64;
   const int S = e_splits(a);
   const long long ka = n2 + a;
-  // device scratch per item: items, W, X, H, Pi (generic path), E partials,
-  // and on the bf16x3 path the split operand S1 and the Pi product Pf
-  size_t per_item = (size_t)(3 * 2 * an2 + 4 * n2 + (long long)S * 2 * n2) * sizeof(T);
+  // device scratch per item: items, W, X (not on the nao = 64 path), H, Pi
+  // (generic path), E partials, and on the bf16x3 path the split operand S1
+  // and the Pi product Pf
+  size_t per_item = (size_t)((f64w ? 2 : 3) * 2 * an2 + (f64w ? 6 : 4) * n2 + (long long)S * 2 * n2) * sizeof(T);
   if (fast) per_item += (size_t)2 * a * ka * 2 * sizeof(uint16_t) + (size_t)4 * a * a * sizeof(float);
   else per_item += (size_t)2 * a * a * sizeof(T);
   const size_t budget = (size_t)4 << 30;
@@ -301,7 +348,8 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   c.sg_items_A.ensure((size_t)P * 2 * n2 * sizeof(T));
   c.sg_items_B.ensure((size_t)P * 2 * n2 * sizeof(T));
   c.sg_W.ensure((size_t)P * 2 * an2 * sizeof(T));
-  c.sg_X.ensure((size_t)P * 2 * an2 * sizeof(T));
+  if (f64w) c.sg_items_Bt.ensure((size_t)P * 2 * n2 * sizeof(T));
+  else c.sg_X.ensure((size_t)P * 2 * an2 * sizeof(T));
   c.sg_H.ensure((size_t)P * 2 * an2 * sizeof(T));
   c.sg_E.ensure((size_t)Ptot * 2 * n2 * sizeof(T));
   c.sg_Ep.ensure((size_t)P * S * 2 * n2 * sizeof(T));
@@ -312,7 +360,8 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   } else {
     c.sg_Pi.ensure((size_t)P * 2 * a * a * sizeof(T));
   }
-  Bufs<T> bf{c.sg_items_A.as<T>(), c.sg_items_B.as<T>(), c.sg_W.as<T>(), c.sg_X.as<T>(), c.sg_H.as<T>(),
+  Bufs<T> bf{c.sg_items_A.as<T>(), c.sg_items_B.as<T>(), f64w ? c.sg_items_Bt.as<T>() : nullptr, c.sg_W.as<T>(),
+             c.sg_X.as<T>(), c.sg_H.as<T>(),
              c.sg_Pi.as<T>(), c.sg_E.as<T>(), c.sg_Ep.as<T>(), c.sg_S1h.as<uint16_t>(), c.sg_S1l.as<uint16_t>(),
              c.sg_Pf.as<float>()};
   const T *Vq = sizeof(T) == 8 ? (const T *)c.Bq.p : (const T *)c.Bq32.p;
@@ -321,7 +370,8 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
     int Pc = std::min(P, Ptot - p0);
     long long tot = (long long)Pc * n2;
     hipLaunchKernelGGL(kern_items<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, c.G.ret.as<z2>(),
-                       c.G.les.as<z2>(), c.G.tv.as<z2>(), tstp, n, np, c.sig, p0, Pc, bf.A, bf.B);
+                       c.G.les.as<z2>(), c.G.tv.as<z2>(), tstp, n, np, c.sig, p0, Pc, bf.A, bf.B, f64w ? 1 : 0,
+                       bf.Bt);
     check_launch("sigma items");
     run_chunk<T>(c, prec, p0, Pc, bf, Vq, U);
   }
@@ -357,14 +407,21 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
                      c.U64.as<double>(), c.U32.as<float>());
   check_launch("coulomb U");
   c.uaug = n2 % 128 == 0 && (2 * a) % 128 == 0 && a % 64 == 0;
+  c.fast64 = c.uaug && n == 64;
   if (c.uaug) {
     const size_t ka = n2 + a;
     c.Uah.ensure(n2 * ka * sizeof(uint16_t));
     c.Ual.ensure(n2 * ka * sizeof(uint16_t));
     long long nt_ = (long long)(n2 * ka);
     hipLaunchKernelGGL(kern_build_Uaug, dim3((unsigned)((nt_ + 255) / 256)), dim3(256), 0, c.stream, c.U32.as<float>(),
-                       c.Bq32.as<float>(), (int)n2, a, c.Uah.as<uint16_t>(), c.Ual.as<uint16_t>());
+                       c.Bq32.as<float>(), n, a, c.fast64 ? 1 : 0, c.Uah.as<uint16_t>(), c.Ual.as<uint16_t>());
     check_launch("coulomb Baug");
+  }
+  if (c.fast64) {
+    c.Vh.ensure(a * n2 * sizeof(uint16_t));
+    c.Vl.ensure(a * n2 * sizeof(uint16_t));
+    split_bf16(c.Bq32.as<float>(), (long long)n2, c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (long long)n2, a, (int)n2,
+               c.stream);
   }
   NED_HIP(hipStreamSynchronize(c.stream));
   c.have_coulomb = true;

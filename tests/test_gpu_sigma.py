@@ -91,3 +91,25 @@ def test_sigma_wrapper_reference_signature(orc):
         orc.sigma_solve(V, t, S_ref, G)
     for c in ("ret", "les", "tv"):
         assert np.max(np.abs(getattr(S, c) - getattr(S_ref, c))) < 1e-12
+
+
+def test_sigma_parity_nao64(orc):
+    """nao = 64: the bf16x3 chain runs on the batched 64 x 64 kernels (W, Z,
+    C, E) with the fused exchange GEMM; FP32 and FP64 take the generic path.
+    One slice against the oracle (OpenMP over its time points)."""
+    import os
+    rng = np.random.default_rng(64)
+    nao, naux, nt, ntau = 64, 64, 2, 3
+    V, B = sym_V(rng, nao, naux)
+    G = random_green(rng, nt, ntau, nao, scale=0.2)
+    S_ref = GreenFunc(nt, ntau, nao)
+    orc.sigma_solve(V, nt, S_ref, G, nthreads=min(os.cpu_count() or 1, 16))
+    for prec in ("bf16x3", "f32", "f64"):
+        print(f"[sigma parity] nao={nao} naux={naux} prec={prec}", flush=True)
+        S = _sigma_gpu(V, B, G, nt, prec)
+        for name, a, b in (("ret", S.ret_row(nt), S_ref.ret_row(nt)),
+                           ("les", S.les_col(nt), S_ref.les_col(nt)),
+                           ("tv", S.tv[nt], S_ref.tv[nt])):
+            scale = max(np.max(np.abs(b)), 1e-30)
+            err = np.max(np.abs(a - b))
+            assert err <= TOL[prec] * scale, (prec, name, err, scale)
