@@ -165,3 +165,114 @@ __device__ __forceinline__ double blk_dist2(const BlkAcc &acc, int n, const z2 *
 }
 
 }  // namespace ned
+
+namespace ned {
+
+// ---------------------------------------------------------------- tiles
+// One 16 x 16 output tile per wave at (r0, c0) (r0, c0 multiples of 16), in
+// the same lane layout as BlkAcc.  Used where one block is split over several
+// workgroups: the retarded recurrence is separable by rows and the lesser one
+// by columns, so a 16-row (16-column) band of a block runs on its own
+// workgroup with its four waves on the four column (row) tiles.
+struct TileAcc {
+  blk_v4d re, im;
+  __device__ void zero() { re = im = blk_v4d{0.0, 0.0, 0.0, 0.0}; }
+  __device__ z2 get(int q) const { return z2{re[q], im[q]}; }
+  __device__ void set(int q, z2 v) {
+    re[q] = v.x;
+    im[q] = v.y;
+  }
+};
+
+__device__ __forceinline__ int tile_i(int r0, int q) { return r0 + 4 * q + ((threadIdx.x & 63) >> 4); }
+__device__ __forceinline__ int tile_j(int c0) { return c0 + (threadIdx.x & 15); }
+
+// acc(r0.., c0..) += coef * op(L)[r0.., :] op(R)[:, c0..], staging only rows
+// [lr0, lr0+lrn) of op(L) and columns [rc0, rc0+rcn) of op(R) (lrn, rcn <= 64)
+__device__ __forceinline__ void tile_mac(TileAcc &acc, BlkLds &s, int n, const z2 *L, int opL, int lr0, int lrn,
+                                         const z2 *R, int opR, int rc0, int rcn, z2 coef, int r0, int c0) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const bool on = r0 < n && c0 < n;
+  for (int k0 = 0; k0 < n; k0 += BO_KT) {
+    __syncthreads();
+    for (int e = tid; e < lrn * BO_KT; e += BO_T) {
+      const int r = e / BO_KT, kk = e % BO_KT;
+      z2 v = {0.0, 0.0};
+      if (lr0 + r < n && k0 + kk < n) v = zmul(coef, blk_get(L, n, opL, lr0 + r, k0 + kk));
+      s.X[r][kk] = v;
+    }
+    for (int e = tid; e < BO_KT * rcn; e += BO_T) {
+      const int kk = e / rcn, c = e % rcn;
+      z2 v = {0.0, 0.0};
+      if (rc0 + c < n && k0 + kk < n) v = blk_get(R, n, opR, k0 + kk, rc0 + c);
+      s.Y[kk][c] = v;
+    }
+    __syncthreads();
+    if (on) {
+      const int ksteps = min(BO_KT, n - k0);
+      for (int ks = 0; ks < ksteps; ks += 4) {
+        const int kk = ks + (lane >> 4);
+        const z2 a = s.X[r0 - lr0 + (lane & 15)][kk];
+        const z2 b = s.Y[kk][c0 - rc0 + (lane & 15)];
+        acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, acc.re, 0, 0, 0);
+        acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, acc.re, 0, 0, 0);
+        acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, acc.im, 0, 0, 0);
+        acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, acc.im, 0, 0, 0);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_axpy(TileAcc &acc, int n, const z2 *X, int opX, z2 coef, int r0, int c0) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = tile_i(r0, q), j = tile_j(c0);
+    if (i < n && j < n) acc.set(q, zadd(acc.get(q), zmul(coef, blk_get(X, n, opX, i, j))));
+  }
+}
+
+__device__ __forceinline__ void tile_load(TileAcc &acc, int n, const z2 *X, int r0, int c0) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = tile_i(r0, q), j = tile_j(c0);
+    acc.set(q, (i < n && j < n) ? X[i * n + j] : z2{0.0, 0.0});
+  }
+}
+
+__device__ __forceinline__ void tile_store(const TileAcc &acc, int n, z2 *X, int r0, int c0) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = tile_i(r0, q), j = tile_j(c0);
+    if (i < n && j < n) X[i * n + j] = acc.get(q);
+  }
+}
+
+__device__ __forceinline__ void tile_scale(TileAcc &acc, z2 f) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) acc.set(q, zmul(acc.get(q), f));
+}
+
+// squared distance between the workgroup's tiles and X, summed over the
+// workgroup (valid in thread 0)
+__device__ __forceinline__ double tile_dist2(const TileAcc &acc, int n, const z2 *X, int r0, int c0, double *red) {
+  const int tid = threadIdx.x;
+  double s = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = tile_i(r0, q), j = tile_j(c0);
+    if (i < n && j < n) {
+      const z2 d = zsub(acc.get(q), X[i * n + j]);
+      s += d.x * d.x + d.y * d.y;
+    }
+  }
+  __syncthreads();
+  red[tid] = s;
+  __syncthreads();
+  for (int w = BO_T / 2; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+}  // namespace ned

@@ -70,7 +70,7 @@ struct Ctx {
   // inverses of the k x k block start-up systems: the retarded one depends on
   // times T-k..T-1 only (valid for every corrector pass of step retinv_T), the
   // lesser one on times 0..k only (valid until the early history changes)
-  DevBuf dy_retinv, dy_lesinv, dy_Xs, dy_pinv;
+  DevBuf dy_retinv, dy_lesinv, dy_Xs, dy_pinv, dy_d2;
   int retinv_T = -1;
   DevBuf wtab_d;  // integrator tables of order wtab_k
   int wtab_k = -1;

@@ -417,6 +417,58 @@ __global__ void __launch_bounds__(256) kern_ret_step(Pt p, const WTab *__restric
   else blk_store(acc, n, Qm);
 }
 
+// The same step split by rows: workgroup (m, b) owns rows 16b..16b+15 of
+// block m (its waves take the four column tiles).  G^R(T, T-m) depends only
+// on the same rows of the history sum, so the bands never exchange data; the
+// squared update of each finished band goes to d2[m * nb + b].
+__global__ void __launch_bounds__(256) kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *Q,
+                                                          const z2 *Minv, double *d2) {
+  const WTab &w = *wtab;
+  __shared__ BlkLds s;
+  __shared__ double red[256];
+  const int m = nn + 1 + blockIdx.x, b = blockIdx.y, nb = gridDim.y, T = p.T, n = p.n;
+  if (m > T) return;
+  const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
+  z2 *Qm = Q + (size_t)m * n * n;
+  TileAcc acc;
+  tile_load(acc, n, Qm, r0, c0);
+  tile_mac(acc, s, n, p.gr(T, T - nn), 0, r0, 16, p.sr(T - nn, T - m), 0, 0, 64, z2{greg(w, m, nn), 0.0}, r0, c0);
+  if (m != nn + 1) {
+    tile_store(acc, n, Qm, r0, c0);
+    return;
+  }
+  // finish the band: Q' = dt Q - i/dt sum_l bd_l G^R(T, T-m+l), X = Q' Minv_{T-m}
+  tile_scale(acc, z2{p.dt, 0.0});
+  for (int l = 1; l <= w.k + 1; l++) tile_axpy(acc, n, p.gr(T, T - m + l), 0, z2{0.0, -w.bd[l] / p.dt}, r0, c0);
+  tile_store(acc, n, Qm, r0, c0);
+  TileAcc x;
+  x.zero();
+  tile_mac(x, s, n, Qm, 0, r0, 16, Minv + (size_t)(T - m) * n * n, 0, 0, 64, z2{1.0, 0.0}, r0, c0);
+  z2 *g = p.gr(T, T - m);
+  const double dd = tile_dist2(x, n, g, r0, c0, red);
+  tile_store(x, n, g, r0, c0);
+  if (threadIdx.x == 0) d2[(size_t)m * nb + b] = dd;
+}
+
+// err += sum over m in [m0, m1] of sqrt(sum_b d2[m][b])  (one workgroup)
+__global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *err) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  double s = 0;
+  for (int m = m0 + tid; m <= m1; m += 256) {
+    double t = 0;
+    for (int b = 0; b < nb; b++) t += d2[(size_t)m * nb + b];
+    s += sqrt(t);
+  }
+  red[tid] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) *err += red[0];
+}
+
 // contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
 // the history sums stay in registers across the whole panel
 __global__ void __launch_bounds__(256) kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q) {
@@ -641,6 +693,33 @@ __global__ void __launch_bounds__(256) kern_les_step(Pt p, const WTab *__restric
   blk_mac(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
   if (m == l + 1 && m <= T - 1) les_finalize(p, w, acc, s, m, Qm, X, Minv);
   else blk_store(acc, n, Qm);
+}
+
+// The same step split by columns: workgroup (m, b) owns columns 16b..16b+15
+// of block m (its waves take the four row tiles); X_m = Minv_m Q'_m keeps the
+// columns of Q'_m apart.
+__global__ void __launch_bounds__(256) kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *Q, z2 *X,
+                                                          const z2 *Minv) {
+  const WTab &w = *wtab;
+  __shared__ BlkLds s;
+  const int m = l + 1 + blockIdx.x, b = blockIdx.y, T = p.T, n = p.n;
+  if (m > T) return;
+  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
+  const size_t es = p.es();
+  z2 *Qm = Q + (size_t)m * es;
+  TileAcc acc;
+  tile_load(acc, n, Qm, r0, c0);
+  tile_mac(acc, s, n, p.sr(m, l), 0, 0, 64, X + (size_t)l * es, 0, c0, 16, z2{p.dt * greg(w, m, l), 0.0}, r0, c0);
+  if (!(m == l + 1 && m <= T - 1)) {
+    tile_store(acc, n, Qm, r0, c0);
+    return;
+  }
+  for (int ll = 1; ll <= w.k + 1; ll++) tile_axpy(acc, n, X + (size_t)(m - ll) * es, 0, z2{0.0, -w.bd[ll] / p.dt}, r0, c0);
+  tile_store(acc, n, Qm, r0, c0);
+  TileAcc x;
+  x.zero();
+  tile_mac(x, s, n, Minv + (size_t)m * es, 0, 0, 64, Qm, 0, c0, 16, z2{1.0, 0.0}, r0, c0);
+  tile_store(x, n, X + (size_t)m * es, r0, c0);
 }
 
 // contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
@@ -1097,6 +1176,9 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   c.dy_M.ensure((size_t)kn * kn * sizeof(z2));
   c.dy_Qs.ensure((size_t)kn * n * sizeof(z2));
   z2 *Q = c.dy_Q.as<z2>(), *X = c.dy_X.as<z2>(), *M = c.dy_M.as<z2>(), *Qs = c.dy_Qs.as<z2>();
+  const int nb = (n + 15) / 16;  // 16-row / 16-column bands of a block
+  c.dy_d2.ensure((size_t)(T + 1) * nb * sizeof(double));
+  double *d2 = c.dy_d2.as<double>();
   const z2 *Minv = c.Minv.as<z2>();
 
   // ---- retarded (src/dyson_step.h:7-127)
@@ -1121,9 +1203,10 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   for (int q0 = k + 1; q0 < T; q0 += RPANEL) {
     const int q1 = std::min(q0 + RPANEL, T);
     for (int nn = q0; nn < q1; nn++)
-      hipLaunchKernelGGL(kern_ret_step, dim3(q1 - nn), dim3(256), 0, s, p, wd, nn, Q, Minv, err + 0);
+      hipLaunchKernelGGL(kern_ret_step_band, dim3(q1 - nn, nb), dim3(256), 0, s, p, wd, nn, Q, Minv, d2);
     if (q1 < T) hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q);
   }
+  if (T > k + 1) hipLaunchKernelGGL(kern_band_err, dim3(1), dim3(256), 0, s, d2, k + 2, T, nb, err + 0);
   check_launch("ret history");
 
   // ---- right mixing (src/dyson_step.h:130-169)
@@ -1163,7 +1246,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     for (int q0 = k + 1; q0 < T; q0 += RPANEL) {
       const int q1 = std::min(q0 + RPANEL, T);
       for (int l = q0; l < q1; l++)
-        hipLaunchKernelGGL(kern_les_step, dim3(q1 - l), dim3(256), 0, s, p, wd, l, Q, X, Minv);
+        hipLaunchKernelGGL(kern_les_step_band, dim3(q1 - l, nb), dim3(256), 0, s, p, wd, l, Q, X, Minv);
       if (q1 < T) hipLaunchKernelGGL(kern_les_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q, X);
     }
     hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, Q, X, c.dy_scr.as<z2>());

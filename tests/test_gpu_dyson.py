@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import nedyson_amd as ned
-from helpers import free_green, poly_green, sym_V
+from helpers import free_green, poly_green, random_green, sym_V
 from nedyson_amd import problem
 from nedyson_amd.green import GreenFunc
 
@@ -282,3 +282,33 @@ def test_long_history_window_parity(orc, nao, naux, T0):
         assert _relerr(d.ret_row(T), G.ret_row(T)) < 1e-10, ("ret", T)
         assert _relerr(d.les_col(T), G.les_col(T)) < 1e-10, ("les", T)
         assert _relerr(d.tv[T], G.tv[T]) < 1e-10, ("tv", T)
+
+
+@pytest.mark.parametrize("nao,T0", [(20, 60), (64, 40)])
+def test_dyson_step_band_parity(orc, nao, T0):
+    """One Dyson step at nao > 16, where each block of the retarded and lesser
+    recurrences is split over 16-row / 16-column bands (2 ragged bands at 20,
+    4 at 64): free G history with a random self-energy history, GPU against
+    the oracle's dyson_step on the same data."""
+    rng = np.random.default_rng(nao)
+    nt, ntau, k, beta, dt, mu = T0 + 1, 15, 5, 10.0, 0.01, 0.0
+    h = rng.normal(0, 0.3, (nao, nao))
+    h = 0.5 * (h + h.T)
+    x, _ = orc.gll(ntau)
+    G = free_green(h, mu, beta, dt, nt, x)
+    S = random_green(rng, nt, ntau, nao, scale=0.02)
+    hmf = np.repeat(h[None].astype(complex), nt + 1, axis=0)
+    G_ref = G.copy()
+    D = orc.Dyson(nt, ntau, nao, k)
+    D.dyson_step(nt, G_ref, S, hmf, mu, beta, dt)
+    print(f"[band] nao={nao} T={nt}", flush=True)
+    with ned.Context(nao, 1, nt, ntau, k) as c:
+        c.upload(ned.NED_G, G)
+        c.upload(ned.NED_SIGMA, S)
+        c.write_hmf(hmf)
+        c.set_matsubara(beta)
+        c.dyson_step(nt, mu, beta, dt)
+        d = c.download(ned.NED_G)
+    assert _relerr(d.ret_row(nt), G_ref.ret_row(nt)) < 1e-10
+    assert _relerr(d.les_col(nt), G_ref.les_col(nt)) < 1e-10
+    assert _relerr(d.tv[nt], G_ref.tv[nt]) < 1e-10
