@@ -144,6 +144,46 @@ ned_status ned_tti_extend(ned_ctx *ctx, int which, int tsrc);
 ned_status ned_profile(ned_ctx *ctx, int enable);
 ned_status ned_profile_read(ned_ctx *ctx, double *out8);
 
+/* ---------------------------------------------------------------------------
+ * Learned initial guess.  No counterpart in the reference: these replace the
+ * polynomial predictor dyson::Extrapolate (src/dyson.cpp:47-117) inside
+ * ned_do_tstp, on 2 x 32 time points per slice (the 16 columns after the
+ * quench, t = k+1..k+16, and t = T-15..T, of the retarded row and the lesser
+ * column), by adding the extrapolation error that a three-layer network
+ * predicts from the errors of the last six steps.  The corrector iteration is
+ * unchanged and decides convergence; a corrector that fails from the learned
+ * guess is rerun from the polynomial one.
+ *
+ * Network: x (n_in) -> tanh(W1 x + b1) -> tanh(W2 . + b2) -> W3 . + b3 (n_out),
+ * FP32, parameters flat in the order W1, b1, W2, b2, W3, b3 (weights row-major
+ * out x in).  Training: minibatch Adam on the mean-squared error. */
+typedef struct ned_mlp ned_mlp;
+ned_status ned_mlp_create(ned_mlp **out, int n_in, int width, int n_out, int device, unsigned long long seed);
+ned_status ned_mlp_destroy(ned_mlp *m);
+ned_status ned_mlp_nparams(ned_mlp *m, long long *n);
+ned_status ned_mlp_get_params(ned_mlp *m, float *out);
+ned_status ned_mlp_set_params(ned_mlp *m, const float *in); /* also clears the Adam moments */
+/* mean-squared loss of rows x n_in inputs X against rows x n_out targets T and
+ * its gradient (nparams floats, may be NULL) */
+ned_status ned_mlp_loss_grad(ned_mlp *m, const float *X, const float *T, int rows, double *loss, float *grads);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient */
+ned_status ned_mlp_adam_step(ned_mlp *m, float lr);
+/* epochs of shuffled minibatches (seeded); loss_per_epoch may be NULL */
+ned_status ned_mlp_train(ned_mlp *m, const float *X, const float *T, int rows, int epochs, int batch, float lr,
+                         unsigned long long seed, float *loss_per_epoch);
+ned_status ned_mlp_predict(ned_mlp *m, const float *X, int rows, float *Y);
+/* initial guess of ned_do_tstp: NULL = polynomial only, else a network with
+ * n_in = 768, n_out = 128 on the context's device.  The context keeps the
+ * pointer: unset it (or destroy the context) before destroying the network,
+ * and do not share one network between contexts that step concurrently. */
+ned_status ned_set_predictor(ned_ctx *ctx, ned_mlp *m);
+/* extrapolation error of step tstp at the predictor points, nao^2 x 128 floats
+ * (pair (i,j) row-major, then (component, point, re/im)); the last seven
+ * steps taken with T >= k + 33 are kept */
+ned_status ned_predictor_residual(ned_ctx *ctx, int tstp, float *out);
+/* steps that used the network, and of those, how many fell back */
+ned_status ned_predictor_stats(ned_ctx *ctx, long long *used, long long *fallbacks);
+
 /* Gauss-Lobatto-Legendre nodes x_0..x_ntau on [-1,1] (ascending). Host-only. */
 ned_status ned_gll_nodes(int ntau, double *x);
 

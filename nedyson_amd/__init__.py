@@ -31,7 +31,9 @@ EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "n
            "ned_green_read", "ned_hmf_write", "ned_hmf_read", "ned_sigma_gf2_decomp", "ned_hf_decomp",
            "ned_set_matsubara", "ned_dyson_step", "ned_extrapolate", "ned_dyson_start", "ned_do_boot", "ned_do_tstp",
            "ned_energy", "ned_g0_from_h0", "ned_tti_extend", "ned_profile", "ned_profile_read", "ned_gll_nodes",
-           "ned_integrator"]
+           "ned_integrator", "ned_mlp_create", "ned_mlp_destroy", "ned_mlp_nparams", "ned_mlp_get_params",
+           "ned_mlp_set_params", "ned_mlp_loss_grad", "ned_mlp_adam_step", "ned_mlp_train", "ned_mlp_predict",
+           "ned_set_predictor", "ned_predictor_residual", "ned_predictor_stats"]
 
 
 class NedError(RuntimeError):
@@ -89,6 +91,19 @@ def lib():
     L.ned_profile_read.argtypes = [_vp, _dp]
     L.ned_gll_nodes.argtypes = [C.c_int, _dp]
     L.ned_integrator.argtypes = [C.c_int] + [_dp] * 7
+    _fp = C.POINTER(C.c_float)
+    L.ned_mlp_create.argtypes = [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong]
+    L.ned_mlp_destroy.argtypes = [_vp]
+    L.ned_mlp_nparams.argtypes = [_vp, C.POINTER(C.c_longlong)]
+    L.ned_mlp_get_params.argtypes = [_vp, _fp]
+    L.ned_mlp_set_params.argtypes = [_vp, _fp]
+    L.ned_mlp_loss_grad.argtypes = [_vp, _fp, _fp, C.c_int, _dp, _fp]
+    L.ned_mlp_adam_step.argtypes = [_vp, C.c_float]
+    L.ned_mlp_train.argtypes = [_vp, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp]
+    L.ned_mlp_predict.argtypes = [_vp, _fp, C.c_int, _fp]
+    L.ned_set_predictor.argtypes = [_vp, _vp]
+    L.ned_predictor_residual.argtypes = [_vp, C.c_int, _fp]
+    L.ned_predictor_stats.argtypes = [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     for name in EXPORTS[1:]:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -255,6 +270,22 @@ class Context:
     def tti_extend(self, which, tsrc):
         _check(lib().ned_tti_extend(self.h, which, tsrc))
 
+    def set_predictor(self, mlp):
+        """Use `mlp` (an Mlp with 768 inputs, 128 outputs) for the initial guess
+        of do_tstp, or the polynomial predictor alone when None."""
+        _check(lib().ned_set_predictor(self.h, mlp.h if mlp is not None else None))
+        self._pred = mlp  # keep the network alive while the context uses it
+
+    def predictor_residual(self, tstp):
+        out = np.zeros((self.nao * self.nao, PRED_OUTPUTS), dtype=np.float32)
+        _check(lib().ned_predictor_residual(self.h, tstp, _fptr(out)))
+        return out
+
+    def predictor_stats(self):
+        u, f = C.c_longlong(0), C.c_longlong(0)
+        _check(lib().ned_predictor_stats(self.h, C.byref(u), C.byref(f)))
+        return u.value, f.value
+
     def profile(self, enable=True):
         _check(lib().ned_profile(self.h, int(enable)))
 
@@ -266,6 +297,85 @@ class Context:
 
     def sync(self):
         _check(lib().ned_sync(self.h))
+
+
+# predictor network shape: residuals of the last six steps at 2 x 32 points
+# (real, imaginary) per orbital pair in, the next step's out
+PRED_OUTPUTS = 128
+PRED_INPUTS = 6 * PRED_OUTPUTS
+
+
+def _fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The predictor network on one GPU (ned_mlp_*, include/nedyson.h)."""
+
+    def __init__(self, n_in=PRED_INPUTS, width=512, n_out=PRED_OUTPUTS, device=0, seed=0):
+        self.n_in, self.width, self.n_out = n_in, width, n_out
+        self.h = _vp()
+        _check(lib().ned_mlp_create(C.byref(self.h), n_in, width, n_out, device, seed))
+        n = C.c_longlong(0)
+        _check(lib().ned_mlp_nparams(self.h, C.byref(n)))
+        self.nparams = n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ned_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().ned_mlp_get_params(self.h, _fptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        _check(lib().ned_mlp_set_params(self.h, _fptr(p)))
+
+    def split(self, flat):
+        """Flat parameters -> [W1, b1, W2, b2, W3, b3]."""
+        shapes = [(self.width, self.n_in), (self.width,), (self.width, self.width), (self.width,),
+                  (self.n_out, self.width), (self.n_out,)]
+        out, o = [], 0
+        for sh in shapes:
+            n = int(np.prod(sh))
+            out.append(flat[o:o + n].reshape(sh))
+            o += n
+        return out
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_double(0)
+        _check(lib().ned_mlp_loss_grad(self.h, _fptr(X), _fptr(T), X.shape[0], C.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def adam_step(self, lr):
+        _check(lib().ned_mlp_adam_step(self.h, lr))
+
+    def train(self, X, T, epochs=10, batch=1024, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        _check(lib().ned_mlp_train(self.h, _fptr(X), _fptr(T), X.shape[0], epochs, batch, lr, seed, _fptr(losses)))
+        return losses
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.n_out), dtype=np.float32)
+        _check(lib().ned_mlp_predict(self.h, _fptr(X), X.shape[0], _fptr(Y)))
+        return Y
 
 
 class molGF2SolverDecomp:

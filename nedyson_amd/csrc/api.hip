@@ -1,7 +1,9 @@
 // C ABI of the nedyson_amd library (declarations: include/nedyson.h).
 #include "../../include/nedyson.h"
 #include "ctx.h"
+#include <memory>
 #include "dyson.h"
+#include "predictor.h"
 #include "quadrature.h"
 #include <cmath>
 #include <cstdlib>
@@ -64,6 +66,19 @@ void state(bool ok, const char *msg) {
   if (!ok) throw std::runtime_error(std::string("state: ") + msg);
 }
 }  // namespace
+
+struct ned_mlp {
+  std::unique_ptr<ned::Mlp> m;
+};
+
+template <class F>
+ned_status mlp_guard(ned_mlp *m, F &&f) {
+  if (!m) { g_err = "null network"; return NED_ERR_ARG; }
+  return guard(nullptr, [&] {
+    NED_HIP(hipSetDevice(m->m->device));
+    f(*m->m);
+  });
+}
 
 extern "C" {
 
@@ -209,6 +224,7 @@ ned_status ned_hf_decomp(ned_ctx *ctx, int tstp, const double *h0) {
   return guard(ctx, [&] {
     state(ctx->c.have_coulomb, "interaction not set (ned_set_coulomb)");
     ned::hf_decomp(ctx->c, tstp, h0);
+    ned::touch_time(ctx->c, tstp);  // hmf(tstp) enters the cached M_tstp
   });
 }
 
@@ -286,21 +302,7 @@ ned_status ned_do_tstp(ned_ctx *ctx, int tstp, const double *h0, double mu, doub
     ned::require(corr_max >= 1, "corr_max must be >= 1");
     state(c.have_coulomb, "interaction not set (ned_set_coulomb)");
     state(c.have_mix && c.mix_beta == beta, "ned_set_matsubara(beta) not called for this beta");
-    ned::extrapolate(c, tstp);
-    double e[3] = {0, 0, 0}, err = 0;
-    int it = 0;
-    for (; it < corr_max;) {
-      ned::hf_decomp(c, tstp, h0);
-      ned::sigma_gf2(c, tstp, (ned::Prec)prec);
-      ned::touch_time(c, tstp);
-      ned::dyson_step(c, tstp, mu, beta, dt, e);
-      err = e[0] + e[1] + e[2];
-      it++;
-      if (!std::isfinite(err)) throw std::runtime_error("non-finite update in corrector");
-      if (corr_tol > 0 && err < corr_tol) break;
-    }
-    if (iters_out) *iters_out = it;
-    if (err_out) *err_out = err;
+    ned::do_tstp(c, tstp, h0, mu, beta, dt, (ned::Prec)prec, corr_max, corr_tol, iters_out, err_out);
   });
 }
 
@@ -322,6 +324,7 @@ ned_status ned_tti_extend(ned_ctx *ctx, int which, int tsrc) {
   return guard(ctx, [&] {
     ned::require(which == NED_G || which == NED_SIGMA, "which must be NED_G or NED_SIGMA");
     ned::tti_extend(ctx->c, which, tsrc);
+    if (which == NED_SIGMA) ned::invalidate_minv(ctx->c);  // Sigma^R(t,t) enters the cached M_t
   });
 }
 
@@ -356,6 +359,121 @@ ned_status ned_profile_read(ned_ctx *ctx, double *out8) {
     out8[7] = 0;
     c.prof_flops = 0;
   });
+}
+
+ned_status ned_mlp_create(ned_mlp **out, int n_in, int width, int n_out, int device, unsigned long long seed) {
+  if (!out) { g_err = "null argument"; return NED_ERR_ARG; }
+  *out = nullptr;
+  ned_mlp *h = nullptr;
+  ned_status st = guard(nullptr, [&] {
+    ned::require(n_in > 0 && width > 0 && n_out > 0, "network sizes must be positive");
+    int ndev = 0;
+    NED_HIP(hipGetDeviceCount(&ndev));
+    ned::require(device >= 0 && device < ndev, "no such HIP device");
+    NED_HIP(hipSetDevice(device));
+    h = new ned_mlp();
+    h->m.reset(new ned::Mlp(n_in, width, n_out, device, seed));
+  });
+  if (st != NED_OK) {
+    delete h;
+    return st;
+  }
+  *out = h;
+  return NED_OK;
+}
+
+ned_status ned_mlp_destroy(ned_mlp *m) {
+  if (!m) return NED_OK;
+  (void)hipSetDevice(m->m->device);
+  delete m;
+  return NED_OK;
+}
+
+ned_status ned_mlp_nparams(ned_mlp *m, long long *n) {
+  if (!n) { g_err = "null argument"; return NED_ERR_ARG; }
+  return mlp_guard(m, [&](ned::Mlp &x) { *n = x.nparam; });
+}
+
+ned_status ned_mlp_get_params(ned_mlp *m, float *out) {
+  if (!out) { g_err = "null argument"; return NED_ERR_ARG; }
+  return mlp_guard(m, [&](ned::Mlp &x) {
+    NED_HIP(hipMemcpy(out, x.params.p, x.nparam * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+ned_status ned_mlp_set_params(ned_mlp *m, const float *in) {
+  if (!in) { g_err = "null argument"; return NED_ERR_ARG; }
+  return mlp_guard(m, [&](ned::Mlp &x) {
+    NED_HIP(hipMemcpy(x.params.p, in, x.nparam * sizeof(float), hipMemcpyHostToDevice));
+    x.reset_adam();
+  });
+}
+
+ned_status ned_mlp_loss_grad(ned_mlp *m, const float *X, const float *T, int rows, double *loss, float *grads) {
+  if (!X || !T || !loss) { g_err = "null argument"; return NED_ERR_ARG; }
+  return mlp_guard(m, [&](ned::Mlp &x) {
+    ned::require(rows > 0, "rows must be positive");
+    ned::DevBuf dX, dT;
+    dX.ensure((size_t)rows * x.n_in * sizeof(float));
+    dT.ensure((size_t)rows * x.n_out * sizeof(float));
+    NED_HIP(hipMemcpy(dX.p, X, (size_t)rows * x.n_in * sizeof(float), hipMemcpyHostToDevice));
+    NED_HIP(hipMemcpy(dT.p, T, (size_t)rows * x.n_out * sizeof(float), hipMemcpyHostToDevice));
+    *loss = x.loss_grad(dX.as<float>(), dT.as<float>(), rows, x.stream);
+    if (grads) {
+      NED_HIP(hipMemcpyAsync(grads, x.grads.p, x.nparam * sizeof(float), hipMemcpyDeviceToHost, x.stream));
+      NED_HIP(hipStreamSynchronize(x.stream));
+    }
+  });
+}
+
+ned_status ned_mlp_adam_step(ned_mlp *m, float lr) {
+  return mlp_guard(m, [&](ned::Mlp &x) {
+    x.adam(lr, x.stream);
+    NED_HIP(hipStreamSynchronize(x.stream));
+  });
+}
+
+ned_status ned_mlp_train(ned_mlp *m, const float *X, const float *T, int rows, int epochs, int batch, float lr,
+                         unsigned long long seed, float *loss_per_epoch) {
+  if (!X || !T) { g_err = "null argument"; return NED_ERR_ARG; }
+  return mlp_guard(m, [&](ned::Mlp &x) { x.train(X, T, rows, epochs, batch, lr, seed, loss_per_epoch); });
+}
+
+ned_status ned_mlp_predict(ned_mlp *m, const float *X, int rows, float *Y) {
+  if (!X || !Y) { g_err = "null argument"; return NED_ERR_ARG; }
+  return mlp_guard(m, [&](ned::Mlp &x) {
+    ned::require(rows > 0, "rows must be positive");
+    ned::DevBuf dX, dY;
+    dX.ensure((size_t)rows * x.n_in * sizeof(float));
+    dY.ensure((size_t)rows * x.n_out * sizeof(float));
+    NED_HIP(hipMemcpy(dX.p, X, (size_t)rows * x.n_in * sizeof(float), hipMemcpyHostToDevice));
+    x.forward(dX.as<float>(), rows, dY.as<float>(), x.stream);
+    NED_HIP(hipMemcpyAsync(Y, dY.p, (size_t)rows * x.n_out * sizeof(float), hipMemcpyDeviceToHost, x.stream));
+    NED_HIP(hipStreamSynchronize(x.stream));
+  });
+}
+
+ned_status ned_set_predictor(ned_ctx *ctx, ned_mlp *m) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] { ned::set_predictor(ctx->c, m ? m->m.get() : nullptr); });
+}
+
+ned_status ned_predictor_residual(ned_ctx *ctx, int tstp, float *out) {
+  if (!ctx || !out) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    const float *r = ned::predictor_residual(ctx->c, tstp);
+    if (!r) throw std::invalid_argument("no extrapolation error recorded for this step (the last 7 steps are kept)");
+    const size_t bytes = (size_t)ctx->c.es * ned::predictor_outputs() * sizeof(float);
+    NED_HIP(hipMemcpyAsync(out, r, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+    NED_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+ned_status ned_predictor_stats(ned_ctx *ctx, long long *used, long long *fallbacks) {
+  if (!ctx || !used || !fallbacks) { g_err = "null argument"; return NED_ERR_ARG; }
+  *used = ctx->c.pred_used;
+  *fallbacks = ctx->c.pred_fallback;
+  return NED_OK;
 }
 
 ned_status ned_gll_nodes(int ntau, double *x) {

@@ -10,6 +10,8 @@
 
 namespace ned {
 
+struct Mlp;
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
@@ -72,6 +74,13 @@ struct Ctx {
   // lesser one on times 0..k only (valid until the early history changes)
   DevBuf dy_retinv, dy_lesinv, dy_Xs, dy_pinv, dy_d2;
   int retinv_T = -1;
+  // initial-guess predictor: extrapolated values at the predictor points,
+  // extrapolation errors of the last 7 steps (ring, slot T % 7 holds step
+  // pred_ring_T[slot]), network inputs / outputs / scales
+  Mlp *pred_mlp = nullptr;
+  DevBuf pred_E, pred_ring, pred_X, pred_Y, pred_sig;
+  int pred_ring_T[7] = {-1, -1, -1, -1, -1, -1, -1};
+  long long pred_used = 0, pred_fallback = 0;
   DevBuf wtab_d;  // integrator tables of order wtab_k
   int wtab_k = -1;
   bool lesinv_valid = false;

@@ -284,3 +284,34 @@ def test_sigma_threaded_bit_identical(orc, nthreads):
     orc.sigma_solve(V, nt, S2, G, nthreads)
     for c in ("ret", "les", "tv"):
         assert np.array_equal(getattr(S1, c), getattr(S2, c))
+
+
+def test_mlp_oracle_backward_matches_finite_differences():
+    """The predictor network's CPU oracle: analytic gradients (backward pass)
+    against central differences of the loss, in float64."""
+    from oracle import mlp as M
+    rng = np.random.default_rng(3)
+    n_in, w, n_out, rows = 7, 9, 4, 5
+    params = [rng.normal(0, 0.5, (w, n_in)), rng.normal(0, 0.1, w), rng.normal(0, 0.5, (w, w)),
+              rng.normal(0, 0.1, w), rng.normal(0, 0.5, (n_out, w)), rng.normal(0, 0.1, n_out)]
+    X = rng.normal(size=(rows, n_in))
+    T = rng.normal(size=(rows, n_out))
+    _, grads = M.loss_and_grads(params, X, T)
+    for which in range(6):
+        for _ in range(4):
+            idx = tuple(rng.integers(0, s) for s in params[which].shape)
+            num = M.numerical_grad(params, X, T, which, idx)
+            assert abs(num - grads[which][idx]) < 1e-7 * max(1.0, abs(num)), (which, idx)
+
+
+def test_mlp_oracle_adam_matches_closed_form_first_step():
+    """First Adam step moves every parameter by lr * sign(g) (bias-corrected
+    moments m = g, v = g^2), up to eps."""
+    from oracle import mlp as M
+    rng = np.random.default_rng(4)
+    p = [rng.normal(size=(3, 2))]
+    g = [rng.normal(size=(3, 2))]
+    m, v = [np.zeros((3, 2))], [np.zeros((3, 2))]
+    p0 = p[0].copy()
+    M.adam_step(p, g, m, v, 1, 0.01)
+    assert np.allclose(p[0], p0 - 0.01 * np.sign(g[0]), atol=1e-8)
