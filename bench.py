@@ -127,6 +127,9 @@ def main():
     ap.add_argument("--b-sd", type=float, default=None,
                     help="sigma_B of the Coulomb factor (default: %s for the nao=64 workload)" % STABLE_B_SD)
     ap.add_argument("--dt", type=float, default=None)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many bytes (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -135,6 +138,17 @@ def main():
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
+
+    if args.copy_bytes:
+        import torch
+        n = args.copy_bytes // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[copy] {4 * n} bytes copied")
+        return
 
     import nedyson_amd as ned
     from nedyson_amd import problem
@@ -211,6 +225,17 @@ def main():
     c.close()
 
     ugemm_ms = prof["ugemm_ms"] / max(prof["ugemm_n"], 1)
+    # HBM bytes per exchange-GEMM launch from the rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs recorded next to this file (profiles/), when they were
+    # taken on this workload's shape
+    traffic, traffic_note = None, "no counter run recorded for this shape"
+    tfile = os.path.join(ROOT, "profiles", "kern_nt_bf16x3_nao64_T1000_traffic.json")
+    if args.prec == "bf16x3" and os.path.exists(tfile) and p.nao == 64 and p.naux == 256:
+        t = json.load(open(tfile))
+        traffic = t["traffic_bytes_per_launch"]
+        traffic_note = (f"bytes per full-chunk launch (M={t['M']}, N={t['N']}, K={t['K']}) from rocprofv3 "
+                        f"FETCH_SIZE x {t['fetch_scale']:.3f} + WRITE_SIZE x {t['write_scale']:.3f}, scaled by a 1 GiB "
+                        f"copy; algorithmic {t['algorithmic_bytes_per_launch']:.3e} B")
     n_sigma = max(prof["sigma_n"], 1)
     # algorithmic flops of one exchange-GEMM launch, 2 M N K with M = 2 naux P
     # rows (P time points of the slice), N = K = nao^2, averaged over launches
@@ -244,7 +269,8 @@ def main():
         "sigma_ms_per_eval": prof["sigma_ms"] / n_sigma,
         "dyson_ms_per_step": prof["dyson_ms"] / max(prof["dyson_n"], 1),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
+                     "traffic_note": traffic_note,
                      "kernel": f"Sigma exchange GEMM ({args.prec}), {flops_per_launch:.3e} algorithmic flop "
                                f"per launch, avg {ugemm_ms:.3f} ms over {prof['ugemm_n']} launches"},
     }

@@ -49,28 +49,39 @@ __device__ __forceinline__ z2 blk_get(const z2 *M, int ld, int op, int r, int c)
 }
 
 // acc += coef * op(X) * op(Y)   (all threads of the workgroup call this);
-// ldX, ldY are the row strides of X and Y as stored (n for a dense block)
+// ldX, ldY are the row strides of X and Y as stored (n for a dense block).
+// The next K-slice is loaded into registers while the current one is
+// multiplied out of LDS.
 __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 *X, int opX, const z2 *Y, int opY,
                                         z2 coef, int ldX = 0, int ldY = 0) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (ldX == 0) ldX = n;
   if (ldY == 0) ldY = n;
   const bool wave_on = 16 * w < n;
+  constexpr int PER = BO_NMAX * BO_KT / BO_T;  // staged elements per thread and operand
+  z2 px[PER], py[PER];
+  auto fetch = [&](int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int e = tid + BO_T * i;
+      const int r = e / BO_KT, kk = e % BO_KT;
+      px[i] = (r < n && k0 + kk < n) ? blk_get(X, ldX, opX, r, k0 + kk) : z2{0.0, 0.0};
+      const int kk2 = e / BO_NMAX, c = e % BO_NMAX;
+      py[i] = (c < n && k0 + kk2 < n) ? blk_get(Y, ldY, opY, k0 + kk2, c) : z2{0.0, 0.0};
+    }
+  };
+  __syncthreads();  // operands written by this workgroup just before the call are visible
+  fetch(0);
   for (int k0 = 0; k0 < n; k0 += BO_KT) {
     __syncthreads();
-    for (int e = tid; e < BO_NMAX * BO_KT; e += BO_T) {
-      int r = e / BO_KT, kk = e % BO_KT;
-      z2 v = {0.0, 0.0};
-      if (r < n && k0 + kk < n) v = zmul(coef, blk_get(X, ldX, opX, r, k0 + kk));
-      s.X[r][kk] = v;
-    }
-    for (int e = tid; e < BO_KT * BO_NMAX; e += BO_T) {
-      int kk = e / BO_NMAX, c = e % BO_NMAX;
-      z2 v = {0.0, 0.0};
-      if (c < n && k0 + kk < n) v = blk_get(Y, ldY, opY, k0 + kk, c);
-      s.Y[kk][c] = v;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int e = tid + BO_T * i;
+      s.X[e / BO_KT][e % BO_KT] = zmul(coef, px[i]);
+      s.Y[e / BO_NMAX][e % BO_NMAX] = py[i];
     }
     __syncthreads();
+    if (k0 + BO_KT < n) fetch(k0 + BO_KT);
     if (wave_on) {
       const int ksteps = min(BO_KT, n - k0);
       for (int ks = 0; ks < ksteps; ks += 4) {
@@ -193,21 +204,31 @@ __device__ __forceinline__ void tile_mac(TileAcc &acc, BlkLds &s, int n, const z
                                          const z2 *R, int opR, int rc0, int rcn, z2 coef, int r0, int c0) {
   const int tid = threadIdx.x, lane = tid & 63;
   const bool on = r0 < n && c0 < n;
+  constexpr int PER = BO_NMAX * BO_KT / BO_T;  // at most 4 elements per thread and operand
+  z2 pl[PER], pr[PER];
+  auto fetch = [&](int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int e = tid + BO_T * i;
+      const int r = e / BO_KT, kk = e % BO_KT;
+      pl[i] = (e < lrn * BO_KT && lr0 + r < n && k0 + kk < n) ? blk_get(L, n, opL, lr0 + r, k0 + kk) : z2{0.0, 0.0};
+      const int kk2 = e / rcn, c = e % rcn;
+      pr[i] = (e < BO_KT * rcn && rc0 + c < n && k0 + kk2 < n) ? blk_get(R, n, opR, k0 + kk2, rc0 + c)
+                                                              : z2{0.0, 0.0};
+    }
+  };
+  __syncthreads();  // operands written by this workgroup just before the call are visible
+  fetch(0);
   for (int k0 = 0; k0 < n; k0 += BO_KT) {
     __syncthreads();
-    for (int e = tid; e < lrn * BO_KT; e += BO_T) {
-      const int r = e / BO_KT, kk = e % BO_KT;
-      z2 v = {0.0, 0.0};
-      if (lr0 + r < n && k0 + kk < n) v = zmul(coef, blk_get(L, n, opL, lr0 + r, k0 + kk));
-      s.X[r][kk] = v;
-    }
-    for (int e = tid; e < BO_KT * rcn; e += BO_T) {
-      const int kk = e / rcn, c = e % rcn;
-      z2 v = {0.0, 0.0};
-      if (rc0 + c < n && k0 + kk < n) v = blk_get(R, n, opR, k0 + kk, rc0 + c);
-      s.Y[kk][c] = v;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int e = tid + BO_T * i;
+      if (e < lrn * BO_KT) s.X[e / BO_KT][e % BO_KT] = zmul(coef, pl[i]);
+      if (e < BO_KT * rcn) s.Y[e / rcn][e % rcn] = pr[i];
     }
     __syncthreads();
+    if (k0 + BO_KT < n) fetch(k0 + BO_KT);
     if (on) {
       const int ksteps = min(BO_KT, n - k0);
       for (int ks = 0; ks < ksteps; ks += 4) {

@@ -95,6 +95,10 @@ struct NtOp {
   long long ld, stride;
 };
 bool nt_bf16x3_ok(int M, int N, int K);
+// tile-shape variants of gemm_nt_bf16x3 (the autotuner's candidates; the
+// environment variable NED_NT_VARIANT=<index> pins one)
+int nt_variant_count();
+const char *nt_variant_name(int v);
 void gemm_nt_bf16x3(const NtOp &A, const NtOp &B, float *C, long long ldc, long long sC, int M, int N, int K,
                     int batch, float alpha, int beta, hipStream_t s);
 // x -> hi + lo with hi = bf16(x), lo = bf16(x - hi), round to nearest even;

@@ -469,21 +469,20 @@ __global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *
   if (tid == 0) *err += red[0];
 }
 
-// contributions of the finished blocks nn in [q0, q1) to every m in (q1, T],
-// one workgroup per (m, 16-row band): the history sums stay in registers
-// across the whole panel
+// contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
+// the history sums stay in registers across the whole panel
 __global__ void __launch_bounds__(256) kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
-  const int r0 = 16 * blockIdx.y, c0 = 16 * (threadIdx.x >> 6);
   z2 *Qm = Q + (size_t)m * n * n;
-  TileAcc acc;
-  tile_load(acc, n, Qm, r0, c0);
+  BlkAcc acc;
+  blk_load(acc, n, Qm);
   for (int nn = q0; nn < q1; nn++)
-    tile_mac(acc, s, n, p.gr(T, T - nn), 0, r0, 16, p.sr(T - nn, T - m), 0, 0, 64, z2{greg(w, m, nn), 0.0}, r0, c0);
-  tile_store(acc, n, Qm, r0, c0);
+    blk_mac(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
+  __syncthreads();
+  blk_store(acc, n, Qm);
 }
 
 // ------------------------------------------------------------ mixing (tv)
@@ -723,22 +722,20 @@ __global__ void __launch_bounds__(256) kern_les_step_band(Pt p, const WTab *__re
   tile_store(x, n, X + (size_t)m * es, r0, c0);
 }
 
-// contributions of the finished X_l, l in [q0, q1), to every m in (q1, T],
-// one workgroup per (m, 16-column band)
+// contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
 __global__ void __launch_bounds__(256) kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q,
                                                       const z2 *X) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
-  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * blockIdx.y;
   z2 *Qm = Q + (size_t)m * p.es();
-  TileAcc acc;
-  tile_load(acc, n, Qm, r0, c0);
+  BlkAcc acc;
+  blk_load(acc, n, Qm);
   for (int l = q0; l < q1; l++)
-    tile_mac(acc, s, n, p.sr(m, l), 0, 0, 64, X + (size_t)l * p.es(), 0, c0, 16, z2{p.dt * greg(w, m, l), 0.0}, r0,
-             c0);
-  tile_store(acc, n, Qm, r0, c0);
+    blk_mac(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+  __syncthreads();
+  blk_store(acc, n, Qm);
 }
 
 // diagonal step, src/dyson_step.h:283-310
@@ -1207,7 +1204,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     const int q1 = std::min(q0 + RPANEL, T);
     for (int nn = q0; nn < q1; nn++)
       hipLaunchKernelGGL(kern_ret_step_band, dim3(q1 - nn, nb), dim3(256), 0, s, p, wd, nn, Q, Minv, d2);
-    if (q1 < T) hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1, nb), dim3(256), 0, s, p, wd, q0, q1, Q);
+    if (q1 < T) hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q);
   }
   if (T > k + 1) hipLaunchKernelGGL(kern_band_err, dim3(1), dim3(256), 0, s, d2, k + 2, T, nb, err + 0);
   check_launch("ret history");
@@ -1250,7 +1247,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
       const int q1 = std::min(q0 + RPANEL, T);
       for (int l = q0; l < q1; l++)
         hipLaunchKernelGGL(kern_les_step_band, dim3(q1 - l, nb), dim3(256), 0, s, p, wd, l, Q, X, Minv);
-      if (q1 < T) hipLaunchKernelGGL(kern_les_panel, dim3(T - q1, nb), dim3(256), 0, s, p, wd, q0, q1, Q, X);
+      if (q1 < T) hipLaunchKernelGGL(kern_les_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q, X);
     }
     hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, Q, X, c.dy_scr.as<z2>());
     hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, X, err + 2);
