@@ -184,6 +184,23 @@ ned_status ned_predictor_residual(ned_ctx *ctx, int tstp, float *out);
 /* steps that used the network, and of those, how many fell back */
 ned_status ned_predictor_stats(ned_ctx *ctx, long long *used, long long *fallbacks);
 
+/* ---------------------------------------------------------------------------
+ * Independent systems stepped together.  No counterpart in the reference
+ * (which propagates one system per process): a batch holds n existing
+ * contexts, possibly of different nao, naux, nt and ntau, on the same or on
+ * different devices; ned_batch_do_tstp runs ned_do_tstp on all of them at
+ * once, one host thread per system, each system's kernels on its own stream
+ * so that the GPU overlaps them.  The contexts stay owned by the caller and
+ * must outlive the batch.  h0[i] is system i's real nao_i x nao_i Hamiltonian;
+ * iters, err and status (arrays of n, each may be NULL) receive the per-system
+ * results; the return value is the first failing system's status. */
+typedef struct ned_batch ned_batch;
+ned_status ned_batch_create(ned_batch **out, ned_ctx *const *ctxs, int n);
+ned_status ned_batch_destroy(ned_batch *b);
+ned_status ned_batch_do_tstp(ned_batch *b, int tstp, const double *const *h0, const double *mu, const double *beta,
+                             const double *dt, int prec, int corr_max, double corr_tol, int *iters, double *err,
+                             int *status);
+
 /* Gauss-Lobatto-Legendre nodes x_0..x_ntau on [-1,1] (ascending). Host-only. */
 ned_status ned_gll_nodes(int ntau, double *x);
 

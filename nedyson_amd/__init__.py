@@ -33,7 +33,8 @@ EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "n
            "ned_energy", "ned_g0_from_h0", "ned_tti_extend", "ned_profile", "ned_profile_read", "ned_gll_nodes",
            "ned_integrator", "ned_mlp_create", "ned_mlp_destroy", "ned_mlp_nparams", "ned_mlp_get_params",
            "ned_mlp_set_params", "ned_mlp_loss_grad", "ned_mlp_adam_step", "ned_mlp_train", "ned_mlp_predict",
-           "ned_set_predictor", "ned_predictor_residual", "ned_predictor_stats"]
+           "ned_set_predictor", "ned_predictor_residual", "ned_predictor_stats", "ned_batch_create",
+           "ned_batch_destroy", "ned_batch_do_tstp"]
 
 
 class NedError(RuntimeError):
@@ -104,6 +105,10 @@ def lib():
     L.ned_set_predictor.argtypes = [_vp, _vp]
     L.ned_predictor_residual.argtypes = [_vp, C.c_int, _fp]
     L.ned_predictor_stats.argtypes = [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.ned_batch_create.argtypes = [C.POINTER(_vp), C.POINTER(_vp), C.c_int]
+    L.ned_batch_destroy.argtypes = [_vp]
+    L.ned_batch_do_tstp.argtypes = [_vp, C.c_int, C.POINTER(_dp), _dp, _dp, _dp, C.c_int, C.c_int, C.c_double,
+                                    C.POINTER(C.c_int), _dp, C.POINTER(C.c_int)]
     for name in EXPORTS[1:]:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -376,6 +381,42 @@ class Mlp:
         Y = np.zeros((X.shape[0], self.n_out), dtype=np.float32)
         _check(lib().ned_mlp_predict(self.h, _fptr(X), X.shape[0], _fptr(Y)))
         return Y
+
+
+class Batch:
+    """Independent systems stepped together (ned_batch_*): each Context keeps
+    its own stream and the timesteps of all of them run concurrently."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (_vp * len(self.contexts))(*[c.h for c in self.contexts])
+        self.h = _vp()
+        _check(lib().ned_batch_create(C.byref(self.h), arr, len(self.contexts)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ned_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def do_tstp(self, tstp, h0s, mus, betas, dts, prec="f64", corr_max=3, corr_tol=0.0):
+        n = len(self.contexts)
+        hs = [np.ascontiguousarray(h, dtype=np.float64) for h in h0s]
+        hp = (_dp * n)(*[_dptr(h) for h in hs])
+        mu = np.ascontiguousarray(mus, dtype=np.float64)
+        be = np.ascontiguousarray(betas, dtype=np.float64)
+        dt = np.ascontiguousarray(dts, dtype=np.float64)
+        it = (C.c_int * n)()
+        err = np.zeros(n)
+        st = (C.c_int * n)()
+        _check(lib().ned_batch_do_tstp(self.h, tstp, hp, _dptr(mu), _dptr(be), _dptr(dt), prec_code(prec), corr_max,
+                                       corr_tol, it, _dptr(err), st))
+        return list(it), err
 
 
 class molGF2SolverDecomp:

@@ -2,6 +2,8 @@
 #include "../../include/nedyson.h"
 #include "ctx.h"
 #include <memory>
+#include <thread>
+#include <vector>
 #include "dyson.h"
 #include "predictor.h"
 #include "quadrature.h"
@@ -69,6 +71,11 @@ void state(bool ok, const char *msg) {
 
 struct ned_mlp {
   std::unique_ptr<ned::Mlp> m;
+};
+
+// independent systems stepped together; each member keeps its own stream
+struct ned_batch {
+  std::vector<ned_ctx *> ctx;
 };
 
 template <class F>
@@ -474,6 +481,50 @@ ned_status ned_predictor_stats(ned_ctx *ctx, long long *used, long long *fallbac
   *used = ctx->c.pred_used;
   *fallbacks = ctx->c.pred_fallback;
   return NED_OK;
+}
+
+ned_status ned_batch_create(ned_batch **out, ned_ctx *const *ctxs, int n) {
+  if (!out || !ctxs || n <= 0) { g_err = "null argument or empty batch"; return NED_ERR_ARG; }
+  *out = nullptr;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < i; j++)
+      if (!ctxs[i] || ctxs[i] == ctxs[j]) { g_err = "batch members must be distinct contexts"; return NED_ERR_ARG; }
+  ned_batch *b = new ned_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  *out = b;
+  return NED_OK;
+}
+
+ned_status ned_batch_destroy(ned_batch *b) {
+  delete b;
+  return NED_OK;
+}
+
+ned_status ned_batch_do_tstp(ned_batch *b, int tstp, const double *const *h0, const double *mu, const double *beta,
+                             const double *dt, int prec, int corr_max, double corr_tol, int *iters, double *err,
+                             int *status) {
+  if (!b || !h0 || !mu || !beta || !dt) { g_err = "null argument"; return NED_ERR_ARG; }
+  const int n = (int)b->ctx.size();
+  std::vector<ned_status> st(n, NED_OK);
+  std::vector<std::string> msg(n);
+  std::vector<std::thread> th;
+  th.reserve(n);
+  for (int i = 0; i < n; i++)
+    th.emplace_back([&, i] {
+      st[i] = ned_do_tstp(b->ctx[i], tstp, h0[i], mu[i], beta[i], dt[i], prec, corr_max, corr_tol,
+                          iters ? iters + i : nullptr, err ? err + i : nullptr);
+      if (st[i] != NED_OK) msg[i] = g_err;
+    });
+  for (auto &t : th) t.join();
+  ned_status first = NED_OK;
+  for (int i = 0; i < n; i++) {
+    if (status) status[i] = st[i];
+    if (st[i] != NED_OK && first == NED_OK) {
+      first = st[i];
+      g_err = "system " + std::to_string(i) + ": " + msg[i];
+    }
+  }
+  return first;
 }
 
 ned_status ned_gll_nodes(int ntau, double *x) {

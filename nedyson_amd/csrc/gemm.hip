@@ -524,15 +524,14 @@ struct NtKey {
 };
 std::mutex g_nt_mu;
 std::map<std::pair<int, NtKey>, int> g_nt_choice;  // (device, shape) -> variant
-int g_nt_forced = -2;                              // -2: not read yet, -1: autotune
 
 int nt_forced() {
-  if (g_nt_forced == -2) {
+  static const int forced = [] {
     const char *e = std::getenv("NED_NT_VARIANT");
-    g_nt_forced = e ? std::atoi(e) : -1;
-    if (g_nt_forced >= kNtCount) g_nt_forced = -1;
-  }
-  return g_nt_forced;
+    const int v = e ? std::atoi(e) : -1;
+    return (v >= 0 && v < kNtCount) ? v : -1;
+  }();
+  return forced;
 }
 
 bool fits(const NtVariant &v, int M, int N) { return M % v.tm == 0 && N % v.tn == 0; }
