@@ -112,6 +112,64 @@ def cpu_baseline(p, T, corr, log):
             "value_1thread": 1.0 / t_step1, "sec_per_step": t_stepN, "gflops_1thread": rate / 1e9}
 
 
+def window_context(ned, p, T0, nt):
+    """A context holding the benchmark history t <= T0: the free G of the
+    equilibrium Fock matrix with Sigma(T0) continued time-translation
+    invariantly."""
+    c = ned.Context(p.nao, p.naux, nt, p.ntau, p.k)
+    c.set_coulomb(p.B)
+    c.write_blocks(ned.NED_G, ned.NED_MAT, 0, p.Gmat)
+    c.set_matsubara(p.beta)
+    c.g0_from_h0(T0, p.fock, p.mu, p.beta, p.dt)
+    c.write_hmf(np.repeat(p.fock[None].astype(complex), T0 + 1, axis=0))
+    c.sigma(T0, "f64")
+    c.tti_extend(ned.NED_SIGMA, T0)
+    c.sync()
+    return c
+
+
+def sweep(args, p, log):
+    """Per-step time at several history lengths T, and the whole-run average
+    over T = k+1..nt from a quadratic fit t(T) = a + b T + c T^2 (the Sigma
+    cost grows linearly and the Dyson cost quadratically in T)."""
+    import nedyson_amd as ned
+    Ts = [int(x) for x in args.sweep.split(",")]
+    rows = []
+    for T0 in Ts:
+        W, K = args.warmup, args.steps
+        c = window_context(ned, p, T0, T0 + W + K)
+        times, split = [], []
+        for i, T in enumerate(range(T0 + 1, T0 + W + K + 1)):
+            if i >= W:
+                c.profile(True)
+            c.sync()
+            t0 = time.perf_counter()
+            c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, args.prec, args.corr, 0.0)
+            c.sync()
+            if i >= W:
+                times.append(time.perf_counter() - t0)
+            c.profile(False)
+        pr = c.profile_read()
+        c.close()
+        ms = 1e3 * float(np.mean(times))
+        rows.append({"T": T0 + W + 1, "ms_per_step": ms, "sigma_ms_per_eval": pr["sigma_ms"] / max(pr["sigma_n"], 1),
+                     "dyson_ms_per_step": pr["dyson_ms"] / max(pr["dyson_n"], 1)})
+        log(f"[sweep] T={T0 + W + 1}: {ms:.1f} ms per step")
+    out = {"metric": "per-step time vs history length", "prec": args.prec, "nao": p.nao, "naux": p.naux,
+           "nt_run": p.nt, "points": rows}
+    if len(rows) >= 3:
+        T = np.array([r["T"] for r in rows], dtype=float)
+        y = np.array([r["ms_per_step"] for r in rows])
+        coef = np.polyfit(T, y, 2)
+        steps = np.arange(p.k + 1, p.nt + 1, dtype=float)
+        avg = float(np.mean(np.polyval(coef, steps)))
+        out["fit_ms"] = {"c2": coef[0], "c1": coef[1], "c0": coef[2]}
+        out["whole_run_avg_ms_per_step"] = avg
+        out["whole_run_timesteps_per_s"] = 1e3 / avg
+        out["extrapolated_ms_per_step"] = {str(int(t)): float(np.polyval(coef, t)) for t in (1500, 2000)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +185,9 @@ def main():
     ap.add_argument("--b-sd", type=float, default=None,
                     help="sigma_B of the Coulomb factor (default: %s for the nao=64 workload)" % STABLE_B_SD)
     ap.add_argument("--dt", type=float, default=None)
+    ap.add_argument("--sweep", default=None,
+                    help="comma-separated history lengths T0: print per-step times at each and the fitted "
+                         "whole-run average instead of the benchmark line")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -164,19 +225,14 @@ def main():
     p = problem.config(args.config, **over)
     log(f"[setup] problem c{args.config}: nao={p.nao} naux={p.naux} ntau+1={p.ntau + 1} mu={p.mu:.6f} "
         f"({time.perf_counter() - t_setup:.1f} s)")
+    if args.sweep:
+        sweep(args, p, log)
+        return
     T0 = p.nt // 2 if args.T0 is None else args.T0
     W, K = args.warmup, args.steps
     nt = T0 + W + K
     n, a = p.nao, p.naux
-    c = ned.Context(n, a, nt, p.ntau, p.k)
-    c.set_coulomb(p.B)
-    c.write_blocks(ned.NED_G, ned.NED_MAT, 0, p.Gmat)
-    c.set_matsubara(p.beta)
-    c.g0_from_h0(T0, p.fock, p.mu, p.beta, p.dt)
-    c.write_hmf(np.repeat(p.fock[None].astype(complex), T0 + 1, axis=0))
-    c.sigma(T0, "f64")
-    c.tti_extend(ned.NED_SIGMA, T0)
-    c.sync()
+    c = window_context(ned, p, T0, nt)
     log(f"[setup] history t<=T0={T0} ready ({time.perf_counter() - t_setup:.1f} s)")
 
     def run(prec, steps, profile):

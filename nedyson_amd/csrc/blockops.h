@@ -60,13 +60,24 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 
   const bool wave_on = 16 * w < n;
   constexpr int PER = BO_NMAX * BO_KT / BO_T;  // staged elements per thread and operand
   z2 px[PER], py[PER];
+  // element e of a slice -> (row, k) of X and (k, col) of Y, chosen so that
+  // consecutive threads read consecutive addresses of the stored matrix
+  // (along k when it is used as stored, along the other index when it is
+  // used transposed or adjoint)
+  auto xmap = [&](int e, int &r, int &kk) __attribute__((always_inline)) {
+    if (opX == 0) { r = e / BO_KT; kk = e % BO_KT; } else { kk = e / BO_NMAX; r = e % BO_NMAX; }
+  };
+  auto ymap = [&](int e, int &kk, int &c) __attribute__((always_inline)) {
+    if (opY == 0) { kk = e / BO_NMAX; c = e % BO_NMAX; } else { c = e / BO_KT; kk = e % BO_KT; }
+  };
   auto fetch = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int e = tid + BO_T * i;
-      const int r = e / BO_KT, kk = e % BO_KT;
+      int r, kk, kk2, c;
+      xmap(e, r, kk);
       px[i] = (r < n && k0 + kk < n) ? blk_get(X, ldX, opX, r, k0 + kk) : z2{0.0, 0.0};
-      const int kk2 = e / BO_NMAX, c = e % BO_NMAX;
+      ymap(e, kk2, c);
       py[i] = (c < n && k0 + kk2 < n) ? blk_get(Y, ldY, opY, k0 + kk2, c) : z2{0.0, 0.0};
     }
   };
@@ -77,8 +88,11 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int e = tid + BO_T * i;
-      s.X[e / BO_KT][e % BO_KT] = zmul(coef, px[i]);
-      s.Y[e / BO_NMAX][e % BO_NMAX] = py[i];
+      int r, kk, kk2, c;
+      xmap(e, r, kk);
+      ymap(e, kk2, c);
+      s.X[r][kk] = zmul(coef, px[i]);
+      s.Y[kk2][c] = py[i];
     }
     __syncthreads();
     if (k0 + BO_KT < n) fetch(k0 + BO_KT);
@@ -206,13 +220,21 @@ __device__ __forceinline__ void tile_mac(TileAcc &acc, BlkLds &s, int n, const z
   const bool on = r0 < n && c0 < n;
   constexpr int PER = BO_NMAX * BO_KT / BO_T;  // at most 4 elements per thread and operand
   z2 pl[PER], pr[PER];
+  // as in blk_mac: consecutive threads read consecutive stored addresses
+  auto lmap = [&](int e, int &r, int &kk) __attribute__((always_inline)) {
+    if (opL == 0) { r = e / BO_KT; kk = e % BO_KT; } else { kk = e / lrn; r = e % lrn; }
+  };
+  auto rmap = [&](int e, int &kk, int &c) __attribute__((always_inline)) {
+    if (opR == 0) { kk = e / rcn; c = e % rcn; } else { c = e / BO_KT; kk = e % BO_KT; }
+  };
   auto fetch = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int e = tid + BO_T * i;
-      const int r = e / BO_KT, kk = e % BO_KT;
+      int r, kk, kk2, c;
+      lmap(e, r, kk);
       pl[i] = (e < lrn * BO_KT && lr0 + r < n && k0 + kk < n) ? blk_get(L, n, opL, lr0 + r, k0 + kk) : z2{0.0, 0.0};
-      const int kk2 = e / rcn, c = e % rcn;
+      rmap(e, kk2, c);
       pr[i] = (e < BO_KT * rcn && rc0 + c < n && k0 + kk2 < n) ? blk_get(R, n, opR, k0 + kk2, rc0 + c)
                                                               : z2{0.0, 0.0};
     }
@@ -224,8 +246,11 @@ __device__ __forceinline__ void tile_mac(TileAcc &acc, BlkLds &s, int n, const z
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int e = tid + BO_T * i;
-      if (e < lrn * BO_KT) s.X[e / BO_KT][e % BO_KT] = zmul(coef, pl[i]);
-      if (e < BO_KT * rcn) s.Y[e / rcn][e % rcn] = pr[i];
+      int r, kk, kk2, c;
+      lmap(e, r, kk);
+      rmap(e, kk2, c);
+      if (e < lrn * BO_KT) s.X[r][kk] = zmul(coef, pl[i]);
+      if (e < BO_KT * rcn) s.Y[kk2][c] = pr[i];
     }
     __syncthreads();
     if (k0 + BO_KT < n) fetch(k0 + BO_KT);
