@@ -312,3 +312,60 @@ def test_dyson_step_band_parity(orc, nao, T0):
     assert _relerr(d.ret_row(nt), G_ref.ret_row(nt)) < 1e-10
     assert _relerr(d.les_col(nt), G_ref.les_col(nt)) < 1e-10
     assert _relerr(d.tv[nt], G_ref.tv[nt]) < 1e-10
+
+
+def test_c0_shape_full_run_parity(orc):
+    """The nao=4 configuration's shape over its whole run: naux=12, 64 GLL
+    nodes, dt=0.02, nt=200, k=5, the h[0,0] += 0.5 quench, FP64; every stored
+    G block against the oracle.  (B entries drawn with sigma 0.1: with the
+    configuration's 0.3, started from the Hartree-Fock state, the oracle itself
+    diverges.)"""
+    p = problem.config(0, b_sd=0.1)
+    print(f"[c0 run] nt={p.nt} ntau={p.ntau}", flush=True)
+    D, G, S, hmf = _oracle_run(orc, p, p.nt)
+    with _ctx_from(p, p.initial_green()) as c:
+        c.do_boot(p.fock, p.mu, p.beta, p.dt, "f64", 50, 1e-12)
+        for T in range(p.k + 1, p.nt + 1):
+            c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, "f64", 3)
+        d = c.download(ned.NED_G)
+        e_first = c.energy(p.k + 1, p.h0, p.beta, p.dt)
+        e_last = c.energy(p.nt, p.h0, p.beta, p.dt)
+    for comp in ("ret", "les", "tv"):
+        assert _relerr(getattr(d, comp), getattr(G, comp)) < 1e-10, comp
+    # GF2 conserves the particle number
+    assert abs(e_last[2] - e_first[2]) < 1e-6
+
+
+def test_c1_shape_window_parity(orc):
+    """The nao=32 configuration's shape (naux=128, 96 GLL nodes, dt=0.02,
+    spacing 0.3, random quench of norm 0.2) on one timestep after a 12-step
+    history, FP64, against the oracle (OpenMP over Sigma points)."""
+    import os
+    nth = min(os.cpu_count() or 1, 16)
+    T0, K = 12, 1
+    p = problem.config(1, nt=T0 + K)
+    print(f"[c1 window] nao={p.nao} naux={p.naux} ntau={p.ntau}", flush=True)
+    G = problem.free_gf(p.fock, p.mu, p.beta, p.dt, p.nt, p.ntau, T0, p.Gmat)
+    S = GreenFunc(p.nt, p.ntau, p.nao)
+    hmf = np.zeros((p.nt + 1, p.nao, p.nao), dtype=complex)
+    hmf[:T0 + 1] = p.fock
+    orc.sigma_solve(p.V, T0, S, G, nthreads=nth)
+    _tti_extend_host(S, T0)
+    D = orc.Dyson(p.nt, p.ntau, p.nao, p.k)
+    for T in range(T0 + 1, T0 + K + 1):
+        D.do_tstp(p.V, T, G, S, hmf, p.h0, p.mu, p.beta, p.dt, 3, nthreads=nth)
+    with ned.Context(p.nao, p.naux, p.nt, p.ntau, p.k) as c:
+        c.set_coulomb(p.B)
+        c.write_blocks(ned.NED_G, ned.NED_MAT, 0, p.Gmat)
+        c.set_matsubara(p.beta)
+        c.g0_from_h0(T0, p.fock, p.mu, p.beta, p.dt)
+        c.write_hmf(np.repeat(p.fock[None].astype(complex), T0 + 1, axis=0))
+        c.sigma(T0, "f64")
+        c.tti_extend(ned.NED_SIGMA, T0)
+        for T in range(T0 + 1, T0 + K + 1):
+            c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, "f64", 3)
+        d = c.download(ned.NED_G)
+    for T in range(T0 + 1, T0 + K + 1):
+        assert _relerr(d.ret_row(T), G.ret_row(T)) < 1e-10, ("ret", T)
+        assert _relerr(d.les_col(T), G.les_col(T)) < 1e-10, ("les", T)
+        assert _relerr(d.tv[T], G.tv[T]) < 1e-10, ("tv", T)
