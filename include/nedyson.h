@@ -42,6 +42,7 @@ typedef enum { NED_PREC_F64 = 0, NED_PREC_F32 = 1, NED_PREC_BF16X3 = 2 } ned_pre
 
 typedef enum { NED_G = 0, NED_SIGMA = 1 } ned_which;
 typedef enum { NED_RET = 0, NED_LES = 1, NED_TV = 2, NED_MAT = 3 } ned_comp;
+typedef enum { NED_MAT_HF = 0, NED_MAT_GF2 = 1 } ned_mat_mode;
 
 typedef struct {
   int nao;   /* orbitals */
@@ -87,6 +88,26 @@ ned_status ned_sigma_gf2_decomp(ned_ctx *ctx, int tstp, int prec);
 /* G.get_dm(tstp, rho); hmf[tstp] = h0; molHFSolverDecomp::solve_HF(tstp, hmf, rho)
  * (src/greens.h:56-65, src/molHFSolver.cpp:81-110).  h0 is real nao x nao. */
 ned_status ned_hf_decomp(ned_ctx *ctx, int tstp, const double *h0);
+
+/* Equilibrium state: the gfmol Matsubara solve the reference runs before
+ * propagating (gfmol::DecompSimulation, constructed at
+ * src/molNEsim_decomp_impl.h:21 and run at :50-52; its G^M and Sigma^M are
+ * sampled on the collocation nodes at :145-165, and its mu() and fock() feed
+ * G0_from_h0 at :44-46).
+ *   G^M = G0 + G0 o Sigma^M o G^M on the ntau+1 GLL nodes, with
+ *   G0 = G0[F - mu], F = h1 + 2J[rho] - K[rho], rho = -G^M(beta), and
+ *   Sigma^M the second-Born self-energy (mode NED_MAT_GF2) or 0 (NED_MAT_HF).
+ * mu is found so that 2 tr rho = nelec, unless fix_mu is nonzero (then *mu_io
+ * is used as given and nelec is ignored).  Starts from the G^M held in the
+ * context, or from G0[h1] if that is zero.  Outputs: G^M and Sigma^M in the
+ * context, the T = 0 slices of G and Sigma and hmf[0] consistent with them,
+ * *mu_io, fock_out (nao x nao, the F of the solution), the number of outer
+ * iterations and the last update max|dG^M|.  Returns NED_ERR_NOCONV if
+ * max|dG^M| >= tol after maxiter iterations.  Real orbitals only (the
+ * Coulomb factor is real symmetric); call ned_set_matsubara(beta) afterwards. */
+ned_status ned_matsubara_gf2(ned_ctx *ctx, const double *h1, double nelec, double beta, int mode, int fix_mu,
+                             double *mu_io, int maxiter, double tol, double *fock_out, int *iters_out,
+                             double *err_out);
 
 /* Imaginary-time setup: builds the mixing kernel of G^M (the decomposed
  * path's GMCT, src/greens.cpp:112-146, initialised here as the full path does,

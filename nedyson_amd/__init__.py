@@ -23,13 +23,14 @@ LIBPATH = os.path.join(_HERE, "libnedyson_amd.so")
 NED_PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
 NED_G, NED_SIGMA = 0, 1
 NED_RET, NED_LES, NED_TV, NED_MAT = 0, 1, 2, 3
+NED_MAT_HF, NED_MAT_GF2 = 0, 1
 _STATUS = {0: "NED_OK", 1: "NED_ERR_ARG", 2: "NED_ERR_NOMEM", 3: "NED_ERR_DEVICE", 4: "NED_ERR_NONFINITE",
            5: "NED_ERR_NOCONV", 6: "NED_ERR_STATE"}
 
 # every symbol declared in include/nedyson.h
 EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "ned_set_coulomb", "ned_green_write",
            "ned_green_read", "ned_hmf_write", "ned_hmf_read", "ned_sigma_gf2_decomp", "ned_hf_decomp",
-           "ned_set_matsubara", "ned_dyson_step", "ned_extrapolate", "ned_dyson_start", "ned_do_boot", "ned_do_tstp",
+           "ned_set_matsubara", "ned_matsubara_gf2", "ned_dyson_step", "ned_extrapolate", "ned_dyson_start", "ned_do_boot", "ned_do_tstp",
            "ned_energy", "ned_g0_from_h0", "ned_tti_extend", "ned_profile", "ned_profile_read", "ned_gll_nodes",
            "ned_integrator", "ned_mlp_create", "ned_mlp_destroy", "ned_mlp_nparams", "ned_mlp_get_params",
            "ned_mlp_set_params", "ned_mlp_loss_grad", "ned_mlp_adam_step", "ned_mlp_train", "ned_mlp_predict",
@@ -78,6 +79,8 @@ def lib():
     L.ned_sigma_gf2_decomp.argtypes = [_vp, C.c_int, C.c_int]
     L.ned_hf_decomp.argtypes = [_vp, C.c_int, _dp]
     L.ned_set_matsubara.argtypes = [_vp, C.c_double]
+    L.ned_matsubara_gf2.argtypes = [_vp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp, C.c_int, C.c_double,
+                                    _dp, C.POINTER(C.c_int), _dp]
     L.ned_dyson_step.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
     L.ned_extrapolate.argtypes = [_vp, C.c_int]
     L.ned_dyson_start.argtypes = [_vp, C.c_double, C.c_double, C.c_double, _dp]
@@ -232,6 +235,23 @@ class Context:
 
     def set_matsubara(self, beta):
         _check(lib().ned_set_matsubara(self.h, beta))
+
+    def matsubara_gf2(self, h1, beta, nelec=None, mu=None, mode="gf2", maxiter=200, tol=1e-10):
+        """Equilibrium G^M (GF2 or HF) on the GLL nodes, written into the
+        context (see ned_matsubara_gf2).  With nelec, mu is solved for (mu is
+        then only the starting value); without it, mu is held fixed.
+        Returns (mu, fock, iterations, last update)."""
+        h1 = np.ascontiguousarray(h1, dtype=np.float64)
+        fix = nelec is None
+        if fix and mu is None:
+            raise ValueError("give nelec or a fixed mu")
+        m = C.c_double(0.0 if mu is None else float(mu))
+        fock = np.zeros((self.nao, self.nao))
+        it, err = C.c_int(0), C.c_double(0.0)
+        _check(lib().ned_matsubara_gf2(self.h, _dptr(h1), 0.0 if fix else float(nelec), float(beta),
+                                       NED_MAT_GF2 if mode == "gf2" else NED_MAT_HF, int(fix), C.byref(m), maxiter,
+                                       tol, _dptr(fock), C.byref(it), C.byref(err)))
+        return m.value, fock, it.value, err.value
 
     def dyson_step(self, tstp, mu, beta, dt):
         e = np.zeros(3)

@@ -243,6 +243,25 @@ ned_status ned_set_matsubara(ned_ctx *ctx, double beta) {
   });
 }
 
+ned_status ned_matsubara_gf2(ned_ctx *ctx, const double *h1, double nelec, double beta, int mode, int fix_mu,
+                             double *mu_io, int maxiter, double tol, double *fock_out, int *iters_out,
+                             double *err_out) {
+  if (!ctx || !h1 || !mu_io || !fock_out) { g_err = "null argument"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::require(beta > 0 && std::isfinite(beta), "beta must be positive");
+    ned::require(mode == NED_MAT_HF || mode == NED_MAT_GF2, "mode must be NED_MAT_HF or NED_MAT_GF2");
+    ned::require(fix_mu || (nelec > 0 && nelec < 2.0 * ctx->c.nao), "nelec must lie in (0, 2 nao)");
+    ned::require(maxiter > 0 && tol > 0, "maxiter and tol must be positive");
+    int it = 0;
+    double err = 0;
+    ned::matsubara_gf2(ctx->c, h1, nelec, beta, mode == NED_MAT_GF2, fix_mu != 0, mu_io, maxiter, tol, fock_out, &it,
+                       &err);
+    if (iters_out) *iters_out = it;
+    if (err_out) *err_out = err;
+    if (!(err < tol)) throw std::runtime_error("matsubara: self-consistency did not converge");
+  });
+}
+
 ned_status ned_dyson_step(ned_ctx *ctx, int tstp, double mu, double beta, double dt, double *err_out) {
   if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
   return guard(ctx, [&] {

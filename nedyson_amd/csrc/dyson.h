@@ -24,6 +24,11 @@ void extrapolate(Ctx &c, int tstp);
 void hf_decomp(Ctx &c, int tstp, const double *h0_host);
 double dyson_start(Ctx &c, double mu, double beta, double dt);
 void energy(Ctx &c, int tstp, const double *h0_host, double beta, double dt, double *out3);
+// Equilibrium GF2 (corr) or HF Matsubara function on the GLL nodes; writes
+// G^M (G.mat), Sigma^M (S.mat), the T = 0 slices of G and Sigma and hmf[0].
+void matsubara_gf2(Ctx &c, const double *h1, double nelec, double beta, bool corr, bool fix_mu, double *mu_io,
+                   int maxiter, double tol, double *fock_out, int *iters_out, double *err_out);
+int lu_solve_aug(hipStream_t s, double *Aug, int D, int nrhs, int *ipiv, int *info);
 void invalidate_minv(Ctx &c);
 // mark hmf(t) / Sigma slice t as changed: drops the cached inverses that use it
 void touch_time(Ctx &c, int t);

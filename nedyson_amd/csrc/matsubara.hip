@@ -1,0 +1,644 @@
+// Equilibrium (Matsubara) GF2 solver on the Gauss-Lobatto-Legendre grid: the
+// state the reference takes from gfmol before propagating
+// (src/molNEsim_decomp_impl.h:14-52 constructs gfmol::DecompSimulation,
+// :50-52 runs it, :145-165 samples its G^M and Sigma^M on the collocation
+// nodes; src/molNEsim.cpp:101-133 is the order of the calls).
+//
+// Unknown: G^M(tau_m) on the np = ntau+1 nodes, real for real orbitals.
+// Equation (fermions, sig = -1):
+//   G = G0 + G0 o Sigma o G,  (A o B)(tau) = int_0^beta A(tau - t) B(t) dt,
+// with A(s < 0) = sig A(s + beta), G0 the Matsubara function of the Fock
+// matrix F[rho] = h + 2J - K, rho = -G(beta), and Sigma the second-Born
+// self-energy.  Sigma is evaluated by the real-time kernels on the T = 0
+// slice: with G^tv(0, tau_m) = -i G^M(beta - tau_m) (the equilibrium form of
+// the tv component), Sigma^M(beta - tau_m) = i Sigma^tv(0, tau_m).  This pairing
+// of conventions is the one under which the tv Dyson equation
+// (src/dyson_step.h:130-169) is stationary at t = 0.
+//
+// Discretisation: the convolution of nodal interpolants is exact
+// (GLL::conv_weights).  In the eigenbasis of F, G0 is diagonal, so its
+// convolution operator is block diagonal (one np x np block C_p per orbital)
+// and the linear system
+//   (I - C K_Sigma) G' = G0'
+// (D = np * nao unknowns per column, rows ordered (m, orbital)) is assembled
+// with an np*D*D product and solved by a blocked LU with partial pivoting on
+// the GPU (panel: one workgroup; trailing update: the FP64 MFMA GEMM).
+//
+// Self-consistency: for fixed Sigma and F the chemical potential is found by
+// a bracketed Illinois iteration on 2 tr rho(mu) = nelec (each evaluation one
+// solve); the outer loop mixes G with Anderson acceleration.
+#include "dyson.h"
+#include "quadrature.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+namespace ned {
+
+namespace {
+
+constexpr int LU_NB = 32;   // panel width
+constexpr int LU_PT = 1024; // panel workgroup
+
+inline unsigned nb256(long long n) { return (unsigned)((n + 255) / 256); }
+
+// T = 0 slice of G from the real G^M: tv(0,m) = -i G^M(tau_{ntau-m}),
+// les(0,0) = tv(0,0), ret(0,0) = -i
+__global__ void kern_fill_t0(const double *Gr, int n, int np, z2 *ret0, z2 *les0, z2 *tv0) {
+  const long long es = (long long)n * n;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)np * es) return;
+  const int m = (int)(idx / es), e = (int)(idx % es);
+  const double g = Gr[(long long)(np - 1 - m) * es + e];
+  tv0[idx] = z2{0.0, -g};
+  if (m == 0) {
+    les0[e] = z2{0.0, -g};
+    ret0[e] = z2{0.0, (e / n == e % n) ? -1.0 : 0.0};
+  }
+}
+
+// Sigma^M(tau_{ntau-m}) = i Sigma^tv(0, tau_m): real part into Sr, full value into Smat
+__global__ void kern_sigma_mat(const z2 *Stv0, int n, int np, double *Sr, z2 *Smat) {
+  const long long es = (long long)n * n;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)np * es) return;
+  const int m = (int)(idx / es), e = (int)(idx % es);
+  const z2 s = Stv0[idx];
+  const long long o = (long long)(np - 1 - m) * es + e;
+  Sr[o] = -s.y;
+  if (Smat) Smat[o] = z2{-s.y, s.x};
+}
+
+// G^M (real) -> mat blocks (complex)
+__global__ void kern_store_mat(const double *Gr, long long cnt, z2 *mat) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < cnt) mat[idx] = z2{Gr[idx], 0.0};
+}
+
+// right-hand side G0' (rows (m,p), columns q): g[p][m] on the diagonal
+__global__ void kern_rhs(const double *gT, int n, int np, double *Aug, long long lda, int D) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)D * n) return;
+  const int row = (int)(idx / n), q = (int)(idx % n);
+  const int m = row / n, p = row % n;
+  Aug[(long long)row * lda + D + q] = (p == q) ? gT[(long long)p * np + m] : 0.0;
+}
+
+__global__ void kern_add_identity(double *A, long long lda, int D) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D) A[(long long)i * lda + i] += 1.0;
+}
+
+// ---------------------------------------------------------------- LU
+// Panel c0..c0+kb-1 of rows c0..D-1, partial pivoting; ipiv[col] = pivot row.
+// One workgroup; info[0] = first zero pivot column + 1.
+__global__ void __launch_bounds__(LU_PT) kern_lu_panel(double *A, long long lda, int D, int c0, int kb, int *ipiv,
+                                                      int *info) {
+  __shared__ double sval[LU_PT];
+  __shared__ int sidx[LU_PT];
+  __shared__ double prow[LU_NB];
+  const int tid = threadIdx.x;
+  for (int c = 0; c < kb; c++) {
+    const int col = c0 + c;
+    double best = -1.0;
+    int bi = col;
+    for (int r = col + tid; r < D; r += LU_PT) {
+      const double v = fabs(A[(long long)r * lda + col]);
+      if (v > best) { best = v; bi = r; }
+    }
+    sval[tid] = best;
+    sidx[tid] = bi;
+    __syncthreads();
+    for (int s = LU_PT / 2; s > 0; s >>= 1) {
+      if (tid < s) {
+        const double o = sval[tid + s];
+        const int oi = sidx[tid + s];
+        if (o > sval[tid] || (o == sval[tid] && oi < sidx[tid])) { sval[tid] = o; sidx[tid] = oi; }
+      }
+      __syncthreads();
+    }
+    const int p = sidx[0];
+    if (tid < kb && p != col) {
+      double *a = A + (long long)col * lda + c0 + tid, *b = A + (long long)p * lda + c0 + tid;
+      const double t = *a;
+      *a = *b;
+      *b = t;
+    }
+    if (tid == 0) ipiv[col] = p;
+    __syncthreads();
+    if (tid < kb) prow[tid] = A[(long long)col * lda + c0 + tid];
+    __syncthreads();
+    const double piv = prow[c];
+    if (piv == 0.0) {
+      if (tid == 0 && info[0] == 0) info[0] = col + 1;
+      continue;
+    }
+    const double inv = 1.0 / piv;
+    for (int r = col + 1 + tid; r < D; r += LU_PT) {
+      double *row = A + (long long)r * lda + c0;
+      const double l = row[c] * inv;
+      row[c] = l;
+      for (int j = c + 1; j < kb; j++) row[j] -= l * prow[j];
+    }
+    __syncthreads();
+  }
+}
+
+// apply the panel's row interchanges to columns j0..ncol-1
+__global__ void kern_lu_swap(double *A, long long lda, int c0, int kb, const int *ipiv, int j0, int ncol) {
+  const int j = j0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  for (int c = 0; c < kb; c++) {
+    const int p = ipiv[c0 + c];
+    if (p != c0 + c) {
+      double *a = A + (long long)(c0 + c) * lda + j, *b = A + (long long)p * lda + j;
+      const double t = *a;
+      *a = *b;
+      *b = t;
+    }
+  }
+}
+
+// rows c0..c0+kb-1, columns j0..ncol-1: X = L11^{-1} X (unit lower L11)
+__global__ void kern_lu_trsm(double *A, long long lda, int c0, int kb, int j0, int ncol) {
+  __shared__ double L[LU_NB][LU_NB + 1];
+  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) L[e / kb][e % kb] = A[(long long)(c0 + e / kb) * lda + c0 + e % kb];
+  __syncthreads();
+  const int j = j0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  double x[LU_NB];
+#pragma unroll
+  for (int r = 0; r < LU_NB; r++)
+    if (r < kb) x[r] = A[(long long)(c0 + r) * lda + j];
+#pragma unroll
+  for (int r = 0; r < LU_NB; r++) {
+    if (r < kb) {
+      double s = x[r];
+#pragma unroll
+      for (int c = 0; c < r; c++) s -= L[r][c] * x[c];
+      x[r] = s;
+      A[(long long)(c0 + r) * lda + j] = s;
+    }
+  }
+}
+
+// rows i0..i0+kb-1 of the right-hand sides (columns D..D+nrhs-1): X = U_II^{-1} X
+__global__ void kern_lu_back_diag(double *A, long long lda, int i0, int kb, int D, int nrhs) {
+  __shared__ double U[LU_NB][LU_NB + 1];
+  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) U[e / kb][e % kb] = A[(long long)(i0 + e / kb) * lda + i0 + e % kb];
+  __syncthreads();
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nrhs) return;
+  double x[LU_NB];
+#pragma unroll
+  for (int r = 0; r < LU_NB; r++)
+    if (r < kb) x[r] = A[(long long)(i0 + r) * lda + D + q];
+#pragma unroll
+  for (int r = LU_NB - 1; r >= 0; r--) {
+    if (r >= kb) continue;
+    double s = x[r];
+#pragma unroll
+    for (int c = 0; c < LU_NB; c++)
+      if (c > r && c < kb) s -= U[r][c] * x[c];
+    x[r] = s / U[r][r];
+    A[(long long)(i0 + r) * lda + D + q] = x[r];
+  }
+}
+
+// ---------------------------------------------------------------- Anderson
+// out[i] += <r, H_i> for i < nh (H rows of length L), out[nh] = max |r| (bits)
+__global__ void kern_dots(const double *r, const double *H, int nh, long long L, double *out,
+                          unsigned long long *amax) {
+  __shared__ double red[8][256];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double mx = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < L; i += (long long)gridDim.x * blockDim.x) {
+    const double v = r[i];
+    mx = fmax(mx, fabs(v));
+#pragma unroll
+    for (int h = 0; h < 8; h++)
+      if (h < nh) acc[h] += v * H[(long long)h * L + i];
+  }
+  for (int h = 0; h < nh; h++) red[h][threadIdx.x] = acc[h];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int h = 0; h < nh; h++) red[h][threadIdx.x] += red[h][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < nh) atomicAdd(&out[threadIdx.x], red[threadIdx.x][0]);
+  // max |r|: non-negative doubles order like their bit patterns
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(amax, (unsigned long long)__double_as_longlong(mx));
+}
+
+// x = sum_i c_i v_i over up to 32 vectors
+constexpr int COMBO_MAX = 32;
+struct Combo {
+  const double *v[COMBO_MAX];
+  double c[COMBO_MAX];
+  int nv;
+};
+__global__ void kern_combo(Combo k, long long L, double *x) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < L; i += (long long)gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int j = 0; j < k.nv; j++) s += k.c[j] * k.v[j][i];
+    x[i] = s;
+  }
+}
+
+__global__ void kern_sub(const double *a, const double *b, long long L, double *out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < L) out[i] = a[i] - b[i];
+}
+
+// real GEMM on row-major operands: C = alpha op(A) op(B) (+ C)
+void dgemm(hipStream_t s, int M, int N, int K, double alpha, const double *A, long long ams, long long aks,
+           const double *B, long long bks, long long bms, double *C, long long crs, int beta, int batch = 1,
+           long long abs_ = 0, long long bbs = 0, long long cbs = 0) {
+  GemmDesc d;
+  d.M = M; d.N = N; d.K = K; d.batch = batch;
+  d.A.ptr = A; d.A.ms = ams; d.A.ks_lo = aks; d.A.bs_lo = abs_;
+  d.B.ptr = B; d.B.ms = bms; d.B.ks_lo = bks; d.B.bs_lo = bbs;
+  d.C.ptr = C; d.C.rs = crs; d.C.cs = 1; d.C.bs_lo = cbs;
+  d.alpha = alpha;
+  d.beta = beta;
+  gemm(d, PREC_F64, s);
+}
+
+}  // namespace
+
+// Solve Aug[:, 0:D] X = Aug[:, D:D+nrhs] in place (X overwrites the
+// right-hand sides); Aug is row-major D x (D + nrhs).  Returns the first zero
+// pivot (1-based) or 0.
+int lu_solve_aug(hipStream_t s, double *Aug, int D, int nrhs, int *ipiv, int *info) {
+  const long long lda = (long long)D + nrhs;
+  const int ncol = D + nrhs;
+  NED_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
+  for (int c0 = 0; c0 < D; c0 += LU_NB) {
+    const int kb = std::min(LU_NB, D - c0);
+    hipLaunchKernelGGL(kern_lu_panel, dim3(1), dim3(LU_PT), 0, s, Aug, lda, D, c0, kb, ipiv, info);
+    const int j0 = c0 + kb, nc = ncol - j0;
+    if (nc <= 0) continue;
+    hipLaunchKernelGGL(kern_lu_swap, dim3(nb256(nc)), dim3(256), 0, s, Aug, lda, c0, kb, ipiv, j0, ncol);
+    hipLaunchKernelGGL(kern_lu_trsm, dim3(nb256(nc)), dim3(256), 0, s, Aug, lda, c0, kb, j0, ncol);
+    const int mr = D - j0;
+    if (mr > 0)
+      dgemm(s, mr, nc, kb, -1.0, Aug + (long long)j0 * lda + c0, lda, 1, Aug + (long long)c0 * lda + j0, lda, 1,
+            Aug + (long long)j0 * lda + j0, lda, 1);
+  }
+  for (int i0 = ((D - 1) / LU_NB) * LU_NB; i0 >= 0; i0 -= LU_NB) {
+    const int kb = std::min(LU_NB, D - i0), kr = D - i0 - kb;
+    if (kr > 0)
+      dgemm(s, kb, nrhs, kr, -1.0, Aug + (long long)i0 * lda + i0 + kb, lda, 1, Aug + (long long)(i0 + kb) * lda + D,
+            lda, 1, Aug + (long long)i0 * lda + D, lda, 1);
+    hipLaunchKernelGGL(kern_lu_back_diag, dim3(nb256(nrhs)), dim3(256), 0, s, Aug, lda, i0, kb, D, nrhs);
+  }
+  check_launch("lu_solve");
+  int h = 0;
+  NED_HIP(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, s));
+  NED_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+namespace {
+
+double fermi(double beta, double w) {
+  const double a = w * beta;
+  if (a > 700) return 0.0;
+  if (a < -700) return 1.0;
+  return 1.0 / (1.0 + std::exp(a));
+}
+
+// -G0 of one level: eigenvalue eps of F, at tau in [0, beta]
+double g0_level(double eps, double mu, double tau, double beta) {
+  const double l = mu - eps;
+  return l < 0 ? -std::exp(l * tau) * fermi(beta, l) : -std::exp((tau - beta) * l) * fermi(beta, -l);
+}
+
+struct MatSolver {
+  Ctx &c;
+  int n, np, D;
+  long long es, L;
+  double beta;
+  std::vector<double> tau;
+  DevBuf w, Gr, Gn, Sr, Sp, tmp, K, Aug, gT, Cp, ipiv, info, V, dots, res;
+  std::vector<double> eps, Vh;
+
+  MatSolver(Ctx &cc, double b) : c(cc), n(cc.nao), np(cc.np), beta(b) {
+    D = np * n;
+    es = (long long)n * n;
+    L = (long long)np * es;
+    GLL g(c.ntau);
+    std::vector<double> wh;
+    g.conv_weights(beta, c.sig, wh);
+    w.ensure(wh.size() * sizeof(double));
+    NED_HIP(hipMemcpyAsync(w.p, wh.data(), wh.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    tau.resize(np);
+    for (int m = 0; m < np; m++) tau[m] = beta * (g.x[m] + 1) / 2;
+    for (DevBuf *b2 : {&Gr, &Gn, &Sr, &Sp, &tmp}) b2->ensure(L * sizeof(double));
+    K.ensure((size_t)D * D * sizeof(double));
+    Aug.ensure((size_t)D * (D + n) * sizeof(double));
+    gT.ensure((size_t)n * np * sizeof(double));
+    Cp.ensure((size_t)n * np * np * sizeof(double));
+    ipiv.ensure((size_t)D * sizeof(int));
+    info.ensure(sizeof(int));
+    V.ensure(es * sizeof(double));
+    dots.ensure(32 * sizeof(double));
+    NED_HIP(hipStreamSynchronize(c.stream));  // wh is released on return
+  }
+
+  // Sigma^M[G] (real) into Sr, F[G] on the host; eigenbasis of F; Sigma' = V^T Sigma V; K_Sigma'
+  void prepare(const double *h1, bool corr, std::vector<double> &F) {
+    hipStream_t s = c.stream;
+    hipLaunchKernelGGL(kern_fill_t0, dim3(nb256(L)), dim3(256), 0, s, Gr.as<double>(), n, np, c.G.ret.as<z2>(),
+                       c.G.les.as<z2>(), c.G.tv.as<z2>());
+    check_launch("matsubara: T=0 slice");
+    hf_decomp(c, 0, h1);
+    F.resize(es);
+    std::vector<z2> Fz(es);
+    NED_HIP(hipMemcpyAsync(Fz.data(), c.hmf.p, es * sizeof(z2), hipMemcpyDeviceToHost, s));
+    NED_HIP(hipStreamSynchronize(s));
+    for (long long e = 0; e < es; e++) F[e] = 0.5 * (Fz[e].x + Fz[(e % n) * n + e / n].x);
+    eps.resize(n);
+    Vh.resize(es);
+    sym_eig(n, F.data(), eps.data(), Vh.data());
+    NED_HIP(hipMemcpyAsync(V.p, Vh.data(), es * sizeof(double), hipMemcpyHostToDevice, s));
+    if (corr) {
+      sigma_gf2(c, 0, PREC_F64);
+      hipLaunchKernelGGL(kern_sigma_mat, dim3(nb256(L)), dim3(256), 0, s, c.S.tv.as<z2>(), n, np, Sr.as<double>(),
+                         c.S.mat.p ? c.S.mat.as<z2>() : nullptr);
+      check_launch("matsubara: Sigma^M");
+    } else {
+      NED_HIP(hipMemsetAsync(Sr.p, 0, L * sizeof(double), s));
+    }
+    // Sigma'_j = V^T Sigma_j V
+    dgemm(s, n, n, n, 1.0, Sr.as<double>(), n, 1, V.as<double>(), n, 1, tmp.as<double>(), n, 0, np, es, 0, es);
+    dgemm(s, n, n, n, 1.0, V.as<double>(), 1, n, tmp.as<double>(), n, 1, Sp.as<double>(), n, 0, np, 0, es, es);
+    // K[(m,p),(m',q)] = sum_j w[m][j][m'] Sigma'_j[p][q]; batch (m, p): rows m', columns q
+    GemmDesc d;
+    d.M = np; d.N = n; d.K = np; d.batch = np * n;
+    d.A.ptr = w.p; d.A.ms = 1; d.A.ks_lo = np; d.A.bdiv = n; d.A.bs_hi = (long long)np * np; d.A.bs_lo = 0;
+    d.B.ptr = Sp.p; d.B.ms = 1; d.B.ks_lo = es; d.B.bdiv = n; d.B.bs_hi = 0; d.B.bs_lo = n;
+    d.C.ptr = K.p; d.C.rs = n; d.C.cs = 1; d.C.bs_lo = D;
+    gemm(d, PREC_F64, s);
+  }
+
+  // G' for the current K and mu into Aug's right-hand-side columns; returns 2 tr rho
+  double solve(double mu) {
+    hipStream_t s = c.stream;
+    std::vector<double> g((size_t)n * np);
+    for (int p = 0; p < n; p++)
+      for (int m = 0; m < np; m++) g[(size_t)p * np + m] = g0_level(eps[p], mu, tau[m], beta);
+    NED_HIP(hipMemcpyAsync(gT.p, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    // C_p[m][m'] = sum_j w[m][j][m'] g_p(tau_j): batch m, rows p, columns m'
+    dgemm(s, n, np, np, 1.0, gT.as<double>(), np, 1, w.as<double>(), np, 1, Cp.as<double>(), (long long)np * np, 0, np,
+          0, (long long)np * np, np);
+    // Aug[(m,p), 0:D] = -sum_m'' C_p[m][m''] K[(m'',p), :]: batch p
+    const long long lda = (long long)D + n;
+    dgemm(s, np, D, np, -1.0, Cp.as<double>(), np, 1, K.as<double>(), (long long)n * D, 1, Aug.as<double>(), n * lda, 0,
+          n, (long long)np * np, D, lda);
+    hipLaunchKernelGGL(kern_add_identity, dim3(nb256(D)), dim3(256), 0, s, Aug.as<double>(), lda, D);
+    hipLaunchKernelGGL(kern_rhs, dim3(nb256((long long)D * n)), dim3(256), 0, s, gT.as<double>(), n, np,
+                       Aug.as<double>(), lda, D);
+    check_launch("matsubara: system");
+    int bad = lu_solve_aug(s, Aug.as<double>(), D, n, ipiv.as<int>(), info.as<int>());
+    if (bad) throw std::runtime_error("matsubara: singular Dyson system (the iteration did not converge)");
+    // 2 tr G'(beta) rows (ntau, p), column p
+    std::vector<double> last((size_t)n * lda);
+    NED_HIP(hipMemcpyAsync(last.data(), Aug.as<double>() + (size_t)(np - 1) * n * lda, last.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    NED_HIP(hipStreamSynchronize(s));
+    double tr = 0;
+    for (int p = 0; p < n; p++) tr += last[(size_t)p * lda + D + p];
+    return -2.0 * tr;
+  }
+
+  // Gn = V G' V^T from Aug's right-hand sides
+  void rotate_back() {
+    hipStream_t s = c.stream;
+    const long long lda = (long long)D + n;
+    // tmp_m = G'_m V^T: G'_m rows p at Aug[(m*n+p)*lda + D + q]
+    dgemm(s, n, n, n, 1.0, Aug.as<double>() + D, lda, 1, V.as<double>(), 1, n, tmp.as<double>(), n, 0, np, n * lda, 0,
+          es);
+    dgemm(s, n, n, n, 1.0, V.as<double>(), n, 1, tmp.as<double>(), n, 1, Gn.as<double>(), n, 0, np, 0, es, es);
+  }
+};
+
+}  // namespace
+
+void matsubara_gf2(Ctx &c, const double *h1, double nelec, double beta, bool corr, bool fix_mu, double *mu_io,
+                   int maxiter, double tol, double *fock_out, int *iters_out, double *err_out) {
+  require(c.have_coulomb, "matsubara: Coulomb tensor not set");
+  require(beta > 0, "matsubara: beta must be positive");
+  require(c.sig == -1, "matsubara: fermions only");
+  require(fix_mu || nelec > 0, "matsubara: nelec must be positive");
+  hipStream_t s = c.stream;
+  MatSolver ms(c, beta);
+  const int n = c.nao, np = c.np;
+  const long long L = ms.L;
+  std::vector<double> F(ms.es);
+  double mu = *mu_io;
+
+  // start: G0 of the Fock matrix of the given G^M, or of h1 if G^M is empty
+  {
+    std::vector<z2> gm(L);
+    NED_HIP(hipMemcpyAsync(gm.data(), c.G.mat.p, L * sizeof(z2), hipMemcpyDeviceToHost, s));
+    NED_HIP(hipStreamSynchronize(s));
+    std::vector<double> gr(L);
+    double nz = 0;
+    for (long long i = 0; i < L; i++) { gr[i] = gm[i].x; nz = std::max(nz, std::fabs(gm[i].x)); }
+    if (nz == 0) {
+      // G^M of h1 with mu placing nelec/2 levels below it
+      std::vector<double> e(n), Vv(ms.es);
+      sym_eig(n, h1, e.data(), Vv.data());
+      if (!fix_mu) {
+        const int occ = std::max(1, std::min(n - 1, (int)std::lround(nelec / 2)));
+        mu = 0.5 * (e[occ - 1] + e[std::min(occ, n - 1)]);
+      }
+      for (int m = 0; m < np; m++)
+        for (int a = 0; a < n; a++)
+          for (int b = 0; b < n; b++) {
+            double v = 0;
+            for (int p = 0; p < n; p++) v += Vv[a * n + p] * g0_level(e[p], mu, ms.tau[m], beta) * Vv[b * n + p];
+            gr[(size_t)m * ms.es + a * n + b] = v;
+          }
+    }
+    NED_HIP(hipMemcpyAsync(ms.Gr.p, gr.data(), L * sizeof(double), hipMemcpyHostToDevice, s));
+    NED_HIP(hipStreamSynchronize(s));
+  }
+
+  // Anderson history (type II): X_i, R_i
+  constexpr int HIST = 6;
+  DevBuf Xh, Rh, r;
+  Xh.ensure((size_t)HIST * L * sizeof(double));
+  Rh.ensure((size_t)HIST * L * sizeof(double));
+  r.ensure(L * sizeof(double));
+  int nh = 0, head = 0;
+  std::vector<double> RR(HIST * HIST, 0.0);  // <R_i, R_j> of the stored residuals
+  double err = INFINITY;
+  double dmu = 0.5;
+  int it = 0;
+  for (; it < maxiter; it++) {
+    ms.prepare(h1, corr, F);
+    if (fix_mu) {
+      ms.solve(mu);
+    } else {
+      auto f = [&](double m) { return ms.solve(m) - nelec; };
+      double step = std::max(1e-3, 2 * dmu);
+      double a = mu - step, b = mu + step;
+      double fa = f(a), fb = f(b);
+      for (int e = 0; fa > 0 && e < 60; e++) { a -= step; step *= 2; fa = f(a); }
+      for (int e = 0; fb < 0 && e < 60; e++) { b += step; step *= 2; fb = f(b); }
+      if (!(fa <= 0 && fb >= 0))
+        throw std::runtime_error("matsubara: the particle number did not converge (no bracketing mu)");
+      double x = mu, fx = 1;
+      int side = 0;
+      for (int e = 0; e < 200; e++) {
+        x = (a * fb - b * fa) / (fb - fa);
+        fx = f(x);
+        if (std::fabs(fx) < 1e-12 * std::max(1.0, nelec) || b - a < 1e-14 * std::max(1.0, std::fabs(x))) break;
+        if (fx < 0) {
+          a = x; fa = fx;
+          if (side == -1) fb *= 0.5;
+          side = -1;
+        } else {
+          b = x; fb = fx;
+          if (side == 1) fa *= 0.5;
+          side = 1;
+        }
+      }
+      dmu = std::fabs(x - mu);
+      mu = x;
+    }
+    ms.rotate_back();
+    // r = Gn - G; <r, R_i>; max |r|
+    hipLaunchKernelGGL(kern_sub, dim3(nb256(L)), dim3(256), 0, s, ms.Gn.as<double>(), ms.Gr.as<double>(), L, r.as<double>());
+    NED_HIP(hipMemsetAsync(ms.dots.p, 0, 32 * sizeof(double), s));
+    unsigned long long *amax = reinterpret_cast<unsigned long long *>(ms.dots.as<double>() + 16);
+    hipLaunchKernelGGL(kern_dots, dim3(256), dim3(256), 0, s, r.as<double>(), Rh.as<double>(), nh, L,
+                       ms.dots.as<double>(), amax);
+    // <r, r> with a second pass (one vector)
+    hipLaunchKernelGGL(kern_dots, dim3(256), dim3(256), 0, s, r.as<double>(), r.as<double>(), 1, L,
+                       ms.dots.as<double>() + 8, amax);
+    check_launch("matsubara: residual");
+    double hd[17];
+    NED_HIP(hipMemcpyAsync(hd, ms.dots.p, sizeof(hd), hipMemcpyDeviceToHost, s));
+    NED_HIP(hipStreamSynchronize(s));
+    unsigned long long bits;
+    std::memcpy(&bits, &hd[16], sizeof(bits));
+    double em;
+    std::memcpy(&em, &bits, sizeof(em));
+    err = em;
+    if (!std::isfinite(err)) throw std::runtime_error("matsubara: non-finite update");
+    if (err < tol) {
+      NED_HIP(hipMemcpyAsync(ms.Gr.p, ms.Gn.p, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+      it++;
+      break;
+    }
+    // order of the stored entries, oldest first
+    std::vector<int> ord;
+    for (int i = 0; i < nh; i++) ord.push_back((head - nh + i + HIST) % HIST);
+    // Anderson step over the differences of consecutive entries plus the current one
+    Combo k{};
+    k.nv = 0;
+    static_assert(2 + 4 * HIST <= COMBO_MAX, "Anderson combination exceeds Combo");
+    auto add = [&](const double *v, double cc) {
+      if (k.nv >= COMBO_MAX) throw std::logic_error("matsubara: Anderson combination too long");
+      k.v[k.nv] = v;
+      k.c[k.nv] = cc;
+      k.nv++;
+    };
+    add(ms.Gr.as<double>(), 1.0);
+    add(r.as<double>(), 1.0);
+    if (nh >= 1) {
+      // dR_i = R_{i+1} - R_i (i = 0..nh-1, with R_nh = r), dX_i likewise with X_nh = G
+      const int md = nh;
+      auto rr = [&](int i, int j) -> double {  // <R_i, R_j> (logical order; index nh is the current r)
+        if (i == nh && j == nh) return hd[8];
+        if (i == nh) return hd[ord[j]];
+        if (j == nh) return hd[ord[i]];
+        return RR[ord[i] * HIST + ord[j]];
+      };
+      std::vector<double> A(md * md), bvec(md);
+      for (int i = 0; i < md; i++) {
+        for (int j = 0; j < md; j++)
+          A[i * md + j] = rr(i + 1, j + 1) - rr(i + 1, j) - rr(i, j + 1) + rr(i, j);
+        bvec[i] = rr(i + 1, nh) - rr(i, nh);
+      }
+      double tr = 0;
+      for (int i = 0; i < md; i++) tr += A[i * md + i];
+      for (int i = 0; i < md; i++) A[i * md + i] += 1e-12 * tr + 1e-300;
+      // solve A gamma = b (Gaussian elimination with partial pivoting, md <= HIST)
+      std::vector<double> gmm = bvec;
+      for (int col = 0; col < md; col++) {
+        int p = col;
+        for (int rr2 = col + 1; rr2 < md; rr2++)
+          if (std::fabs(A[rr2 * md + col]) > std::fabs(A[p * md + col])) p = rr2;
+        if (p != col) {
+          for (int j = 0; j < md; j++) std::swap(A[col * md + j], A[p * md + j]);
+          std::swap(gmm[col], gmm[p]);
+        }
+        for (int rr2 = col + 1; rr2 < md; rr2++) {
+          const double l = A[rr2 * md + col] / A[col * md + col];
+          for (int j = col; j < md; j++) A[rr2 * md + j] -= l * A[col * md + j];
+          gmm[rr2] -= l * gmm[col];
+        }
+      }
+      for (int col = md - 1; col >= 0; col--) {
+        double sacc = gmm[col];
+        for (int j = col + 1; j < md; j++) sacc -= A[col * md + j] * gmm[j];
+        gmm[col] = sacc / A[col * md + col];
+      }
+      // G_new = G + r - sum_i gamma_i (dX_i + dR_i); dX_i + dR_i = (X+R)_{i+1} - (X+R)_i
+      for (int i = 0; i < md; i++) {
+        const double gi = gmm[i];
+        // entry i+1 (the current one when i+1 == nh)
+        if (i + 1 == nh) {
+          add(ms.Gr.as<double>(), -gi);
+          add(r.as<double>(), -gi);
+        } else {
+          add(Xh.as<double>() + (size_t)ord[i + 1] * L, -gi);
+          add(Rh.as<double>() + (size_t)ord[i + 1] * L, -gi);
+        }
+        add(Xh.as<double>() + (size_t)ord[i] * L, gi);
+        add(Rh.as<double>() + (size_t)ord[i] * L, gi);
+      }
+    }
+    // store the current (G, r) as the newest entry, then form the update in Gn
+    hipLaunchKernelGGL(kern_combo, dim3(1024), dim3(256), 0, s, k, L, ms.Gn.as<double>());
+    {
+      // the slot about to be overwritten may be referenced by k: k ran first on the stream
+      const int slot = head;
+      NED_HIP(hipMemcpyAsync(Xh.as<double>() + (size_t)slot * L, ms.Gr.p, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+      NED_HIP(hipMemcpyAsync(Rh.as<double>() + (size_t)slot * L, r.p, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+      // Gram entries of the new slot: <r, R_i> were computed before the overwrite
+      for (int i = 0; i < nh; i++) {
+        const int o = ord[i];
+        if (o == slot) continue;
+        RR[slot * HIST + o] = RR[o * HIST + slot] = hd[o];
+      }
+      RR[slot * HIST + slot] = hd[8];
+      head = (head + 1) % HIST;
+      nh = std::min(nh + 1, HIST);
+    }
+    NED_HIP(hipMemcpyAsync(ms.Gr.p, ms.Gn.p, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+    check_launch("matsubara: Anderson");
+  }
+  *iters_out = it;
+  *err_out = err;
+  *mu_io = mu;
+  // final state: G^M, and Sigma^M / F consistent with it
+  hipLaunchKernelGGL(kern_store_mat, dim3(nb256(L)), dim3(256), 0, s, ms.Gr.as<double>(), L, c.G.mat.as<z2>());
+  check_launch("matsubara: store");
+  std::vector<double> Ff;
+  ms.prepare(h1, corr, Ff);
+  for (long long e = 0; e < ms.es; e++) fock_out[e] = Ff[e];
+  NED_HIP(hipStreamSynchronize(s));
+  invalidate_minv(c);
+  c.have_mix = false;
+}
+
+}  // namespace ned

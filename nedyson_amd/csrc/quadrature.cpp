@@ -3,6 +3,7 @@
 #include <cmath>
 #include <algorithm>
 #include <stdexcept>
+#include <thread>
 
 namespace ned {
 
@@ -230,6 +231,47 @@ void GLL::mixing_rows(double beta, int sig, std::vector<double> &la, std::vector
       }
     }
   }
+}
+
+void GLL::conv_weights(double beta, int sig, std::vector<double> &w) const {
+  std::vector<ld> g, gw;
+  gauss_legendre(np, g, gw);
+  w.assign((size_t)np * np * np, 0.0);
+  auto rows = [&](int m0, int m1) {
+    std::vector<double> ba((size_t)np * np), bb((size_t)np * np), wq(np);
+    for (int m = m0; m < m1; m++) {
+      const double tm = beta * (x[m] + 1) / 2;
+      double *wm = &w[(size_t)m * np * np];
+      for (int piece = 0; piece < 2; piece++) {
+        // piece 0: t in [0, tau_m], A at tau_m - t; piece 1: t in [tau_m, beta], A at tau_m - t + beta
+        const double lo = piece ? tm : 0.0, hi = piece ? beta : tm, len = hi - lo;
+        if (len <= 0) continue;
+        const double fac = piece ? (double)sig : 1.0;
+        for (int q = 0; q < np; q++) {
+          const double t = lo + len * ((double)g[q] + 1) / 2;
+          const double s = piece ? tm - t + beta : tm - t;
+          basis(2 * s / beta - 1, &ba[(size_t)q * np]);
+          basis(2 * t / beta - 1, &bb[(size_t)q * np]);
+          wq[q] = fac * (double)gw[q] * len / 2;
+        }
+        for (int q = 0; q < np; q++)
+          for (int j = 0; j < np; j++) {
+            const double a = wq[q] * ba[(size_t)q * np + j];
+            const double *b = &bb[(size_t)q * np];
+            double *o = wm + (size_t)j * np;
+            for (int k = 0; k < np; k++) o[k] += a * b[k];
+          }
+      }
+    }
+  };
+  const int nth = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  const int per = (np + nth - 1) / nth;
+  for (int t = 0; t < nth; t++) {
+    const int m0 = t * per, m1 = std::min(np, m0 + per);
+    if (m0 < m1) th.emplace_back(rows, m0, m1);
+  }
+  for (auto &t : th) t.join();
 }
 
 }  // namespace ned

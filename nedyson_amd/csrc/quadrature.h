@@ -26,6 +26,13 @@ struct GLL {
   // for every node i, nq = 2*np points t_q with weight w_q (sig folded in),
   // interpolation rows la[i][q][j] = w_q l_j(t_q) and lb[i][q][k] = l_k(s_q).
   void mixing_rows(double beta, int sig, std::vector<double> &la, std::vector<double> &lb) const;
+  // Weights of the antiperiodic imaginary-time convolution
+  //   C(tau_m) = int_0^beta A(tau_m - t) B(t) dt,  A(s<0) = sig A(s+beta),
+  // of the nodal interpolants of A and B:
+  //   C_m = sum_{j,k} w[(m*np + j)*np + k] A_j B_k.
+  // The integrand is a polynomial of degree 2*ntau on [0, tau_m] and on
+  // [tau_m, beta], so np-point Gauss-Legendre on each piece is exact.
+  void conv_weights(double beta, int sig, std::vector<double> &w) const;
 };
 
 }  // namespace ned

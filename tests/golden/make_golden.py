@@ -11,9 +11,12 @@ Outputs (committed):
   integrals.json   -- the expected scalars of tests/integrals-test.cpp:8-212
                       (energy_conv, CTV1, Ctv_tstp, Cles2_tstp, Cles3_tstp on
                       the closed-form contour functions of tests/set_up_g.h)
-  dsim_invariants.json -- a few derived numbers from tests/data/Dsim.h5 (a
-                      decomposed-GF2 equilibrium run, nao=2, nt=ntau=100),
-                      used as invariant checks only.
+  dsim_invariants.json -- slices of tests/data/Dsim.h5 (a decomposed-GF2
+                      equilibrium run, nao=2, nt=ntau=100): G^M (on a uniform
+                      tau grid), the first retarded column, the lesser
+                      diagonal and a few more blocks.
+  h2_hf.json       -- tests/data/hf.hdf5 (h1, Fock matrix, Coulomb integrals
+                      (ij|kl) of the H2 system Dsim.h5 propagates).
 """
 import json
 import os
@@ -109,6 +112,23 @@ def dsim():
     out["GL_5_10"] = [[c.real, c.imag] for c in les(5, 10).reshape(-1)]
     out["GR_10_5"] = [[c.real, c.imag] for c in ret(10, 5).reshape(-1)]
     out["density"] = [float(np.real(np.trace(1j * -1 * les(t, t)))) * 2 for t in range(0, nt + 1, 10)]
+    # every time step of the first retarded column and of the lesser diagonal
+    out["GR_t0"] = [[c.real, c.imag] for t in range(nt + 1) for c in ret(t, 0).reshape(-1)]
+    out["GL_tt"] = [[c.real, c.imag] for t in range(nt + 1) for c in les(t, t).reshape(-1)]
+    return out
+
+
+def h2_system():
+    """tests/data/hf.hdf5: the restricted-HF H2 system (minimal basis) whose
+    equilibrium and propagation Dsim.h5 holds."""
+    import h5py
+    f = h5py.File(os.path.join(REF, "hf.hdf5"), "r")
+    out = {"_source": "tests/data/hf.hdf5"}
+    for k in ("hcore", "fock", "uchem"):
+        out[k] = np.asarray(f[k][()], dtype=float).reshape(-1).tolist()
+    out["nelectron"] = int(f["nelectron"][()])
+    out["enuc"] = float(f["enuc"][()])
+    out["nao"] = int(f["hcore"].shape[0])
     return out
 
 
@@ -119,6 +139,7 @@ def main():
         json.dump(dsim(), open(os.path.join(OUT, "dsim_invariants.json"), "w"), indent=1)
     except Exception as e:  # pragma: no cover
         print("Dsim.h5 not read:", e, file=sys.stderr)
+    json.dump(h2_system(), open(os.path.join(OUT, "h2_hf.json"), "w"), indent=1)
 
 
 if __name__ == "__main__":
