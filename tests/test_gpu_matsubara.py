@@ -129,3 +129,26 @@ def test_gpu_matsubara_stationarity_detects_non_equilibrium():
           f"HF start tti {tti_hf:.2e} drift {dn_hf:.2e}", flush=True)
     assert tti < 1e-9 and dn < 1e-9
     assert tti_hf > 1e4 * tti
+
+
+def test_gpu_replays_reference_h2_propagation():
+    """End to end on the GPU: ned_matsubara_gf2, bootstrap and 95 FP64 time
+    steps of H2 against tests/data/Dsim.h5 (the same bounds as the oracle's
+    replay), and against the oracle's replay to 1e-11."""
+    from test_matsubara_oracle import H2_DT, H2_K, H2_NT, check_h2_replay
+    h1, B = MS.h2_system(json.load(open(os.path.join(GOLD, "h2_hf.json"))))
+    with ned.Context(2, B.shape[0], H2_NT, H2_NTAU, H2_K) as c:
+        c.set_coulomb(B)
+        mu, F, _, _ = c.matsubara_gf2(h1, H2_BETA, mu=H2_MU, tol=1e-13)
+        c.set_matsubara(H2_BETA)
+        c.g0_from_h0(H2_K, F, mu, H2_BETA, H2_DT)
+        c.do_boot(F, mu, H2_BETA, H2_DT, "f64", 60, 1e-13)
+        for T in range(H2_K + 1, H2_NT + 1):
+            c.do_tstp(T, h1, mu, H2_BETA, H2_DT, "f64", 4, 1e-13)
+        ntri = (H2_NT + 1) * (H2_NT + 2) // 2
+        R = c.read_blocks(ned.NED_G, ned.NED_RET, 0, ntri)
+        Lz = c.read_blocks(ned.NED_G, ned.NED_LES, 0, ntri)
+    GR0 = np.array([R[t * (t + 1) // 2] for t in range(H2_NT + 1)])
+    GLtt = np.array([Lz[t * (t + 1) // 2 + t] for t in range(H2_NT + 1)])
+    dR, dL = check_h2_replay(GR0, GLtt)
+    print(f"[matsubara] H2 replay: max|G^R(t,0) - Dsim| t<=10 {dR[:11].max():.2e}, all t {dR.max():.2e}", flush=True)
