@@ -28,7 +28,16 @@ void energy(Ctx &c, int tstp, const double *h0_host, double beta, double dt, dou
 // G^M (G.mat), Sigma^M (S.mat), the T = 0 slices of G and Sigma and hmf[0].
 void matsubara_gf2(Ctx &c, const double *h1, double nelec, double beta, bool corr, bool fix_mu, double *mu_io,
                    int maxiter, double tol, double *fock_out, int *iters_out, double *err_out);
-int lu_solve_aug(hipStream_t s, double *Aug, int D, int nrhs, int *ipiv, int *info);
+// Scratch of lu_solve_aug: prow (32 doubles), pval/pidx (one per workgroup
+// of a step: D/32 + 1 entries), ticket (one counter, zeroed by the call).
+struct LuWork {
+  double *prow, *pval;
+  int *pidx;
+  unsigned *ticket;
+};
+// Solves Aug[:, :D] X = Aug[:, D:D+nrhs] in place (row-major D x (D+nrhs));
+// returns the first zero pivot (1-based) or 0.
+int lu_solve_aug(hipStream_t s, double *Aug, int D, int nrhs, int *ipiv, int *info, const LuWork &ws);
 void invalidate_minv(Ctx &c);
 // mark hmf(t) / Sigma slice t as changed: drops the cached inverses that use it
 void touch_time(Ctx &c, int t);

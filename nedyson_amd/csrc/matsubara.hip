@@ -31,6 +31,8 @@
 #include "quadrature.h"
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -40,7 +42,6 @@ namespace ned {
 namespace {
 
 constexpr int LU_NB = 32;   // panel width
-constexpr int LU_PT = 1024; // panel workgroup
 
 inline unsigned nb256(long long n) { return (unsigned)((n + 255) / 256); }
 
@@ -92,57 +93,96 @@ __global__ void kern_add_identity(double *A, long long lda, int D) {
 }
 
 // ---------------------------------------------------------------- LU
-// Panel c0..c0+kb-1 of rows c0..D-1, partial pivoting; ipiv[col] = pivot row.
-// One workgroup; info[0] = first zero pivot column + 1.
-__global__ void __launch_bounds__(LU_PT) kern_lu_panel(double *A, long long lda, int D, int c0, int kb, int *ipiv,
-                                                      int *info) {
-  __shared__ double sval[LU_PT];
-  __shared__ int sidx[LU_PT];
-  __shared__ double prow[LU_NB];
-  const int tid = threadIdx.x;
-  for (int c = 0; c < kb; c++) {
-    const int col = c0 + c;
-    double best = -1.0;
-    int bi = col;
-    for (int r = col + tid; r < D; r += LU_PT) {
-      const double v = fabs(A[(long long)r * lda + col]);
+// One elimination step of the panel c0..c0+kb-1, spread over the whole chip.
+// Step c >= 0 eliminates column col = c0+c from the rows below it with the
+// pivot row already in prow (and stores the multipliers), and tracks the
+// largest |entry| of column col+1 over those rows; step c = -1 only tracks
+// column c0 over rows c0..D-1.  Each workgroup leaves its candidate in
+// (pval, pidx); the last workgroup to finish (atomic ticket, fences on both
+// sides) picks the pivot of the next column, swaps the two rows within the
+// panel, records ipiv and loads the new pivot row into prow.  Rows map to
+// half-waves, panel columns to lanes, so each row segment is one 256-byte access.
+constexpr int LU_ROWS = 32;  // rows per workgroup and step
+__global__ void __launch_bounds__(256) kern_lu_step(double *A, long long lda, int D, int c0, int kb, int c, double *prow,
+                                                  double *pval, int *pidx, unsigned *ticket, int *ipiv, int *info) {
+  __shared__ double sv[8];
+  __shared__ int si[8];
+  __shared__ bool last;
+  const int tid = threadIdx.x, j = tid & 31, hw = tid >> 5;
+  const int col = c0 + c, nxt = c + 1;  // nxt: panel index of the column searched
+  const int r0 = col + 1 + blockIdx.x * LU_ROWS;
+  const double pc = c >= 0 ? prow[c] : 1.0;
+  const double pj = (c >= 0 && j < kb) ? prow[j] : 0.0;
+  const double inv = 1.0 / pc;
+  double best = -1.0;
+  int bi = 0x7fffffff;
+  for (int r = r0 + hw; r < min(D, r0 + LU_ROWS); r += 8) {
+    double *row = A + (long long)r * lda + c0;
+    double a = j < kb ? row[j] : 0.0;
+    if (c >= 0) {
+      const double l = __shfl(a, c, 32) * inv;
+      if (j == c) a = l;
+      else if (j > c) a -= l * pj;
+      if (j >= c && j < kb) row[j] = a;
+    }
+    if (j == nxt && nxt < kb) {
+      const double v = fabs(a);
       if (v > best) { best = v; bi = r; }
     }
-    sval[tid] = best;
-    sidx[tid] = bi;
-    __syncthreads();
-    for (int s = LU_PT / 2; s > 0; s >>= 1) {
-      if (tid < s) {
-        const double o = sval[tid + s];
-        const int oi = sidx[tid + s];
-        if (o > sval[tid] || (o == sval[tid] && oi < sidx[tid])) { sval[tid] = o; sidx[tid] = oi; }
-      }
-      __syncthreads();
+  }
+  // candidate of this half-wave sits in lane nxt; workgroup reduction
+  if (nxt < kb) {
+    const double bv = __shfl(best, nxt, 32);
+    const int bx = __shfl(bi, nxt, 32);
+    if (j == 0) { sv[hw] = bv; si[hw] = bx; }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double v = -1.0;
+    int x = 0x7fffffff;
+    if (nxt < kb)
+      for (int h = 0; h < 8; h++)
+        if (sv[h] > v || (sv[h] == v && si[h] < x)) { v = sv[h]; x = si[h]; }
+    pval[blockIdx.x] = v;
+    pidx[blockIdx.x] = x;
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (nxt >= kb) {
+    if (tid == 0) *ticket = 0;
+    return;
+  }
+  // pick the pivot of column c0+nxt over all workgroups
+  if (tid < 64) {
+    double v = -1.0;
+    int x = 0x7fffffff;
+    for (int b = tid; b < (int)gridDim.x; b += 64) {
+      const double o = pval[b];
+      const int oi = pidx[b];
+      if (o > v || (o == v && oi < x)) { v = o; x = oi; }
     }
-    const int p = sidx[0];
-    if (tid < kb && p != col) {
-      double *a = A + (long long)col * lda + c0 + tid, *b = A + (long long)p * lda + c0 + tid;
-      const double t = *a;
-      *a = *b;
-      *b = t;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v, off);
+      const int oi = __shfl_xor(x, off);
+      if (o > v || (o == v && oi < x)) { v = o; x = oi; }
     }
-    if (tid == 0) ipiv[col] = p;
-    __syncthreads();
-    if (tid < kb) prow[tid] = A[(long long)col * lda + c0 + tid];
-    __syncthreads();
-    const double piv = prow[c];
-    if (piv == 0.0) {
-      if (tid == 0 && info[0] == 0) info[0] = col + 1;
-      continue;
-    }
-    const double inv = 1.0 / piv;
-    for (int r = col + 1 + tid; r < D; r += LU_PT) {
-      double *row = A + (long long)r * lda + c0;
-      const double l = row[c] * inv;
-      row[c] = l;
-      for (int j = c + 1; j < kb; j++) row[j] -= l * prow[j];
-    }
-    __syncthreads();
+    if (tid == 0) { sv[0] = v; si[0] = x; }
+  }
+  __syncthreads();
+  const int pr = c0 + nxt, p = si[0];
+  if (tid < kb) {
+    double *a = A + (long long)pr * lda + c0 + tid, *b = A + (long long)p * lda + c0 + tid;
+    const double ta = *a, tb = *b;
+    if (p != pr) { *a = tb; *b = ta; }
+    prow[tid] = p != pr ? tb : ta;
+    if (tid == nxt && (p != pr ? tb : ta) == 0.0 && info[0] == 0) info[0] = pr + 1;
+  }
+  if (tid == 0) {
+    ipiv[pr] = p;
+    *ticket = 0;
   }
 }
 
@@ -273,13 +313,19 @@ void dgemm(hipStream_t s, int M, int N, int K, double alpha, const double *A, lo
 // Solve Aug[:, 0:D] X = Aug[:, D:D+nrhs] in place (X overwrites the
 // right-hand sides); Aug is row-major D x (D + nrhs).  Returns the first zero
 // pivot (1-based) or 0.
-int lu_solve_aug(hipStream_t s, double *Aug, int D, int nrhs, int *ipiv, int *info) {
+int lu_solve_aug(hipStream_t s, double *Aug, int D, int nrhs, int *ipiv, int *info, const LuWork &ws) {
   const long long lda = (long long)D + nrhs;
   const int ncol = D + nrhs;
   NED_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
+  NED_HIP(hipMemsetAsync(ws.ticket, 0, sizeof(unsigned), s));
   for (int c0 = 0; c0 < D; c0 += LU_NB) {
     const int kb = std::min(LU_NB, D - c0);
-    hipLaunchKernelGGL(kern_lu_panel, dim3(1), dim3(LU_PT), 0, s, Aug, lda, D, c0, kb, ipiv, info);
+    for (int c = -1; c < kb; c++) {
+      const int rows = D - (c0 + c + 1);
+      const unsigned grid = (unsigned)std::max(1, (rows + LU_ROWS - 1) / LU_ROWS);
+      hipLaunchKernelGGL(kern_lu_step, dim3(grid), dim3(256), 0, s, Aug, lda, D, c0, kb, c, ws.prow, ws.pval, ws.pidx,
+                         ws.ticket, ipiv, info);
+    }
     const int j0 = c0 + kb, nc = ncol - j0;
     if (nc <= 0) continue;
     hipLaunchKernelGGL(kern_lu_swap, dim3(nb256(nc)), dim3(256), 0, s, Aug, lda, c0, kb, ipiv, j0, ncol);
@@ -324,8 +370,10 @@ struct MatSolver {
   long long es, L;
   double beta;
   std::vector<double> tau;
-  DevBuf w, Gr, Gn, Sr, Sp, tmp, K, Aug, gT, Cp, ipiv, info, V, dots, res;
+  DevBuf w, Gr, Gn, Sr, Sp, tmp, K, Aug, gT, Cp, ipiv, info, V, dots, luw;
+  LuWork ws{};
   std::vector<double> eps, Vh;
+  int nsolve = 0;
 
   MatSolver(Ctx &cc, double b) : c(cc), n(cc.nao), np(cc.np), beta(b) {
     D = np * n;
@@ -345,6 +393,12 @@ struct MatSolver {
     Cp.ensure((size_t)n * np * np * sizeof(double));
     ipiv.ensure((size_t)D * sizeof(int));
     info.ensure(sizeof(int));
+    const size_t nwg = (size_t)D / 32 + 2;
+    luw.ensure((32 + nwg) * sizeof(double) + (nwg + 1) * sizeof(int));
+    ws.prow = luw.as<double>();
+    ws.pval = ws.prow + 32;
+    ws.pidx = reinterpret_cast<int *>(ws.pval + nwg);
+    ws.ticket = reinterpret_cast<unsigned *>(ws.pidx + nwg);
     V.ensure(es * sizeof(double));
     dots.ensure(32 * sizeof(double));
     NED_HIP(hipStreamSynchronize(c.stream));  // wh is released on return
@@ -389,6 +443,7 @@ struct MatSolver {
   // G' for the current K and mu into Aug's right-hand-side columns; returns 2 tr rho
   double solve(double mu) {
     hipStream_t s = c.stream;
+    nsolve++;
     std::vector<double> g((size_t)n * np);
     for (int p = 0; p < n; p++)
       for (int m = 0; m < np; m++) g[(size_t)p * np + m] = g0_level(eps[p], mu, tau[m], beta);
@@ -404,7 +459,7 @@ struct MatSolver {
     hipLaunchKernelGGL(kern_rhs, dim3(nb256((long long)D * n)), dim3(256), 0, s, gT.as<double>(), n, np,
                        Aug.as<double>(), lda, D);
     check_launch("matsubara: system");
-    int bad = lu_solve_aug(s, Aug.as<double>(), D, n, ipiv.as<int>(), info.as<int>());
+    int bad = lu_solve_aug(s, Aug.as<double>(), D, n, ipiv.as<int>(), info.as<int>(), ws);
     if (bad) throw std::runtime_error("matsubara: singular Dyson system (the iteration did not converge)");
     // 2 tr G'(beta) rows (ntau, p), column p
     std::vector<double> last((size_t)n * lda);
@@ -479,37 +534,65 @@ void matsubara_gf2(Ctx &c, const double *h1, double nelec, double beta, bool cor
   int nh = 0, head = 0;
   std::vector<double> RR(HIST * HIST, 0.0);  // <R_i, R_j> of the stored residuals
   double err = INFINITY;
-  double dmu = 0.5;
+  double dmu = 0.5, slope = 0.0;
+  const bool verbose = std::getenv("NED_MAT_VERBOSE") != nullptr;
   int it = 0;
   for (; it < maxiter; it++) {
     ms.prepare(h1, corr, F);
     if (fix_mu) {
       ms.solve(mu);
     } else {
-      auto f = [&](double m) { return ms.solve(m) - nelec; };
-      double step = std::max(1e-3, 2 * dmu);
-      double a = mu - step, b = mu + step;
-      double fa = f(a), fb = f(b);
-      for (int e = 0; fa > 0 && e < 60; e++) { a -= step; step *= 2; fa = f(a); }
-      for (int e = 0; fb < 0 && e < 60; e++) { b += step; step *= 2; fb = f(b); }
-      if (!(fa <= 0 && fb >= 0))
-        throw std::runtime_error("matsubara: the particle number did not converge (no bracketing mu)");
-      double x = mu, fx = 1;
-      int side = 0;
-      for (int e = 0; e < 200; e++) {
-        x = (a * fb - b * fa) / (fb - fa);
-        fx = f(x);
-        if (std::fabs(fx) < 1e-12 * std::max(1.0, nelec) || b - a < 1e-14 * std::max(1.0, std::fabs(x))) break;
-        if (fx < 0) {
-          a = x; fa = fx;
-          if (side == -1) fb *= 0.5;
-          side = -1;
-        } else {
-          b = x; fb = fx;
-          if (side == 1) fa *= 0.5;
-          side = 1;
+      // 2 tr rho(mu) increases with mu: secant from the previous mu (with the
+      // previous slope), extended until the root is bracketed, then Illinois.
+      // The last solve is always at the returned mu (rotate_back uses it).
+      const double ftol = 1e-12 * std::max(1.0, nelec);
+      double last_x = NAN;
+      auto f = [&](double m) {
+        last_x = m;
+        return ms.solve(m) - nelec;
+      };
+      double x0 = mu, f0 = f(x0), x = x0, fx = f0;
+      if (std::fabs(f0) > ftol) {
+        double st = slope > 0 ? -f0 / slope : (f0 < 0 ? 1.0 : -1.0) * std::max(1e-3, 2 * dmu);
+        st = std::copysign(std::min(std::max(std::fabs(st), 1e-10), 2.0), f0 < 0 ? 1.0 : -1.0);
+        double x1 = x0 + st, f1 = f(x1);
+        int e = 0;
+        while (f0 * f1 > 0 && std::fabs(f1) > ftol && e++ < 60) {
+          // same side: secant step if it moves the right way, else double the step
+          const double dir = f1 < 0 ? 1.0 : -1.0;
+          double sx = (f1 != f0) ? -f1 * (x1 - x0) / (f1 - f0) : 0.0;
+          const double lim = 4 * std::fabs(x1 - x0);
+          if (!(sx * dir > 0)) sx = dir * 2 * std::fabs(x1 - x0);
+          if (std::fabs(sx) > lim) sx = dir * lim;
+          x0 = x1; f0 = f1;
+          x1 = x0 + sx;
+          f1 = f(x1);
+        }
+        if (f0 * f1 > 0 && std::fabs(f1) > ftol)
+          throw std::runtime_error("matsubara: the particle number did not converge (no bracketing mu)");
+        if (f1 != f0) slope = (f1 - f0) / (x1 - x0);
+        x = x1; fx = f1;
+        if (std::fabs(f1) > ftol) {
+          double a = x0, fa = f0, b = x1, fb = f1;
+          if (fa > 0) { std::swap(a, b); std::swap(fa, fb); }
+          int side = 0;
+          for (int it2 = 0; it2 < 200; it2++) {
+            x = (a * fb - b * fa) / (fb - fa);
+            fx = f(x);
+            if (std::fabs(fx) < ftol || std::fabs(b - a) < 1e-14 * std::max(1.0, std::fabs(x))) break;
+            if (fx < 0) {
+              a = x; fa = fx;
+              if (side == -1) fb *= 0.5;
+              side = -1;
+            } else {
+              b = x; fb = fx;
+              if (side == 1) fa *= 0.5;
+              side = 1;
+            }
+          }
         }
       }
+      if (last_x != x) f(x);
       dmu = std::fabs(x - mu);
       mu = x;
     }
@@ -532,6 +615,7 @@ void matsubara_gf2(Ctx &c, const double *h1, double nelec, double beta, bool cor
     double em;
     std::memcpy(&em, &bits, sizeof(em));
     err = em;
+    if (verbose) fprintf(stderr, "[matsubara] iteration %d: max|dG| %.3e mu %.12f solves %d\n", it, err, mu, ms.nsolve);
     if (!std::isfinite(err)) throw std::runtime_error("matsubara: non-finite update");
     if (err < tol) {
       NED_HIP(hipMemcpyAsync(ms.Gr.p, ms.Gn.p, L * sizeof(double), hipMemcpyDeviceToDevice, s));

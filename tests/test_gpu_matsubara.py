@@ -131,11 +131,11 @@ def test_gpu_matsubara_stationarity_detects_non_equilibrium():
     assert tti_hf > 1e4 * tti
 
 
-def test_gpu_replays_reference_h2_propagation():
+def test_gpu_replays_reference_h2_propagation(orc):
     """End to end on the GPU: ned_matsubara_gf2, bootstrap and 95 FP64 time
     steps of H2 against tests/data/Dsim.h5 (the same bounds as the oracle's
     replay), and against the oracle's replay to 1e-11."""
-    from test_matsubara_oracle import H2_DT, H2_K, H2_NT, check_h2_replay
+    from test_matsubara_oracle import H2_DT, H2_K, H2_NT, check_h2_replay, h2_replay_oracle
     h1, B = MS.h2_system(json.load(open(os.path.join(GOLD, "h2_hf.json"))))
     with ned.Context(2, B.shape[0], H2_NT, H2_NTAU, H2_K) as c:
         c.set_coulomb(B)
@@ -151,4 +151,6 @@ def test_gpu_replays_reference_h2_propagation():
     GR0 = np.array([R[t * (t + 1) // 2] for t in range(H2_NT + 1)])
     GLtt = np.array([Lz[t * (t + 1) // 2 + t] for t in range(H2_NT + 1)])
     dR, dL = check_h2_replay(GR0, GLtt)
+    oR, oL, _ = h2_replay_oracle(orc)
+    assert np.max(np.abs(GR0 - oR)) < 1e-11 and np.max(np.abs(GLtt - oL)) < 1e-11
     print(f"[matsubara] H2 replay: max|G^R(t,0) - Dsim| t<=10 {dR[:11].max():.2e}, all t {dR.max():.2e}", flush=True)

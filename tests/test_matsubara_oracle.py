@@ -101,9 +101,10 @@ def check_h2_replay(GR0, GLtt):
     return dR, dL
 
 
-def test_oracle_replays_reference_h2_propagation(orc):
-    """End to end on the CPU: equilibrium GF2 solve, bootstrap and 95
-    time steps of the H2 system against tests/data/Dsim.h5."""
+def h2_replay_oracle(orc):
+    """The oracle's run: equilibrium GF2 solve, bootstrap and 95 time steps of
+    the H2 system.  Returns (G^R(t, 0), G^<(t, t), the largest
+    |G^R(t + 50, 50) - G^R(t, 0)|)."""
     from oracle import matsubara as MS
     h1, B = MS.h2_system(json.load(open(os.path.join(GOLD, "h2_hf.json"))))
     V = np.ascontiguousarray(np.transpose(B, (1, 2, 0)))
@@ -118,6 +119,13 @@ def test_oracle_replays_reference_h2_propagation(orc):
         D.do_tstp(V, T, G, S, hmf, h1, mu, H2_BETA, H2_DT, 4, 1e-13)
     GR0 = np.array([G.retptr(t, 0) for t in range(H2_NT + 1)])
     GLtt = np.array([G.lesptr(t, t) for t in range(H2_NT + 1)])
-    check_h2_replay(GR0, GLtt)
     tti = max(np.abs(G.retptr(t + 50, 50) - G.retptr(t, 0)).max() for t in range(51))
+    return GR0, GLtt, tti
+
+
+def test_oracle_replays_reference_h2_propagation(orc):
+    """End to end on the CPU against tests/data/Dsim.h5; the oracle's own run
+    is time-translation invariant to 1e-12."""
+    GR0, GLtt, tti = h2_replay_oracle(orc)
+    check_h2_replay(GR0, GLtt)
     assert tti < 1e-12
