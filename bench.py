@@ -3,11 +3,13 @@
 Workload (default): nao=64, naux=256, 128 GLL nodes, dt=0.01, k=5, GF2,
 spin-restricted -- the nao=64 configuration -- timed at mid-run, T = 1000 of
 nt = 2000, where the per-step cost equals the whole-run average (the Sigma cost
-grows linearly with T).  The history t <= T0 is the non-interacting solution
-of the equilibrium Fock matrix (ned_g0_from_h0) with its self-energy slice
-continued time-translation invariantly (ned_tti_extend); the timed steps
-T0+W+1 .. T0+W+K then run the full timestep -- predictor, and `corr` corrector
-passes of {HF, Sigma, Dyson} -- on the quenched Hamiltonian.
+grows linearly with T).  The initial state is the equilibrium GF2 solution
+(ned_matsubara_gf2: G^M, Fock matrix, mu).  The history t <= T0 is the
+non-interacting solution of that Fock matrix (ned_g0_from_h0) with its
+self-energy slice continued time-translation invariantly (ned_tti_extend);
+the timed steps T0+W+1 .. T0+W+K then run the full timestep -- predictor,
+and `corr` corrector passes of {HF, Sigma, Dyson} -- on the quenched
+Hamiltonian.
 
 The same steps are re-run with FP64 Sigma contractions from the same history;
 the line reports the mixed-precision rate as `value`, the FP64 rate, their
@@ -33,14 +35,27 @@ sys.path.insert(0, ROOT)
 
 METRIC = "KBE timesteps/s at nao=64 GF2 vs plain FP64 at ≤1e-6 G error; % MFMA peak"
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0 / 3}
-# Coulomb-factor spread used for the nao=64 workload.  With B[Q,i,j] ~ N(0, 0.1)
-# the GF2 propagation started from the Hartree-Fock state (the equilibrium
-# GF2 Matsubara solver is not part of this build) leaves the stable range
-# within a few steps, and on the 1000-step synthetic history 0.02 still grows
-# the corrector update to O(100); at 0.01 the window stays smooth (corrector
-# update ~1e-3, particle number constant to 1e-7).  The work per step does
-# not depend on the value.
+# Coulomb-factor spread used for the nao=64 workload.  B[Q,i,j] ~ N(0, 0.1)
+# makes this 64-orbital system strongly correlated (the GF2 equilibrium mu
+# lies 18 Hartree below the Hartree-Fock one), and on the 1000-step synthetic
+# history below even 0.02 grows the corrector update to O(100); at 0.01 the
+# window stays smooth (corrector update ~1e-3, particle number constant to
+# 1e-7).  The work per step does not depend on the value.
 STABLE_B_SD = {2: 0.01}
+
+
+def gf2_equilibrium(ned, p, log):
+    """Replace the Hartree-Fock initial state of problem p by the equilibrium
+    GF2 one (ned_matsubara_gf2): G^M on the GLL nodes, its Fock matrix and mu."""
+    t0 = time.perf_counter()
+    with ned.Context(p.nao, p.naux, p.k, p.ntau, p.k) as c:
+        c.set_coulomb(p.B)
+        c.write_blocks(ned.NED_G, ned.NED_MAT, 0, p.Gmat)
+        mu, fock, it, err = c.matsubara_gf2(p.h1, p.beta, nelec=float(p.nelec), mu=p.mu, tol=1e-10)
+        p.Gmat = c.read_blocks(ned.NED_G, ned.NED_MAT, 0, p.ntau + 1)
+    log(f"[setup] GF2 equilibrium: mu {p.mu:.6f} -> {mu:.6f}, {it} iterations, last update {err:.1e} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    p.fock, p.mu = fock, mu
 
 
 def sigma_flops_reassoc(n, a, items):
@@ -185,6 +200,8 @@ def main():
     ap.add_argument("--b-sd", type=float, default=None,
                     help="sigma_B of the Coulomb factor (default: %s for the nao=64 workload)" % STABLE_B_SD)
     ap.add_argument("--dt", type=float, default=None)
+    ap.add_argument("--initial", default="gf2", choices=["gf2", "hf"],
+                    help="initial state: equilibrium GF2 (default) or Hartree-Fock")
     ap.add_argument("--sweep", default=None,
                     help="comma-separated history lengths T0: print per-step times at each and the fitted "
                          "whole-run average instead of the benchmark line")
@@ -225,6 +242,8 @@ def main():
     p = problem.config(args.config, **over)
     log(f"[setup] problem c{args.config}: nao={p.nao} naux={p.naux} ntau+1={p.ntau + 1} mu={p.mu:.6f} "
         f"({time.perf_counter() - t_setup:.1f} s)")
+    if args.initial == "gf2":
+        gf2_equilibrium(ned, p, log)
     if args.sweep:
         sweep(args, p, log)
         return
@@ -310,7 +329,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": f"{args.prec} Sigma contractions, f64 Dyson",
-        "data": "synthetic (seeded h1, B[Q,i,j]; HF equilibrium; history = free G of the Fock matrix)",
+        "data": f"synthetic (seeded h1, B[Q,i,j]; {args.initial.upper()} equilibrium; history = free G of its Fock "
+                "matrix)",
         "config": {"workload": f"GF2 KBE timestep at T={T0 + W + 1}..{nt}, nao={n}, naux={a}, "
                                f"ntau+1={p.ntau + 1}, dt={p.dt}, k={p.k}, {args.corr} corrector passes",
                    "nao": n, "naux": a, "nL": p.ntau + 1, "T": T0 + W + 1, "nt_run": p.nt,

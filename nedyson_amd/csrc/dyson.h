@@ -28,10 +28,11 @@ void energy(Ctx &c, int tstp, const double *h0_host, double beta, double dt, dou
 // G^M (G.mat), Sigma^M (S.mat), the T = 0 slices of G and Sigma and hmf[0].
 void matsubara_gf2(Ctx &c, const double *h1, double nelec, double beta, bool corr, bool fix_mu, double *mu_io,
                    int maxiter, double tol, double *fock_out, int *iters_out, double *err_out);
-// Scratch of lu_solve_aug: prow (32 doubles), pval/pidx (one per workgroup
-// of a step: D/32 + 1 entries), ticket (one counter, zeroed by the call).
+// Scratch of lu_solve_aug: prow (64 doubles), tinv (64 x 64), tmp
+// (64 x (D + nrhs)), pval/pidx (one per workgroup of a step: D/32 + 2
+// entries), ticket (one counter, zeroed by the call).
 struct LuWork {
-  double *prow, *pval;
+  double *prow, *tinv, *tmp, *pval;
   int *pidx;
   unsigned *ticket;
 };

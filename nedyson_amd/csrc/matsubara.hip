@@ -41,7 +41,7 @@ namespace ned {
 
 namespace {
 
-constexpr int LU_NB = 32;   // panel width
+constexpr int LU_NB = 64;   // panel width (one wave64 lane per panel column)
 
 inline unsigned nb256(long long n) { return (unsigned)((n + 255) / 256); }
 
@@ -101,14 +101,14 @@ __global__ void kern_add_identity(double *A, long long lda, int D) {
 // (pval, pidx); the last workgroup to finish (atomic ticket, fences on both
 // sides) picks the pivot of the next column, swaps the two rows within the
 // panel, records ipiv and loads the new pivot row into prow.  Rows map to
-// half-waves, panel columns to lanes, so each row segment is one 256-byte access.
+// waves, panel columns to lanes, so each row segment is one 512-byte access.
 constexpr int LU_ROWS = 32;  // rows per workgroup and step
 __global__ void __launch_bounds__(256) kern_lu_step(double *A, long long lda, int D, int c0, int kb, int c, double *prow,
                                                   double *pval, int *pidx, unsigned *ticket, int *ipiv, int *info) {
-  __shared__ double sv[8];
-  __shared__ int si[8];
+  __shared__ double sv[4];
+  __shared__ int si[4];
   __shared__ bool last;
-  const int tid = threadIdx.x, j = tid & 31, hw = tid >> 5;
+  const int tid = threadIdx.x, j = tid & 63, wv = tid >> 6;
   const int col = c0 + c, nxt = c + 1;  // nxt: panel index of the column searched
   const int r0 = col + 1 + blockIdx.x * LU_ROWS;
   const double pc = c >= 0 ? prow[c] : 1.0;
@@ -116,11 +116,11 @@ __global__ void __launch_bounds__(256) kern_lu_step(double *A, long long lda, in
   const double inv = 1.0 / pc;
   double best = -1.0;
   int bi = 0x7fffffff;
-  for (int r = r0 + hw; r < min(D, r0 + LU_ROWS); r += 8) {
+  for (int r = r0 + wv; r < min(D, r0 + LU_ROWS); r += 4) {
     double *row = A + (long long)r * lda + c0;
     double a = j < kb ? row[j] : 0.0;
     if (c >= 0) {
-      const double l = __shfl(a, c, 32) * inv;
+      const double l = __shfl(a, c, 64) * inv;
       if (j == c) a = l;
       else if (j > c) a -= l * pj;
       if (j >= c && j < kb) row[j] = a;
@@ -130,18 +130,18 @@ __global__ void __launch_bounds__(256) kern_lu_step(double *A, long long lda, in
       if (v > best) { best = v; bi = r; }
     }
   }
-  // candidate of this half-wave sits in lane nxt; workgroup reduction
+  // candidate of this wave sits in lane nxt; workgroup reduction
   if (nxt < kb) {
-    const double bv = __shfl(best, nxt, 32);
-    const int bx = __shfl(bi, nxt, 32);
-    if (j == 0) { sv[hw] = bv; si[hw] = bx; }
+    const double bv = __shfl(best, nxt, 64);
+    const int bx = __shfl(bi, nxt, 64);
+    if (j == 0) { sv[wv] = bv; si[wv] = bx; }
   }
   __syncthreads();
   if (tid == 0) {
     double v = -1.0;
     int x = 0x7fffffff;
     if (nxt < kb)
-      for (int h = 0; h < 8; h++)
+      for (int h = 0; h < 4; h++)
         if (sv[h] > v || (sv[h] == v && si[h] < x)) { v = sv[h]; x = si[h]; }
     pval[blockIdx.x] = v;
     pidx[blockIdx.x] = x;
@@ -201,50 +201,39 @@ __global__ void kern_lu_swap(double *A, long long lda, int c0, int kb, const int
   }
 }
 
-// rows c0..c0+kb-1, columns j0..ncol-1: X = L11^{-1} X (unit lower L11)
-__global__ void kern_lu_trsm(double *A, long long lda, int c0, int kb, int j0, int ncol) {
-  __shared__ double L[LU_NB][LU_NB + 1];
-  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) L[e / kb][e % kb] = A[(long long)(c0 + e / kb) * lda + c0 + e % kb];
+// Inverse of the kb x kb diagonal block at (i0, i0): unit lower (upper = 0)
+// or upper triangular (upper = 1), into X (row-major, ld LU_NB).  One wave:
+// lane c forms column c in LDS.
+__global__ void __launch_bounds__(64) kern_tri_inv(const double *A, long long lda, int i0, int kb, int upper, double *X) {
+  __shared__ double T[LU_NB][LU_NB + 1];
+  __shared__ double Y[LU_NB][LU_NB + 1];
+  const int c = threadIdx.x;
+  for (int e = c; e < kb * kb; e += 64) T[e / kb][e % kb] = A[(long long)(i0 + e / kb) * lda + i0 + e % kb];
   __syncthreads();
-  const int j = j0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ncol) return;
-  double x[LU_NB];
-#pragma unroll
-  for (int r = 0; r < LU_NB; r++)
-    if (r < kb) x[r] = A[(long long)(c0 + r) * lda + j];
-#pragma unroll
-  for (int r = 0; r < LU_NB; r++) {
-    if (r < kb) {
-      double s = x[r];
-#pragma unroll
-      for (int c = 0; c < r; c++) s -= L[r][c] * x[c];
-      x[r] = s;
-      A[(long long)(c0 + r) * lda + j] = s;
+  if (c < kb) {
+    if (!upper) {
+      for (int r = 0; r < kb; r++) {
+        double v = (r == c) ? 1.0 : 0.0;
+        if (r > c) {
+          v = 0.0;
+          for (int k = c; k < r; k++) v -= T[r][k] * Y[k][c];
+        }
+        Y[r][c] = v;
+      }
+    } else {
+      for (int r = kb - 1; r >= 0; r--) {
+        double v = 0.0;
+        if (r <= c) {
+          v = (r == c) ? 1.0 : 0.0;
+          for (int k = r + 1; k <= c; k++) v -= T[r][k] * Y[k][c];
+          v /= T[r][r];
+        }
+        Y[r][c] = v;
+      }
     }
   }
-}
-
-// rows i0..i0+kb-1 of the right-hand sides (columns D..D+nrhs-1): X = U_II^{-1} X
-__global__ void kern_lu_back_diag(double *A, long long lda, int i0, int kb, int D, int nrhs) {
-  __shared__ double U[LU_NB][LU_NB + 1];
-  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) U[e / kb][e % kb] = A[(long long)(i0 + e / kb) * lda + i0 + e % kb];
   __syncthreads();
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nrhs) return;
-  double x[LU_NB];
-#pragma unroll
-  for (int r = 0; r < LU_NB; r++)
-    if (r < kb) x[r] = A[(long long)(i0 + r) * lda + D + q];
-#pragma unroll
-  for (int r = LU_NB - 1; r >= 0; r--) {
-    if (r >= kb) continue;
-    double s = x[r];
-#pragma unroll
-    for (int c = 0; c < LU_NB; c++)
-      if (c > r && c < kb) s -= U[r][c] * x[c];
-    x[r] = s / U[r][r];
-    A[(long long)(i0 + r) * lda + D + q] = x[r];
-  }
+  for (int e = c; e < kb * kb; e += 64) X[(e / kb) * LU_NB + e % kb] = Y[e / kb][e % kb];
 }
 
 // ---------------------------------------------------------------- Anderson
@@ -329,7 +318,11 @@ int lu_solve_aug(hipStream_t s, double *Aug, int D, int nrhs, int *ipiv, int *in
     const int j0 = c0 + kb, nc = ncol - j0;
     if (nc <= 0) continue;
     hipLaunchKernelGGL(kern_lu_swap, dim3(nb256(nc)), dim3(256), 0, s, Aug, lda, c0, kb, ipiv, j0, ncol);
-    hipLaunchKernelGGL(kern_lu_trsm, dim3(nb256(nc)), dim3(256), 0, s, Aug, lda, c0, kb, j0, ncol);
+    // U12 = L11^{-1} A12 as a GEMM with the explicit inverse
+    hipLaunchKernelGGL(kern_tri_inv, dim3(1), dim3(64), 0, s, Aug, lda, c0, kb, 0, ws.tinv);
+    NED_HIP(hipMemcpy2DAsync(ws.tmp, (size_t)nc * sizeof(double), Aug + (long long)c0 * lda + j0, (size_t)lda * sizeof(double),
+                             (size_t)nc * sizeof(double), kb, hipMemcpyDeviceToDevice, s));
+    dgemm(s, kb, nc, kb, 1.0, ws.tinv, LU_NB, 1, ws.tmp, nc, 1, Aug + (long long)c0 * lda + j0, lda, 0);
     const int mr = D - j0;
     if (mr > 0)
       dgemm(s, mr, nc, kb, -1.0, Aug + (long long)j0 * lda + c0, lda, 1, Aug + (long long)c0 * lda + j0, lda, 1,
@@ -340,7 +333,10 @@ int lu_solve_aug(hipStream_t s, double *Aug, int D, int nrhs, int *ipiv, int *in
     if (kr > 0)
       dgemm(s, kb, nrhs, kr, -1.0, Aug + (long long)i0 * lda + i0 + kb, lda, 1, Aug + (long long)(i0 + kb) * lda + D,
             lda, 1, Aug + (long long)i0 * lda + D, lda, 1);
-    hipLaunchKernelGGL(kern_lu_back_diag, dim3(nb256(nrhs)), dim3(256), 0, s, Aug, lda, i0, kb, D, nrhs);
+    hipLaunchKernelGGL(kern_tri_inv, dim3(1), dim3(64), 0, s, Aug, lda, i0, kb, 1, ws.tinv);
+    NED_HIP(hipMemcpy2DAsync(ws.tmp, (size_t)nrhs * sizeof(double), Aug + (long long)i0 * lda + D, (size_t)lda * sizeof(double),
+                             (size_t)nrhs * sizeof(double), kb, hipMemcpyDeviceToDevice, s));
+    dgemm(s, kb, nrhs, kb, 1.0, ws.tinv, LU_NB, 1, ws.tmp, nrhs, 1, Aug + (long long)i0 * lda + D, lda, 0);
   }
   check_launch("lu_solve");
   int h = 0;
@@ -394,9 +390,12 @@ struct MatSolver {
     ipiv.ensure((size_t)D * sizeof(int));
     info.ensure(sizeof(int));
     const size_t nwg = (size_t)D / 32 + 2;
-    luw.ensure((32 + nwg) * sizeof(double) + (nwg + 1) * sizeof(int));
+    const size_t ntmp = (size_t)LU_NB * (D + n);
+    luw.ensure((LU_NB + LU_NB * LU_NB + ntmp + nwg) * sizeof(double) + (nwg + 1) * sizeof(int));
     ws.prow = luw.as<double>();
-    ws.pval = ws.prow + 32;
+    ws.tinv = ws.prow + LU_NB;
+    ws.tmp = ws.tinv + LU_NB * LU_NB;
+    ws.pval = ws.tmp + ntmp;
     ws.pidx = reinterpret_cast<int *>(ws.pval + nwg);
     ws.ticket = reinterpret_cast<unsigned *>(ws.pidx + nwg);
     V.ensure(es * sizeof(double));
