@@ -253,6 +253,37 @@ class dyson {
   System *sys_;
 };
 
+// The equilibrium solve the reference delegates to gfmol::DecompSimulation
+// (constructed at src/molNEsim_decomp_impl.h:21, run by do_mat at :50-52,
+// read through mu() and fock() at :44-46 and sampled on the collocation
+// nodes at :145-165): run() leaves G^M and Sigma^M in the System and keeps mu
+// and the Fock matrix for DecompSimulation.  mode NED_MAT_GF2 or NED_MAT_HF;
+// nelec > 0 solves for mu, otherwise mu stays at the constructor's value.
+class MatsubaraSolver {
+ public:
+  MatsubaraSolver(System &sys, std::vector<double> h1, double nelec, double beta, int mode = NED_MAT_GF2,
+                  double mu = 0.0)
+      : sys_(&sys), h1_(std::move(h1)), fock_(h1_.size()), nelec_(nelec), beta_(beta), mu_(mu), mode_(mode) {}
+
+  // do_mat: returns the outer iterations used; throws on NED_ERR_NOCONV
+  int run(int maxiter, double tol) {
+    int it = 0;
+    check(ned_matsubara_gf2(sys_->ctx(), h1_.data(), nelec_ > 0 ? nelec_ : 0.0, beta_, mode_, nelec_ > 0 ? 0 : 1, &mu_,
+                            maxiter, tol, fock_.data(), &it, &err_));
+    return it;
+  }
+  double mu() const { return mu_; }
+  const std::vector<double> &fock() const { return fock_; }
+  double beta() const { return beta_; }
+  double last_update() const { return err_; }
+
+ private:
+  System *sys_;
+  std::vector<double> h1_, fock_;
+  double nelec_, beta_, mu_, err_ = 0;
+  int mode_;
+};
+
 // DecompSimulation's propagation loop (src/molNEsim_decomp_impl.h:63-144)
 // on a System.  h0 is the post-quench one-body Hamiltonian, fock the
 // equilibrium mean field; both real nao x nao.

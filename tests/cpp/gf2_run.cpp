@@ -9,7 +9,11 @@
 //         <dir>/Viaj.bin    nao*naux*nao doubles, Viaj(i,Q,j)
 //         <dir>/h0.bin, fock.bin   nao*nao doubles
 //         <dir>/Gmat.bin    (ntau+1) complex nao x nao blocks
+// optional <dir>/equilibrium.txt "nelec" and <dir>/h1.bin (nao*nao doubles):
+//         the initial state then comes from MatsubaraSolver (the gfmol
+//         role: G^M, mu and the Fock matrix), not from Gmat/mu/fock
 // writes  <dir>/Gret.bin, Gles.bin, Gtv.bin  (packed, complex128)
+//         <dir>/Gmat_out.bin  G^M used (complex128 blocks)
 //         <dir>/energy.txt  one line "t eKin ePot N" per step
 #include <cstdio>
 #include <fstream>
@@ -64,7 +68,20 @@ int main(int argc, char **argv) {
     GREEN G = sys.G();
     molGF2SolverDecomp solver(sys, Vija.data(), Viaj.data());
     dyson Dyson(sys);
-    G.write(NED_MAT, 0, (long long)np, Gmat.data());
+    double nelec = 0;
+    std::ifstream feq(dir + "/equilibrium.txt");
+    if (feq >> nelec) {
+      auto h1 = load<double>(dir + "/h1.bin", es);
+      MatsubaraSolver mat(sys, h1, nelec, beta);
+      int it = mat.run(200, 1e-11);
+      mu = mat.mu();
+      fock = mat.fock();
+      std::cout << "matsubara: " << it << " iterations, mu " << mu << std::endl;
+    } else {
+      G.write(NED_MAT, 0, (long long)np, Gmat.data());
+    }
+    G.read(NED_MAT, 0, (long long)np, Gmat.data());
+    save(dir + "/Gmat_out.bin", Gmat);
     Dyson.set_matsubara(beta);
     Dyson.G0_from_h0(G, mu, fock.data(), beta, dt);
 

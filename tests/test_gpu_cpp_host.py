@@ -69,3 +69,40 @@ def test_cpp_mirror_gf2_run_vs_oracle(tmp_path, orc):
     assert e.shape == (p.nt + 1, 4)
     # particle number 2 tr rho stays at the filling after the quench (GF2 conserves it)
     assert np.max(np.abs(e[p.k + 1:, 3] - e[p.k + 1, 3])) < 1e-6
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_equilibrium_then_propagation_vs_oracle(tmp_path, orc):
+    """MatsubaraSolver (the gfmol role) through the mirror, then the same
+    propagation: G^M against the Matsubara oracle, the contour functions
+    against the oracle's propagation from that equilibrium."""
+    from oracle import matsubara as MS
+    p = problem.Problem(nao=4, naux=12, nelec=2, beta=10.0, ntau=15, nt=12, dt=0.02, seed=0, spacing=1.0,
+                        b_sd=0.1, quench="h00", quench_norm=0.5)
+    corr, bootmax, boottol = 3, 50, 1e-12
+    d = str(tmp_path)
+    _write_inputs(d, p, "f64", corr, bootmax, boottol)
+    np.ascontiguousarray(p.h1, dtype=np.float64).tofile(os.path.join(d, "h1.bin"))
+    with open(os.path.join(d, "equilibrium.txt"), "w") as f:
+        f.write(f"{p.nelec}\n")
+    print(f"[gf2_run] equilibrium + nao={p.nao} nt={p.nt} dir={d}", flush=True)
+    r = subprocess.run([EXE, d], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    GM, F, mu, _ = MS.solve(p.h1, p.B, p.beta, p.ntau, nelec=float(p.nelec), tol=1e-12)
+    gm = np.fromfile(os.path.join(d, "Gmat_out.bin"), dtype=np.complex128).reshape(p.ntau + 1, p.nao, p.nao)
+    assert np.max(np.abs(gm.real - GM)) < 1e-9 and np.max(np.abs(gm.imag)) == 0.0
+    # oracle propagation from the oracle's equilibrium
+    G = GreenFunc(p.nt, p.ntau, p.nao)
+    S = GreenFunc(p.nt, p.ntau, p.nao)
+    hmf = np.zeros((p.nt + 1, p.nao, p.nao), dtype=complex)
+    D = orc.Dyson(p.nt, p.ntau, p.nao, p.k)
+    D.g0_from_h0(G, mu, F, p.beta, p.dt)
+    G.mat[:] = GM
+    D.do_boot(p.V, G, S, hmf, F, mu, p.beta, p.dt, bootmax, boottol)
+    for T in range(p.k + 1, p.nt + 1):
+        D.do_tstp(p.V, T, G, S, hmf, p.h0, mu, p.beta, p.dt, corr)
+    ret = np.fromfile(os.path.join(d, "Gret.bin"), dtype=np.complex128).reshape(G.ret.shape)
+    les = np.fromfile(os.path.join(d, "Gles.bin"), dtype=np.complex128).reshape(G.les.shape)
+    tv = np.fromfile(os.path.join(d, "Gtv.bin"), dtype=np.complex128).reshape(G.tv.shape)
+    err = max(np.max(np.abs(ret - G.ret)), np.max(np.abs(les - G.les)), np.max(np.abs(tv - G.tv)))
+    assert err < 1e-9, err
