@@ -348,29 +348,40 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int NT_TK = 64, NT_LDK = NT_TK + 8;
 
-template <int TM, int TN, int WM, int WN>
+template <int TM, int TN, int WM, int WN, int TK = NT_TK>
 struct NtCfg {
   static constexpr int WAVES = (TM / WM) * (TN / WN);
   static constexpr int THREADS = 64 * WAVES;
   static constexpr int BI = WM / 16, BJ = WN / 16;  // MFMA blocks per wave
-  static constexpr int LA = TM * NT_TK / 8 / THREADS, LB = TN * NT_TK / 8 / THREADS;  // 16-byte loads per thread
+  static constexpr int SEG = TK / 8;                // 16-byte segments per row and K slice
+  static constexpr int LA = TM * TK / 8 / THREADS, LB = TN * TK / 8 / THREADS;  // 16-byte loads per thread
 };
 
-template <int TM, int TN, int WM, int WN>
+// TK: depth of the K slice staged in LDS (rows padded by 8 elements, which
+// keeps 16-byte ds_read_b128 conflict-free for 64 and 32).  STREAM: hold the
+// B fragments of a 32-deep step and stream the A fragments one row block at
+// a time (fewer live registers for tall wave tiles); the three products of a
+// row block run as three passes over its column blocks, so consecutive MFMAs
+// never share an accumulator.
+// DB: two LDS stages; the next slice is written into the idle stage while the
+// current one is consumed, one barrier per slice.
+template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false>
 __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
     kern_nt_bf16x3(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo, long long lda, long long strA,
                    const uint16_t *__restrict__ Bhi, const uint16_t *__restrict__ Blo, long long ldb, long long strB,
                    float *__restrict__ C, long long ldc, long long strC, int M, int N, int K, float alpha, int beta) {
-  using Cfg = NtCfg<TM, TN, WM, WN>;
-  constexpr int NTH = Cfg::THREADS, BI = Cfg::BI, BJ = Cfg::BJ, LA = Cfg::LA, LB = Cfg::LB;
+  using Cfg = NtCfg<TM, TN, WM, WN, TK>;
+  constexpr int NTH = Cfg::THREADS, BI = Cfg::BI, BJ = Cfg::BJ, LA = Cfg::LA, LB = Cfg::LB, SEG = Cfg::SEG;
+  constexpr int LDK = TK + 8;
   constexpr int WCOLS = TN / WN;
   Ahi += (size_t)blockIdx.y * strA;
   Alo += (size_t)blockIdx.y * strA;
   Bhi += (size_t)blockIdx.y * strB;
   Blo += (size_t)blockIdx.y * strB;
   C += (size_t)blockIdx.y * strC;
-  __shared__ __attribute__((aligned(16))) uint16_t sA[2][TM][NT_LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][TN][NT_LDK];
+  constexpr int NS = DB ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[NS][2][TM][LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[NS][2][TN][LDK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WCOLS, wn = wave % WCOLS;
   const int tilesN = N / TN, tilesM = M / TM;
@@ -397,62 +408,106 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
   auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < LA; i++) {
-      const int idx = tid + NTH * i, row = idx >> 3, seg = idx & 7;
+      const int idx = tid + NTH * i, row = idx / SEG, seg = idx % SEG;
       const size_t o = (size_t)(m0 + row) * lda + k0 + seg * 8;
       ra0[i] = *reinterpret_cast<const u32x4 *>(Ahi + o);
       ra1[i] = *reinterpret_cast<const u32x4 *>(Alo + o);
     }
 #pragma unroll
     for (int i = 0; i < LB; i++) {
-      const int idx = tid + NTH * i, row = idx >> 3, seg = idx & 7;
+      const int idx = tid + NTH * i, row = idx / SEG, seg = idx % SEG;
       const size_t o = (size_t)(n0 + row) * ldb + k0 + seg * 8;
       rb0[i] = *reinterpret_cast<const u32x4 *>(Bhi + o);
       rb1[i] = *reinterpret_cast<const u32x4 *>(Blo + o);
     }
   };
-  gload(0);
-  for (int k0 = 0; k0 < K; k0 += NT_TK) {
-    __syncthreads();
+  auto lstore = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < LA; i++) {
-      const int idx = tid + NTH * i, row = idx >> 3, seg = idx & 7;
-      *reinterpret_cast<u32x4 *>(&sA[0][row][seg * 8]) = ra0[i];
-      *reinterpret_cast<u32x4 *>(&sA[1][row][seg * 8]) = ra1[i];
+      const int idx = tid + NTH * i, row = idx / SEG, seg = idx % SEG;
+      *reinterpret_cast<u32x4 *>(&sA[st][0][row][seg * 8]) = ra0[i];
+      *reinterpret_cast<u32x4 *>(&sA[st][1][row][seg * 8]) = ra1[i];
     }
 #pragma unroll
     for (int i = 0; i < LB; i++) {
-      const int idx = tid + NTH * i, row = idx >> 3, seg = idx & 7;
-      *reinterpret_cast<u32x4 *>(&sB[0][row][seg * 8]) = rb0[i];
-      *reinterpret_cast<u32x4 *>(&sB[1][row][seg * 8]) = rb1[i];
+      const int idx = tid + NTH * i, row = idx / SEG, seg = idx % SEG;
+      *reinterpret_cast<u32x4 *>(&sB[st][0][row][seg * 8]) = rb0[i];
+      *reinterpret_cast<u32x4 *>(&sB[st][1][row][seg * 8]) = rb1[i];
     }
-    __syncthreads();
-    if (k0 + NT_TK < K) gload(k0 + NT_TK);
+  };
+  auto compute = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
-    for (int kk = 0; kk < NT_TK; kk += 32) {
+    for (int kk = 0; kk < TK; kk += 32) {
       const int kq = kk + 8 * (lane >> 4);
-      u32x4 ah[BI], al[BI], bh[BJ], bl[BJ];
-#pragma unroll
-      for (int i = 0; i < BI; i++) {
-        const int row = wm * WM + i * 16 + (lane & 15);
-        ah[i] = *reinterpret_cast<const u32x4 *>(&sA[0][row][kq]);
-        al[i] = *reinterpret_cast<const u32x4 *>(&sA[1][row][kq]);
-      }
-#pragma unroll
-      for (int j = 0; j < BJ; j++) {
-        const int col = wn * WN + j * 16 + (lane & 15);
-        bh[j] = *reinterpret_cast<const u32x4 *>(&sB[0][col][kq]);
-        bl[j] = *reinterpret_cast<const u32x4 *>(&sB[1][col][kq]);
-      }
-#pragma unroll
-      for (int i = 0; i < BI; i++)
+      if constexpr (STREAM) {
+        v8bf bh[BJ], bl[BJ];
 #pragma unroll
         for (int j = 0; j < BJ; j++) {
-          v8bf a_h = __builtin_bit_cast(v8bf, ah[i]), a_l = __builtin_bit_cast(v8bf, al[i]);
-          v8bf b_h = __builtin_bit_cast(v8bf, bh[j]), b_l = __builtin_bit_cast(v8bf, bl[j]);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, b_h, acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_l, acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_h, acc[i][j], 0, 0, 0);
+          const int col = wn * WN + j * 16 + (lane & 15);
+          bh[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][0][col][kq]));
+          bl[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][1][col][kq]));
         }
+#pragma unroll
+        for (int i = 0; i < BI; i++) {
+          const int row = wm * WM + i * 16 + (lane & 15);
+          const v8bf a_h = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][0][row][kq]));
+          const v8bf a_l = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][1][row][kq]));
+#pragma unroll
+          for (int j = 0; j < BJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < BJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < BJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bh[j], acc[i][j], 0, 0, 0);
+        }
+      } else {
+        u32x4 ah[BI], al[BI], bh[BJ], bl[BJ];
+#pragma unroll
+        for (int i = 0; i < BI; i++) {
+          const int row = wm * WM + i * 16 + (lane & 15);
+          ah[i] = *reinterpret_cast<const u32x4 *>(&sA[st][0][row][kq]);
+          al[i] = *reinterpret_cast<const u32x4 *>(&sA[st][1][row][kq]);
+        }
+#pragma unroll
+        for (int j = 0; j < BJ; j++) {
+          const int col = wn * WN + j * 16 + (lane & 15);
+          bh[j] = *reinterpret_cast<const u32x4 *>(&sB[st][0][col][kq]);
+          bl[j] = *reinterpret_cast<const u32x4 *>(&sB[st][1][col][kq]);
+        }
+#pragma unroll
+        for (int i = 0; i < BI; i++)
+#pragma unroll
+          for (int j = 0; j < BJ; j++) {
+            v8bf a_h = __builtin_bit_cast(v8bf, ah[i]), a_l = __builtin_bit_cast(v8bf, al[i]);
+            v8bf b_h = __builtin_bit_cast(v8bf, bh[j]), b_l = __builtin_bit_cast(v8bf, bl[j]);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, b_h, acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_l, acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_h, acc[i][j], 0, 0, 0);
+          }
+      }
+    }
+  };
+  gload(0);
+  if constexpr (DB) {
+    lstore(0);
+    if (TK < K) gload(TK);
+    __syncthreads();
+    int st = 0;
+    for (int k0 = 0; k0 < K; k0 += TK) {
+      if (k0 + TK < K) {
+        lstore(st ^ 1);
+        if (k0 + 2 * TK < K) gload(k0 + 2 * TK);
+      }
+      compute(st);
+      __syncthreads();
+      st ^= 1;
+    }
+  } else {
+    for (int k0 = 0; k0 < K; k0 += TK) {
+      __syncthreads();
+      lstore(0);
+      __syncthreads();
+      if (k0 + TK < K) gload(k0 + TK);
+      compute(0);
     }
   }
 #pragma unroll
@@ -499,12 +554,13 @@ struct NtVariant {
   const char *name;
 };
 
-template <int TM, int TN, int WM, int WN>
+template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false>
 void nt_launch(const NtOp &A, const NtOp &B, float *C, long long ldc, long long strC, int M, int N, int K, int batch,
                float alpha, int beta, hipStream_t s) {
   const int tiles = (M / TM) * (N / TN);
-  hipLaunchKernelGGL((kern_nt_bf16x3<TM, TN, WM, WN>), dim3(tiles, batch), dim3(NtCfg<TM, TN, WM, WN>::THREADS), 0, s,
-                     A.hi, A.lo, A.ld, A.stride, B.hi, B.lo, B.ld, B.stride, C, ldc, strC, M, N, K, alpha, beta);
+  hipLaunchKernelGGL((kern_nt_bf16x3<TM, TN, WM, WN, TK, STREAM, DB>), dim3(tiles, batch),
+                     dim3(NtCfg<TM, TN, WM, WN, TK>::THREADS), 0, s, A.hi, A.lo, A.ld, A.stride, B.hi, B.lo, B.ld,
+                     B.stride, C, ldc, strC, M, N, K, alpha, beta);
 }
 
 const NtVariant kNt[] = {
@@ -513,6 +569,11 @@ const NtVariant kNt[] = {
     {128, 256, nt_launch<128, 256, 64, 128>, "128x256 / 4 waves of 64x128"},
     {256, 128, nt_launch<256, 128, 64, 64>, "256x128 / 8 waves of 64x64"},
     {128, 128, nt_launch<128, 128, 32, 64>, "128x128 / 8 waves of 32x64"},
+    {256, 256, nt_launch<256, 256, 128, 64, 32, true>, "256x256 / 8 waves of 128x64, K32, streamed A"},
+    {128, 128, nt_launch<128, 128, 64, 64, 64, true>, "128x128 / 4 waves of 64x64, streamed A"},
+    {256, 256, nt_launch<256, 256, 128, 64, 32, true, true>, "256x256 / 8 waves of 128x64, K32, streamed A, 2 stages"},
+    {128, 128, nt_launch<128, 128, 64, 64, 32, true, true>, "128x128 / 4 waves of 64x64, K32, streamed A, 2 stages"},
+    {256, 128, nt_launch<256, 128, 64, 64, 32, true, true>, "256x128 / 8 waves of 64x64, K32, streamed A, 2 stages"},
 };
 constexpr int kNtCount = sizeof(kNt) / sizeof(kNt[0]);
 
