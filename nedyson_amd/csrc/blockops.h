@@ -52,6 +52,11 @@ __device__ __forceinline__ z2 blk_get(const z2 *M, int ld, int op, int r, int c)
 // ldX, ldY are the row strides of X and Y as stored (n for a dense block).
 // The next K-slice is loaded into registers while the current one is
 // multiplied out of LDS.
+// LATE: 0 prefetches both operands' next K slice into registers during the
+// current slice; 1 (2) reads X (Y) only when it is staged -- for an operand
+// that every workgroup shares and that therefore sits in L2, this saves the
+// prefetch registers (and buys occupancy) at little cost.
+template <int LATE = 0>
 __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 *X, int opX, const z2 *Y, int opY,
                                         z2 coef, int ldX = 0, int ldY = 0) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -59,7 +64,8 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 
   if (ldY == 0) ldY = n;
   const bool wave_on = 16 * w < n;
   constexpr int PER = BO_NMAX * BO_KT / BO_T;  // staged elements per thread and operand
-  z2 px[PER], py[PER];
+  constexpr int PX = LATE == 1 ? 1 : PER, PY = LATE == 2 ? 1 : PER;
+  z2 px[PX], py[PY];
   // element e of a slice -> (row, k) of X and (k, col) of Y, chosen so that
   // consecutive threads read consecutive addresses of the stored matrix
   // (along k when it is used as stored, along the other index when it is
@@ -70,15 +76,22 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 
   auto ymap = [&](int e, int &kk, int &c) __attribute__((always_inline)) {
     if (opY == 0) { kk = e / BO_NMAX; c = e % BO_NMAX; } else { c = e / BO_KT; kk = e % BO_KT; }
   };
+  auto getx = [&](int e, int k0) __attribute__((always_inline)) {
+    int r, kk;
+    xmap(e, r, kk);
+    return (r < n && k0 + kk < n) ? blk_get(X, ldX, opX, r, k0 + kk) : z2{0.0, 0.0};
+  };
+  auto gety = [&](int e, int k0) __attribute__((always_inline)) {
+    int kk, c;
+    ymap(e, kk, c);
+    return (c < n && k0 + kk < n) ? blk_get(Y, ldY, opY, k0 + kk, c) : z2{0.0, 0.0};
+  };
   auto fetch = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int e = tid + BO_T * i;
-      int r, kk, kk2, c;
-      xmap(e, r, kk);
-      px[i] = (r < n && k0 + kk < n) ? blk_get(X, ldX, opX, r, k0 + kk) : z2{0.0, 0.0};
-      ymap(e, kk2, c);
-      py[i] = (c < n && k0 + kk2 < n) ? blk_get(Y, ldY, opY, k0 + kk2, c) : z2{0.0, 0.0};
+      if constexpr (LATE != 1) px[i] = getx(e, k0);
+      if constexpr (LATE != 2) py[i] = gety(e, k0);
     }
   };
   __syncthreads();  // operands written by this workgroup just before the call are visible
@@ -91,8 +104,8 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 
       int r, kk, kk2, c;
       xmap(e, r, kk);
       ymap(e, kk2, c);
-      s.X[r][kk] = zmul(coef, px[i]);
-      s.Y[kk2][c] = py[i];
+      s.X[r][kk] = zmul(coef, LATE == 1 ? getx(e, k0) : px[LATE == 1 ? 0 : i]);
+      s.Y[kk2][c] = LATE == 2 ? gety(e, k0) : py[LATE == 2 ? 0 : i];
     }
     __syncthreads();
     if (k0 + BO_KT < n) fetch(k0 + BO_KT);

@@ -471,7 +471,7 @@ __global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *
 
 // contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
 // the history sums stay in registers across the whole panel
-__global__ void __launch_bounds__(256) kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(256) kern_ret_panel(Pt p, const WTab *__restri
   BlkAcc acc;
   blk_load(acc, n, Qm);
   for (int nn = q0; nn < q1; nn++)
-    blk_mac(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
+    blk_mac<1>(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
   __syncthreads();
   blk_store(acc, n, Qm);
 }
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(256) kern_ret_panel(Pt p, const WTab *__restri
 // src/dyson_step.h:130-169 with Ctv_tstp/CTV1/Conv.mixing
 // (src/dyson_integrals.h:80-116, 254-324): partial sums over chunks of the
 // time history, one workgroup per (node m, chunk s).
-__global__ void __launch_bounds__(256) kern_tv_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *mixK, int nchunk, z2 *part) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_tv_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *mixK, int nchunk, z2 *part) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
@@ -499,10 +499,10 @@ __global__ void __launch_bounds__(256) kern_tv_rhs(Pt p, const WTab *__restrict_
   BlkAcc acc;
   acc.zero();
   for (int j = j0; j < j1; j++)
-    blk_mac(acc, s, n, p.sr(T, j), 0, p.gtv(j, m), 0, z2{p.dt * greg(w, T, j), 0.0});
+    blk_mac<1>(acc, s, n, p.sr(T, j), 0, p.gtv(j, m), 0, z2{p.dt * greg(w, T, j), 0.0});
   if (ch == 0) {
     for (int jp = 0; jp < np; jp++)
-      blk_mac(acc, s, n, p.stv(T, jp), 0, mixK + ((size_t)m * np + jp) * p.es(), 0, z2{1.0, 0.0});
+      blk_mac<1>(acc, s, n, p.stv(T, jp), 0, mixK + ((size_t)m * np + jp) * p.es(), 0, z2{1.0, 0.0});
     for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, p.gtv(T - l, m), 0, z2{0.0, -w.bd[l] / p.dt});
   }
   blk_store(acc, n, part + ((size_t)ch * np + m) * p.es());
@@ -565,7 +565,7 @@ __global__ void kern_les_Y(Pt p, const double *mass, z2 *Y) {
 
 // Q_row = Cles3 + Cles2 (src/dyson_integrals.h:336-380, 393-427), partial
 // sums over chunks of the (tau, j) terms.
-__global__ void __launch_bounds__(256) kern_les_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *Y, int nchunk, z2 *part) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_les_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *Y, int nchunk, z2 *part) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int row = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
@@ -574,16 +574,19 @@ __global__ void __launch_bounds__(256) kern_les_rhs(Pt p, const WTab *__restrict
   const int t0 = ch * per, t1 = min(nterms, t0 + per);
   BlkAcc acc;
   acc.zero();
-  for (int t = t0; t < t1; t++) {
-    if (t < np) {
-      blk_mac(acc, s, n, p.stv(row, t), 0, Y + (size_t)t * p.es(), 0, z2{1.0, 0.0});
-    } else {
-      int j = t - np;
-      double wt = p.dt * greg(w, T, j);
-      // X_j = dt w(T,j) G^R(T,j)^H ; Sigma^<(row,j) for j >= row, -Sigma^<(j,row)^H for j < row
-      if (j >= row) blk_mac(acc, s, n, p.sl(row, j), 0, p.gr(T, j), 1, z2{wt, 0.0});
-      else blk_mac(acc, s, n, p.sl(j, row), 1, p.gr(T, j), 1, z2{-wt, 0.0});
-    }
+  // three runs of terms, each with fixed operand forms (one inlined product
+  // per run keeps the live registers to one copy):
+  // tv terms t < np; then j = t - np: -Sigma^<(j,row)^H X_j for j < row and
+  // Sigma^<(row,j) X_j for j >= row, with X_j = dt w(T,j) G^R(T,j)^H
+  for (int t = t0; t < min(t1, np); t++)
+    blk_mac<2>(acc, s, n, p.stv(row, t), 0, Y + (size_t)t * p.es(), 0, z2{1.0, 0.0});
+  for (int t = max(t0, np); t < min(t1, np + row); t++) {
+    const int j = t - np;
+    blk_mac<2>(acc, s, n, p.sl(j, row), 1, p.gr(T, j), 1, z2{-p.dt * greg(w, T, j), 0.0});
+  }
+  for (int t = max(t0, np + row); t < t1; t++) {
+    const int j = t - np;
+    blk_mac<2>(acc, s, n, p.sl(row, j), 0, p.gr(T, j), 1, z2{p.dt * greg(w, T, j), 0.0});
   }
   blk_store(acc, n, part + ((size_t)ch * (T + 1) + row) * p.es());
 }
@@ -677,7 +680,7 @@ __global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restri
   BlkAcc acc;
   blk_load(acc, n, Qm);
   for (int l = 0; l <= w.k; l++)
-    blk_mac(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+    blk_mac<2>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
   if (m == w.k + 1 && m <= T - 1) les_finalize(p, w, acc, s, m, Qm, X, Minv);
   else blk_store(acc, n, Qm);
 }
@@ -723,7 +726,7 @@ __global__ void __launch_bounds__(256) kern_les_step_band(Pt p, const WTab *__re
 }
 
 // contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
-__global__ void __launch_bounds__(256) kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q,
                                                       const z2 *X) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
@@ -733,7 +736,7 @@ __global__ void __launch_bounds__(256) kern_les_panel(Pt p, const WTab *__restri
   BlkAcc acc;
   blk_load(acc, n, Qm);
   for (int l = q0; l < q1; l++)
-    blk_mac(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+    blk_mac<2>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
   __syncthreads();
   blk_store(acc, n, Qm);
 }
