@@ -26,7 +26,7 @@ inline void check_launch(const char *what) {
 
 // complex FP64 on device: interleaved (re, im) pairs, the layout of
 // std::complex<double> used by green_func.
-struct z2 {
+struct alignas(16) z2 {
   double x, y;
 };
 
