@@ -17,6 +17,9 @@ Outputs (committed):
                       diagonal and a few more blocks.
   h2_hf.json       -- tests/data/hf.hdf5 (h1, Fock matrix, Coulomb integrals
                       (ij|kl) of the H2 system Dsim.h5 propagates).
+  reference_dyson_fixtures.npz -- the contour functions of
+                      tests/data/dyson_test_data.h5 (Timesteps, Sigma, Extrap,
+                      epsmf/ft) and tests/data/Gfree_const.h5 (free-test.cpp).
 """
 import json
 import os
@@ -132,6 +135,24 @@ def h2_system():
     return out
 
 
+def dyson_fixtures():
+    """Contour functions of the reference's Dyson-solver test data
+    (tests/data/dyson_test_data.h5: groups Timesteps, Sigma, Extrap and the
+    mean field epsmf/ft; nao = 2, nt = 20, ntau = 30) and of the free-GF test
+    (tests/data/Gfree_const.h5, checked by tests/free-test.cpp:8-44)."""
+    import h5py
+    out = {}
+    f = h5py.File(os.path.join(REF, "dyson_test_data.h5"), "r")
+    for g in ("Timesteps", "Sigma", "Extrap"):
+        for c in ("GR", "GL", "GTV", "GM"):
+            out[f"{g}_{c}"] = np.asarray(f[g][c][()], dtype=np.complex128)
+    out["epsmf_ft"] = np.asarray(f["epsmf"]["ft"][()], dtype=np.complex128)
+    f = h5py.File(os.path.join(REF, "Gfree_const.h5"), "r")
+    for c in ("GR", "GL", "GTV", "GM"):
+        out[f"Gfree_{c}"] = np.asarray(f[c][()], dtype=np.complex128)
+    return out
+
+
 def main():
     json.dump(calccoeff(), open(os.path.join(OUT, "calccoeff.json"), "w"), indent=1)
     json.dump(integrals(), open(os.path.join(OUT, "integrals.json"), "w"), indent=1)
@@ -140,6 +161,7 @@ def main():
     except Exception as e:  # pragma: no cover
         print("Dsim.h5 not read:", e, file=sys.stderr)
     json.dump(h2_system(), open(os.path.join(OUT, "h2_hf.json"), "w"), indent=1)
+    np.savez_compressed(os.path.join(OUT, "reference_dyson_fixtures.npz"), **dyson_fixtures())
 
 
 if __name__ == "__main__":

@@ -315,3 +315,64 @@ def test_mlp_oracle_adam_matches_closed_form_first_step():
     p0 = p[0].copy()
     M.adam_step(p, g, m, v, 1, 0.01)
     assert np.allclose(p[0], p0 - 0.01 * np.sign(g[0]), atol=1e-8)
+
+
+def test_dyson_step_ret_against_reference_timesteps(orc):
+    """The retarded rows of the reference's stored GF propagation
+    (tests/data/dyson_test_data.h5, Timesteps with Sigma and a time-dependent
+    mean field) for T = k+1..20, each from the stored history: 1e-12."""
+    import reference_fixtures as RF
+    S, Gt, h = RF.green("Sigma"), RF.green("Timesteps"), RF.hmf()
+    D = orc.Dyson(RF.NT, RF.NTAU, RF.NAO, RF.K)
+    for T in range(RF.K + 1, RF.NT + 1):
+        G = RF.green("Timesteps")
+        D.dyson_step(T, G, S, h, RF.MU, RF.BETA, RF.DT)
+        err = max(np.abs(G.retptr(T, j) - Gt.retptr(T, j)).max() for j in range(T + 1))
+        assert err < 1e-12, (T, err)
+
+
+def test_extrapolate_against_reference(orc):
+    """dyson::Extrapolate against the reference's stored extrapolation run
+    (dyson_test_data.h5, Extrap): every slice T = k+1..20 from the slices
+    before it, all three components, 1e-12 relative."""
+    import reference_fixtures as RF
+    Ge = RF.green("Extrap")
+    D = orc.Dyson(RF.NT, RF.NTAU, RF.NAO, RF.K)
+    for T in range(RF.K + 1, RF.NT + 1):
+        G = RF.green("Extrap")
+        D.extrapolate(T, G)
+        for a, b in ((G.ret_row(T), Ge.ret_row(T)), (G.les_col(T), Ge.les_col(T)), (G.tv[T], Ge.tv[T])):
+            assert np.max(np.abs(a - b)) <= 1e-12 * max(1.0, np.max(np.abs(b))), T
+
+
+def test_g0_from_h0_against_reference_free_test(orc):
+    """G0_from_h0 against tests/data/Gfree_const.h5, the golden file of
+    tests/free-test.cpp:8-44: G^R and G^< of steps 0..k directly; G^M and
+    G^tv through the closed form, which the file holds on a uniform tau grid."""
+    import reference_fixtures as RF
+    ref = RF.green("Gfree")
+    G = GreenFunc(RF.NT, RF.NTAU, RF.NAO)
+    D = orc.Dyson(RF.NT, RF.NTAU, RF.NAO, RF.K)
+    D.g0_from_h0(G, RF.MU, RF.H0_FREE, RF.BETA, RF.DT)
+    nb = (RF.K + 1) * (RF.K + 2) // 2
+    assert np.max(np.abs(G.ret[:nb] - ref.ret[:nb])) < 1e-12
+    assert np.max(np.abs(G.les[:nb] - ref.les[:nb])) < 1e-12
+    # the closed form on the fixture's uniform grid reproduces its G^M and G^tv
+    from nedyson_amd.problem import fermi
+    lam, V = np.linalg.eigh(RF.MU * np.eye(RF.NAO) - RF.H0_FREE)
+    taus = np.linspace(0, RF.BETA, RF.NTAU + 1)
+
+    def fexp(t, w):
+        return np.where(w < 0, np.exp(w * t) * fermi(RF.BETA, w), np.exp((t - RF.BETA) * w) * fermi(RF.BETA, -w))
+    gm = np.array([-(V * fexp(t, lam)) @ V.T for t in taus])
+    assert np.max(np.abs(gm - ref.mat)) < 1e-12
+    for m in range(RF.K + 1):
+        U = V @ np.diag(np.exp(1j * RF.DT * m * lam)) @ V.T
+        tv = np.array([1j * U @ ((V * fexp(t, -lam)) @ V.T) for t in taus])
+        assert np.max(np.abs(tv - ref.tv[m])) < 1e-12
+    # and the oracle's G^tv on its own (GLL) nodes is the same closed form
+    x, _ = orc.gll(RF.NTAU)
+    for m in range(RF.K + 1):
+        U = V @ np.diag(np.exp(1j * RF.DT * m * lam)) @ V.T
+        tv = np.array([1j * U @ ((V * fexp(t, -lam)) @ V.T) for t in RF.BETA * (x + 1) / 2])
+        assert np.max(np.abs(tv - G.tv[m])) < 1e-12
