@@ -63,6 +63,8 @@ const char *ned_last_error(void);
 ned_status ned_ctx_create(ned_ctx **out, const ned_dims *dims, int device);
 ned_status ned_ctx_destroy(ned_ctx *ctx);
 ned_status ned_sync(ned_ctx *ctx);
+/* the sizes the context was created with (GREEN::nt(), ntau(), size1()) */
+ned_status ned_ctx_get_dims(const ned_ctx *ctx, ned_dims *out);
 
 /* Replaces molGF2SolverDecomp(Vija, Viaj) (src/mol2bSolver.h:101-111):
  * B_Qij is naux x nao x nao, B[Q][i][j] = Vija(i,j,Q); must be symmetric in

@@ -28,7 +28,7 @@ _STATUS = {0: "NED_OK", 1: "NED_ERR_ARG", 2: "NED_ERR_NOMEM", 3: "NED_ERR_DEVICE
            5: "NED_ERR_NOCONV", 6: "NED_ERR_STATE"}
 
 # every symbol declared in include/nedyson.h
-EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "ned_set_coulomb", "ned_green_write",
+EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "ned_ctx_get_dims", "ned_set_coulomb", "ned_green_write",
            "ned_green_read", "ned_hmf_write", "ned_hmf_read", "ned_sigma_gf2_decomp", "ned_hf_decomp",
            "ned_set_matsubara", "ned_matsubara_gf2", "ned_dyson_step", "ned_extrapolate", "ned_dyson_start", "ned_do_boot", "ned_do_tstp",
            "ned_energy", "ned_g0_from_h0", "ned_tti_extend", "ned_profile", "ned_profile_read", "ned_gll_nodes",

@@ -153,6 +153,13 @@ ned_status ned_ctx_destroy(ned_ctx *ctx) {
   return st;
 }
 
+ned_status ned_ctx_get_dims(const ned_ctx *ctx, ned_dims *out) {
+  if (!ctx || !out) { g_err = "null argument"; return NED_ERR_ARG; }
+  const ned::Ctx &c = ctx->c;
+  out->nao = c.nao; out->naux = c.naux; out->nt = c.nt; out->ntau = c.ntau; out->k = c.k; out->sig = c.sig;
+  return NED_OK;
+}
+
 ned_status ned_sync(ned_ctx *ctx) {
   if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
   return guard(ctx, [&] { NED_HIP(hipStreamSynchronize(ctx->c.stream)); });

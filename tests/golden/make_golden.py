@@ -20,6 +20,8 @@ Outputs (committed):
   reference_dyson_fixtures.npz -- the contour functions of
                       tests/data/dyson_test_data.h5 (Timesteps, Sigma, Extrap,
                       epsmf/ft) and tests/data/Gfree_const.h5 (free-test.cpp).
+  Gfree_const.h5   -- tests/data/Gfree_const.h5 itself (a GREEN file written by
+                      the reference), read by the HDF5 GREEN reader's tests.
 """
 import json
 import os
@@ -162,6 +164,9 @@ def main():
         print("Dsim.h5 not read:", e, file=sys.stderr)
     json.dump(h2_system(), open(os.path.join(OUT, "h2_hf.json"), "w"), indent=1)
     np.savez_compressed(os.path.join(OUT, "reference_dyson_fixtures.npz"), **dyson_fixtures())
+    # one of the reference's GREEN files as it is, for the HDF5 reader
+    import shutil
+    shutil.copyfile(os.path.join(REF, "Gfree_const.h5"), os.path.join(OUT, "Gfree_const.h5"))
 
 
 if __name__ == "__main__":
