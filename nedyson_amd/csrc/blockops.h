@@ -311,6 +311,89 @@ __device__ __forceinline__ void tile_scale(TileAcc &acc, z2 f) {
   for (int q = 0; q < 4; q++) acc.set(q, zmul(acc.get(q), f));
 }
 
+// Operand fragments of v_mfma_f64_16x16x4f64 for one wave's 16 x 16 tile,
+// read straight from memory: no LDS staging and no barrier, so a product
+// whose operands are read once costs one memory round trip plus the matrix
+// core time.  This is the form for the dependency chains, where one
+// workgroup's latency is the whole cost.  Step s covers k = 4s..4s+3; lane l
+// holds
+//   left  operand: op(L)[r0 + l%16][4s + l/16]
+//   right operand: op(R)[4s + l/16][c0 + l%16]
+// and zero outside nao x nao.
+struct Frag {
+  z2 v[BO_NMAX / 4];
+};
+
+__device__ __forceinline__ void frag_left(Frag &f, int n, const z2 *L, int opL, int r0) {
+  const int lane = threadIdx.x & 63, r = r0 + (lane & 15);
+#pragma unroll
+  for (int s = 0; s < BO_NMAX / 4; s++) {
+    const int kk = 4 * s + (lane >> 4);
+    f.v[s] = (r < n && kk < n) ? blk_get(L, n, opL, r, kk) : z2{0.0, 0.0};
+  }
+}
+
+__device__ __forceinline__ void frag_right(Frag &f, int n, const z2 *R, int opR, int c0) {
+  const int lane = threadIdx.x & 63, c = c0 + (lane & 15);
+#pragma unroll
+  for (int s = 0; s < BO_NMAX / 4; s++) {
+    const int kk = 4 * s + (lane >> 4);
+    f.v[s] = (kk < n && c < n) ? blk_get(R, n, opR, kk, c) : z2{0.0, 0.0};
+  }
+}
+
+// acc += coef * L R from fragments; coef scales the left operand, as the
+// LDS-staged products do, so both forms round alike
+__device__ __forceinline__ void frag_mac(TileAcc &acc, const Frag &a, const Frag &b, int n, z2 coef) {
+#pragma unroll
+  for (int s = 0; s < BO_NMAX / 4; s++) {
+    if (4 * s < n) {
+      const z2 x = zmul(coef, a.v[s]), y = b.v[s];
+      acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, acc.re, 0, 0, 0);
+      acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, acc.re, 0, 0, 0);
+      acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, acc.im, 0, 0, 0);
+      acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, acc.im, 0, 0, 0);
+    }
+  }
+}
+
+// the four tiles of a 16-row band (tile (0, 16w) from wave w) as the left
+// operand of the next product: band[16][NMAX + 1] in LDS
+__device__ __forceinline__ void frag_left_band(Frag &f, const TileAcc &acc, z2 (*band)[BO_NMAX + 1], int n) {
+  const int lane = threadIdx.x & 63, c0 = 16 * (threadIdx.x >> 6);
+#pragma unroll
+  for (int q = 0; q < 4; q++) band[4 * q + (lane >> 4)][c0 + (lane & 15)] = acc.get(q);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < BO_NMAX / 4; s++) {
+    const int kk = 4 * s + (lane >> 4);
+    f.v[s] = kk < n ? band[lane & 15][kk] : z2{0.0, 0.0};
+  }
+}
+
+// the four tiles of a 16-column band (tile (16w, 0) from wave w) as the
+// right operand of the next product: band[NMAX][16 + 1] in LDS
+__device__ __forceinline__ void frag_right_band(Frag &f, const TileAcc &acc, z2 (*band)[17], int n) {
+  const int lane = threadIdx.x & 63, r0 = 16 * (threadIdx.x >> 6);
+#pragma unroll
+  for (int q = 0; q < 4; q++) band[r0 + 4 * q + (lane >> 4)][lane & 15] = acc.get(q);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < BO_NMAX / 4; s++) {
+    const int kk = 4 * s + (lane >> 4);
+    f.v[s] = kk < n ? band[kk][lane & 15] : z2{0.0, 0.0};
+  }
+}
+
+// sum of v over the workgroup, valid in thread 0 (red: 4 doubles of LDS)
+__device__ __forceinline__ double wg_sum(double v, double *red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
 // squared distance between the workgroup's tiles and X, summed over the
 // workgroup (valid in thread 0)
 __device__ __forceinline__ double tile_dist2(const TileAcc &acc, int n, const z2 *X, int r0, int c0, double *red) {

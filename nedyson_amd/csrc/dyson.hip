@@ -420,33 +420,61 @@ __global__ void __launch_bounds__(256) kern_ret_step(Pt p, const WTab *__restric
 // The same step split by rows: workgroup (m, b) owns rows 16b..16b+15 of
 // block m (its waves take the four column tiles).  G^R(T, T-m) depends only
 // on the same rows of the history sum, so the bands never exchange data; the
-// squared update of each finished band goes to d2[m * nb + b].
+// squared update of each finished band goes to d2[m * nb + b].  This launch
+// sits on the dependency chain, so its time is the latency of the finishing
+// workgroup: every operand is read straight into registers at the start (no
+// LDS staging), and only the finished band Q' passes through LDS.
 __global__ void __launch_bounds__(256) kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *Q,
                                                           const z2 *Minv, double *d2) {
   const WTab &w = *wtab;
-  __shared__ BlkLds s;
-  __shared__ double red[256];
+  __shared__ z2 band[16][BO_NMAX + 1];
+  __shared__ double red[4];
   const int m = nn + 1 + blockIdx.x, b = blockIdx.y, nb = gridDim.y, T = p.T, n = p.n;
   if (m > T) return;
-  const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
+  const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6), k1 = w.k + 1;
   z2 *Qm = Q + (size_t)m * n * n;
+  const z2 coef = {greg(w, m, nn), 0.0};
   TileAcc acc;
+  Frag fa, fb;
   tile_load(acc, n, Qm, r0, c0);
-  tile_mac(acc, s, n, p.gr(T, T - nn), 0, r0, 16, p.sr(T - nn, T - m), 0, 0, 64, z2{greg(w, m, nn), 0.0}, r0, c0);
+  frag_left(fa, n, p.gr(T, T - nn), 0, r0);
+  frag_right(fb, n, p.sr(T - nn, T - m), 0, c0);
   if (m != nn + 1) {
+    frag_mac(acc, fa, fb, n, coef);
     tile_store(acc, n, Qm, r0, c0);
     return;
   }
   // finish the band: Q' = dt Q - i/dt sum_l bd_l G^R(T, T-m+l), X = Q' Minv_{T-m}
+  TileAcc hist[KMAX + 1], old;
+  Frag fm;
+#pragma unroll
+  for (int l = 1; l <= KMAX + 1; l++)
+    if (l <= k1) tile_load(hist[l - 1], n, p.gr(T, T - m + l), r0, c0);
+  frag_right(fm, n, Minv + (size_t)(T - m) * n * n, 0, c0);
+  tile_load(old, n, p.gr(T, T - m), r0, c0);
+  frag_mac(acc, fa, fb, n, coef);
   tile_scale(acc, z2{p.dt, 0.0});
-  for (int l = 1; l <= w.k + 1; l++) tile_axpy(acc, n, p.gr(T, T - m + l), 0, z2{0.0, -w.bd[l] / p.dt}, r0, c0);
-  tile_store(acc, n, Qm, r0, c0);
+#pragma unroll
+  for (int l = 1; l <= KMAX + 1; l++)
+    if (l <= k1) {
+      const z2 cl = {0.0, -w.bd[l] / p.dt};
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc.set(q, zadd(acc.get(q), zmul(cl, hist[l - 1].get(q))));
+    }
+  Frag fq;
+  frag_left_band(fq, acc, band, n);
   TileAcc x;
   x.zero();
-  tile_mac(x, s, n, Qm, 0, r0, 16, Minv + (size_t)(T - m) * n * n, 0, 0, 64, z2{1.0, 0.0}, r0, c0);
-  z2 *g = p.gr(T, T - m);
-  const double dd = tile_dist2(x, n, g, r0, c0, red);
-  tile_store(x, n, g, r0, c0);
+  frag_mac(x, fq, fm, n, z2{1.0, 0.0});
+  double s2 = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (tile_i(r0, q) < n && tile_j(c0) < n) {
+      const z2 d = zsub(x.get(q), old.get(q));
+      s2 += d.x * d.x + d.y * d.y;
+    }
+  const double dd = wg_sum(s2, red);
+  tile_store(x, n, p.gr(T, T - m), r0, c0);
   if (threadIdx.x == 0) d2[(size_t)m * nb + b] = dd;
 }
 
@@ -700,28 +728,47 @@ __global__ void __launch_bounds__(256) kern_les_step(Pt p, const WTab *__restric
 
 // The same step split by columns: workgroup (m, b) owns columns 16b..16b+15
 // of block m (its waves take the four row tiles); X_m = Minv_m Q'_m keeps the
-// columns of Q'_m apart.
+// columns of Q'_m apart.  Operands go straight into registers, as in
+// kern_ret_step_band.
 __global__ void __launch_bounds__(256) kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *Q, z2 *X,
                                                           const z2 *Minv) {
   const WTab &w = *wtab;
-  __shared__ BlkLds s;
+  __shared__ z2 band[BO_NMAX][17];
   const int m = l + 1 + blockIdx.x, b = blockIdx.y, T = p.T, n = p.n;
   if (m > T) return;
-  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
+  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b, k1 = w.k + 1;
   const size_t es = p.es();
   z2 *Qm = Q + (size_t)m * es;
+  const z2 coef = {p.dt * greg(w, m, l), 0.0};
   TileAcc acc;
+  Frag fa, fb;
   tile_load(acc, n, Qm, r0, c0);
-  tile_mac(acc, s, n, p.sr(m, l), 0, 0, 64, X + (size_t)l * es, 0, c0, 16, z2{p.dt * greg(w, m, l), 0.0}, r0, c0);
+  frag_left(fa, n, p.sr(m, l), 0, r0);
+  frag_right(fb, n, X + (size_t)l * es, 0, c0);
   if (!(m == l + 1 && m <= T - 1)) {
+    frag_mac(acc, fa, fb, n, coef);
     tile_store(acc, n, Qm, r0, c0);
     return;
   }
-  for (int ll = 1; ll <= w.k + 1; ll++) tile_axpy(acc, n, X + (size_t)(m - ll) * es, 0, z2{0.0, -w.bd[ll] / p.dt}, r0, c0);
-  tile_store(acc, n, Qm, r0, c0);
+  TileAcc hist[KMAX + 1];
+  Frag fm;
+#pragma unroll
+  for (int ll = 1; ll <= KMAX + 1; ll++)
+    if (ll <= k1) tile_load(hist[ll - 1], n, X + (size_t)(m - ll) * es, r0, c0);
+  frag_left(fm, n, Minv + (size_t)m * es, 0, r0);
+  frag_mac(acc, fa, fb, n, coef);
+#pragma unroll
+  for (int ll = 1; ll <= KMAX + 1; ll++)
+    if (ll <= k1) {
+      const z2 cl = {0.0, -w.bd[ll] / p.dt};
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc.set(q, zadd(acc.get(q), zmul(cl, hist[ll - 1].get(q))));
+    }
+  Frag fq;
+  frag_right_band(fq, acc, band, n);
   TileAcc x;
   x.zero();
-  tile_mac(x, s, n, Minv + (size_t)m * es, 0, 0, 64, Qm, 0, c0, 16, z2{1.0, 0.0}, r0, c0);
+  frag_mac(x, fm, fq, n, z2{1.0, 0.0});
   tile_store(x, n, X + (size_t)m * es, r0, c0);
 }
 
