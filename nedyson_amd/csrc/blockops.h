@@ -320,34 +320,42 @@ __device__ __forceinline__ void tile_scale(TileAcc &acc, z2 f) {
 //   left  operand: op(L)[r0 + l%16][4s + l/16]
 //   right operand: op(R)[4s + l/16][c0 + l%16]
 // and zero outside nao x nao.
-struct Frag {
-  z2 v[BO_NMAX / 4];
+// A fragment may hold only the NS steps from s0 on (half the K range in
+// throughput kernels, which keeps registers for occupancy).
+template <int NS = BO_NMAX / 4>
+struct FragT {
+  z2 v[NS];
 };
+using Frag = FragT<>;
 
-__device__ __forceinline__ void frag_left(Frag &f, int n, const z2 *L, int opL, int r0) {
+template <int NS>
+__device__ __forceinline__ void frag_left(FragT<NS> &f, int n, const z2 *L, int opL, int r0, int s0 = 0) {
   const int lane = threadIdx.x & 63, r = r0 + (lane & 15);
 #pragma unroll
-  for (int s = 0; s < BO_NMAX / 4; s++) {
-    const int kk = 4 * s + (lane >> 4);
+  for (int s = 0; s < NS; s++) {
+    const int kk = 4 * (s0 + s) + (lane >> 4);
     f.v[s] = (r < n && kk < n) ? blk_get(L, n, opL, r, kk) : z2{0.0, 0.0};
   }
 }
 
-__device__ __forceinline__ void frag_right(Frag &f, int n, const z2 *R, int opR, int c0) {
+template <int NS>
+__device__ __forceinline__ void frag_right(FragT<NS> &f, int n, const z2 *R, int opR, int c0, int s0 = 0) {
   const int lane = threadIdx.x & 63, c = c0 + (lane & 15);
 #pragma unroll
-  for (int s = 0; s < BO_NMAX / 4; s++) {
-    const int kk = 4 * s + (lane >> 4);
+  for (int s = 0; s < NS; s++) {
+    const int kk = 4 * (s0 + s) + (lane >> 4);
     f.v[s] = (kk < n && c < n) ? blk_get(R, n, opR, kk, c) : z2{0.0, 0.0};
   }
 }
 
-// acc += coef * L R from fragments; coef scales the left operand, as the
-// LDS-staged products do, so both forms round alike
-__device__ __forceinline__ void frag_mac(TileAcc &acc, const Frag &a, const Frag &b, int n, z2 coef) {
+// acc += coef * L R over the fragments' steps; coef scales the left operand,
+// as the LDS-staged products do, so both forms round alike
+template <int NS>
+__device__ __forceinline__ void frag_mac(TileAcc &acc, const FragT<NS> &a, const FragT<NS> &b, int n, z2 coef,
+                                         int s0 = 0) {
 #pragma unroll
-  for (int s = 0; s < BO_NMAX / 4; s++) {
-    if (4 * s < n) {
+  for (int s = 0; s < NS; s++) {
+    if (4 * (s0 + s) < n) {
       const z2 x = zmul(coef, a.v[s]), y = b.v[s];
       acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, acc.re, 0, 0, 0);
       acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, acc.re, 0, 0, 0);

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <vector>
 #include <algorithm>
+#include <cstdlib>
 
 namespace ned {
 
@@ -513,6 +514,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   blk_store(acc, n, Qm);
 }
 
+// kern_ret_panel split by rows, for the late panels whose (T - q1) blocks
+// alone leave most of the chip idle: workgroup (m, b) owns rows
+// 16b..16b+15 of block m, its waves the four column tiles, four times the
+// workgroups at a quarter of the work each
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_ret_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q) {
+  const WTab &w = *wtab;
+  const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
+  if (m > T) return;
+  const int r0 = 16 * blockIdx.y, c0 = 16 * (threadIdx.x >> 6);
+  z2 *Qm = Q + (size_t)m * n * n;
+  TileAcc acc;
+  tile_load(acc, n, Qm, r0, c0);
+  for (int nn = q0; nn < q1; nn++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      FragT<BO_NMAX / 8> fa, fb;
+      frag_left(fa, n, p.gr(T, T - nn), 0, r0, h * BO_NMAX / 8);
+      frag_right(fb, n, p.sr(T - nn, T - m), 0, c0, h * BO_NMAX / 8);
+      frag_mac(acc, fa, fb, n, z2{greg(w, m, nn), 0.0}, h * BO_NMAX / 8);
+    }
+  tile_store(acc, n, Qm, r0, c0);
+}
+
 // ------------------------------------------------------------ mixing (tv)
 // src/dyson_step.h:130-169 with Ctv_tstp/CTV1/Conv.mixing
 // (src/dyson_integrals.h:80-116, 254-324): partial sums over chunks of the
@@ -786,6 +811,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     blk_mac<2>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
   __syncthreads();
   blk_store(acc, n, Qm);
+}
+
+// kern_les_panel split by columns (workgroup (m, b) owns columns
+// 16b..16b+15 of block m), for the late panels, as kern_ret_panel_band
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_les_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q, const z2 *X) {
+  const WTab &w = *wtab;
+  const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
+  if (m > T) return;
+  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * blockIdx.y;
+  const size_t es = p.es();
+  z2 *Qm = Q + (size_t)m * es;
+  TileAcc acc;
+  tile_load(acc, n, Qm, r0, c0);
+  for (int l = q0; l < q1; l++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      FragT<BO_NMAX / 8> fa, fb;
+      frag_left(fa, n, p.sr(m, l), 0, r0, h * BO_NMAX / 8);
+      frag_right(fb, n, X + (size_t)l * es, 0, c0, h * BO_NMAX / 8);
+      frag_mac(acc, fa, fb, n, z2{p.dt * greg(w, m, l), 0.0}, h * BO_NMAX / 8);
+    }
+  tile_store(acc, n, Qm, r0, c0);
 }
 
 // diagonal step, src/dyson_step.h:283-310
@@ -1207,6 +1255,16 @@ void set_matsubara(Ctx &c, double beta) {
   c.have_mix = true;
 }
 
+// Panels with at most this many later blocks take the band-split update
+// (NED_PANEL_BAND_MAX overrides; 0 turns it off).
+static int panel_band_max() {
+  static const int v = [] {
+    const char *e = std::getenv("NED_PANEL_BAND_MAX");
+    return e ? std::atoi(e) : 512;
+  }();
+  return v;
+}
+
 void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) {
   require(c.have_mix && c.mix_beta == beta, "dyson_step: call ned_set_matsubara(beta) first");
   require(T > c.k && T <= c.nt, "dyson_step: tstp must be in (k, nt]");
@@ -1254,7 +1312,12 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     const int q1 = std::min(q0 + RPANEL, T);
     for (int nn = q0; nn < q1; nn++)
       hipLaunchKernelGGL(kern_ret_step_band, dim3(q1 - nn, nb), dim3(256), 0, s, p, wd, nn, Q, Minv, d2);
-    if (q1 < T) hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q);
+    if (q1 < T) {
+      if (T - q1 <= panel_band_max())
+        hipLaunchKernelGGL(kern_ret_panel_band, dim3(T - q1, nb), dim3(256), 0, s, p, wd, q0, q1, Q);
+      else
+        hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q);
+    }
   }
   if (T > k + 1) hipLaunchKernelGGL(kern_band_err, dim3(1), dim3(256), 0, s, d2, k + 2, T, nb, err + 0);
   check_launch("ret history");
@@ -1297,7 +1360,12 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
       const int q1 = std::min(q0 + RPANEL, T);
       for (int l = q0; l < q1; l++)
         hipLaunchKernelGGL(kern_les_step_band, dim3(q1 - l, nb), dim3(256), 0, s, p, wd, l, Q, X, Minv);
-      if (q1 < T) hipLaunchKernelGGL(kern_les_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q, X);
+      if (q1 < T) {
+        if (T - q1 <= panel_band_max())
+          hipLaunchKernelGGL(kern_les_panel_band, dim3(T - q1, nb), dim3(256), 0, s, p, wd, q0, q1, Q, X);
+        else
+          hipLaunchKernelGGL(kern_les_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q, X);
+      }
     }
     hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, Q, X, c.dy_scr.as<z2>());
     hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, X, err + 2);

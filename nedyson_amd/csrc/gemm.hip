@@ -365,15 +365,25 @@ struct NtCfg {
 // never share an accumulator.
 // DB: two LDS stages; the next slice is written into the idle stage while the
 // current one is consumed, one barrier per slice.
-template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false>
+// SWZ (TK = 32 only): unpadded 64-byte LDS rows whose four 16-byte segments
+// are permuted by row, segment s of row r stored at s ^ ((r >> 1 ^ r >> 3) & 3).
+// The 16 lanes of a fragment read (16 rows, one segment) then fall on
+// distinct banks, and so do the 16 lanes of a staging write (4 rows, all
+// segments), which the padded layout serves with two-way conflicts.
+template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false, bool SWZ = false>
 __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
     kern_nt_bf16x3(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo, long long lda, long long strA,
                    const uint16_t *__restrict__ Bhi, const uint16_t *__restrict__ Blo, long long ldb, long long strB,
                    float *__restrict__ C, long long ldc, long long strC, int M, int N, int K, float alpha, int beta) {
   using Cfg = NtCfg<TM, TN, WM, WN, TK>;
   constexpr int NTH = Cfg::THREADS, BI = Cfg::BI, BJ = Cfg::BJ, LA = Cfg::LA, LB = Cfg::LB, SEG = Cfg::SEG;
-  constexpr int LDK = TK + 8;
+  static_assert(!SWZ || TK == 32, "the swizzled layout is for 32-deep slices");
+  constexpr int LDK = SWZ ? TK : TK + 8;
   constexpr int WCOLS = TN / WN;
+  // element offset of 8-element segment sg of LDS row rw
+  auto lofs = [](int rw, int sg) __attribute__((always_inline)) {
+    return SWZ ? ((sg ^ (((rw >> 1) ^ (rw >> 3)) & 3)) << 3) : (sg << 3);
+  };
   Ahi += (size_t)blockIdx.y * strA;
   Alo += (size_t)blockIdx.y * strA;
   Bhi += (size_t)blockIdx.y * strB;
@@ -425,33 +435,33 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
 #pragma unroll
     for (int i = 0; i < LA; i++) {
       const int idx = tid + NTH * i, row = idx / SEG, seg = idx % SEG;
-      *reinterpret_cast<u32x4 *>(&sA[st][0][row][seg * 8]) = ra0[i];
-      *reinterpret_cast<u32x4 *>(&sA[st][1][row][seg * 8]) = ra1[i];
+      *reinterpret_cast<u32x4 *>(&sA[st][0][row][lofs(row, seg)]) = ra0[i];
+      *reinterpret_cast<u32x4 *>(&sA[st][1][row][lofs(row, seg)]) = ra1[i];
     }
 #pragma unroll
     for (int i = 0; i < LB; i++) {
       const int idx = tid + NTH * i, row = idx / SEG, seg = idx % SEG;
-      *reinterpret_cast<u32x4 *>(&sB[st][0][row][seg * 8]) = rb0[i];
-      *reinterpret_cast<u32x4 *>(&sB[st][1][row][seg * 8]) = rb1[i];
+      *reinterpret_cast<u32x4 *>(&sB[st][0][row][lofs(row, seg)]) = rb0[i];
+      *reinterpret_cast<u32x4 *>(&sB[st][1][row][lofs(row, seg)]) = rb1[i];
     }
   };
   auto compute = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
     for (int kk = 0; kk < TK; kk += 32) {
-      const int kq = kk + 8 * (lane >> 4);
+      const int kseg = (kk >> 3) + (lane >> 4);
       if constexpr (STREAM) {
         v8bf bh[BJ], bl[BJ];
 #pragma unroll
         for (int j = 0; j < BJ; j++) {
           const int col = wn * WN + j * 16 + (lane & 15);
-          bh[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][0][col][kq]));
-          bl[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][1][col][kq]));
+          bh[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][0][col][lofs(col, kseg)]));
+          bl[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][1][col][lofs(col, kseg)]));
         }
 #pragma unroll
         for (int i = 0; i < BI; i++) {
           const int row = wm * WM + i * 16 + (lane & 15);
-          const v8bf a_h = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][0][row][kq]));
-          const v8bf a_l = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][1][row][kq]));
+          const v8bf a_h = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][0][row][lofs(row, kseg)]));
+          const v8bf a_l = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][1][row][lofs(row, kseg)]));
 #pragma unroll
           for (int j = 0; j < BJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
@@ -464,14 +474,14 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
 #pragma unroll
         for (int i = 0; i < BI; i++) {
           const int row = wm * WM + i * 16 + (lane & 15);
-          ah[i] = *reinterpret_cast<const u32x4 *>(&sA[st][0][row][kq]);
-          al[i] = *reinterpret_cast<const u32x4 *>(&sA[st][1][row][kq]);
+          ah[i] = *reinterpret_cast<const u32x4 *>(&sA[st][0][row][lofs(row, kseg)]);
+          al[i] = *reinterpret_cast<const u32x4 *>(&sA[st][1][row][lofs(row, kseg)]);
         }
 #pragma unroll
         for (int j = 0; j < BJ; j++) {
           const int col = wn * WN + j * 16 + (lane & 15);
-          bh[j] = *reinterpret_cast<const u32x4 *>(&sB[st][0][col][kq]);
-          bl[j] = *reinterpret_cast<const u32x4 *>(&sB[st][1][col][kq]);
+          bh[j] = *reinterpret_cast<const u32x4 *>(&sB[st][0][col][lofs(col, kseg)]);
+          bl[j] = *reinterpret_cast<const u32x4 *>(&sB[st][1][col][lofs(col, kseg)]);
         }
 #pragma unroll
         for (int i = 0; i < BI; i++)
@@ -554,11 +564,11 @@ struct NtVariant {
   const char *name;
 };
 
-template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false>
+template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false, bool SWZ = false>
 void nt_launch(const NtOp &A, const NtOp &B, float *C, long long ldc, long long strC, int M, int N, int K, int batch,
                float alpha, int beta, hipStream_t s) {
   const int tiles = (M / TM) * (N / TN);
-  hipLaunchKernelGGL((kern_nt_bf16x3<TM, TN, WM, WN, TK, STREAM, DB>), dim3(tiles, batch),
+  hipLaunchKernelGGL((kern_nt_bf16x3<TM, TN, WM, WN, TK, STREAM, DB, SWZ>), dim3(tiles, batch),
                      dim3(NtCfg<TM, TN, WM, WN, TK>::THREADS), 0, s, A.hi, A.lo, A.ld, A.stride, B.hi, B.lo, B.ld,
                      B.stride, C, ldc, strC, M, N, K, alpha, beta);
 }
@@ -574,6 +584,12 @@ const NtVariant kNt[] = {
     {256, 256, nt_launch<256, 256, 128, 64, 32, true, true>, "256x256 / 8 waves of 128x64, K32, streamed A, 2 stages"},
     {128, 128, nt_launch<128, 128, 64, 64, 32, true, true>, "128x128 / 4 waves of 64x64, K32, streamed A, 2 stages"},
     {256, 128, nt_launch<256, 128, 64, 64, 32, true, true>, "256x128 / 8 waves of 64x64, K32, streamed A, 2 stages"},
+    {256, 256, nt_launch<256, 256, 128, 64, 32, true, true, true>,
+     "256x256 / 8 waves of 128x64, K32, streamed A, 2 stages, swizzled"},
+    {128, 128, nt_launch<128, 128, 64, 64, 32, true, true, true>,
+     "128x128 / 4 waves of 64x64, K32, streamed A, 2 stages, swizzled"},
+    {256, 128, nt_launch<256, 128, 64, 64, 32, true, true, true>,
+     "256x128 / 8 waves of 64x64, K32, streamed A, 2 stages, swizzled"},
 };
 constexpr int kNtCount = sizeof(kNt) / sizeof(kNt[0]);
 
