@@ -67,6 +67,14 @@ long long nblocks_of(const ned::Ctx &c, int comp) {
 void state(bool ok, const char *msg) {
   if (!ok) throw std::runtime_error(std::string("state: ") + msg);
 }
+
+void release_streams(ned::Ctx &c) {
+  for (hipEvent_t &e : c.ev_side)
+    if (e) (void)hipEventDestroy(e);
+  if (c.ev_main) (void)hipEventDestroy(c.ev_main);
+  if (c.side) (void)hipStreamDestroy(c.side);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+}
 }  // namespace
 
 struct ned_mlp {
@@ -113,6 +121,9 @@ ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
     c.es = c.nao * c.nao;
     c.np = c.ntau + 1;
     NED_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    NED_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    NED_HIP(hipEventCreateWithFlags(&c.ev_main, hipEventDisableTiming));
+    for (hipEvent_t &e : c.ev_side) NED_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const size_t es = c.es, ntri = (size_t)(c.nt + 1) * (c.nt + 2) / 2;
     for (ned::Green *g : {&c.G, &c.S}) {
       g->ret.ensure(ntri * es * sizeof(ned::z2));
@@ -135,7 +146,7 @@ ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
   });
   if (st != NED_OK) {
     if (ctx) {
-      if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+      release_streams(ctx->c);
       delete ctx;
     }
     if (g_err.find("out of memory") != std::string::npos) st = NED_ERR_NOMEM;
@@ -147,8 +158,11 @@ ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
 
 ned_status ned_ctx_destroy(ned_ctx *ctx) {
   if (!ctx) return NED_OK;
-  ned_status st = guard(ctx, [&] { NED_HIP(hipStreamSynchronize(ctx->c.stream)); });
-  (void)hipStreamDestroy(ctx->c.stream);
+  ned_status st = guard(ctx, [&] {
+    NED_HIP(hipStreamSynchronize(ctx->c.side));
+    NED_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+  release_streams(ctx->c);
   delete ctx;
   return st;
 }

@@ -43,6 +43,10 @@ struct Ctx {
   int nao = 0, naux = 0, nt = 0, ntau = 0, k = 0, sig = -1, device = 0;
   int es = 0, np = 0;
   hipStream_t stream = nullptr;
+  // second stream of the Dyson recurrences (bulk panel updates overlapped
+  // with the dependency chain), and the events that order the two
+  hipStream_t side = nullptr;
+  hipEvent_t ev_main = nullptr, ev_side[2] = {nullptr, nullptr};
   Green G, S;
   DevBuf hmf;      // (nt+1) x es complex
   DevBuf Bq;       // naux x nao x nao double, B[Q][i][j] = V(i,j,Q)
@@ -67,7 +71,7 @@ struct Ctx {
   double minv_mu = 0.0 / 0.0, minv_dt = 0.0 / 0.0;
   double mix_beta = 0;
   bool have_mix = false;
-  DevBuf dy_info, dy_js, dy_gA, dy_gB, dy_err, dy_Q, dy_X, dy_M, dy_Qs, dy_part, dy_scr, dy_Y;
+  DevBuf dy_info, dy_js, dy_gA, dy_gB, dy_err, dy_Q, dy_QA, dy_X, dy_M, dy_Qs, dy_part, dy_scr, dy_Y;
   DevBuf dy_la, dy_lb, dy_mixtmp, dy_h0, dy_xq, dy_bootI, dy_en;
   // inverses of the k x k block start-up systems: the retarded one depends on
   // times T-k..T-1 only (valid for every corrector pass of step retinv_T), the

@@ -51,7 +51,6 @@ const WTab *dev_wtab(Ctx &c) {
 namespace {
 
 // blocks per panel of the blocked Volterra recurrences
-constexpr int RPANEL = 16;
 
 __device__ __forceinline__ size_t dtri(int t) { return (size_t)t * (t + 1) / 2; }
 
@@ -389,12 +388,15 @@ __device__ void ret_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s,
 }
 
 // history sums from the start-up blocks l = 0..k, for m = k+1..T
-__global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, const WTab *__restrict__ wtab, z2 *Q, const z2 *Minv, double *err) {
+// (QA: the dependency chain's own accumulator, zeroed here)
+__global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA,
+                                                      const z2 *Minv, double *err) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   __shared__ double red[256];
   const int m = w.k + 1 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
+  for (int e = threadIdx.x; e < n * n; e += 256) QA[(size_t)m * n * n + e] = z2{0.0, 0.0};
   BlkAcc acc;
   acc.zero();
   for (int l = 0; l <= w.k; l++) blk_mac(acc, s, n, p.gr(T, T - l), 0, p.sr(T - l, T - m), 0, z2{greg(w, m, l), 0.0});
@@ -426,7 +428,7 @@ __global__ void __launch_bounds__(256) kern_ret_step(Pt p, const WTab *__restric
 // workgroup: every operand is read straight into registers at the start (no
 // LDS staging), and only the finished band Q' passes through LDS.
 __global__ void __launch_bounds__(256) kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *Q,
-                                                          const z2 *Minv, double *d2) {
+                                                          const z2 *QB, const z2 *Minv, double *d2) {
   const WTab &w = *wtab;
   __shared__ z2 band[16][BO_NMAX + 1];
   __shared__ double red[4];
@@ -445,14 +447,18 @@ __global__ void __launch_bounds__(256) kern_ret_step_band(Pt p, const WTab *__re
     tile_store(acc, n, Qm, r0, c0);
     return;
   }
-  // finish the band: Q' = dt Q - i/dt sum_l bd_l G^R(T, T-m+l), X = Q' Minv_{T-m}
-  TileAcc hist[KMAX + 1], old;
+  // finish the band: Q' = dt Q - i/dt sum_l bd_l G^R(T, T-m+l), X = Q' Minv_{T-m},
+  // with Q the sum of the chain's and the bulk updates' accumulators
+  TileAcc hist[KMAX + 1], old, qb;
   Frag fm;
+  tile_load(qb, n, QB + (size_t)m * n * n, r0, c0);
 #pragma unroll
   for (int l = 1; l <= KMAX + 1; l++)
     if (l <= k1) tile_load(hist[l - 1], n, p.gr(T, T - m + l), r0, c0);
   frag_right(fm, n, Minv + (size_t)(T - m) * n * n, 0, c0);
   tile_load(old, n, p.gr(T, T - m), r0, c0);
+  acc.re += qb.re;
+  acc.im += qb.im;
   frag_mac(acc, fa, fb, n, coef);
   tile_scale(acc, z2{p.dt, 0.0});
 #pragma unroll
@@ -500,10 +506,11 @@ __global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *
 
 // contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
 // the history sums stay in registers across the whole panel
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
-  const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
+  const int m = m0 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * n * n;
   BlkAcc acc;
@@ -519,9 +526,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 // 16b..16b+15 of block m, its waves the four column tiles, four times the
 // workgroups at a quarter of the work each
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_ret_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q) {
+kern_ret_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
   const WTab &w = *wtab;
-  const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
+  const int m = m0 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
   const int r0 = 16 * blockIdx.y, c0 = 16 * (threadIdx.x >> 6);
   z2 *Qm = Q + (size_t)m * n * n;
@@ -724,11 +731,14 @@ __device__ void les_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s,
   blk_store(x, n, X + (size_t)m * p.es());
 }
 
-__global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restrict__ wtab, z2 *Q, z2 *X, const z2 *Minv) {
+// (QA: the dependency chain's own accumulator, zeroed here)
+__global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA, z2 *X,
+                                                      const z2 *Minv) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = w.k + 1 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
+  for (int e = threadIdx.x; e < n * n; e += 256) QA[(size_t)m * p.es() + e] = z2{0.0, 0.0};
   z2 *Qm = Q + (size_t)m * p.es();
   BlkAcc acc;
   blk_load(acc, n, Qm);
@@ -755,8 +765,8 @@ __global__ void __launch_bounds__(256) kern_les_step(Pt p, const WTab *__restric
 // of block m (its waves take the four row tiles); X_m = Minv_m Q'_m keeps the
 // columns of Q'_m apart.  Operands go straight into registers, as in
 // kern_ret_step_band.
-__global__ void __launch_bounds__(256) kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *Q, z2 *X,
-                                                          const z2 *Minv) {
+__global__ void __launch_bounds__(256) kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *Q,
+                                                          const z2 *QB, z2 *X, const z2 *Minv) {
   const WTab &w = *wtab;
   __shared__ z2 band[BO_NMAX][17];
   const int m = l + 1 + blockIdx.x, b = blockIdx.y, T = p.T, n = p.n;
@@ -775,12 +785,15 @@ __global__ void __launch_bounds__(256) kern_les_step_band(Pt p, const WTab *__re
     tile_store(acc, n, Qm, r0, c0);
     return;
   }
-  TileAcc hist[KMAX + 1];
+  TileAcc hist[KMAX + 1], qb;
   Frag fm;
+  tile_load(qb, n, QB + (size_t)m * es, r0, c0);
 #pragma unroll
   for (int ll = 1; ll <= KMAX + 1; ll++)
     if (ll <= k1) tile_load(hist[ll - 1], n, X + (size_t)(m - ll) * es, r0, c0);
   frag_left(fm, n, Minv + (size_t)m * es, 0, r0);
+  acc.re += qb.re;
+  acc.im += qb.im;
   frag_mac(acc, fa, fb, n, coef);
 #pragma unroll
   for (int ll = 1; ll <= KMAX + 1; ll++)
@@ -798,11 +811,11 @@ __global__ void __launch_bounds__(256) kern_les_step_band(Pt p, const WTab *__re
 }
 
 // contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q,
-                                                      const z2 *X) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
-  const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
+  const int m = m0 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * p.es();
   BlkAcc acc;
@@ -816,9 +829,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 // kern_les_panel split by columns (workgroup (m, b) owns columns
 // 16b..16b+15 of block m), for the late panels, as kern_ret_panel_band
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_les_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q, const z2 *X) {
+kern_les_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
   const WTab &w = *wtab;
-  const int m = q1 + 1 + blockIdx.x, T = p.T, n = p.n;
+  const int m = m0 + blockIdx.x, T = p.T, n = p.n;
   if (m > T) return;
   const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * blockIdx.y;
   const size_t es = p.es();
@@ -837,7 +850,8 @@ kern_les_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, z2 *Q, 
 }
 
 // diagonal step, src/dyson_step.h:283-310
-__global__ void __launch_bounds__(256) kern_les_diag(Pt p, const WTab *__restrict__ wtab, z2 *Q, z2 *X, z2 *scr) {
+__global__ void __launch_bounds__(256) kern_les_diag(Pt p, const WTab *__restrict__ wtab, z2 *Q, const z2 *QA, z2 *X,
+                                                     z2 *scr) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int T = p.T, n = p.n;
@@ -850,6 +864,7 @@ __global__ void __launch_bounds__(256) kern_les_diag(Pt p, const WTab *__restric
   __syncthreads();
   // Q_T += dt w(T,T) Sigma^R(T,T) X_T + (hmf(T) - mu) X_T
   blk_load(acc, n, Q + (size_t)T * es);
+  blk_axpy(acc, n, QA + (size_t)T * es, 0, z2{1.0, 0.0});
   blk_mac(acc, s, n, p.sr(T, T), 0, XT, 0, z2{p.dt * greg(w, T, T), 0.0});
   blk_mac(acc, s, n, p.h(T), 0, XT, 0, z2{1.0, 0.0});
   blk_axpy(acc, n, XT, 0, z2{-p.mu, 0.0});
@@ -1255,6 +1270,22 @@ void set_matsubara(Ctx &c, double beta) {
   c.have_mix = true;
 }
 
+// Blocks per panel of the Dyson recurrences (NED_RPANEL overrides).
+static int rpanel() {
+  static const int v = [] {
+    const char *e = std::getenv("NED_RPANEL");
+    const int r = e ? std::atoi(e) : 16;
+    return r >= 1 ? r : 16;
+  }();
+  return v;
+}
+
+// The context's stream waits for everything issued so far on the side stream.
+static void join_side(Ctx &c) {
+  NED_HIP(hipEventRecord(c.ev_main, c.side));
+  NED_HIP(hipStreamWaitEvent(c.stream, c.ev_main, 0));
+}
+
 // Panels with at most this many later blocks take the band-split update
 // (NED_PANEL_BAND_MAX overrides; 0 turns it off).
 static int panel_band_max() {
@@ -1280,10 +1311,13 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   double *err = c.dy_err.as<double>();
   NED_HIP(hipMemsetAsync(err, 0, 4 * sizeof(double), s));
   c.dy_Q.ensure((size_t)(T + 1) * es * sizeof(z2));
+  c.dy_QA.ensure((size_t)(T + 1) * es * sizeof(z2));
   c.dy_X.ensure((size_t)(T + 1) * es * sizeof(z2));
   c.dy_M.ensure((size_t)kn * kn * sizeof(z2));
   c.dy_Qs.ensure((size_t)kn * n * sizeof(z2));
   z2 *Q = c.dy_Q.as<z2>(), *X = c.dy_X.as<z2>(), *M = c.dy_M.as<z2>(), *Qs = c.dy_Qs.as<z2>();
+  z2 *QA = c.dy_QA.as<z2>();
+  const int RP = rpanel();
   const int nb = (n + 15) / 16;  // 16-row / 16-column bands of a block
   c.dy_d2.ensure((size_t)(T + 1) * nb * sizeof(double));
   double *d2 = c.dy_d2.as<double>();
@@ -1303,22 +1337,31 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_retinv.as<z2>(), Qs, kn, n, Xs);
   check_launch("ret start");
   hipLaunchKernelGGL(kern_ret_start_store, dim3(k), dim3(256), 0, s, p, Xs, err + 0);
-  hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, Minv, err + 0);
-  // blocked right-looking recurrence: inside a panel of RPANEL blocks each
-  // finished block updates only the panel's own later blocks (small launches
-  // along the dependency chain); then one launch applies the whole panel to
-  // every later block
-  for (int q0 = k + 1; q0 < T; q0 += RPANEL) {
-    const int q1 = std::min(q0 + RPANEL, T);
+  hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, QA, Minv, err + 0);
+  // Blocked right-looking recurrence with look-ahead.  Panel p holds the
+  // links nn in [q0, q1).  Each link finishes block nn+1 and adds block nn to
+  // the later blocks of its own panel and of the next one (small launches on
+  // the context's stream, the dependency chain; accumulator QA).  The bulk
+  // update -- panel p applied to every block beyond the next panel -- runs on
+  // the side stream into a second accumulator Q while the chain goes on, and
+  // the chain of panel p+2 waits for it.  A finishing link adds both sums.
+  int pi = 0;
+  for (int q0 = k + 1; q0 < T; q0 += RP, pi++) {
+    const int q1 = std::min(q0 + RP, T), q1x = std::min(q1 + RP, T);
+    if (pi >= 2) NED_HIP(hipStreamWaitEvent(s, c.ev_side[pi & 1], 0));
     for (int nn = q0; nn < q1; nn++)
-      hipLaunchKernelGGL(kern_ret_step_band, dim3(q1 - nn, nb), dim3(256), 0, s, p, wd, nn, Q, Minv, d2);
-    if (q1 < T) {
-      if (T - q1 <= panel_band_max())
-        hipLaunchKernelGGL(kern_ret_panel_band, dim3(T - q1, nb), dim3(256), 0, s, p, wd, q0, q1, Q);
+      hipLaunchKernelGGL(kern_ret_step_band, dim3(q1x - nn, nb), dim3(256), 0, s, p, wd, nn, QA, Q, Minv, d2);
+    if (q1x < T) {
+      NED_HIP(hipEventRecord(c.ev_main, s));
+      NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
+      if (T - q1x <= panel_band_max())
+        hipLaunchKernelGGL(kern_ret_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q);
       else
-        hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q);
+        hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q);
+      NED_HIP(hipEventRecord(c.ev_side[pi & 1], c.side));
     }
   }
+  join_side(c);
   if (T > k + 1) hipLaunchKernelGGL(kern_band_err, dim3(1), dim3(256), 0, s, d2, k + 2, T, nb, err + 0);
   check_launch("ret history");
 
@@ -1355,19 +1398,27 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     }
     hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_lesinv.as<z2>(), Qs, kn, n,
                        X + es);
-    hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, X, Minv);
-    for (int q0 = k + 1; q0 < T; q0 += RPANEL) {
-      const int q1 = std::min(q0 + RPANEL, T);
+    hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, QA, X, Minv);
+    // the same look-ahead schedule as the retarded recurrence
+    int pi = 0;
+    for (int q0 = k + 1; q0 < T; q0 += RP, pi++) {
+      const int q1 = std::min(q0 + RP, T), q1x = std::min(q1 + RP, T);
+      if (pi >= 2) NED_HIP(hipStreamWaitEvent(s, c.ev_side[pi & 1], 0));
       for (int l = q0; l < q1; l++)
-        hipLaunchKernelGGL(kern_les_step_band, dim3(q1 - l, nb), dim3(256), 0, s, p, wd, l, Q, X, Minv);
-      if (q1 < T) {
-        if (T - q1 <= panel_band_max())
-          hipLaunchKernelGGL(kern_les_panel_band, dim3(T - q1, nb), dim3(256), 0, s, p, wd, q0, q1, Q, X);
+        hipLaunchKernelGGL(kern_les_step_band, dim3(q1x - l, nb), dim3(256), 0, s, p, wd, l, QA, Q, X, Minv);
+      if (q1x < T) {
+        NED_HIP(hipEventRecord(c.ev_main, s));
+        NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
+        if (T - q1x <= panel_band_max())
+          hipLaunchKernelGGL(kern_les_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1,
+                             Q, X);
         else
-          hipLaunchKernelGGL(kern_les_panel, dim3(T - q1), dim3(256), 0, s, p, wd, q0, q1, Q, X);
+          hipLaunchKernelGGL(kern_les_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q, X);
+        NED_HIP(hipEventRecord(c.ev_side[pi & 1], c.side));
       }
     }
-    hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, Q, X, c.dy_scr.as<z2>());
+    join_side(c);
+    hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, Q, QA, X, c.dy_scr.as<z2>());
     hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, X, err + 2);
     check_launch("les");
   }
