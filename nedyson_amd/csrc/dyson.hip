@@ -50,8 +50,6 @@ const WTab *dev_wtab(Ctx &c) {
 
 namespace {
 
-// blocks per panel of the blocked Volterra recurrences
-
 __device__ __forceinline__ size_t dtri(int t) { return (size_t)t * (t + 1) / 2; }
 
 // gregory_weights(n, j), src/integration.h:78-91
@@ -401,8 +399,15 @@ __global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, const WTab *__restri
   acc.zero();
   for (int l = 0; l <= w.k; l++) blk_mac(acc, s, n, p.gr(T, T - l), 0, p.sr(T - l, T - m), 0, z2{greg(w, m, l), 0.0});
   z2 *Qm = Q + (size_t)m * n * n;
-  if (m == w.k + 1) ret_finalize(p, w, acc, s, m, Qm, Minv, red, err);
-  else blk_store(acc, n, Qm);
+  if (m == w.k + 1) {
+    ret_finalize(p, w, acc, s, m, Qm, Minv, red, err);
+    return;
+  }
+  // the start blocks' share of the finishing terms of block m (added by the
+  // chain links for every later block): -i/dt^2 bd_{m-j} G^R(T, T-j)
+  for (int j = m - w.k - 1; j <= w.k; j++)
+    blk_axpy(acc, n, p.gr(T, T - j), 0, z2{0.0, -w.bd[m - j] / (p.dt * p.dt)});
+  blk_store(acc, n, Qm);
 }
 
 // add block nn's contribution to every later history sum; finish nn+1
@@ -423,51 +428,52 @@ __global__ void __launch_bounds__(256) kern_ret_step(Pt p, const WTab *__restric
 // The same step split by rows: workgroup (m, b) owns rows 16b..16b+15 of
 // block m (its waves take the four column tiles).  G^R(T, T-m) depends only
 // on the same rows of the history sum, so the bands never exchange data; the
-// squared update of each finished band goes to d2[m * nb + b].  This launch
-// sits on the dependency chain, so its time is the latency of the finishing
-// workgroup: every operand is read straight into registers at the start (no
-// LDS staging), and only the finished band Q' passes through LDS.
-__global__ void __launch_bounds__(256) kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *Q,
-                                                          const z2 *QB, const z2 *Minv, double *d2) {
+// squared update of each finished band goes to d2[m * nb + b].
+//
+// This launch is the dependency chain, so its time is the latency of the
+// finishing workgroup, and it runs beside the bulk panel updates of the side
+// stream.  Hence: operands go straight into registers (no LDS staging), the
+// finishing terms -i/dt bd_l G^R(T, T-m+l) of Q' = dt Q - ... are folded
+// into the accumulator by the link that finishes each block (so that the
+// finishing link adds one of them, not k+1), the register budget (at most
+// 224 a wave) lets a chain wave share a SIMD with two bulk panel waves (at
+// most 136 each), and the waves raise their issue priority.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *QA, const z2 *QB, const z2 *Minv, double *d2) {
+  __builtin_amdgcn_s_setprio(3);
   const WTab &w = *wtab;
   __shared__ z2 band[16][BO_NMAX + 1];
   __shared__ double red[4];
   const int m = nn + 1 + blockIdx.x, b = blockIdx.y, nb = gridDim.y, T = p.T, n = p.n;
   if (m > T) return;
-  const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6), k1 = w.k + 1;
-  z2 *Qm = Q + (size_t)m * n * n;
-  const z2 coef = {greg(w, m, nn), 0.0};
-  TileAcc acc;
+  const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
+  const bool fin = m == nn + 1, bdt = m - nn <= w.k + 1;
+  z2 *Qm = QA + (size_t)m * n * n;
+  TileAcc acc, gt, qb, old;
   Frag fa, fb;
   tile_load(acc, n, Qm, r0, c0);
   frag_left(fa, n, p.gr(T, T - nn), 0, r0);
   frag_right(fb, n, p.sr(T - nn, T - m), 0, c0);
-  if (m != nn + 1) {
-    frag_mac(acc, fa, fb, n, coef);
+  if (bdt) tile_load(gt, n, p.gr(T, T - nn), r0, c0);
+  if (fin) {
+    tile_load(qb, n, QB + (size_t)m * n * n, r0, c0);
+    tile_load(old, n, p.gr(T, T - m), r0, c0);
+  }
+  frag_mac(acc, fa, fb, n, z2{greg(w, m, nn), 0.0});
+  if (bdt) {
+    const z2 cl = {0.0, -w.bd[m - nn] / (p.dt * p.dt)};
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc.set(q, zadd(acc.get(q), zmul(cl, gt.get(q))));
+  }
+  if (!fin) {
     tile_store(acc, n, Qm, r0, c0);
     return;
   }
-  // finish the band: Q' = dt Q - i/dt sum_l bd_l G^R(T, T-m+l), X = Q' Minv_{T-m},
-  // with Q the sum of the chain's and the bulk updates' accumulators
-  TileAcc hist[KMAX + 1], old, qb;
+  // finish the band: Q' = dt (QA + QB), X = Q' Minv_{T-m}
   Frag fm;
-  tile_load(qb, n, QB + (size_t)m * n * n, r0, c0);
-#pragma unroll
-  for (int l = 1; l <= KMAX + 1; l++)
-    if (l <= k1) tile_load(hist[l - 1], n, p.gr(T, T - m + l), r0, c0);
   frag_right(fm, n, Minv + (size_t)(T - m) * n * n, 0, c0);
-  tile_load(old, n, p.gr(T, T - m), r0, c0);
-  acc.re += qb.re;
-  acc.im += qb.im;
-  frag_mac(acc, fa, fb, n, coef);
-  tile_scale(acc, z2{p.dt, 0.0});
-#pragma unroll
-  for (int l = 1; l <= KMAX + 1; l++)
-    if (l <= k1) {
-      const z2 cl = {0.0, -w.bd[l] / p.dt};
-#pragma unroll
-      for (int q = 0; q < 4; q++) acc.set(q, zadd(acc.get(q), zmul(cl, hist[l - 1].get(q))));
-    }
+  acc.re = (acc.re + qb.re) * p.dt;
+  acc.im = (acc.im + qb.im) * p.dt;
   Frag fq;
   frag_left_band(fq, acc, band, n);
   TileAcc x;
@@ -744,8 +750,15 @@ __global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restri
   blk_load(acc, n, Qm);
   for (int l = 0; l <= w.k; l++)
     blk_mac<2>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
-  if (m == w.k + 1 && m <= T - 1) les_finalize(p, w, acc, s, m, Qm, X, Minv);
-  else blk_store(acc, n, Qm);
+  if (m == w.k + 1 && m <= T - 1) {
+    les_finalize(p, w, acc, s, m, Qm, X, Minv);
+    return;
+  }
+  // the start blocks' share of the finishing terms -i/dt bd_{m-j} X_j of
+  // block m (not for m = T, which the diagonal step finishes)
+  if (m <= T - 1)
+    for (int j = m - w.k - 1; j <= w.k; j++) blk_axpy(acc, n, X + (size_t)j * p.es(), 0, z2{0.0, -w.bd[m - j] / p.dt});
+  blk_store(acc, n, Qm);
 }
 
 __global__ void __launch_bounds__(256) kern_les_step(Pt p, const WTab *__restrict__ wtab, int l, z2 *Q, z2 *X, const z2 *Minv) {
@@ -763,45 +776,40 @@ __global__ void __launch_bounds__(256) kern_les_step(Pt p, const WTab *__restric
 
 // The same step split by columns: workgroup (m, b) owns columns 16b..16b+15
 // of block m (its waves take the four row tiles); X_m = Minv_m Q'_m keeps the
-// columns of Q'_m apart.  Operands go straight into registers, as in
-// kern_ret_step_band.
-__global__ void __launch_bounds__(256) kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *Q,
-                                                          const z2 *QB, z2 *X, const z2 *Minv) {
+// columns of Q'_m apart.  Built as kern_ret_step_band (the finishing terms
+// -i/dt bd_l X_{m-l} folded in by the link that finishes X_{m-l}).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *QA, const z2 *QB, z2 *X, const z2 *Minv) {
+  __builtin_amdgcn_s_setprio(3);
   const WTab &w = *wtab;
   __shared__ z2 band[BO_NMAX][17];
   const int m = l + 1 + blockIdx.x, b = blockIdx.y, T = p.T, n = p.n;
   if (m > T) return;
-  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b, k1 = w.k + 1;
+  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
+  const bool fin = m == l + 1 && m <= T - 1, bdt = m - l <= w.k + 1 && m <= T - 1;
   const size_t es = p.es();
-  z2 *Qm = Q + (size_t)m * es;
-  const z2 coef = {p.dt * greg(w, m, l), 0.0};
-  TileAcc acc;
+  z2 *Qm = QA + (size_t)m * es;
+  TileAcc acc, gt, qb;
   Frag fa, fb;
   tile_load(acc, n, Qm, r0, c0);
   frag_left(fa, n, p.sr(m, l), 0, r0);
   frag_right(fb, n, X + (size_t)l * es, 0, c0);
-  if (!(m == l + 1 && m <= T - 1)) {
-    frag_mac(acc, fa, fb, n, coef);
+  if (bdt) tile_load(gt, n, X + (size_t)l * es, r0, c0);
+  if (fin) tile_load(qb, n, QB + (size_t)m * es, r0, c0);
+  frag_mac(acc, fa, fb, n, z2{p.dt * greg(w, m, l), 0.0});
+  if (bdt) {
+    const z2 cl = {0.0, -w.bd[m - l] / p.dt};
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc.set(q, zadd(acc.get(q), zmul(cl, gt.get(q))));
+  }
+  if (!fin) {
     tile_store(acc, n, Qm, r0, c0);
     return;
   }
-  TileAcc hist[KMAX + 1], qb;
   Frag fm;
-  tile_load(qb, n, QB + (size_t)m * es, r0, c0);
-#pragma unroll
-  for (int ll = 1; ll <= KMAX + 1; ll++)
-    if (ll <= k1) tile_load(hist[ll - 1], n, X + (size_t)(m - ll) * es, r0, c0);
   frag_left(fm, n, Minv + (size_t)m * es, 0, r0);
   acc.re += qb.re;
   acc.im += qb.im;
-  frag_mac(acc, fa, fb, n, coef);
-#pragma unroll
-  for (int ll = 1; ll <= KMAX + 1; ll++)
-    if (ll <= k1) {
-      const z2 cl = {0.0, -w.bd[ll] / p.dt};
-#pragma unroll
-      for (int q = 0; q < 4; q++) acc.set(q, zadd(acc.get(q), zmul(cl, hist[ll - 1].get(q))));
-    }
   Frag fq;
   frag_right_band(fq, acc, band, n);
   TileAcc x;
@@ -1355,9 +1363,11 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
       NED_HIP(hipEventRecord(c.ev_main, s));
       NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
       if (T - q1x <= panel_band_max())
-        hipLaunchKernelGGL(kern_ret_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q);
+        hipLaunchKernelGGL(kern_ret_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
+                           q1x + 1, Q);
       else
-        hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q);
+        hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1,
+                           q1x + 1, Q);
       NED_HIP(hipEventRecord(c.ev_side[pi & 1], c.side));
     }
   }
@@ -1410,10 +1420,11 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
         NED_HIP(hipEventRecord(c.ev_main, s));
         NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
         if (T - q1x <= panel_band_max())
-          hipLaunchKernelGGL(kern_les_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1,
-                             Q, X);
+          hipLaunchKernelGGL(kern_les_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
+                             q1x + 1, Q, X);
         else
-          hipLaunchKernelGGL(kern_les_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q, X);
+          hipLaunchKernelGGL(kern_les_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0,
+                             q1, q1x + 1, Q, X);
         NED_HIP(hipEventRecord(c.ev_side[pi & 1], c.side));
       }
     }
