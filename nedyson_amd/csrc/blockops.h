@@ -35,9 +35,14 @@ struct BlkAcc {
   }
 };
 
+// Both operands' K-slices are held with K innermost (X[row][k], Y[col][k])
+// in rows of 17: an MFMA fragment read (16 rows or columns, one k each per
+// lane group) then reads 16 rows at the same column, which the LDS serves
+// without conflicts; a k-major Y (16 consecutive columns of one k per lane
+// group) measured four times slower.
 struct BlkLds {
   z2 X[BO_NMAX][BO_KT + 1];
-  z2 Y[BO_KT][BO_NMAX + 1];
+  z2 Y[BO_NMAX][BO_KT + 1];
 };
 
 // op: 0 = as stored, 1 = adjoint (conjugate transpose), 2 = transpose;
@@ -105,7 +110,7 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 
       xmap(e, r, kk);
       ymap(e, kk2, c);
       s.X[r][kk] = zmul(coef, LATE == 1 ? getx(e, k0) : px[LATE == 1 ? 0 : i]);
-      s.Y[kk2][c] = LATE == 2 ? gety(e, k0) : py[LATE == 2 ? 0 : i];
+      s.Y[c][kk2] = LATE == 2 ? gety(e, k0) : py[LATE == 2 ? 0 : i];
     }
     __syncthreads();
     if (k0 + BO_KT < n) fetch(k0 + BO_KT);
@@ -117,7 +122,7 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 
 #pragma unroll
         for (int c = 0; c < 4; c++) {
           if (16 * c < n) {
-            const z2 b = s.Y[kk][16 * c + (lane & 15)];
+            const z2 b = s.Y[16 * c + (lane & 15)][kk];
             acc.re[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, acc.re[c], 0, 0, 0);
             acc.re[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, acc.re[c], 0, 0, 0);
             acc.im[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, acc.im[c], 0, 0, 0);
@@ -263,7 +268,7 @@ __device__ __forceinline__ void tile_mac(TileAcc &acc, BlkLds &s, int n, const z
       lmap(e, r, kk);
       rmap(e, kk2, c);
       if (e < lrn * BO_KT) s.X[r][kk] = zmul(coef, pl[i]);
-      if (e < BO_KT * rcn) s.Y[kk2][c] = pr[i];
+      if (e < BO_KT * rcn) s.Y[c][kk2] = pr[i];
     }
     __syncthreads();
     if (k0 + BO_KT < n) fetch(k0 + BO_KT);
@@ -272,7 +277,7 @@ __device__ __forceinline__ void tile_mac(TileAcc &acc, BlkLds &s, int n, const z
       for (int ks = 0; ks < ksteps; ks += 4) {
         const int kk = ks + (lane >> 4);
         const z2 a = s.X[r0 - lr0 + (lane & 15)][kk];
-        const z2 b = s.Y[kk][c0 - rc0 + (lane & 15)];
+        const z2 b = s.Y[c0 - rc0 + (lane & 15)][kk];
         acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, acc.re, 0, 0, 0);
         acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, acc.re, 0, 0, 0);
         acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, acc.im, 0, 0, 0);

@@ -366,10 +366,11 @@ struct NtCfg {
 // DB: two LDS stages; the next slice is written into the idle stage while the
 // current one is consumed, one barrier per slice.
 // SWZ (TK = 32 only): unpadded 64-byte LDS rows whose four 16-byte segments
-// are permuted by row, segment s of row r stored at s ^ ((r >> 1 ^ r >> 3) & 3).
-// The 16 lanes of a fragment read (16 rows, one segment) then fall on
-// distinct banks, and so do the 16 lanes of a staging write (4 rows, all
-// segments), which the padded layout serves with two-way conflicts.
+// are permuted by row, segment s of row r stored at s ^ ((r >> 1) & 3).  A
+// fragment read (16 rows, one segment per lane group) is then free of bank
+// conflicts on gfx950, where the padded 80-byte rows measured a conflict
+// cycle for every five active ones; a staging write (4 rows, all segments)
+// stays one contiguous 256-byte run.
 template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false, bool SWZ = false>
 __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
     kern_nt_bf16x3(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo, long long lda, long long strA,
@@ -382,7 +383,7 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
   constexpr int WCOLS = TN / WN;
   // element offset of 8-element segment sg of LDS row rw
   auto lofs = [](int rw, int sg) __attribute__((always_inline)) {
-    return SWZ ? ((sg ^ (((rw >> 1) ^ (rw >> 3)) & 3)) << 3) : (sg << 3);
+    return SWZ ? ((sg ^ ((rw >> 1) & 3)) << 3) : (sg << 3);
   };
   Ahi += (size_t)blockIdx.y * strA;
   Alo += (size_t)blockIdx.y * strA;
