@@ -230,71 +230,6 @@ struct TileAcc {
 __device__ __forceinline__ int tile_i(int r0, int q) { return r0 + 4 * q + ((threadIdx.x & 63) >> 4); }
 __device__ __forceinline__ int tile_j(int c0) { return c0 + (threadIdx.x & 15); }
 
-// acc(r0.., c0..) += coef * op(L)[r0.., :] op(R)[:, c0..], staging only rows
-// [lr0, lr0+lrn) of op(L) and columns [rc0, rc0+rcn) of op(R) (lrn, rcn <= 64)
-__device__ __forceinline__ void tile_mac(TileAcc &acc, BlkLds &s, int n, const z2 *L, int opL, int lr0, int lrn,
-                                         const z2 *R, int opR, int rc0, int rcn, z2 coef, int r0, int c0) {
-  const int tid = threadIdx.x, lane = tid & 63;
-  const bool on = r0 < n && c0 < n;
-  constexpr int PER = BO_NMAX * BO_KT / BO_T;  // at most 4 elements per thread and operand
-  z2 pl[PER], pr[PER];
-  // as in blk_mac: consecutive threads read consecutive stored addresses
-  auto lmap = [&](int e, int &r, int &kk) __attribute__((always_inline)) {
-    if (opL == 0) { r = e / BO_KT; kk = e % BO_KT; } else { kk = e / lrn; r = e % lrn; }
-  };
-  auto rmap = [&](int e, int &kk, int &c) __attribute__((always_inline)) {
-    if (opR == 0) { kk = e / rcn; c = e % rcn; } else { c = e / BO_KT; kk = e % BO_KT; }
-  };
-  auto fetch = [&](int k0) __attribute__((always_inline)) {
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int e = tid + BO_T * i;
-      int r, kk, kk2, c;
-      lmap(e, r, kk);
-      pl[i] = (e < lrn * BO_KT && lr0 + r < n && k0 + kk < n) ? blk_get(L, n, opL, lr0 + r, k0 + kk) : z2{0.0, 0.0};
-      rmap(e, kk2, c);
-      pr[i] = (e < BO_KT * rcn && rc0 + c < n && k0 + kk2 < n) ? blk_get(R, n, opR, k0 + kk2, rc0 + c)
-                                                              : z2{0.0, 0.0};
-    }
-  };
-  __syncthreads();  // operands written by this workgroup just before the call are visible
-  fetch(0);
-  for (int k0 = 0; k0 < n; k0 += BO_KT) {
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int e = tid + BO_T * i;
-      int r, kk, kk2, c;
-      lmap(e, r, kk);
-      rmap(e, kk2, c);
-      if (e < lrn * BO_KT) s.X[r][kk] = zmul(coef, pl[i]);
-      if (e < BO_KT * rcn) s.Y[c][kk2] = pr[i];
-    }
-    __syncthreads();
-    if (k0 + BO_KT < n) fetch(k0 + BO_KT);
-    if (on) {
-      const int ksteps = min(BO_KT, n - k0);
-      for (int ks = 0; ks < ksteps; ks += 4) {
-        const int kk = ks + (lane >> 4);
-        const z2 a = s.X[r0 - lr0 + (lane & 15)][kk];
-        const z2 b = s.Y[c0 - rc0 + (lane & 15)][kk];
-        acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, acc.re, 0, 0, 0);
-        acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, acc.re, 0, 0, 0);
-        acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, acc.im, 0, 0, 0);
-        acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, acc.im, 0, 0, 0);
-      }
-    }
-  }
-}
-
-__device__ __forceinline__ void tile_axpy(TileAcc &acc, int n, const z2 *X, int opX, z2 coef, int r0, int c0) {
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int i = tile_i(r0, q), j = tile_j(c0);
-    if (i < n && j < n) acc.set(q, zadd(acc.get(q), zmul(coef, blk_get(X, n, opX, i, j))));
-  }
-}
-
 __device__ __forceinline__ void tile_load(TileAcc &acc, int n, const z2 *X, int r0, int c0) {
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -309,11 +244,6 @@ __device__ __forceinline__ void tile_store(const TileAcc &acc, int n, z2 *X, int
     const int i = tile_i(r0, q), j = tile_j(c0);
     if (i < n && j < n) X[i * n + j] = acc.get(q);
   }
-}
-
-__device__ __forceinline__ void tile_scale(TileAcc &acc, z2 f) {
-#pragma unroll
-  for (int q = 0; q < 4; q++) acc.set(q, zmul(acc.get(q), f));
 }
 
 // Operand fragments of v_mfma_f64_16x16x4f64 for one wave's 16 x 16 tile,
@@ -363,8 +293,8 @@ __device__ __forceinline__ void frag_mac(TileAcc &acc, const FragT<NS> &a, const
     if (4 * (s0 + s) < n) {
       const z2 x = zmul(coef, a.v[s]), y = b.v[s];
       acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, acc.re, 0, 0, 0);
-      acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, acc.re, 0, 0, 0);
       acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, acc.im, 0, 0, 0);
+      acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, acc.re, 0, 0, 0);
       acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, acc.im, 0, 0, 0);
     }
   }
@@ -407,27 +337,5 @@ __device__ __forceinline__ double wg_sum(double v, double *red) {
   return red[0] + red[1] + red[2] + red[3];
 }
 
-// squared distance between the workgroup's tiles and X, summed over the
-// workgroup (valid in thread 0)
-__device__ __forceinline__ double tile_dist2(const TileAcc &acc, int n, const z2 *X, int r0, int c0, double *red) {
-  const int tid = threadIdx.x;
-  double s = 0;
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int i = tile_i(r0, q), j = tile_j(c0);
-    if (i < n && j < n) {
-      const z2 d = zsub(acc.get(q), X[i * n + j]);
-      s += d.x * d.x + d.y * d.y;
-    }
-  }
-  __syncthreads();
-  red[tid] = s;
-  __syncthreads();
-  for (int w = BO_T / 2; w > 0; w >>= 1) {
-    if (tid < w) red[tid] += red[tid + w];
-    __syncthreads();
-  }
-  return red[0];
-}
 
 }  // namespace ned
