@@ -44,26 +44,33 @@ __device__ __forceinline__ f32x4 mma3(u32x4_t ah, u32x4_t al, u32x4_t bh, u32x4_
 
 __device__ __forceinline__ u32x4_t negbf(u32x4_t v) { return v ^ 0x80008000u; }
 
+// thread tid's share of an FP32 [row][k] plane: four float4, element
+// e = tid + 256 it at row e/16, columns 4(e%16)..+3
+__device__ __forceinline__ int f32_row(int it) { return (threadIdx.x + 256 * it) >> 4; }
+__device__ __forceinline__ int f32_col(int it) { return ((threadIdx.x + 256 * it) & 15) * 4; }
+
+// split four FP32 values into the image's hi and lo planes
+__device__ __forceinline__ void put_f32(Img &im, int pl, int it, float4 v) {
+  const int row = f32_row(it), c4 = f32_col(it);
+  const float f[4] = {v.x, v.y, v.z, v.w};
+  uint16_t hh[4], ll[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    hh[q] = f2bf(f[q]);
+    ll[q] = f2bf(f[q] - bf2f(hh[q]));
+  }
+  *reinterpret_cast<uint2 *>(&im.h[pl][row][c4]) =
+      uint2{(uint32_t)hh[0] | ((uint32_t)hh[1] << 16), (uint32_t)hh[2] | ((uint32_t)hh[3] << 16)};
+  *reinterpret_cast<uint2 *>(&im.l[pl][row][c4]) =
+      uint2{(uint32_t)ll[0] | ((uint32_t)ll[1] << 16), (uint32_t)ll[2] | ((uint32_t)ll[3] << 16)};
+}
+
 // FP32 [row][k] planes (plane stride ps) -> split image
 __device__ __forceinline__ void load_f32(Img &im, const float *src, long long ps, int planes) {
-  const int tid = threadIdx.x;
   for (int pl = 0; pl < planes; pl++)
 #pragma unroll
-    for (int it = 0; it < 4; it++) {
-      const int e = tid + 256 * it, row = e >> 4, c4 = (e & 15) * 4;
-      const float4 v = *reinterpret_cast<const float4 *>(src + pl * ps + row * N64 + c4);
-      const float f[4] = {v.x, v.y, v.z, v.w};
-      uint16_t hh[4], ll[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        hh[q] = f2bf(f[q]);
-        ll[q] = f2bf(f[q] - bf2f(hh[q]));
-      }
-      *reinterpret_cast<uint2 *>(&im.h[pl][row][c4]) =
-          uint2{(uint32_t)hh[0] | ((uint32_t)hh[1] << 16), (uint32_t)hh[2] | ((uint32_t)hh[3] << 16)};
-      *reinterpret_cast<uint2 *>(&im.l[pl][row][c4]) =
-          uint2{(uint32_t)ll[0] | ((uint32_t)ll[1] << 16), (uint32_t)ll[2] | ((uint32_t)ll[3] << 16)};
-    }
+    for (int it = 0; it < 4; it++)
+      put_f32(im, pl, it, *reinterpret_cast<const float4 *>(src + pl * ps + f32_row(it) * N64 + f32_col(it)));
 }
 
 // pre-split [row][k] planes -> image
@@ -254,16 +261,36 @@ __global__ void __launch_bounds__(256) kern_c64_c(const float *__restrict__ Bm, 
 __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, const float *__restrict__ Ht, int a,
                                                   int G, float *__restrict__ Ep) {
   __shared__ __attribute__((aligned(16))) Img S, X;
-  const int p = blockIdx.y, s = blockIdx.x, a0 = s * G;
+  const int p = blockIdx.y, s = blockIdx.x, a0 = s * G, a1 = min(a0 + G, a);
   const long long an2 = (long long)a * E2;
+  // both operands are read once: the next index's are loaded into registers
+  // while the current product runs
+  float4 rw[2][4], rh[2][4];
+  auto gload = [&](int x) __attribute__((always_inline)) {
+    const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
+#pragma unroll
+    for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+      for (int it = 0; it < 4; it++) {
+        const size_t e = o + pl * an2 + f32_row(it) * N64 + f32_col(it);
+        rw[pl][it] = *reinterpret_cast<const float4 *>(W + e);
+        rh[pl][it] = *reinterpret_cast<const float4 *>(Ht + e);
+      }
+  };
   f32x4 acc[4][2];
   zero(acc);
-  for (int x = a0; x < a0 + G && x < a; x++) {
+  if (a0 < a1) gload(a0);
+  for (int x = a0; x < a1; x++) {
     __syncthreads();
-    const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
-    load_f32(S, W + o, an2, 2);
-    load_f32(X, Ht + o, an2, 2);
+#pragma unroll
+    for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+      for (int it = 0; it < 4; it++) {
+        put_f32(S, pl, it, rw[pl][it]);
+        put_f32(X, pl, it, rh[pl][it]);
+      }
     __syncthreads();
+    if (x + 1 < a1) gload(x + 1);
     mac64<2, 2>(acc, S, X);
   }
   store_f32(Ep + ((size_t)p * gridDim.x + s) * 2 * E2, E2, acc);
