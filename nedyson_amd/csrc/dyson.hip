@@ -410,21 +410,6 @@ __global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, const WTab *__restri
   blk_store(acc, n, Qm);
 }
 
-// add block nn's contribution to every later history sum; finish nn+1
-__global__ void __launch_bounds__(256) kern_ret_step(Pt p, const WTab *__restrict__ wtab, int nn, z2 *Q, const z2 *Minv, double *err) {
-  const WTab &w = *wtab;
-  __shared__ BlkLds s;
-  __shared__ double red[256];
-  const int m = nn + 1 + blockIdx.x, T = p.T, n = p.n;
-  if (m > T) return;
-  z2 *Qm = Q + (size_t)m * n * n;
-  BlkAcc acc;
-  blk_load(acc, n, Qm);
-  blk_mac(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
-  if (m == nn + 1) ret_finalize(p, w, acc, s, m, Qm, Minv, red, err);
-  else blk_store(acc, n, Qm);
-}
-
 // The same step split by rows: workgroup (m, b) owns rows 16b..16b+15 of
 // block m (its waves take the four column tiles).  G^R(T, T-m) depends only
 // on the same rows of the history sum, so the bands never exchange data; the
@@ -759,19 +744,6 @@ __global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restri
   if (m <= T - 1)
     for (int j = m - w.k - 1; j <= w.k; j++) blk_axpy(acc, n, X + (size_t)j * p.es(), 0, z2{0.0, -w.bd[m - j] / p.dt});
   blk_store(acc, n, Qm);
-}
-
-__global__ void __launch_bounds__(256) kern_les_step(Pt p, const WTab *__restrict__ wtab, int l, z2 *Q, z2 *X, const z2 *Minv) {
-  const WTab &w = *wtab;
-  __shared__ BlkLds s;
-  const int m = l + 1 + blockIdx.x, T = p.T, n = p.n;
-  if (m > T) return;
-  z2 *Qm = Q + (size_t)m * p.es();
-  BlkAcc acc;
-  blk_load(acc, n, Qm);
-  blk_mac(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
-  if (m == l + 1 && m <= T - 1) les_finalize(p, w, acc, s, m, Qm, X, Minv);
-  else blk_store(acc, n, Qm);
 }
 
 // The same step split by columns: workgroup (m, b) owns columns 16b..16b+15
