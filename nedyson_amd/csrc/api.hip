@@ -142,6 +142,9 @@ ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
     c.dy_info.ensure(sizeof(int));
     // items per Sigma chunk (tests and tuning; 0 = sized to ~3 GiB of scratch)
     if (const char *ch = std::getenv("NED_SIGMA_CHUNK")) c.sigma_chunk = std::atoi(ch);
+    // schedule of the Dyson recurrences (tests and tuning)
+    if (const char *e = std::getenv("NED_RPANEL")) c.rpanel = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("NED_PANEL_BAND_MAX")) c.panel_band_max = std::atoi(e);
     NED_HIP(hipStreamSynchronize(c.stream));
   });
   if (st != NED_OK) {

@@ -63,6 +63,9 @@ struct Ctx {
   // Sigma scratch
   DevBuf sg_items_A, sg_items_B, sg_items_Bt, sg_W, sg_X, sg_H, sg_Pi, sg_E, sg_Ep, sg_S1h, sg_S1l, sg_Pf;
   int sigma_chunk = 0;  // items per chunk (0 = choose automatically)
+  // Dyson recurrences: blocks per panel, and the largest number of later
+  // blocks for which a panel's bulk update is split into 16-row/column bands
+  int rpanel = 16, panel_band_max = 512;
   // Dyson state
   DevBuf mixK;                // mixing kernel of G^M (np x np blocks)
   DevBuf mass;                // (np x np) GLL mass matrix

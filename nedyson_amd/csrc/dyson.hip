@@ -18,7 +18,6 @@
 #include <cmath>
 #include <vector>
 #include <algorithm>
-#include <cstdlib>
 
 namespace ned {
 
@@ -1250,30 +1249,10 @@ void set_matsubara(Ctx &c, double beta) {
   c.have_mix = true;
 }
 
-// Blocks per panel of the Dyson recurrences (NED_RPANEL overrides).
-static int rpanel() {
-  static const int v = [] {
-    const char *e = std::getenv("NED_RPANEL");
-    const int r = e ? std::atoi(e) : 16;
-    return r >= 1 ? r : 16;
-  }();
-  return v;
-}
-
 // The context's stream waits for everything issued so far on the side stream.
 static void join_side(Ctx &c) {
   NED_HIP(hipEventRecord(c.ev_main, c.side));
   NED_HIP(hipStreamWaitEvent(c.stream, c.ev_main, 0));
-}
-
-// Panels with at most this many later blocks take the band-split update
-// (NED_PANEL_BAND_MAX overrides; 0 turns it off).
-static int panel_band_max() {
-  static const int v = [] {
-    const char *e = std::getenv("NED_PANEL_BAND_MAX");
-    return e ? std::atoi(e) : 512;
-  }();
-  return v;
 }
 
 void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) {
@@ -1297,7 +1276,9 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   c.dy_Qs.ensure((size_t)kn * n * sizeof(z2));
   z2 *Q = c.dy_Q.as<z2>(), *X = c.dy_X.as<z2>(), *M = c.dy_M.as<z2>(), *Qs = c.dy_Qs.as<z2>();
   z2 *QA = c.dy_QA.as<z2>();
-  const int RP = rpanel();
+  // blocks per panel; at least k, so that the chain links of a panel reach
+  // every block whose finishing terms they fold in (up to k+1 blocks ahead)
+  const int RP = std::max(c.rpanel, k);
   const int nb = (n + 15) / 16;  // 16-row / 16-column bands of a block
   c.dy_d2.ensure((size_t)(T + 1) * nb * sizeof(double));
   double *d2 = c.dy_d2.as<double>();
@@ -1334,7 +1315,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     if (q1x < T) {
       NED_HIP(hipEventRecord(c.ev_main, s));
       NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
-      if (T - q1x <= panel_band_max())
+      if (T - q1x <= c.panel_band_max)
         hipLaunchKernelGGL(kern_ret_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
                            q1x + 1, Q);
       else
@@ -1391,7 +1372,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
       if (q1x < T) {
         NED_HIP(hipEventRecord(c.ev_main, s));
         NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
-        if (T - q1x <= panel_band_max())
+        if (T - q1x <= c.panel_band_max)
           hipLaunchKernelGGL(kern_les_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
                              q1x + 1, Q, X);
         else

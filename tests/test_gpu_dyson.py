@@ -314,6 +314,42 @@ def test_dyson_step_band_parity(orc, nao, T0):
     assert _relerr(d.tv[nt], G_ref.tv[nt]) < 1e-10
 
 
+@pytest.mark.parametrize("rpanel,band_max", [(2, 0), (6, 10**6), (16, 40)])
+def test_dyson_step_schedule_parity(orc, monkeypatch, rpanel, band_max):
+    """One Dyson step at T = 150, nao = 20 under different schedules of the
+    blocked recurrences: the panel size (2 is raised to k = 5, the least that
+    lets a panel's links reach every block whose finishing terms they fold
+    in) and the bulk update as whole blocks (band_max 0), as 16-row/column
+    bands throughout (10**6), or both within one step (40).  Covers the
+    look-ahead order of the two streams, the folded finishing terms and both
+    bulk kernels, GPU against the oracle's dyson_step on the same data."""
+    monkeypatch.setenv("NED_RPANEL", str(rpanel))
+    monkeypatch.setenv("NED_PANEL_BAND_MAX", str(band_max))
+    nao, nt, ntau, k, beta, dt, mu = 20, 150, 15, 5, 10.0, 0.01, 0.0
+    rng = np.random.default_rng(7)
+    h = rng.normal(0, 0.3, (nao, nao))
+    h = 0.5 * (h + h.T)
+    x, _ = orc.gll(ntau)
+    G = free_green(h, mu, beta, dt, nt, x)
+    S = random_green(rng, nt, ntau, nao, scale=0.02)
+    hmf = np.repeat(h[None].astype(complex), nt + 1, axis=0)
+    G_ref = G.copy()
+    D = orc.Dyson(nt, ntau, nao, k)
+    D.dyson_step(nt, G_ref, S, hmf, mu, beta, dt)
+    print(f"[schedule] rpanel={rpanel} band_max={band_max}", flush=True)
+    with ned.Context(nao, 1, nt, ntau, k) as c:
+        c.upload(ned.NED_G, G)
+        c.upload(ned.NED_SIGMA, S)
+        c.write_hmf(hmf)
+        c.set_matsubara(beta)
+        c.dyson_step(nt, mu, beta, dt)
+        c.dyson_step(nt, mu, beta, dt)  # a second pass reuses the cached inverses
+        d = c.download(ned.NED_G)
+    assert _relerr(d.ret_row(nt), G_ref.ret_row(nt)) < 1e-10
+    assert _relerr(d.les_col(nt), G_ref.les_col(nt)) < 1e-10
+    assert _relerr(d.tv[nt], G_ref.tv[nt]) < 1e-10
+
+
 def test_c0_shape_full_run_parity(orc):
     """The nao=4 configuration's shape over its whole run: naux=12, 64 GLL
     nodes, dt=0.02, nt=200, k=5, the h[0,0] += 0.5 quench, FP64; every stored
