@@ -428,7 +428,9 @@ kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *QA, const z2
   const WTab &w = *wtab;
   __shared__ z2 band[16][BO_NMAX + 1];
   __shared__ double red[4];
-  const int m = nn + 1 + blockIdx.x, b = blockIdx.y, nb = gridDim.y, T = p.T, n = p.n;
+  // bands on x: the finishing workgroups (m = nn + 1) are the first
+  // dispatched, ahead of the updates of later blocks
+  const int m = nn + 1 + blockIdx.y, b = blockIdx.x, nb = gridDim.x, T = p.T, n = p.n;
   if (m > T) return;
   const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
   const bool fin = m == nn + 1, bdt = m - nn <= w.k + 1;
@@ -754,7 +756,7 @@ kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *QA, const z2 
   __builtin_amdgcn_s_setprio(3);
   const WTab &w = *wtab;
   __shared__ z2 band[BO_NMAX][17];
-  const int m = l + 1 + blockIdx.x, b = blockIdx.y, T = p.T, n = p.n;
+  const int m = l + 1 + blockIdx.y, b = blockIdx.x, T = p.T, n = p.n;  // finishing bands first
   if (m > T) return;
   const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
   const bool fin = m == l + 1 && m <= T - 1, bdt = m - l <= w.k + 1 && m <= T - 1;
@@ -1311,7 +1313,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     const int q1 = std::min(q0 + RP, T), q1x = std::min(q1 + RP, T);
     if (pi >= 2) NED_HIP(hipStreamWaitEvent(s, c.ev_side[pi & 1], 0));
     for (int nn = q0; nn < q1; nn++)
-      hipLaunchKernelGGL(kern_ret_step_band, dim3(q1x - nn, nb), dim3(256), 0, s, p, wd, nn, QA, Q, Minv, d2);
+      hipLaunchKernelGGL(kern_ret_step_band, dim3(nb, q1x - nn), dim3(256), 0, s, p, wd, nn, QA, Q, Minv, d2);
     if (q1x < T) {
       NED_HIP(hipEventRecord(c.ev_main, s));
       NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
@@ -1368,7 +1370,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
       const int q1 = std::min(q0 + RP, T), q1x = std::min(q1 + RP, T);
       if (pi >= 2) NED_HIP(hipStreamWaitEvent(s, c.ev_side[pi & 1], 0));
       for (int l = q0; l < q1; l++)
-        hipLaunchKernelGGL(kern_les_step_band, dim3(q1x - l, nb), dim3(256), 0, s, p, wd, l, QA, Q, X, Minv);
+        hipLaunchKernelGGL(kern_les_step_band, dim3(nb, q1x - l), dim3(256), 0, s, p, wd, l, QA, Q, X, Minv);
       if (q1x < T) {
         NED_HIP(hipEventRecord(c.ev_main, s));
         NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
