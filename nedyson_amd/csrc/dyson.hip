@@ -296,9 +296,9 @@ __global__ void kern_scatter_blocks(const z2 *src, const int *js, int cnt, size_
 __global__ void __launch_bounds__(256) kern_ret_start(Pt p, const WTab *__restrict__ wtab, int build_M, z2 *M, z2 *Q) {
   const WTab &w = *wtab;
   const int n = p.n, k = w.k, kn = k * n, T = p.T, k1 = k + 1;
-  const int tid = threadIdx.x;
+  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x, gsz = (long long)gridDim.x * 256;
   if (build_M) {
-    for (long long e = tid; e < (long long)kn * kn; e += 256) {
+    for (long long e = gid; e < (long long)kn * kn; e += gsz) {
       int row = (int)(e / kn), col = (int)(e % kn);
       int nn = row / n + 1, a = row % n, l = col / n + 1, b = col % n;
       z2 v = {0.0, 0.0};
@@ -322,19 +322,18 @@ __global__ void __launch_bounds__(256) kern_ret_start(Pt p, const WTab *__restri
     }
     return;
   }
+  // G^R(T,T) = -i 1 (src/dyson_step.h:29)
   z2 *Gtt = p.gr(T, T);
-  for (int e = tid; e < n * n; e += 256) Gtt[e] = (e / n == e % n) ? z2{0.0, -1.0} : z2{0.0, 0.0};
-  __syncthreads();
-  // Q: (-i/dt) pd(n,0) G(T,T)^T + dt w(n,0) (G(T,T) Sigma^R(T,T-n))^T
-  for (long long e = tid; e < (long long)kn * n; e += 256) {
+  for (long long e = gid; e < (long long)n * n; e += gsz) Gtt[e] = (e / n == e % n) ? z2{0.0, -1.0} : z2{0.0, 0.0};
+  // Q: (-i/dt) pd(n,0) G(T,T)^T + dt w(n,0) (G(T,T) Sigma^R(T,T-n))^T, with
+  // G(T,T) = -i 1: the product is -i Sigma^R(T,T-n)^T
+  for (long long e = gid; e < (long long)kn * n; e += gsz) {
     int row = (int)(e / n), b = (int)(e % n);
     int nn = row / n + 1, a = row % n;
-    z2 g = Gtt[b * n + a];
+    const z2 g = (a == b) ? z2{0.0, -1.0} : z2{0.0, 0.0};
     double pd = w.pdiff[nn * k1 + 0] / p.dt;
     z2 v = {pd * g.y, -pd * g.x};  // -i * pd * g
-    const z2 *S = p.sr(T, T - nn);
-    z2 s = {0.0, 0.0};
-    for (int c = 0; c < n; c++) s = zadd(s, zmul(Gtt[b * n + c], S[c * n + a]));
+    const z2 s = zmul(z2{0.0, -1.0}, p.sr(T, T - nn)[b * n + a]);
     double wt = p.dt * greg(w, nn, 0);
     v.x += wt * s.x; v.y += wt * s.y;
     Q[e] = v;
@@ -654,26 +653,29 @@ __global__ void kern_sum_parts(const z2 *part, int nchunk, long long nel, z2 *ou
 // start-up of src/dyson_step.h:178-251: initial condition, kn x kn system.
 // The matrix (times 0..k only) is built only when build_M is set, with the
 // identity in MI for its inversion.
-__global__ void __launch_bounds__(256) kern_les_start(Pt p, const WTab *__restrict__ wtab, int build_M, const z2 *Q, z2 *X, z2 *M, z2 *MI,
-                                                      z2 *Qs, double *err) {
-  const WTab &w = *wtab;
+// G^<(0,T) = -G^rceil(T,0)^H ; X_0 = same   (one workgroup)
+__global__ void __launch_bounds__(256) kern_les_x0(Pt p, z2 *X, double *err) {
   __shared__ double red[256];
-  const int n = p.n, k = w.k, kn = k * n, T = p.T, k1 = k + 1, tid = threadIdx.x;
-  // G^<(0,T) = -G^rceil(T,0)^H ; X_0 = same
-  {
-    BlkAcc acc;
-    acc.zero();
-    blk_axpy(acc, n, p.gtv(T, 0), 1, z2{-1.0, 0.0});
-    z2 *g = p.gl(0, T);
-    double d2 = blk_dist2(acc, n, g, red);
-    blk_store(acc, n, g);
-    blk_store(acc, n, X);
-    if (tid == 0) atomicAdd(err, sqrt(d2));
-  }
-  __syncthreads();
-  const z2 *X0 = X;
+  const int n = p.n, T = p.T;
+  BlkAcc acc;
+  acc.zero();
+  blk_axpy(acc, n, p.gtv(T, 0), 1, z2{-1.0, 0.0});
+  z2 *g = p.gl(0, T);
+  double d2 = blk_dist2(acc, n, g, red);
+  blk_store(acc, n, g);
+  blk_store(acc, n, X);
+  if (threadIdx.x == 0) atomicAdd(err, sqrt(d2));
+}
+
+// the start-up system (build_M) and its right-hand sides from X_0 (written
+// by kern_les_x0), every element independent
+__global__ void __launch_bounds__(256) kern_les_start(Pt p, const WTab *__restrict__ wtab, int build_M, const z2 *Q,
+                                                      const z2 *X0, z2 *M, z2 *MI, z2 *Qs) {
+  const WTab &w = *wtab;
+  const int n = p.n, k = w.k, kn = k * n, k1 = k + 1;
+  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x, gsz = (long long)gridDim.x * 256;
   if (build_M)
-    for (long long e = tid; e < (long long)kn * kn; e += 256) {
+    for (long long e = gid; e < (long long)kn * kn; e += gsz) {
       int row = (int)(e / kn), col = (int)(e % kn);
       int m = row / n + 1, a = row % n, l = col / n + 1, b = col % n;
       z2 v = {0.0, 0.0};
@@ -694,7 +696,7 @@ __global__ void __launch_bounds__(256) kern_les_start(Pt p, const WTab *__restri
       M[e] = v;
       MI[e] = (row == col) ? z2{1.0, 0.0} : z2{0.0, 0.0};
     }
-  for (long long e = tid; e < (long long)kn * n; e += 256) {
+  for (long long e = gid; e < (long long)kn * n; e += gsz) {
     int row = (int)(e / n), b = (int)(e % n);
     int m = row / n + 1, a = row % n;
     z2 v = Q[(size_t)m * n * n + a * n + b];
@@ -1292,11 +1294,12 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   c.dy_Xs.ensure((size_t)kn * n * sizeof(z2));
   z2 *Xs = c.dy_Xs.as<z2>();
   if (c.retinv_T != T) {
-    hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, wd, 1, M, c.dy_retinv.as<z2>());
+    hipLaunchKernelGGL(kern_ret_start, dim3(nblk((long long)kn * kn)), dim3(256), 0, s, p, wd, 1, M,
+                       c.dy_retinv.as<z2>());
     gesv(c, M, 0, c.dy_retinv.as<z2>(), 0, kn, kn, 1);
     c.retinv_T = T;
   }
-  hipLaunchKernelGGL(kern_ret_start, dim3(1), dim3(256), 0, s, p, wd, 0, M, Qs);
+  hipLaunchKernelGGL(kern_ret_start, dim3(nblk((long long)kn * n)), dim3(256), 0, s, p, wd, 0, M, Qs);
   hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_retinv.as<z2>(), Qs, kn, n, Xs);
   check_launch("ret start");
   hipLaunchKernelGGL(kern_ret_start_store, dim3(k), dim3(256), 0, s, p, Xs, err + 0);
@@ -1355,8 +1358,9 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     long long nel = (long long)(T + 1) * es;
     hipLaunchKernelGGL(kern_sum_parts, dim3(nblk(nel)), dim3(256), 0, s, c.dy_part.as<z2>(), nch, nel, Q);
     const int build = c.lesinv_valid ? 0 : 1;
-    hipLaunchKernelGGL(kern_les_start, dim3(1), dim3(256), 0, s, p, wd, build, Q, X, M, c.dy_lesinv.as<z2>(), Qs,
-                       err + 2);
+    hipLaunchKernelGGL(kern_les_x0, dim3(1), dim3(256), 0, s, p, X, err + 2);
+    hipLaunchKernelGGL(kern_les_start, dim3(nblk((long long)kn * (build ? kn : n))), dim3(256), 0, s, p, wd, build, Q,
+                       X, M, c.dy_lesinv.as<z2>(), Qs);
     if (build) {
       gesv(c, M, 0, c.dy_lesinv.as<z2>(), 0, kn, kn, 1);
       c.lesinv_valid = true;
