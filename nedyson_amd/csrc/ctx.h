@@ -65,7 +65,7 @@ struct Ctx {
   int sigma_chunk = 0;  // items per chunk (0 = choose automatically)
   // Dyson recurrences: blocks per panel, and the largest number of later
   // blocks for which a panel's bulk update is split into 16-row/column bands
-  int rpanel = 16, panel_band_max = 512;
+  int rpanel = 16, panel_band_max = 128;
   // Dyson state
   DevBuf mixK;                // mixing kernel of G^M (np x np blocks)
   DevBuf mass;                // (np x np) GLL mass matrix
