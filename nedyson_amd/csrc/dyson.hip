@@ -544,18 +544,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
-  const int nt_terms = T;  // j = 0..T-1 (G^rceil(T,.) is the unknown)
-  const int per = (nt_terms + nchunk - 1) / nchunk;
-  const int j0 = ch * per, j1 = min(nt_terms, j0 + per);
+  // terms t < T: the history j = t (G^rceil(T,.) is the unknown); t >= T:
+  // the mixing node jp = t - T; chunks of equal length over both
+  const int nterms = T + np;
+  const int per = (nterms + nchunk - 1) / nchunk;
+  const int t0 = ch * per, t1 = min(nterms, t0 + per);
   BlkAcc acc;
   acc.zero();
-  for (int j = j0; j < j1; j++)
+  for (int j = t0; j < min(t1, T); j++)
     blk_mac<1>(acc, s, n, p.sr(T, j), 0, p.gtv(j, m), 0, z2{p.dt * greg(w, T, j), 0.0});
-  if (ch == 0) {
-    for (int jp = 0; jp < np; jp++)
-      blk_mac<1>(acc, s, n, p.stv(T, jp), 0, mixK + ((size_t)m * np + jp) * p.es(), 0, z2{1.0, 0.0});
+  for (int jp = max(t0, T) - T; jp < t1 - T; jp++)
+    blk_mac<1>(acc, s, n, p.stv(T, jp), 0, mixK + ((size_t)m * np + jp) * p.es(), 0, z2{1.0, 0.0});
+  if (ch == 0)
     for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, p.gtv(T - l, m), 0, z2{0.0, -w.bd[l] / p.dt});
-  }
   blk_store(acc, n, part + ((size_t)ch * np + m) * p.es());
 }
 
@@ -1207,8 +1208,9 @@ void update_minv(Ctx &c, int T, double mu, double dt, const WTab &w) {
 }
 
 int chunks_for(int rows, int terms) {
-  // enough workgroups to cover the chip (~4 per CU) without tiny chunks
-  int want = (1024 + rows - 1) / std::max(rows, 1);
+  // about 32 workgroups per CU, so that the last partial round of workgroups
+  // is a small part of the launch, without chunks of fewer than 8 terms
+  int want = (8192 + rows - 1) / std::max(rows, 1);
   want = std::min(want, std::max(1, terms / 8));
   return std::max(1, std::min(want, 64));
 }
@@ -1335,7 +1337,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
 
   // ---- right mixing (src/dyson_step.h:130-169)
   {
-    int nch = chunks_for(np, T);
+    int nch = chunks_for(np, T + np);
     c.dy_part.ensure((size_t)std::max(nch * np, 1) * es * sizeof(z2));
     c.dy_gA.ensure(es * sizeof(z2));
     c.dy_gB.ensure(es * sizeof(z2));
