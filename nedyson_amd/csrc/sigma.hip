@@ -344,6 +344,13 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   else per_item += (size_t)2 * a * a * sizeof(T);
   const size_t budget = (size_t)4 << 30;
   int P = c.sigma_chunk > 0 ? c.sigma_chunk : (int)std::max<size_t>(1, budget / std::max<size_t>(per_item, 1));
+  if (c.sigma_chunk <= 0 && P >= 16) {
+    // a multiple of 8 items: the exchange GEMM of a full chunk then has
+    // naux*P/8 tiles of 256 x 256 (at nao = 64), whole rounds of 256 CUs;
+    // rounded up when that stays within 5% of the scratch budget
+    const int up = (P + 7) / 8 * 8;
+    P = (double)up * per_item <= 1.05 * budget ? up : P / 8 * 8;
+  }
   P = std::min(P, Ptot);
   c.sg_items_A.ensure((size_t)P * 2 * n2 * sizeof(T));
   c.sg_items_B.ensure((size_t)P * 2 * n2 * sizeof(T));
