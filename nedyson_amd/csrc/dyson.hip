@@ -408,19 +408,21 @@ __global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, const WTab *__restri
   blk_store(acc, n, Qm);
 }
 
-// The same step split by rows: workgroup (m, b) owns rows 16b..16b+15 of
-// block m (its waves take the four column tiles).  G^R(T, T-m) depends only
-// on the same rows of the history sum, so the bands never exchange data; the
+// One link nn of the retarded recurrence: add block nn's contribution to
+// the later blocks m of its panel and of the next one, and finish block
+// nn+1.  Split by rows: workgroup (b, m) owns rows 16b..16b+15 of block m
+// (its waves take the four column tiles).  G^R(T, T-m) depends only on the
+// same rows of the history sum, so the bands never exchange data; the
 // squared update of each finished band goes to d2[m * nb + b].
 //
-// This launch is the dependency chain, so its time is the latency of the
-// finishing workgroup, and it runs beside the bulk panel updates of the side
-// stream.  Hence: operands go straight into registers (no LDS staging), the
-// finishing terms -i/dt bd_l G^R(T, T-m+l) of Q' = dt Q - ... are folded
-// into the accumulator by the link that finishes each block (so that the
-// finishing link adds one of them, not k+1), the register budget (at most
-// 224 a wave) lets a chain wave share a SIMD with two bulk panel waves (at
-// most 136 each), and the waves raise their issue priority.
+// The links are the dependency chain, so a launch's time is the latency of
+// its finishing workgroups, and it runs beside the bulk panel updates of the
+// side stream.  Hence: operands go straight into registers (no LDS staging);
+// of the finishing terms -i/dt bd_l G^R(T, T-m+l) in Q' = dt Q - ..., each
+// link adds those of block nn to the blocks up to k+1 ahead, so a finishing
+// workgroup adds only the newest one; the register budget (at most 224 a
+// wave) lets a chain wave share a SIMD with two bulk panel waves (at most 136
+// each); and the waves raise their issue priority.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *QA, const z2 *QB, const z2 *Minv, double *d2) {
   __builtin_amdgcn_s_setprio(3);
@@ -750,10 +752,11 @@ __global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restri
   blk_store(acc, n, Qm);
 }
 
-// The same step split by columns: workgroup (m, b) owns columns 16b..16b+15
-// of block m (its waves take the four row tiles); X_m = Minv_m Q'_m keeps the
-// columns of Q'_m apart.  Built as kern_ret_step_band (the finishing terms
-// -i/dt bd_l X_{m-l} folded in by the link that finishes X_{m-l}).
+// One link l of the lesser recurrence, split by columns: workgroup (b, m)
+// owns columns 16b..16b+15 of block m (its waves take the four row tiles);
+// X_m = Minv_m Q'_m keeps the columns of Q'_m apart.  Built as
+// kern_ret_step_band (the finishing terms -i/dt bd_l X_{m-l} added ahead by
+// link m-l).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *QA, const z2 *QB, z2 *X, const z2 *Minv) {
   __builtin_amdgcn_s_setprio(3);
