@@ -6,6 +6,7 @@
 // tiles are staged in LDS; the loader follows whichever operand dimension is
 // unit-stride so that consecutive lanes read consecutive addresses.
 #include "common.h"
+#include <algorithm>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -661,13 +662,19 @@ int nt_choose(const NtOp &A, const NtOp &B, float *C, long long ldc, long long s
   for (int v = 0; v < kNtCount; v++) {
     if (!fits(kNt[v], M, N)) continue;
     kNt[v].launch(A, B, C, ldc, strC, M, N, K, batch, 1.0f, 0, s);  // warm-up
-    NED_HIP(hipEventRecord(e0, s));
-    kNt[v].launch(A, B, C, ldc, strC, M, N, K, batch, 1.0f, 0, s);
-    NED_HIP(hipEventRecord(e1, s));
-    NED_HIP(hipEventSynchronize(e1));
-    check_launch("gemm_nt_bf16x3 autotune");
-    float ms = 0;
-    NED_HIP(hipEventElapsedTime(&ms, e0, e1));
+    // the best of three timed launches: variants a few percent apart would
+    // otherwise trade places from run to run
+    float ms = 1e30f;
+    for (int rep = 0; rep < 3; rep++) {
+      NED_HIP(hipEventRecord(e0, s));
+      kNt[v].launch(A, B, C, ldc, strC, M, N, K, batch, 1.0f, 0, s);
+      NED_HIP(hipEventRecord(e1, s));
+      NED_HIP(hipEventSynchronize(e1));
+      check_launch("gemm_nt_bf16x3 autotune");
+      float t = 0;
+      NED_HIP(hipEventElapsedTime(&t, e0, e1));
+      ms = std::min(ms, t);
+    }
     if (std::getenv("NED_NT_VERBOSE"))
       std::fprintf(stderr, "[nt autotune] M=%d N=%d K=%d batch=%d  %-28s %.3f ms\n", M, N, K, batch, kNt[v].name, ms);
     if (ms < best_ms) {
