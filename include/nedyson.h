@@ -171,6 +171,15 @@ ned_status ned_tti_extend(ned_ctx *ctx, int which, int tsrc);
 ned_status ned_profile(ned_ctx *ctx, int enable);
 ned_status ned_profile_read(ned_ctx *ctx, double *out8);
 
+/* The tile variants of the split-bf16 NT GEMMs (the exchange and Pi
+ * products), for tuning and tests; no counterpart in the reference.  The
+ * first call for a new shape times every fitting variant and keeps the
+ * fastest; NED_NT_VARIANT=<index> in the environment (read on every call)
+ * pins one wherever it fits.  ned_gemm_variant_name returns a static string
+ * ("?" out of range). */
+int ned_gemm_variant_count(void);
+const char *ned_gemm_variant_name(int v);
+
 /* ---------------------------------------------------------------------------
  * Learned initial guess.  No counterpart in the reference: these replace the
  * polynomial predictor dyson::Extrapolate (src/dyson.cpp:47-117) inside

@@ -35,7 +35,7 @@ EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "n
            "ned_integrator", "ned_mlp_create", "ned_mlp_destroy", "ned_mlp_nparams", "ned_mlp_get_params",
            "ned_mlp_set_params", "ned_mlp_loss_grad", "ned_mlp_adam_step", "ned_mlp_train", "ned_mlp_predict",
            "ned_set_predictor", "ned_predictor_residual", "ned_predictor_stats", "ned_batch_create",
-           "ned_batch_destroy", "ned_batch_do_tstp"]
+           "ned_batch_destroy", "ned_batch_do_tstp", "ned_gemm_variant_count", "ned_gemm_variant_name"]
 
 
 class NedError(RuntimeError):
@@ -114,6 +114,8 @@ def lib():
                                     C.POINTER(C.c_int), _dp, C.POINTER(C.c_int)]
     for name in EXPORTS[1:]:
         getattr(L, name).restype = C.c_int
+    L.ned_gemm_variant_name.argtypes = [C.c_int]
+    L.ned_gemm_variant_name.restype = C.c_char_p
     _lib = L
     return L
 
@@ -140,6 +142,13 @@ def prec_code(prec):
     if isinstance(prec, str):
         return NED_PREC[prec]
     return int(prec)
+
+
+def gemm_variants():
+    """Names of the split-bf16 GEMM tile variants; NED_NT_VARIANT=<index>
+    pins one (read on every GEMM call)."""
+    L = lib()
+    return [L.ned_gemm_variant_name(v).decode() for v in range(L.ned_gemm_variant_count())]
 
 
 def gll_nodes(ntau):

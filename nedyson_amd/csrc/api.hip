@@ -378,6 +378,9 @@ ned_status ned_tti_extend(ned_ctx *ctx, int which, int tsrc) {
   });
 }
 
+int ned_gemm_variant_count(void) { return ned::nt_variant_count(); }
+const char *ned_gemm_variant_name(int v) { return ned::nt_variant_name(v); }
+
 ned_status ned_profile(ned_ctx *ctx, int enable) {
   if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
   return guard(ctx, [&] { ctx->c.prof = enable != 0; });

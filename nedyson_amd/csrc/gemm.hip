@@ -614,13 +614,12 @@ struct NtKey {
 std::mutex g_nt_mu;
 std::map<std::pair<int, NtKey>, int> g_nt_choice;  // (device, shape) -> variant
 
+// NED_NT_VARIANT, read on every call so that a process can pin the variants
+// one after another
 int nt_forced() {
-  static const int forced = [] {
-    const char *e = std::getenv("NED_NT_VARIANT");
-    const int v = e ? std::atoi(e) : -1;
-    return (v >= 0 && v < kNtCount) ? v : -1;
-  }();
-  return forced;
+  const char *e = std::getenv("NED_NT_VARIANT");
+  const int v = e ? std::atoi(e) : -1;
+  return (v >= 0 && v < kNtCount) ? v : -1;
 }
 
 bool fits(const NtVariant &v, int M, int N) { return M % v.tm == 0 && N % v.tn == 0; }

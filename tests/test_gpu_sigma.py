@@ -113,3 +113,29 @@ def test_sigma_parity_nao64(orc):
             scale = max(np.max(np.abs(b)), 1e-30)
             err = np.max(np.abs(a - b))
             assert err <= TOL[prec] * scale, (prec, name, err, scale)
+
+
+def test_sigma_every_gemm_variant_nao64(orc, monkeypatch):
+    """Every tile variant of the split-bf16 exchange and Pi GEMMs, pinned in
+    turn with NED_NT_VARIANT, on the nao = 64 slice of
+    test_sigma_parity_nao64 (M = 1280 and N = 4096, so that every tile shape
+    fits): each against the oracle within the bf16x3 bound."""
+    import os
+    names = ned.gemm_variants()
+    assert len(names) >= 10
+    rng = np.random.default_rng(64)
+    nao, naux, nt, ntau = 64, 64, 2, 3
+    V, B = sym_V(rng, nao, naux)
+    G = random_green(rng, nt, ntau, nao, scale=0.2)
+    S_ref = GreenFunc(nt, ntau, nao)
+    orc.sigma_solve(V, nt, S_ref, G, nthreads=min(os.cpu_count() or 1, 16))
+    for v, name in enumerate(names):
+        print(f"[gemm variant] {v}: {name}", flush=True)
+        monkeypatch.setenv("NED_NT_VARIANT", str(v))
+        S = _sigma_gpu(V, B, G, nt, "bf16x3")
+        for comp, a, b in (("ret", S.ret_row(nt), S_ref.ret_row(nt)),
+                           ("les", S.les_col(nt), S_ref.les_col(nt)),
+                           ("tv", S.tv[nt], S_ref.tv[nt])):
+            scale = max(np.max(np.abs(b)), 1e-30)
+            err = np.max(np.abs(a - b))
+            assert err <= TOL["bf16x3"] * scale, (v, name, comp, err, scale)
