@@ -923,7 +923,7 @@ __global__ void kern_extrap(Pt p, const WTab *__restrict__ wtab, int which) {
 // X_Q = sum_kl rho_kl B[Q][k][l]; rho = i sig G^<(T,T)
 __global__ void kern_hf_x(Pt p, const double *Bq, z2 *Xq) {
   __shared__ double rx[256], ry[256];
-  const int Q = blockIdx.x, n = p.n;
+  const int Q = blockIdx.x;
   const size_t es = p.es();
   const z2 *gl = p.gl(p.T, p.T);
   double sx = 0, sy = 0;

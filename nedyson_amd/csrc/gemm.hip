@@ -347,7 +347,7 @@ namespace {
 // native 4 x 32-bit vector: keeps the prefetch registers out of scratch
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int NT_TK = 64, NT_LDK = NT_TK + 8;
+constexpr int NT_TK = 64;
 
 template <int TM, int TN, int WM, int WN, int TK = NT_TK>
 struct NtCfg {
