@@ -72,7 +72,10 @@ void release_streams(ned::Ctx &c) {
   for (hipEvent_t &e : c.ev_side)
     if (e) (void)hipEventDestroy(e);
   if (c.ev_main) (void)hipEventDestroy(c.ev_main);
+  if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+  if (c.ev_tv) (void)hipEventDestroy(c.ev_tv);
   if (c.side) (void)hipStreamDestroy(c.side);
+  if (c.side2) (void)hipStreamDestroy(c.side2);
   if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 }  // namespace
@@ -122,7 +125,10 @@ ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
     c.np = c.ntau + 1;
     NED_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     NED_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    NED_HIP(hipStreamCreateWithFlags(&c.side2, hipStreamNonBlocking));
     NED_HIP(hipEventCreateWithFlags(&c.ev_main, hipEventDisableTiming));
+    NED_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    NED_HIP(hipEventCreateWithFlags(&c.ev_tv, hipEventDisableTiming));
     for (hipEvent_t &e : c.ev_side) NED_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const size_t es = c.es, ntri = (size_t)(c.nt + 1) * (c.nt + 2) / 2;
     for (ned::Green *g : {&c.G, &c.S}) {
@@ -163,6 +169,7 @@ ned_status ned_ctx_destroy(ned_ctx *ctx) {
   if (!ctx) return NED_OK;
   ned_status st = guard(ctx, [&] {
     NED_HIP(hipStreamSynchronize(ctx->c.side));
+    NED_HIP(hipStreamSynchronize(ctx->c.side2));
     NED_HIP(hipStreamSynchronize(ctx->c.stream));
   });
   release_streams(ctx->c);

@@ -44,9 +44,10 @@ struct Ctx {
   int es = 0, np = 0;
   hipStream_t stream = nullptr;
   // second stream of the Dyson recurrences (bulk panel updates overlapped
-  // with the dependency chain), and the events that order the two
-  hipStream_t side = nullptr;
-  hipEvent_t ev_main = nullptr, ev_side[2] = {nullptr, nullptr};
+  // with the dependency chain), third stream of the Dyson step (the mixing
+  // row, beside the retarded recurrence), and the events that order them
+  hipStream_t side = nullptr, side2 = nullptr;
+  hipEvent_t ev_main = nullptr, ev_side[2] = {nullptr, nullptr}, ev_fork = nullptr, ev_tv = nullptr;
   Green G, S;
   DevBuf hmf;      // (nt+1) x es complex
   DevBuf Bq;       // naux x nao x nao double, B[Q][i][j] = V(i,j,Q)
@@ -74,6 +75,7 @@ struct Ctx {
   double minv_mu = 0.0 / 0.0, minv_dt = 0.0 / 0.0;
   double mix_beta = 0;
   bool have_mix = false;
+  DevBuf dy_tvA, dy_tvB, dy_tvpart, dy_tvinfo;  // the mixing row's own scratch
   DevBuf dy_info, dy_js, dy_gA, dy_gB, dy_err, dy_Q, dy_QA, dy_X, dy_M, dy_Qs, dy_part, dy_scr, dy_Y;
   DevBuf dy_la, dy_lb, dy_mixtmp, dy_h0, dy_xq, dy_bootI, dy_en;
   // inverses of the k x k block start-up systems: the retarded one depends on

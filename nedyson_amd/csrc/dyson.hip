@@ -1163,23 +1163,27 @@ Pt make_pt(Ctx &c, int T, double mu, double beta, double dt) {
 
 inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-void gesv(Ctx &c, z2 *A, long long sA, z2 *B, long long sB, int N, int R, int nsys) {
+// (on stream st, c.stream if null, counting zero pivots in *info, the
+// context's counter if null)
+void gesv(Ctx &c, z2 *A, long long sA, z2 *B, long long sB, int N, int R, int nsys, hipStream_t st = nullptr,
+          int *info = nullptr) {
   require(N <= 320, "linear system larger than 320");
-  int *info = c.dy_info.as<int>();
-  NED_HIP(hipMemsetAsync(info, 0, sizeof(int), c.stream));
+  if (!st) st = c.stream;
+  if (!info) info = c.dy_info.as<int>();
+  NED_HIP(hipMemsetAsync(info, 0, sizeof(int), st));
   if (N <= LDS_N && R <= LDS_N) {
-    hipLaunchKernelGGL(kern_gesv_lds, dim3(nsys), dim3(256), 0, c.stream, A, sA, B, sB, N, R, info);
+    hipLaunchKernelGGL(kern_gesv_lds, dim3(nsys), dim3(256), 0, st, A, sA, B, sB, N, R, info);
   } else if (nsys == 1) {
     c.dy_pinv.ensure((size_t)N * sizeof(z2));
     z2 *pinv = c.dy_pinv.as<z2>();
     for (int col = 0; col < N; col++) {
-      hipLaunchKernelGGL(kern_gj_pivot, dim3(1), dim3(256), 0, c.stream, A, B, N, R, col, pinv, info);
+      hipLaunchKernelGGL(kern_gj_pivot, dim3(1), dim3(256), 0, st, A, B, N, R, col, pinv, info);
       const long long work = (long long)(N - 1) * (N - col - 1 + R);
-      hipLaunchKernelGGL(kern_gj_elim, dim3(nblk(work)), dim3(256), 0, c.stream, A, B, N, R, col, pinv);
+      hipLaunchKernelGGL(kern_gj_elim, dim3(nblk(work)), dim3(256), 0, st, A, B, N, R, col, pinv);
     }
-    hipLaunchKernelGGL(kern_gj_scale, dim3(nblk((long long)N * R)), dim3(256), 0, c.stream, B, N, R, pinv);
+    hipLaunchKernelGGL(kern_gj_scale, dim3(nblk((long long)N * R)), dim3(256), 0, st, B, N, R, pinv);
   } else {
-    hipLaunchKernelGGL(kern_gesv, dim3(nsys), dim3(256), 0, c.stream, A, sA, B, sB, N, R, info);
+    hipLaunchKernelGGL(kern_gesv, dim3(nsys), dim3(256), 0, st, A, sA, B, sB, N, R, info);
   }
   check_launch("gesv");
 }
@@ -1278,18 +1282,47 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   c.dy_err.ensure(4 * sizeof(double));
   double *err = c.dy_err.as<double>();
   NED_HIP(hipMemsetAsync(err, 0, 4 * sizeof(double), s));
-  c.dy_Q.ensure((size_t)(T + 1) * es * sizeof(z2));
-  c.dy_QA.ensure((size_t)(T + 1) * es * sizeof(z2));
-  c.dy_X.ensure((size_t)(T + 1) * es * sizeof(z2));
+  // scratch sized for the last step (nt), so that it is allocated once and
+  // no allocation (an implicit device synchronisation) falls between the
+  // launches of a step
+  c.dy_Q.ensure((size_t)(c.nt + 1) * es * sizeof(z2));
+  c.dy_QA.ensure((size_t)(c.nt + 1) * es * sizeof(z2));
+  c.dy_X.ensure((size_t)(c.nt + 1) * es * sizeof(z2));
+  // the lesser history sums' partial sums: nch * (T + 1) <= 8192 + T + 1 blocks
+  c.dy_part.ensure((size_t)(8192 + c.nt + 1) * es * sizeof(z2));
+  c.dy_Y.ensure((size_t)np * es * sizeof(z2));
   c.dy_M.ensure((size_t)kn * kn * sizeof(z2));
   c.dy_Qs.ensure((size_t)kn * n * sizeof(z2));
   z2 *Q = c.dy_Q.as<z2>(), *X = c.dy_X.as<z2>(), *M = c.dy_M.as<z2>(), *Qs = c.dy_Qs.as<z2>();
   z2 *QA = c.dy_QA.as<z2>();
+  // ---- right mixing (src/dyson_step.h:130-169) on the third stream: it
+  // reads only Sigma and the G^rceil and G^M history, not the retarded row,
+  // so it runs beside the retarded recurrence (its own scratch throughout)
+  {
+    const int nch = chunks_for(np, T + np);
+    c.dy_tvpart.ensure((size_t)std::max(nch * np, 1) * es * sizeof(z2));
+    c.dy_tvA.ensure(es * sizeof(z2));
+    c.dy_tvB.ensure(es * sizeof(z2));
+    c.dy_tvinfo.ensure(sizeof(int));
+    c.dy_scr.ensure((size_t)std::max(np, c.nt + 1) * es * sizeof(z2));
+    NED_HIP(hipEventRecord(c.ev_fork, s));
+    NED_HIP(hipStreamWaitEvent(c.side2, c.ev_fork, 0));
+    hipStream_t s2 = c.side2;
+    hipLaunchKernelGGL(kern_tv_matrix, dim3(1), dim3(256), 0, s2, p, wd, c.dy_tvA.as<z2>(), c.dy_tvB.as<z2>());
+    gesv(c, c.dy_tvA.as<z2>(), 0, c.dy_tvB.as<z2>(), 0, n, n, 1, s2, c.dy_tvinfo.as<int>());
+    hipLaunchKernelGGL(kern_tv_rhs, dim3(np, nch), dim3(256), 0, s2, p, wd, c.mixK.as<z2>(), nch,
+                       c.dy_tvpart.as<z2>());
+    hipLaunchKernelGGL(kern_tv_fin, dim3(np), dim3(256), 0, s2, p, c.dy_tvpart.as<z2>(), nch, c.dy_tvB.as<z2>(),
+                       c.dy_scr.as<z2>(), err + 1);
+    check_launch("tv");
+    NED_HIP(hipEventRecord(c.ev_tv, s2));
+  }
+
   // blocks per panel; at least k, so that the chain links of a panel reach
   // every block whose finishing terms they fold in (up to k+1 blocks ahead)
   const int RP = std::max(c.rpanel, k);
   const int nb = (n + 15) / 16;  // 16-row / 16-column bands of a block
-  c.dy_d2.ensure((size_t)(T + 1) * nb * sizeof(double));
+  c.dy_d2.ensure((size_t)(c.nt + 1) * nb * sizeof(double));
   double *d2 = c.dy_d2.as<double>();
   const z2 *Minv = c.Minv.as<z2>();
 
@@ -1338,27 +1371,14 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   if (T > k + 1) hipLaunchKernelGGL(kern_band_err, dim3(1), dim3(256), 0, s, d2, k + 2, T, nb, err + 0);
   check_launch("ret history");
 
-  // ---- right mixing (src/dyson_step.h:130-169)
-  {
-    int nch = chunks_for(np, T + np);
-    c.dy_part.ensure((size_t)std::max(nch * np, 1) * es * sizeof(z2));
-    c.dy_gA.ensure(es * sizeof(z2));
-    c.dy_gB.ensure(es * sizeof(z2));
-    c.dy_scr.ensure((size_t)std::max(np, T + 1) * es * sizeof(z2));
-    hipLaunchKernelGGL(kern_tv_matrix, dim3(1), dim3(256), 0, s, p, wd, c.dy_gA.as<z2>(), c.dy_gB.as<z2>());
-    gesv(c, c.dy_gA.as<z2>(), 0, c.dy_gB.as<z2>(), 0, n, n, 1);
-    hipLaunchKernelGGL(kern_tv_rhs, dim3(np, nch), dim3(256), 0, s, p, wd, c.mixK.as<z2>(), nch, c.dy_part.as<z2>());
-    hipLaunchKernelGGL(kern_tv_fin, dim3(np), dim3(256), 0, s, p, c.dy_part.as<z2>(), nch, c.dy_gB.as<z2>(),
-                       c.dy_scr.as<z2>(), err + 1);
-    check_launch("tv");
-  }
+  // ---- right mixing: joined back before the lesser column, which reads it
+  NED_HIP(hipStreamWaitEvent(s, c.ev_tv, 0));
 
   // ---- lesser (src/dyson_step.h:171-326)
   {
-    c.dy_Y.ensure((size_t)np * es * sizeof(z2));
     hipLaunchKernelGGL(kern_les_Y, dim3(np), dim3(256), 0, s, p, c.mass.as<double>(), c.dy_Y.as<z2>());
     int nch = chunks_for(T + 1, np + T + 1);
-    c.dy_part.ensure((size_t)nch * (T + 1) * es * sizeof(z2));
+    require((long long)nch * (T + 1) <= 8192LL + c.nt + 1, "dyson_step: partial-sum buffer too small");
     hipLaunchKernelGGL(kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, wd, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
     long long nel = (long long)(T + 1) * es;
     hipLaunchKernelGGL(kern_sum_parts, dim3(nblk(nel)), dim3(256), 0, s, c.dy_part.as<z2>(), nch, nel, Q);
