@@ -5,11 +5,11 @@
  * One context owns one propagated system: the contour functions G and Sigma
  * in the packing of the reference's green_func (src/greens.cpp:177-195), the
  * mean field hmf, the Cholesky-factored interaction and all scratch, resident
- * in HBM, plus its HIP streams (the context's stream, and a side stream on
- * which the Dyson step runs its bulk history updates beside the dependency
- * chain; every call's work is joined back into the context's stream before
- * the call returns or the next phase starts).  Calls on different contexts
- * are independent;
+ * in HBM, plus its HIP streams (the context's stream, and two more on which
+ * the Dyson step runs its bulk history updates beside the dependency chain
+ * and its mixing row beside the retarded row; every call's work is joined
+ * back into the context's stream before the call returns or the next phase
+ * starts).  Calls on different contexts are independent;
  * calls on one context are serialised by the caller (the reference's solver
  * objects are likewise not re-entrant, src/mol2bSolver.h:80-88).
  *
