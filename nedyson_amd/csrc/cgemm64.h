@@ -10,8 +10,10 @@ void c64_w(const uint16_t *Vh, const uint16_t *Vl, const float *At, int P, int a
            uint16_t *Wtl, hipStream_t s);
 void c64_z(const uint16_t *Vh, const uint16_t *Vl, const float *Bt, int P, int a, int G, uint16_t *Zth,
            uint16_t *Ztl, hipStream_t s);
+// pairs != nullptr: the row in the split layout of the exchange (pair table,
+// antisymmetric block from column ks); otherwise the 64 x 64 block row-major
 void c64_c(const float *Bm, const uint16_t *Wth, const uint16_t *Wtl, int P, int a, int G, uint16_t *Sh,
-           uint16_t *Sl, long long ld, hipStream_t s);
+           uint16_t *Sl, long long ld, hipStream_t s, const uint16_t *pairs = nullptr, long long ks = 0);
 void c64_e(const float *W, const float *Ht, int P, int a, int S, float *Ep, hipStream_t s);
 
 }  // namespace ned

@@ -224,9 +224,15 @@ __global__ void __launch_bounds__(256) kern_c64_z(const uint16_t *__restrict__ V
 
 // C^a = B_p W^a: A = B_p ([i][l], FP32), B = Wt^a ([k][l], pre-split); rows
 // (p, pl, a) of the exchange operand, row stride ld
+// With SYM the row is written in the split layout of the exchange: groups of
+// 8 pairs from the table (m << 8 | q), pairs m <= q as X_mq + X_qm (X_mm on the
+// diagonal) in columns [0, ns), pairs m < q as X_mq - X_qm from column ks, and
+// zeros in the 32 padding columns after the Pi block and after the pairs m < q.
+template <bool SYM>
 __global__ void __launch_bounds__(256) kern_c64_c(const float *__restrict__ Bm, const uint16_t *__restrict__ Wth,
                                                   const uint16_t *__restrict__ Wtl, int a, int G,
-                                                  uint16_t *__restrict__ Sh, uint16_t *__restrict__ Sl, long long ld) {
+                                                  uint16_t *__restrict__ Sh, uint16_t *__restrict__ Sl, long long ld,
+                                                  const uint16_t *__restrict__ pairs, long long ks) {
   __shared__ __attribute__((aligned(16))) Img S, X;
   const int p = blockIdx.y, a0 = blockIdx.x * G;
   const long long an2 = (long long)a * E2;
@@ -240,11 +246,63 @@ __global__ void __launch_bounds__(256) kern_c64_c(const float *__restrict__ Bm, 
     zero(acc);
     mac64<2, 2>(acc, S, X);
     __syncthreads();
+    const size_t r0 = ((size_t)p * 2 * a + x) * ld;  // plane 0 row; plane 1 is a rows further
+    const int tid = threadIdx.x;
+    if constexpr (SYM) {
+      constexpr int NS = N64 * (N64 + 1) / 2, NA = N64 * (N64 - 1) / 2, NS8 = NS / 8;
+      // fp32 image of the block (rows m, columns q, stride 65 words) in X's space
+      float(*F)[N64][65] = reinterpret_cast<float(*)[N64][65]>(&X);
+      {
+        const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+          for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) F[pl][16 * w + 4 * (lane >> 4) + q][16 * j + (lane & 15)] = acc[j][pl][q];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < 2; it++) {
+        const int g = tid + 256 * it;  // 512 groups: NS8 symmetric, then NA / 8 antisymmetric
+        const uint4 pk = *reinterpret_cast<const uint4 *>(pairs + 8 * g);
+        const uint32_t pw[4] = {pk.x, pk.y, pk.z, pk.w};
+        const size_t col = g < NS8 ? (size_t)8 * g : (size_t)ks + 8 * (g - NS8);
+#pragma unroll
+        for (int pl = 0; pl < 2; pl++) {
+          uint32_t hw[4], lw[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            hw[u] = 0;
+            lw[u] = 0;
+#pragma unroll
+            for (int hf = 0; hf < 2; hf++) {
+              const uint32_t e = (pw[u] >> (16 * hf)) & 0xffffu;
+              const int m = (int)(e >> 8), q = (int)(e & 255u);
+              const float xv = F[pl][m][q], yv = F[pl][q][m];
+              const float v = g < NS8 ? (m == q ? xv : xv + yv) : xv - yv;
+              const uint16_t h = f2bf(v), l = f2bf(v - bf2f(h));
+              hw[u] |= (uint32_t)h << (16 * hf);
+              lw[u] |= (uint32_t)l << (16 * hf);
+            }
+          }
+          const size_t go = r0 + (size_t)pl * a * ld + col;
+          *reinterpret_cast<uint4 *>(Sh + go) = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+          *reinterpret_cast<uint4 *>(Sl + go) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+        }
+      }
+      if (tid < 16) {
+        const int pl = tid >> 3, w = tid & 7;
+        const size_t col = w < 4 ? (size_t)(NS + a + 8 * w) : (size_t)ks + NA + 8 * (w - 4);
+        const size_t go = r0 + (size_t)pl * a * ld + col;
+        *reinterpret_cast<uint4 *>(Sh + go) = make_uint4(0, 0, 0, 0);
+        *reinterpret_cast<uint4 *>(Sl + go) = make_uint4(0, 0, 0, 0);
+      }
+      continue;
+    }
     stage(X, acc, false);
     __syncthreads();
     // the 64 x 64 block is one row of the operand: image row i -> columns i*64..
-    const size_t r0 = ((size_t)p * 2 * a + x) * ld;  // plane 0 row; plane 1 is a rows further
-    const int tid = threadIdx.x;
 #pragma unroll
     for (int pl = 0; pl < 2; pl++)
 #pragma unroll
@@ -311,8 +369,13 @@ void c64_z(const uint16_t *Vh, const uint16_t *Vl, const float *Bt, int P, int a
 }
 
 void c64_c(const float *Bm, const uint16_t *Wth, const uint16_t *Wtl, int P, int a, int G, uint16_t *Sh,
-           uint16_t *Sl, long long ld, hipStream_t s) {
-  hipLaunchKernelGGL(kern_c64_c, dim3((a + G - 1) / G, P), dim3(256), 0, s, Bm, Wth, Wtl, a, G, Sh, Sl, ld);
+           uint16_t *Sl, long long ld, hipStream_t s, const uint16_t *pairs, long long ks) {
+  if (pairs)
+    hipLaunchKernelGGL(kern_c64_c<true>, dim3((a + G - 1) / G, P), dim3(256), 0, s, Bm, Wth, Wtl, a, G, Sh, Sl, ld,
+                       pairs, ks);
+  else
+    hipLaunchKernelGGL(kern_c64_c<false>, dim3((a + G - 1) / G, P), dim3(256), 0, s, Bm, Wth, Wtl, a, G, Sh, Sl, ld,
+                       pairs, ks);
   check_launch("c64_c");
 }
 

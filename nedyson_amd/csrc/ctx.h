@@ -39,6 +39,28 @@ struct Green {
   DevBuf ret, les, tv, mat;
 };
 
+// Shapes of the exchange contraction split by the symmetry of U' (nao = n,
+// naux = a): ns pairs m <= q and na pairs m < q; the symmetric GEMM has K =
+// ns + a (the Pi columns ride on it) padded to ks, the antisymmetric K = na
+// padded to ka; operand A rows hold [sym | Pi | 0 | anti | 0] (stride kx); the
+// outputs are padded to nsp / nap columns and share rows of stride ldh.
+struct SymGeom {
+  int ns, na, nsp, nap, ldh;
+  long long ks, ka, kx;
+};
+inline SymGeom sym_geom(int n, int a) {
+  SymGeom g;
+  g.ns = n * (n + 1) / 2;
+  g.na = n * (n - 1) / 2;
+  g.ks = (g.ns + a + 63) / 64 * 64;
+  g.ka = (g.na + 63) / 64 * 64;
+  g.kx = g.ks + g.ka;
+  g.nsp = (g.ns + 127) / 128 * 128;
+  g.nap = (g.na + 127) / 128 * 128;
+  g.ldh = g.nsp + g.nap;
+  return g;
+}
+
 struct Ctx {
   int nao = 0, naux = 0, nt = 0, ntau = 0, k = 0, sig = -1, device = 0;
   int es = 0, np = 0;
@@ -60,6 +82,10 @@ struct Ctx {
   bool uaug = false;  // Baug built (nao^2 and 2 naux multiples of 128, naux of 64)
   bool fast64 = false;  // nao = 64 as well: the batched 64 x 64 kernels carry the chain
   DevBuf Vh, Vl;        // B[Q][i][l] split into bf16 hi / lo (fast64)
+  // fast64 with the exchange split into its parts symmetric and antisymmetric
+  // under (m,q) -> (q,m): the pair table (m << 8 | q) and the two right operands
+  bool symx = false;
+  DevBuf sym_pairs, Ush, Usl, Uxh, Uxl;
   bool have_coulomb = false;
   // Sigma scratch
   DevBuf sg_items_A, sg_items_B, sg_items_Bt, sg_W, sg_X, sg_H, sg_Pi, sg_E, sg_Ep, sg_S1h, sg_S1l, sg_Pf;

@@ -23,6 +23,7 @@
 #include "ctx.h"
 #include "cgemm64.h"
 #include <cmath>
+#include <cstdlib>
 #include <algorithm>
 
 namespace ned {
@@ -63,6 +64,75 @@ __global__ void kern_build_Uaug(const float *U, const float *Bq, int n, int a, i
   const uint16_t h = f2bf(v);
   hi[idx] = h;
   lo[idx] = f2bf(v - bf2f(h));
+}
+
+// Pair table of the split exchange: entries [0, ns) are the pairs m <= q and
+// [ns, ns + na) the pairs m < q, each row-major over the upper triangle,
+// packed m << 8 | q.
+__global__ void kern_sym_pairs(int n, uint16_t *pairs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, ns = n * (n + 1) / 2, na = n * (n - 1) / 2;
+  if (t >= ns + na) return;
+  const int strict = t >= ns ? 1 : 0;
+  int r = strict ? t - ns : t, m = 0;
+  while (r >= n - m - strict) { r -= n - m - strict; m++; }
+  pairs[t] = (uint16_t)(m << 8 | (m + strict + r));
+}
+
+// Right operands of the split exchange.  U' is invariant under swapping both
+// pairs, U'[(q,m)][(j,k)] = U'[(m,q)][(k,j)] (B symmetric), so with
+//   Us[(m,q)][(k,j)] = -(U'[(m,q)][(k,j)] + U'[(m,q)][(j,k)]) / 2  (even in m,q)
+//   Ua[(m,q)][(k,j)] = -(U'[(m,q)][(k,j)] - U'[(m,q)][(j,k)]) / 2  (odd in m,q)
+// the exchange -C U' splits into Cs = xs Us over pairs m <= q (xs = X_mq + X_qm,
+// X_mm on the diagonal) and Ca = xa Ua over pairs m < q (xa = X_mq - X_qm),
+// with (-C U')_kj = Cs_kj + Ca_kj and (-C U')_jk = Cs_kj - Ca_kj for k < j.
+// Rows are the output pairs (k,j), columns the K pairs; the symmetric operand
+// carries 2 B[b][k][j] in columns ns + b (Pi V is symmetric in (k,j)).  Rows and
+// columns past the pairs are zero.  Split into bf16 hi / lo.
+__global__ void kern_build_Usym(const double *U, const double *Bq, const uint16_t *pairs, int n, int a, int anti,
+                                int rows, int cols, uint16_t *hi, uint16_t *lo) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)rows * cols) return;
+  const int r = (int)(idx / cols), c = (int)(idx % cols);
+  const int n2 = n * n, ns = n * (n + 1) / 2, np = anti ? n * (n - 1) / 2 : ns, off = anti ? ns : 0;
+  double v = 0;
+  if (r < np) {
+    const int rk = pairs[off + r], k = rk >> 8, j = rk & 255;
+    if (c < np) {
+      const int mq = pairs[off + c], m = mq >> 8, q = mq & 255;
+      const double *Ur = U + (size_t)(m * n + q) * n2;
+      v = -0.5 * (Ur[k * n + j] + (anti ? -1.0 : 1.0) * Ur[j * n + k]);
+    } else if (!anti && c < ns + a) {
+      v = 2.0 * Bq[(size_t)(c - ns) * n2 + k * n + j];
+    }
+  }
+  const float f = (float)v;
+  const uint16_t h = f2bf(f);
+  hi[idx] = h;
+  lo[idx] = f2bf(f - bf2f(h));
+}
+
+// Ht^a from the split outputs: row r of Y holds Cs (pairs m <= q) in columns
+// [0, nsp) and Ca (pairs m < q) from column nsp; Ht[r][j n + k] = H^a_kj with
+// H^a_kj = Cs + Ca for k < j, Cs - Ca for k > j, Cs on the diagonal.
+__global__ void __launch_bounds__(256) kern_sym_unfold(const float *__restrict__ Y, int ldy, int nsp,
+                                                       float *__restrict__ Ht) {
+  constexpr int n = 64, ns = n * (n + 1) / 2, na = n * (n - 1) / 2;
+  __shared__ float cs[ns], ca[na];
+  const float *y = Y + (size_t)blockIdx.x * ldy;
+  for (int t = threadIdx.x; t < ns; t += 256) cs[t] = y[t];
+  for (int t = threadIdx.x; t < na; t += 256) ca[t] = y[nsp + t];
+  __syncthreads();
+  float *h = Ht + (size_t)blockIdx.x * n * n;
+  for (int o = threadIdx.x; o < n * n; o += 256) {
+    const int j = o / n, k = o % n, lo = min(j, k), hi = max(j, k);
+    const int ps = lo * n - lo * (lo - 1) / 2 + (hi - lo);
+    float v = cs[ps];
+    if (lo != hi) {
+      const float d = ca[lo * (n - 1) - lo * (lo - 1) / 2 + (hi - lo - 1)];
+      v = k < j ? v + d : v - d;
+    }
+    h[o] = v;
+  }
 }
 
 // Pi of every item from the real product Pf = [Wr; Wi] [Zr; Zi]^T (2a x 2a):
@@ -195,7 +265,9 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     // split once, intermediates handed over in the layout the next step reads.
     // kern_items left A^T in A, B in B and B^T in Bt.
     constexpr int G = 16;  // aux indices per workgroup
-    const long long ka = n2 + a;
+    const SymGeom sg = sym_geom(n, a);
+    const bool sym = c.symx;
+    const long long ka = sym ? sg.kx : n2 + a;  // row stride of the exchange operand
     const int M = 2 * a * P, NN = (int)n2;
     float *W = (float *)bf.W, *H = (float *)bf.H;
     uint16_t *wth = reinterpret_cast<uint16_t *>(H), *wtl = wth + (size_t)P * 2 * an2;  // Wt lives in H until H is formed
@@ -206,15 +278,32 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
                    2 * a, NN, P, 1.0f, 0, s);
     long long tot = (long long)P * 2 * a * a;
     hipLaunchKernelGGL(kern_pi_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Pf, a, P, bf.S1h,
-                       bf.S1l, ka, NN);
+                       bf.S1l, ka, sym ? sg.ns : NN);
     check_launch("sigma pi combine");
-    c64_c((const float *)bf.B, wth, wtl, P, a, G, bf.S1h, bf.S1l, ka, s);
-    // Ht^a = ([C | Pi] Baug^T) with Baug rows ordered (j,k): H^a transposed
-    c.mark(0);
-    gemm_nt_bf16x3(NtOp{bf.S1h, bf.S1l, ka, 0}, NtOp{c.Uah.as<uint16_t>(), c.Ual.as<uint16_t>(), ka, 0}, H, n2, 0, M,
-                   NN, (int)ka, 1, 1.0f, 0, s);
-    c.mark(1);
-    if (c.prof) c.prof_flops += 2.0 * M * (double)NN * (double)ka;
+    if (sym) {
+      // the exchange in its symmetric and antisymmetric parts (half the flops
+      // of the full contraction), each row unfolded into Ht^a afterwards
+      c64_c((const float *)bf.B, wth, wtl, P, a, G, bf.S1h, bf.S1l, ka, s, c.sym_pairs.as<uint16_t>(), sg.ks);
+      float *Y = (float *)bf.X;
+      c.mark(0);
+      gemm_nt_bf16x3(NtOp{bf.S1h, bf.S1l, ka, 0}, NtOp{c.Ush.as<uint16_t>(), c.Usl.as<uint16_t>(), sg.ks, 0}, Y,
+                     sg.ldh, 0, M, sg.nsp, (int)sg.ks, 1, 1.0f, 0, s);
+      gemm_nt_bf16x3(NtOp{bf.S1h + sg.ks, bf.S1l + sg.ks, ka, 0},
+                     NtOp{c.Uxh.as<uint16_t>(), c.Uxl.as<uint16_t>(), sg.ka, 0}, Y + sg.nsp, sg.ldh, 0, M, sg.nap,
+                     (int)sg.ka, 1, 1.0f, 0, s);
+      c.mark(1);
+      if (c.prof) c.prof_flops += 2.0 * M * ((double)sg.ns * (sg.ns + a) + (double)sg.na * sg.na);
+      hipLaunchKernelGGL(kern_sym_unfold, dim3((unsigned)M), dim3(256), 0, s, Y, sg.ldh, sg.nsp, H);
+      check_launch("sigma exchange unfold");
+    } else {
+      c64_c((const float *)bf.B, wth, wtl, P, a, G, bf.S1h, bf.S1l, ka, s);
+      // Ht^a = ([C | Pi] Baug^T) with Baug rows ordered (j,k): H^a transposed
+      c.mark(0);
+      gemm_nt_bf16x3(NtOp{bf.S1h, bf.S1l, ka, 0}, NtOp{c.Uah.as<uint16_t>(), c.Ual.as<uint16_t>(), ka, 0}, H, n2, 0,
+                     M, NN, (int)ka, 1, 1.0f, 0, s);
+      c.mark(1);
+      if (c.prof) c.prof_flops += 2.0 * M * (double)NN * (double)ka;
+    }
     const int S = e_splits(a);
     c64_e(W, H, P, a, S, (float *)bf.Ep, s);
     tot = (long long)P * 2 * n2;
@@ -335,14 +424,17 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   const bool fast = fast_bf16(c, prec);
   const bool f64w = fast && c.fast64;
   const int S = e_splits(a);
-  const long long ka = n2 + a;
-  // device scratch per item: items, W, X (not on the nao = 64 path), H, Pi
-  // (generic path), E partials, and on the bf16x3 path the split operand S1
-  // and the Pi product Pf
+  const bool sym = f64w && c.symx;
+  const SymGeom sg = sym_geom(n, a);
+  const long long ka = sym ? sg.kx : n2 + a;
+  // device scratch per item: items, W, X (not on the nao = 64 path unless the
+  // exchange is split: then it holds the split outputs), H, Pi (generic path),
+  // E partials, and on the bf16x3 path the split operand S1 and the Pi product Pf
   size_t per_item = (size_t)((f64w ? 2 : 3) * 2 * an2 + (f64w ? 6 : 4) * n2 + (long long)S * 2 * n2) * sizeof(T);
   if (fast) per_item += (size_t)2 * a * ka * 2 * sizeof(uint16_t) + (size_t)4 * a * a * sizeof(float);
   else per_item += (size_t)2 * a * a * sizeof(T);
-  const size_t budget = (size_t)4 << 30;
+  if (sym) per_item += (size_t)2 * a * sg.ldh * sizeof(float);
+  const size_t budget = (size_t)6 << 30;
   int P = c.sigma_chunk > 0 ? c.sigma_chunk : (int)std::max<size_t>(1, budget / std::max<size_t>(per_item, 1));
   if (c.sigma_chunk <= 0 && P >= 16) {
     // a multiple of 8 items: the exchange GEMM of a full chunk then has
@@ -357,6 +449,7 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   c.sg_W.ensure((size_t)P * 2 * an2 * sizeof(T));
   if (f64w) c.sg_items_Bt.ensure((size_t)P * 2 * n2 * sizeof(T));
   else c.sg_X.ensure((size_t)P * 2 * an2 * sizeof(T));
+  if (sym) c.sg_X.ensure((size_t)P * 2 * a * sg.ldh * sizeof(float));
   c.sg_H.ensure((size_t)P * 2 * an2 * sizeof(T));
   c.sg_E.ensure((size_t)Ptot * 2 * n2 * sizeof(T));
   c.sg_Ep.ensure((size_t)P * S * 2 * n2 * sizeof(T));
@@ -429,6 +522,30 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
     c.Vl.ensure(a * n2 * sizeof(uint16_t));
     split_bf16(c.Bq32.as<float>(), (long long)n2, c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (long long)n2, a, (int)n2,
                c.stream);
+  }
+  // at nao = 64 the exchange runs split by the symmetry of U' (NED_SYMX=0
+  // keeps the full contraction)
+  const char *sx = std::getenv("NED_SYMX");
+  c.symx = c.fast64 && !(sx && std::atoi(sx) == 0);
+  if (c.symx) {
+    const SymGeom sg = sym_geom(n, a);
+    const int npair = sg.ns + sg.na;
+    c.sym_pairs.ensure((size_t)npair * sizeof(uint16_t));
+    hipLaunchKernelGGL(kern_sym_pairs, dim3((npair + 255) / 256), dim3(256), 0, c.stream, n,
+                       c.sym_pairs.as<uint16_t>());
+    check_launch("coulomb pairs");
+    c.Ush.ensure((size_t)sg.nsp * sg.ks * sizeof(uint16_t));
+    c.Usl.ensure((size_t)sg.nsp * sg.ks * sizeof(uint16_t));
+    c.Uxh.ensure((size_t)sg.nap * sg.ka * sizeof(uint16_t));
+    c.Uxl.ensure((size_t)sg.nap * sg.ka * sizeof(uint16_t));
+    long long ns_ = (long long)sg.nsp * sg.ks, na_ = (long long)sg.nap * sg.ka;
+    hipLaunchKernelGGL(kern_build_Usym, dim3((unsigned)((ns_ + 255) / 256)), dim3(256), 0, c.stream,
+                       c.U64.as<double>(), c.Bq.as<double>(), c.sym_pairs.as<uint16_t>(), n, a, 0, sg.nsp,
+                       (int)sg.ks, c.Ush.as<uint16_t>(), c.Usl.as<uint16_t>());
+    hipLaunchKernelGGL(kern_build_Usym, dim3((unsigned)((na_ + 255) / 256)), dim3(256), 0, c.stream,
+                       c.U64.as<double>(), c.Bq.as<double>(), c.sym_pairs.as<uint16_t>(), n, a, 1, sg.nap,
+                       (int)sg.ka, c.Uxh.as<uint16_t>(), c.Uxl.as<uint16_t>());
+    check_launch("coulomb split U");
   }
   NED_HIP(hipStreamSynchronize(c.stream));
   c.have_coulomb = true;
