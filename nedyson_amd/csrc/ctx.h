@@ -55,8 +55,8 @@ inline SymGeom sym_geom(int n, int a) {
   g.ks = (g.ns + a + 63) / 64 * 64;
   g.ka = (g.na + 63) / 64 * 64;
   g.kx = g.ks + g.ka;
-  g.nsp = (g.ns + 127) / 128 * 128;
-  g.nap = (g.na + 127) / 128 * 128;
+  g.nsp = (g.ns + 255) / 256 * 256;  // whole 256-column tiles
+  g.nap = (g.na + 255) / 256 * 256;
   g.ldh = g.nsp + g.nap;
   return g;
 }

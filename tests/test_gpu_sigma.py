@@ -121,9 +121,8 @@ def test_sigma_every_gemm_variant_nao64(orc, monkeypatch, symx):
     turn with NED_NT_VARIANT, on the nao = 64 slice of
     test_sigma_parity_nao64 (M = 1280): each against the oracle within the
     bf16x3 bound.  With NED_SYMX=1 (the default) the exchange runs as its
-    symmetric and antisymmetric parts under (m,q) -> (q,m) (N = 2176 and
-    2048, where a variant whose tile does not divide N falls back to the
-    tuned choice); with NED_SYMX=0 as the full contraction (N = 4096)."""
+    symmetric and antisymmetric parts under (m,q) -> (q,m) (N = 2304 and
+    2048); with NED_SYMX=0 as the full contraction (N = 4096)."""
     import os
     monkeypatch.setenv("NED_SYMX", symx)
     names = ned.gemm_variants()
