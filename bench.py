@@ -15,9 +15,10 @@ The same steps are re-run with FP64 Sigma contractions from the same history;
 the line reports the mixed-precision rate as `value`, the FP64 rate, their
 ratio, and the max-abs difference of every G block written by the steps.
 
-`roofline` is for the dominant kernel, the exchange GEMM of Sigma
-(kern_gemm_*<1,1>: rows (item, plane, aux) x U'), timed with HIP events on
-the library's stream.  `cpu_baseline` times the CPU oracle (the restated
+`roofline` is for the dominant kernel, the exchange GEMM of Sigma (at
+nao = 64 two launches of kern_nt_bf16x3, the parts of rows (item, plane, aux)
+x U' symmetric and antisymmetric in the contracted pair), timed with HIP
+events on the library's stream around both launches.  `cpu_baseline` times the CPU oracle (the restated
 reference algorithm) on one lesser, one retarded and one mixing time point of
 Sigma at the same sizes on one thread, and on a small slice with OpenMP over
 time points on all host threads, and scales to a whole timestep.
@@ -56,13 +57,6 @@ def gf2_equilibrium(ned, p, log):
     log(f"[setup] GF2 equilibrium: mu {p.mu:.6f} -> {mu:.6f}, {it} iterations, last update {err:.1e} "
         f"({time.perf_counter() - t0:.1f} s)")
     p.fock, p.mu = fock, mu
-
-
-def sigma_flops_reassoc(n, a, items):
-    """Flops of the executed Sigma algorithm per slice (U-reassociated exchange):
-    per item 4 n^4 a (exchange GEMM) + bubble and V-products."""
-    per = 4 * n ** 4 * a + 8 * n ** 3 * a * 4 + 12 * n ** 2 * a ** 2
-    return per * items
 
 
 def ref_sigma_flops(n, a, T, ntau1):
@@ -300,7 +294,7 @@ def main():
     c.close()
 
     ugemm_ms = prof["ugemm_ms"] / max(prof["ugemm_n"], 1)
-    # HBM bytes per exchange-GEMM launch from the rocprofv3 FETCH_SIZE /
+    # HBM bytes per exchange (both GEMM launches) from the rocprofv3 FETCH_SIZE /
     # WRITE_SIZE runs recorded next to this file (profiles/), when they were
     # taken on this workload's shape
     traffic, traffic_note = None, "no counter run recorded for this shape"
@@ -308,12 +302,14 @@ def main():
     if args.prec == "bf16x3" and os.path.exists(tfile) and p.nao == 64 and p.naux == 256:
         t = json.load(open(tfile))
         traffic = t["traffic_bytes_per_launch"]
-        traffic_note = (f"bytes per full-chunk launch (M={t['M']}, N={t['N']}, K={t['K']}) from rocprofv3 "
+        traffic_note = (f"bytes per full-chunk exchange ({t['shape']}) from rocprofv3 "
                         f"FETCH_SIZE x {t['fetch_scale']:.3f} + WRITE_SIZE x {t['write_scale']:.3f}, scaled by a 1 GiB "
                         f"copy; algorithmic {t['algorithmic_bytes_per_launch']:.3e} B")
     n_sigma = max(prof["sigma_n"], 1)
-    # algorithmic flops of one exchange-GEMM launch, 2 M N K with M = 2 naux P
-    # rows (P time points of the slice), N = K = nao^2, averaged over launches
+    # algorithmic flops of one exchange with M = 2 naux P rows (P time points
+    # of the slice): 2 M (ns (ns + naux) + na^2) over the ns = nao(nao+1)/2
+    # and na = nao(nao-1)/2 pairs at nao = 64 (padding not counted), 2 M n^2
+    # (n^2 + naux) for the full contraction elsewhere; averaged over exchanges
     flops_per_launch = prof["ugemm_flops"] / max(prof["ugemm_n"], 1)
     achieved = flops_per_launch / (ugemm_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.prec]
@@ -348,7 +344,8 @@ def main():
                      "frac": achieved / peak, "traffic": traffic,
                      "traffic_note": traffic_note,
                      "kernel": f"Sigma exchange GEMM ({args.prec}), {flops_per_launch:.3e} algorithmic flop "
-                               f"per launch, avg {ugemm_ms:.3f} ms over {prof['ugemm_n']} launches"},
+                               f"per exchange, avg {ugemm_ms:.3f} ms over {prof['ugemm_n']} exchanges (each the "
+                               f"symmetric and antisymmetric launch at nao=64)"},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, T0 + W + 1, args.corr, log)
