@@ -72,7 +72,9 @@ ned_status ned_ctx_get_dims(const ned_ctx *ctx, ned_dims *out);
 
 /* Replaces molGF2SolverDecomp(Vija, Viaj) (src/mol2bSolver.h:101-111):
  * B_Qij is naux x nao x nao, B[Q][i][j] = Vija(i,j,Q); must be symmetric in
- * (i,j). */
+ * (i,j).  At nao = 64 that symmetry also splits the exchange contraction into
+ * its parts even and odd in the contracted pair (half the flops); the
+ * environment variable NED_SYMX=0, read here, keeps the full contraction. */
 ned_status ned_set_coulomb(ned_ctx *ctx, const double *B_Qij);
 
 /* Copy nblocks packed blocks starting at first_block between host and device
@@ -165,7 +167,8 @@ ned_status ned_g0_from_h0(ned_ctx *ctx, int tmax, const double *fock, double mu,
 ned_status ned_tti_extend(ned_ctx *ctx, int which, int tsrc);
 
 /* Event timing of the hot phases on the context's stream.  ned_profile_read
- * fills 8 doubles: {ms, count} for the Sigma exchange GEMM, the whole Sigma
+ * fills 8 doubles: {ms, count} for the Sigma exchange GEMM (both launches of
+ * the split form counted as one), the whole Sigma
  * and the whole Dyson step, then the algorithmic flops of the timed exchange
  * GEMM launches and a reserved 0, accumulated since the last read. */
 ned_status ned_profile(ned_ctx *ctx, int enable);
