@@ -11,7 +11,8 @@
 // The exchange is reassociated through U, so its n^4 naux work is a single
 // real GEMM [(item, plane, a) x (m,q)] x U'[(m,q) x (k,j)] that streams the
 // whole slice through the MFMA units at once (the reference does two n^4 naux
-// GEMMs per time point, lines 206-226).
+// GEMMs per time point, lines 206-226).  At nao = 64 that GEMM is split by the
+// pair symmetry of U' into two of half the flops (kern_build_Usym).
 //
 // Items of slice T (L = G^<(t,T), R = G^R(T,t)):
 //   "<" t:  A = G^<(T,t) = -L^H,      B = G^>(t,T) = L - R^H   -> Sigma^<(T,t)
@@ -529,6 +530,8 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
   c.symx = c.fast64 && !(sx && std::atoi(sx) == 0);
   if (c.symx) {
     const SymGeom sg = sym_geom(n, a);
+    // kern_c64_c zeroes 32 padding columns after each block (naux % 64 == 0)
+    require(sg.ks - sg.ns - a == 32 && sg.ka - sg.na == 32, "split exchange: unexpected padding");
     const int npair = sg.ns + sg.na;
     c.sym_pairs.ensure((size_t)npair * sizeof(uint16_t));
     hipLaunchKernelGGL(kern_sym_pairs, dim3((npair + 255) / 256), dim3(256), 0, c.stream, n,
