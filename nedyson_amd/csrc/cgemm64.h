@@ -14,6 +14,9 @@ void c64_z(const uint16_t *Vh, const uint16_t *Vl, const float *Bt, int P, int a
 // antisymmetric block from column ks); otherwise the 64 x 64 block row-major
 void c64_c(const float *Bm, const uint16_t *Wth, const uint16_t *Wtl, int P, int a, int G, uint16_t *Sh,
            uint16_t *Sl, long long ld, hipStream_t s, const uint16_t *pairs = nullptr, long long ks = 0);
-void c64_e(const float *W, const float *Ht, int P, int a, int S, float *Ep, hipStream_t s);
+// Y != nullptr: H^a from the split exchange's outputs (row stride ldy, the
+// antisymmetric part from column nsp) instead of Ht
+void c64_e(const float *W, const float *Ht, int P, int a, int S, float *Ep, hipStream_t s, const float *Y = nullptr,
+           long long ldy = 0, int nsp = 0, const uint16_t *pairs = nullptr);
 
 }  // namespace ned

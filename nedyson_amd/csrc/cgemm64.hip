@@ -315,25 +315,56 @@ __global__ void __launch_bounds__(256) kern_c64_c(const float *__restrict__ Bm, 
   }
 }
 
-// partial E over a in [a0, a0+G): A = W^a ([i][k], FP32), B = Ht^a ([j][k], FP32)
+// one FP32 value split into the image's hi and lo planes at [row][col]
+__device__ __forceinline__ void put1(Img &im, int pl, int row, int col, float v) {
+  const uint16_t h = f2bf(v);
+  im.h[pl][row][col] = h;
+  im.l[pl][row][col] = f2bf(v - bf2f(h));
+}
+
+// partial E over a in [a0, a0+G): A = W^a ([i][k], FP32), B = Ht^a ([j][k], FP32).
+// With SYM, H^a comes as the two outputs of the split exchange instead: row
+// (item, plane, a) of Y holds Cs over the strict pairs k < j (in the order of
+// the pair table's entries [ns, ns + na)) and then the diagonal, and Ca over
+// the same strict pairs from column nsp; H_kj = Cs + Ca, H_jk = Cs - Ca, H_kk = Cs.
+template <bool SYM>
 __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, const float *__restrict__ Ht, int a,
-                                                  int G, float *__restrict__ Ep) {
+                                                  int G, float *__restrict__ Ep, const float *__restrict__ Y,
+                                                  long long ldy, int nsp, const uint16_t *__restrict__ pairs) {
   __shared__ __attribute__((aligned(16))) Img S, X;
-  const int p = blockIdx.y, s = blockIdx.x, a0 = s * G, a1 = min(a0 + G, a);
+  const int p = blockIdx.y, s = blockIdx.x, a0 = s * G, a1 = min(a0 + G, a), tid = threadIdx.x;
   const long long an2 = (long long)a * E2;
+  constexpr int NS = N64 * (N64 + 1) / 2, NA = N64 * (N64 - 1) / 2, NA8 = NA / 8;  // 252 threads of 8 pairs
+  // SYM: threads below NA8 take 8 strict pairs, the other 4 take 16 diagonal entries each
+  uint32_t pw[4] = {0, 0, 0, 0};
+  if constexpr (SYM)
+    if (tid < NA8) {
+      const uint4 pk = *reinterpret_cast<const uint4 *>(pairs + NS + 8 * tid);
+      pw[0] = pk.x; pw[1] = pk.y; pw[2] = pk.z; pw[3] = pk.w;
+    }
   // both operands are read once: the next index's are loaded into registers
   // while the current product runs
   float4 rw[2][4], rh[2][4];
   auto gload = [&](int x) __attribute__((always_inline)) {
     const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
 #pragma unroll
-    for (int pl = 0; pl < 2; pl++)
+    for (int pl = 0; pl < 2; pl++) {
 #pragma unroll
       for (int it = 0; it < 4; it++) {
         const size_t e = o + pl * an2 + f32_row(it) * N64 + f32_col(it);
         rw[pl][it] = *reinterpret_cast<const float4 *>(W + e);
-        rh[pl][it] = *reinterpret_cast<const float4 *>(Ht + e);
+        if constexpr (!SYM) rh[pl][it] = *reinterpret_cast<const float4 *>(Ht + e);
       }
+      if constexpr (SYM) {
+        const float *y = Y + (((size_t)p * 2 + pl) * a + x) * ldy;
+        const float *q0 = tid < NA8 ? y + 8 * tid : y + NA + 16 * (tid - NA8);
+        const float *q1 = tid < NA8 ? y + nsp + 8 * tid : q0 + 8;
+        rh[pl][0] = *reinterpret_cast<const float4 *>(q0);
+        rh[pl][1] = *reinterpret_cast<const float4 *>(q0 + 4);
+        rh[pl][2] = *reinterpret_cast<const float4 *>(q1);
+        rh[pl][3] = *reinterpret_cast<const float4 *>(q1 + 4);
+      }
+    }
   };
   f32x4 acc[4][2];
   zero(acc);
@@ -341,12 +372,33 @@ __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, c
   for (int x = a0; x < a1; x++) {
     __syncthreads();
 #pragma unroll
-    for (int pl = 0; pl < 2; pl++)
+    for (int pl = 0; pl < 2; pl++) {
 #pragma unroll
       for (int it = 0; it < 4; it++) {
         put_f32(S, pl, it, rw[pl][it]);
-        put_f32(X, pl, it, rh[pl][it]);
+        if constexpr (!SYM) put_f32(X, pl, it, rh[pl][it]);
       }
+      if constexpr (SYM) {
+        const float v[16] = {rh[pl][0].x, rh[pl][0].y, rh[pl][0].z, rh[pl][0].w, rh[pl][1].x, rh[pl][1].y,
+                             rh[pl][1].z, rh[pl][1].w, rh[pl][2].x, rh[pl][2].y, rh[pl][2].z, rh[pl][2].w,
+                             rh[pl][3].x, rh[pl][3].y, rh[pl][3].z, rh[pl][3].w};
+        if (tid < NA8) {
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const uint32_t e = (pw[u >> 1] >> (16 * (u & 1))) & 0xffffu;
+            const int k = (int)(e >> 8), j = (int)(e & 255u);
+            put1(X, pl, j, k, v[u] + v[8 + u]);  // Ht[j][k] = H_kj
+            put1(X, pl, k, j, v[u] - v[8 + u]);  // Ht[k][j] = H_jk
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 16; u++) {
+            const int k = 16 * (tid - NA8) + u;
+            put1(X, pl, k, k, v[u]);
+          }
+        }
+      }
+    }
     __syncthreads();
     if (x + 1 < a1) gload(x + 1);
     mac64<2, 2>(acc, S, X);
@@ -379,9 +431,13 @@ void c64_c(const float *Bm, const uint16_t *Wth, const uint16_t *Wtl, int P, int
   check_launch("c64_c");
 }
 
-void c64_e(const float *W, const float *Ht, int P, int a, int S, float *Ep, hipStream_t s) {
+void c64_e(const float *W, const float *Ht, int P, int a, int S, float *Ep, hipStream_t s, const float *Y,
+           long long ldy, int nsp, const uint16_t *pairs) {
   if (a % S) throw std::invalid_argument("c64_e: naux must be a multiple of the split count");
-  hipLaunchKernelGGL(kern_c64_e, dim3(S, P), dim3(256), 0, s, W, Ht, a, a / S, Ep);
+  if (Y)
+    hipLaunchKernelGGL(kern_c64_e<true>, dim3(S, P), dim3(256), 0, s, W, Ht, a, a / S, Ep, Y, ldy, nsp, pairs);
+  else
+    hipLaunchKernelGGL(kern_c64_e<false>, dim3(S, P), dim3(256), 0, s, W, Ht, a, a / S, Ep, Y, ldy, nsp, pairs);
   check_launch("c64_e");
 }
 

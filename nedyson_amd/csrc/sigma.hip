@@ -86,18 +86,21 @@ __global__ void kern_sym_pairs(int n, uint16_t *pairs) {
 // the exchange -C U' splits into Cs = xs Us over pairs m <= q (xs = X_mq + X_qm,
 // X_mm on the diagonal) and Ca = xa Ua over pairs m < q (xa = X_mq - X_qm),
 // with (-C U')_kj = Cs_kj + Ca_kj and (-C U')_jk = Cs_kj - Ca_kj for k < j.
-// Rows are the output pairs (k,j), columns the K pairs; the symmetric operand
-// carries 2 B[b][k][j] in columns ns + b (Pi V is symmetric in (k,j)).  Rows and
-// columns past the pairs are zero.  Split into bf16 hi / lo.
+// Rows are the output pairs (k,j): for both operands the na pairs k < j in the
+// table's order, so that Cs and Ca of one pair share a column of the outputs,
+// and for the symmetric one the diagonal (k,k) after them.  Columns are the K
+// pairs; the symmetric operand carries 2 B[b][k][j] in columns ns + b (Pi V is
+// symmetric in (k,j)).  Rows and columns past the pairs are zero.  Split into
+// bf16 hi / lo.
 __global__ void kern_build_Usym(const double *U, const double *Bq, const uint16_t *pairs, int n, int a, int anti,
                                 int rows, int cols, uint16_t *hi, uint16_t *lo) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)rows * cols) return;
   const int r = (int)(idx / cols), c = (int)(idx % cols);
-  const int n2 = n * n, ns = n * (n + 1) / 2, np = anti ? n * (n - 1) / 2 : ns, off = anti ? ns : 0;
+  const int n2 = n * n, ns = n * (n + 1) / 2, na = n * (n - 1) / 2, np = anti ? na : ns, off = anti ? ns : 0;
   double v = 0;
   if (r < np) {
-    const int rk = pairs[off + r], k = rk >> 8, j = rk & 255;
+    const int rk = r < na ? pairs[ns + r] : (r - na) * 257, k = rk >> 8, j = rk & 255;
     if (c < np) {
       const int mq = pairs[off + c], m = mq >> 8, q = mq & 255;
       const double *Ur = U + (size_t)(m * n + q) * n2;
@@ -110,30 +113,6 @@ __global__ void kern_build_Usym(const double *U, const double *Bq, const uint16_
   const uint16_t h = f2bf(f);
   hi[idx] = h;
   lo[idx] = f2bf(f - bf2f(h));
-}
-
-// Ht^a from the split outputs: row r of Y holds Cs (pairs m <= q) in columns
-// [0, nsp) and Ca (pairs m < q) from column nsp; Ht[r][j n + k] = H^a_kj with
-// H^a_kj = Cs + Ca for k < j, Cs - Ca for k > j, Cs on the diagonal.
-__global__ void __launch_bounds__(256) kern_sym_unfold(const float *__restrict__ Y, int ldy, int nsp,
-                                                       float *__restrict__ Ht) {
-  constexpr int n = 64, ns = n * (n + 1) / 2, na = n * (n - 1) / 2;
-  __shared__ float cs[ns], ca[na];
-  const float *y = Y + (size_t)blockIdx.x * ldy;
-  for (int t = threadIdx.x; t < ns; t += 256) cs[t] = y[t];
-  for (int t = threadIdx.x; t < na; t += 256) ca[t] = y[nsp + t];
-  __syncthreads();
-  float *h = Ht + (size_t)blockIdx.x * n * n;
-  for (int o = threadIdx.x; o < n * n; o += 256) {
-    const int j = o / n, k = o % n, lo = min(j, k), hi = max(j, k);
-    const int ps = lo * n - lo * (lo - 1) / 2 + (hi - lo);
-    float v = cs[ps];
-    if (lo != hi) {
-      const float d = ca[lo * (n - 1) - lo * (lo - 1) / 2 + (hi - lo - 1)];
-      v = k < j ? v + d : v - d;
-    }
-    h[o] = v;
-  }
 }
 
 // Pi of every item from the real product Pf = [Wr; Wi] [Zr; Zi]^T (2a x 2a):
@@ -283,7 +262,7 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     check_launch("sigma pi combine");
     if (sym) {
       // the exchange in its symmetric and antisymmetric parts (half the flops
-      // of the full contraction), each row unfolded into Ht^a afterwards
+      // of the full contraction); E reads the two outputs as they are
       c64_c((const float *)bf.B, wth, wtl, P, a, G, bf.S1h, bf.S1l, ka, s, c.sym_pairs.as<uint16_t>(), sg.ks);
       float *Y = (float *)bf.X;
       c.mark(0);
@@ -294,8 +273,6 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
                      (int)sg.ka, 1, 1.0f, 0, s);
       c.mark(1);
       if (c.prof) c.prof_flops += 2.0 * M * ((double)sg.ns * (sg.ns + a) + (double)sg.na * sg.na);
-      hipLaunchKernelGGL(kern_sym_unfold, dim3((unsigned)M), dim3(256), 0, s, Y, sg.ldh, sg.nsp, H);
-      check_launch("sigma exchange unfold");
     } else {
       c64_c((const float *)bf.B, wth, wtl, P, a, G, bf.S1h, bf.S1l, ka, s);
       // Ht^a = ([C | Pi] Baug^T) with Baug rows ordered (j,k): H^a transposed
@@ -306,7 +283,8 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
       if (c.prof) c.prof_flops += 2.0 * M * (double)NN * (double)ka;
     }
     const int S = e_splits(a);
-    c64_e(W, H, P, a, S, (float *)bf.Ep, s);
+    if (sym) c64_e(W, nullptr, P, a, S, (float *)bf.Ep, s, (const float *)bf.X, sg.ldh, sg.nsp, c.sym_pairs.as<uint16_t>());
+    else c64_e(W, H, P, a, S, (float *)bf.Ep, s);
     tot = (long long)P * 2 * n2;
     hipLaunchKernelGGL(kern_sum_splits<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, P, 2 * n2,
                        bf.E + (size_t)p0 * 2 * n2);
