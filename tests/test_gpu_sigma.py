@@ -143,3 +143,24 @@ def test_sigma_every_gemm_variant_nao64(orc, monkeypatch, symx):
             scale = max(np.max(np.abs(b)), 1e-30)
             err = np.max(np.abs(a - b))
             assert err <= TOL["bf16x3"] * scale, (v, name, comp, err, scale)
+
+
+def test_sigma_parity_nao64_naux256(orc):
+    """The benchmark's shape, nao = 64 and naux = 256: the split exchange with
+    K = 2080 + 256 padded to 2368 and the Pi block in its symmetric operand, one
+    slice against the oracle (OpenMP over its time points) in bf16x3."""
+    import os
+    rng = np.random.default_rng(256)
+    nao, naux, nt, ntau = 64, 256, 2, 3
+    V, B = sym_V(rng, nao, naux)
+    G = random_green(rng, nt, ntau, nao, scale=0.2)
+    S_ref = GreenFunc(nt, ntau, nao)
+    orc.sigma_solve(V, nt, S_ref, G, nthreads=min(os.cpu_count() or 1, 16))
+    print(f"[sigma parity] nao={nao} naux={naux} prec=bf16x3", flush=True)
+    S = _sigma_gpu(V, B, G, nt, "bf16x3")
+    for name, a, b in (("ret", S.ret_row(nt), S_ref.ret_row(nt)),
+                       ("les", S.les_col(nt), S_ref.les_col(nt)),
+                       ("tv", S.tv[nt], S_ref.tv[nt])):
+        scale = max(np.max(np.abs(b)), 1e-30)
+        err = np.max(np.abs(a - b))
+        assert err <= TOL["bf16x3"] * scale, (name, err, scale)
