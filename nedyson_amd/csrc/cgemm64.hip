@@ -327,6 +327,9 @@ __device__ __forceinline__ void put1(Img &im, int pl, int row, int col, float v)
 // (item, plane, a) of Y holds Cs over the strict pairs k < j (in the order of
 // the pair table's entries [ns, ns + na)) and then the diagonal, and Ca over
 // the same strict pairs from column nsp; H_kj = Cs + Ca, H_jk = Cs - Ca, H_kk = Cs.
+// Thread t takes the strict pairs t + 256 u (neighbouring lanes hold
+// neighbouring pairs: coalesced loads, and image writes of neighbouring rows)
+// and, below 64, the diagonal entry t.
 template <bool SYM>
 __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, const float *__restrict__ Ht, int a,
                                                   int G, float *__restrict__ Ep, const float *__restrict__ Y,
@@ -334,17 +337,15 @@ __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, c
   __shared__ __attribute__((aligned(16))) Img S, X;
   const int p = blockIdx.y, s = blockIdx.x, a0 = s * G, a1 = min(a0 + G, a), tid = threadIdx.x;
   const long long an2 = (long long)a * E2;
-  constexpr int NS = N64 * (N64 + 1) / 2, NA = N64 * (N64 - 1) / 2, NA8 = NA / 8;  // 252 threads of 8 pairs
-  // SYM: threads below NA8 take 8 strict pairs, the other 4 take 16 diagonal entries each
-  uint32_t pw[4] = {0, 0, 0, 0};
+  constexpr int NS = N64 * (N64 + 1) / 2, NA = N64 * (N64 - 1) / 2, PU = (NA + 255) / 256;  // 8 pairs a thread
+  uint16_t pk[PU];
   if constexpr (SYM)
-    if (tid < NA8) {
-      const uint4 pk = *reinterpret_cast<const uint4 *>(pairs + NS + 8 * tid);
-      pw[0] = pk.x; pw[1] = pk.y; pw[2] = pk.z; pw[3] = pk.w;
-    }
+#pragma unroll
+    for (int u = 0; u < PU; u++) pk[u] = tid + 256 * u < NA ? pairs[NS + tid + 256 * u] : (uint16_t)0;
   // both operands are read once: the next index's are loaded into registers
   // while the current product runs
   float4 rw[2][4], rh[2][4];
+  float ys[2][PU], ya[2][PU], yd[2];
   auto gload = [&](int x) __attribute__((always_inline)) {
     const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
 #pragma unroll
@@ -357,12 +358,13 @@ __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, c
       }
       if constexpr (SYM) {
         const float *y = Y + (((size_t)p * 2 + pl) * a + x) * ldy;
-        const float *q0 = tid < NA8 ? y + 8 * tid : y + NA + 16 * (tid - NA8);
-        const float *q1 = tid < NA8 ? y + nsp + 8 * tid : q0 + 8;
-        rh[pl][0] = *reinterpret_cast<const float4 *>(q0);
-        rh[pl][1] = *reinterpret_cast<const float4 *>(q0 + 4);
-        rh[pl][2] = *reinterpret_cast<const float4 *>(q1);
-        rh[pl][3] = *reinterpret_cast<const float4 *>(q1 + 4);
+#pragma unroll
+        for (int u = 0; u < PU; u++) {
+          const int q = tid + 256 * u;
+          ys[pl][u] = q < NA ? y[q] : 0.0f;
+          ya[pl][u] = q < NA ? y[nsp + q] : 0.0f;
+        }
+        yd[pl] = tid < N64 ? y[NA + tid] : 0.0f;
       }
     }
   };
@@ -379,24 +381,14 @@ __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, c
         if constexpr (!SYM) put_f32(X, pl, it, rh[pl][it]);
       }
       if constexpr (SYM) {
-        const float v[16] = {rh[pl][0].x, rh[pl][0].y, rh[pl][0].z, rh[pl][0].w, rh[pl][1].x, rh[pl][1].y,
-                             rh[pl][1].z, rh[pl][1].w, rh[pl][2].x, rh[pl][2].y, rh[pl][2].z, rh[pl][2].w,
-                             rh[pl][3].x, rh[pl][3].y, rh[pl][3].z, rh[pl][3].w};
-        if (tid < NA8) {
 #pragma unroll
-          for (int u = 0; u < 8; u++) {
-            const uint32_t e = (pw[u >> 1] >> (16 * (u & 1))) & 0xffffu;
-            const int k = (int)(e >> 8), j = (int)(e & 255u);
-            put1(X, pl, j, k, v[u] + v[8 + u]);  // Ht[j][k] = H_kj
-            put1(X, pl, k, j, v[u] - v[8 + u]);  // Ht[k][j] = H_jk
+        for (int u = 0; u < PU; u++)
+          if (tid + 256 * u < NA) {
+            const int k = pk[u] >> 8, j = pk[u] & 255;
+            put1(X, pl, j, k, ys[pl][u] + ya[pl][u]);  // Ht[j][k] = H_kj
+            put1(X, pl, k, j, ys[pl][u] - ya[pl][u]);  // Ht[k][j] = H_jk
           }
-        } else {
-#pragma unroll
-          for (int u = 0; u < 16; u++) {
-            const int k = 16 * (tid - NA8) + u;
-            put1(X, pl, k, k, v[u]);
-          }
-        }
+        if (tid < N64) put1(X, pl, tid, tid, yd[pl]);
       }
     }
     __syncthreads();
