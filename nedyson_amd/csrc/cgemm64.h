@@ -6,6 +6,7 @@
 
 namespace ned {
 
+// W == nullptr: only the transposed split Wth / Wtl is written
 void c64_w(const uint16_t *Vh, const uint16_t *Vl, const float *At, int P, int a, int G, float *W, uint16_t *Wth,
            uint16_t *Wtl, hipStream_t s);
 void c64_z(const uint16_t *Vh, const uint16_t *Vl, const float *Bt, int P, int a, int G, uint16_t *Zth,
@@ -15,8 +16,10 @@ void c64_z(const uint16_t *Vh, const uint16_t *Vl, const float *Bt, int P, int a
 void c64_c(const float *Bm, const uint16_t *Wth, const uint16_t *Wtl, int P, int a, int G, uint16_t *Sh,
            uint16_t *Sl, long long ld, hipStream_t s, const uint16_t *pairs = nullptr, long long ks = 0);
 // Y != nullptr: H^a from the split exchange's outputs (row stride ldy, the
-// antisymmetric part from column nsp) instead of Ht
+// antisymmetric part from column nsp) instead of Ht, and W^a from its
+// transposed split Wth / Wtl (as c64_w writes it) instead of W
 void c64_e(const float *W, const float *Ht, int P, int a, int S, float *Ep, hipStream_t s, const float *Y = nullptr,
-           long long ldy = 0, int nsp = 0, const uint16_t *pairs = nullptr);
+           long long ldy = 0, int nsp = 0, const uint16_t *pairs = nullptr, const uint16_t *Wth = nullptr,
+           const uint16_t *Wtl = nullptr);
 
 }  // namespace ned

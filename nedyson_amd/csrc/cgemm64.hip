@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(256) kern_c64_w(const uint16_t *__restrict__ V
     zero(acc);
     mac64<1, 2>(acc, X, S);
     const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
-    store_f32(W + o, an2, acc);
+    if (W) store_f32(W + o, an2, acc);  // only the split form when E reads that
     __syncthreads();
     stage(X, acc, true);
     __syncthreads();
@@ -330,10 +330,14 @@ __device__ __forceinline__ void put1(Img &im, int pl, int row, int col, float v)
 // Thread t takes the strict pairs t + 256 u (neighbouring lanes hold
 // neighbouring pairs: coalesced loads, and image writes of neighbouring rows)
 // and, below 64, the diagonal entry t.
+// Also with SYM, W^a comes as its transposed split (Wt[k][i] = W^a[i][k], the
+// layout c64_w leaves for C and Pi): lane t of a wave takes row k = t of Wt,
+// 8 columns a load, and writes them down column k of the image.
 template <bool SYM>
 __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, const float *__restrict__ Ht, int a,
                                                   int G, float *__restrict__ Ep, const float *__restrict__ Y,
-                                                  long long ldy, int nsp, const uint16_t *__restrict__ pairs) {
+                                                  long long ldy, int nsp, const uint16_t *__restrict__ pairs,
+                                                  const uint16_t *__restrict__ Wth, const uint16_t *__restrict__ Wtl) {
   __shared__ __attribute__((aligned(16))) Img S, X;
   const int p = blockIdx.y, s = blockIdx.x, a0 = s * G, a1 = min(a0 + G, a), tid = threadIdx.x;
   const long long an2 = (long long)a * E2;
@@ -346,17 +350,26 @@ __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, c
   // while the current product runs
   float4 rw[2][4], rh[2][4];
   float ys[2][PU], ya[2][PU], yd[2];
+  uint4 wh[2][2], wl[2][2];
   auto gload = [&](int x) __attribute__((always_inline)) {
     const size_t o = (size_t)p * 2 * an2 + (size_t)x * E2;
 #pragma unroll
     for (int pl = 0; pl < 2; pl++) {
+      if constexpr (!SYM)
 #pragma unroll
-      for (int it = 0; it < 4; it++) {
-        const size_t e = o + pl * an2 + f32_row(it) * N64 + f32_col(it);
-        rw[pl][it] = *reinterpret_cast<const float4 *>(W + e);
-        if constexpr (!SYM) rh[pl][it] = *reinterpret_cast<const float4 *>(Ht + e);
-      }
+        for (int it = 0; it < 4; it++) {
+          const size_t e = o + pl * an2 + f32_row(it) * N64 + f32_col(it);
+          rw[pl][it] = *reinterpret_cast<const float4 *>(W + e);
+          rh[pl][it] = *reinterpret_cast<const float4 *>(Ht + e);
+        }
       if constexpr (SYM) {
+#pragma unroll
+        for (int it = 0; it < 2; it++) {
+          const int e = tid + 256 * it, k = e & 63, i0 = (e >> 6) * 8;
+          const size_t g = o + pl * an2 + (size_t)k * N64 + i0;
+          wh[pl][it] = *reinterpret_cast<const uint4 *>(Wth + g);
+          wl[pl][it] = *reinterpret_cast<const uint4 *>(Wtl + g);
+        }
         const float *y = Y + (((size_t)p * 2 + pl) * a + x) * ldy;
 #pragma unroll
         for (int u = 0; u < PU; u++) {
@@ -375,12 +388,24 @@ __global__ void __launch_bounds__(256) kern_c64_e(const float *__restrict__ W, c
     __syncthreads();
 #pragma unroll
     for (int pl = 0; pl < 2; pl++) {
+      if constexpr (!SYM)
 #pragma unroll
-      for (int it = 0; it < 4; it++) {
-        put_f32(S, pl, it, rw[pl][it]);
-        if constexpr (!SYM) put_f32(X, pl, it, rh[pl][it]);
-      }
+        for (int it = 0; it < 4; it++) {
+          put_f32(S, pl, it, rw[pl][it]);
+          put_f32(X, pl, it, rh[pl][it]);
+        }
       if constexpr (SYM) {
+#pragma unroll
+        for (int it = 0; it < 2; it++) {
+          const int e = tid + 256 * it, k = e & 63, i0 = (e >> 6) * 8;
+          const uint32_t hw[4] = {wh[pl][it].x, wh[pl][it].y, wh[pl][it].z, wh[pl][it].w};
+          const uint32_t lw[4] = {wl[pl][it].x, wl[pl][it].y, wl[pl][it].z, wl[pl][it].w};
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            S.h[pl][i0 + u][k] = (uint16_t)(hw[u >> 1] >> (16 * (u & 1)));
+            S.l[pl][i0 + u][k] = (uint16_t)(lw[u >> 1] >> (16 * (u & 1)));
+          }
+        }
 #pragma unroll
         for (int u = 0; u < PU; u++)
           if (tid + 256 * u < NA) {
@@ -424,12 +449,14 @@ void c64_c(const float *Bm, const uint16_t *Wth, const uint16_t *Wtl, int P, int
 }
 
 void c64_e(const float *W, const float *Ht, int P, int a, int S, float *Ep, hipStream_t s, const float *Y,
-           long long ldy, int nsp, const uint16_t *pairs) {
+           long long ldy, int nsp, const uint16_t *pairs, const uint16_t *Wth, const uint16_t *Wtl) {
   if (a % S) throw std::invalid_argument("c64_e: naux must be a multiple of the split count");
   if (Y)
-    hipLaunchKernelGGL(kern_c64_e<true>, dim3(S, P), dim3(256), 0, s, W, Ht, a, a / S, Ep, Y, ldy, nsp, pairs);
+    hipLaunchKernelGGL(kern_c64_e<true>, dim3(S, P), dim3(256), 0, s, W, Ht, a, a / S, Ep, Y, ldy, nsp, pairs, Wth,
+                       Wtl);
   else
-    hipLaunchKernelGGL(kern_c64_e<false>, dim3(S, P), dim3(256), 0, s, W, Ht, a, a / S, Ep, Y, ldy, nsp, pairs);
+    hipLaunchKernelGGL(kern_c64_e<false>, dim3(S, P), dim3(256), 0, s, W, Ht, a, a / S, Ep, Y, ldy, nsp, pairs, Wth,
+                       Wtl);
   check_launch("c64_e");
 }
 

@@ -250,8 +250,8 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     const long long ka = sym ? sg.kx : n2 + a;  // row stride of the exchange operand
     const int M = 2 * a * P, NN = (int)n2;
     float *W = (float *)bf.W, *H = (float *)bf.H;
-    uint16_t *wth = reinterpret_cast<uint16_t *>(H), *wtl = wth + (size_t)P * 2 * an2;  // Wt lives in H until H is formed
-    c64_w(c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (const float *)bf.A, P, a, G, W, wth, wtl, s);
+    uint16_t *wth = reinterpret_cast<uint16_t *>(H), *wtl = wth + (size_t)P * 2 * an2;  // Wt lives in H until H is formed (for the whole chunk when split)
+    c64_w(c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (const float *)bf.A, P, a, G, sym ? nullptr : W, wth, wtl, s);
     c64_z(c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (const float *)bf.Bt, P, a, G, bf.S1h, bf.S1l, s);
     // Pf_p = [Wt_r; Wt_i] [Zt_r; Zt_i]^T (K index (k,i) in both)
     gemm_nt_bf16x3(NtOp{wth, wtl, n2, 2 * an2}, NtOp{bf.S1h, bf.S1l, n2, 2 * an2}, bf.Pf, 2LL * a, 4LL * a * a, 2 * a,
@@ -283,7 +283,9 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
       if (c.prof) c.prof_flops += 2.0 * M * (double)NN * (double)ka;
     }
     const int S = e_splits(a);
-    if (sym) c64_e(W, nullptr, P, a, S, (float *)bf.Ep, s, (const float *)bf.X, sg.ldh, sg.nsp, c.sym_pairs.as<uint16_t>());
+    if (sym)
+      c64_e(nullptr, nullptr, P, a, S, (float *)bf.Ep, s, (const float *)bf.X, sg.ldh, sg.nsp,
+            c.sym_pairs.as<uint16_t>(), wth, wtl);
     else c64_e(W, H, P, a, S, (float *)bf.Ep, s);
     tot = (long long)P * 2 * n2;
     hipLaunchKernelGGL(kern_sum_splits<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, P, 2 * n2,
