@@ -99,8 +99,10 @@ bool nt_bf16x3_ok(int M, int N, int K);
 // environment variable NED_NT_VARIANT=<index> pins one)
 int nt_variant_count();
 const char *nt_variant_name(int v);
+// tune = false: no timing runs (the tile variant chosen earlier for this
+// shape, else a fixed default)
 void gemm_nt_bf16x3(const NtOp &A, const NtOp &B, float *C, long long ldc, long long sC, int M, int N, int K,
-                    int batch, float alpha, int beta, hipStream_t s);
+                    int batch, float alpha, int beta, hipStream_t s, bool tune = true);
 // x -> hi + lo with hi = bf16(x), lo = bf16(x - hi), round to nearest even;
 // rows x cols (cols a multiple of 4) from row stride ldx to row stride ldh
 void split_bf16(const float *x, long long ldx, uint16_t *hi, uint16_t *lo, long long ldh, long long rows, int cols,

@@ -87,8 +87,9 @@ struct Ctx {
   bool symx = false;
   DevBuf sym_pairs, Ush, Usl, Uxh, Uxl;
   bool have_coulomb = false;
-  // Sigma scratch
-  DevBuf sg_items_A, sg_items_B, sg_items_Bt, sg_W, sg_X, sg_H, sg_Pi, sg_E, sg_Ep, sg_S1h, sg_S1l, sg_Pf;
+  // Sigma scratch: the buffers of one chunk of time points, carved from one
+  // allocation
+  DevBuf sg_arena;
   int sigma_chunk = 0;  // items per chunk (0 = choose automatically)
   // Dyson recurrences: blocks per panel, and the largest number of later
   // blocks for which a panel's bulk update is split into 16-row/column bands

@@ -575,6 +575,9 @@ template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, b
           int IGLP = -1>
 void nt_launch(const NtOp &A, const NtOp &B, float *C, long long ldc, long long strC, int M, int N, int K, int batch,
                float alpha, int beta, hipStream_t s) {
+  // the grid covers whole tiles only: a ragged M or N would leave rows or
+  // columns unwritten
+  if (M % TM || N % TN) throw std::invalid_argument("gemm_nt_bf16x3: tile does not divide M or N");
   const int tiles = (M / TM) * (N / TN);
   hipLaunchKernelGGL((kern_nt_bf16x3<TM, TN, WM, WN, TK, STREAM, DB, SWZ, IGLP>), dim3(tiles, batch),
                      dim3(NtCfg<TM, TN, WM, WN, TK>::THREADS), 0, s, A.hi, A.lo, A.ld, A.stride, B.hi, B.lo, B.ld,
@@ -606,9 +609,9 @@ const NtVariant kNt[] = {
 constexpr int kNtCount = sizeof(kNt) / sizeof(kNt[0]);
 
 struct NtKey {
-  int M, N, K, batch;
+  int M, N, K, batch, mfit;
   bool operator<(const NtKey &o) const {
-    return std::tie(M, N, K, batch) < std::tie(o.M, o.N, o.K, o.batch);
+    return std::tie(M, N, K, batch, mfit) < std::tie(o.M, o.N, o.K, o.batch, o.mfit);
   }
 };
 std::mutex g_nt_mu;
@@ -635,7 +638,7 @@ const char *nt_variant_name(int v) { return v >= 0 && v < kNtCount ? kNt[v].name
 // variants run on the caller's stream (C is overwritten by the timing runs
 // and then by the real one) and the fastest is kept for later calls.
 int nt_choose(const NtOp &A, const NtOp &B, float *C, long long ldc, long long strC, int M, int N, int K, int batch,
-              hipStream_t s) {
+              hipStream_t s, bool tune) {
   const int forced = nt_forced();
   if (forced >= 0 && fits(kNt[forced], M, N)) return forced;
   int dev = 0;
@@ -647,11 +650,22 @@ int nt_choose(const NtOp &A, const NtOp &B, float *C, long long ldc, long long s
     while (x > 1) { x >>= 1; o++; }
     return o;
   };
-  const std::pair<int, NtKey> key{dev, NtKey{octave(M), N, K, octave(batch)}};
+  // ... and the largest row tile that divides M is part of the key: an M in
+  // the same octave that only 128-row tiles divide must not reuse a 256-row
+  // choice
+  const int mfit = M % 256 == 0 ? 256 : 128;
+  const std::pair<int, NtKey> key{dev, NtKey{octave(M), N, K, octave(batch), mfit}};
   {
     std::lock_guard<std::mutex> lk(g_nt_mu);
     auto it = g_nt_choice.find(key);
-    if (it != g_nt_choice.end()) return it->second;
+    if (it != g_nt_choice.end() && fits(kNt[it->second], M, N)) return it->second;
+  }
+  if (!tune) {
+    // untimed default: the streamed, double-buffered, swizzled tiles, the
+    // largest that fits
+    for (int v : {10, 12, 11})
+      if (fits(kNt[v], M, N)) return v;
+    return 0;
   }
   hipEvent_t e0, e1;
   NED_HIP(hipEventCreate(&e0));
@@ -689,11 +703,11 @@ int nt_choose(const NtOp &A, const NtOp &B, float *C, long long ldc, long long s
 }
 
 void gemm_nt_bf16x3(const NtOp &A, const NtOp &B, float *C, long long ldc, long long strC, int M, int N, int K,
-                    int batch, float alpha, int beta, hipStream_t s) {
+                    int batch, float alpha, int beta, hipStream_t s, bool tune) {
   if (!nt_bf16x3_ok(M, N, K) || A.ld % 8 || B.ld % 8 || A.stride % 8 || B.stride % 8)
     throw std::invalid_argument("gemm_nt_bf16x3: M, N must be multiples of 128, K of 64, strides of 8");
   // beta = 1 reads C, which the timing runs would overwrite: autotune only on beta = 0
-  const int v = beta ? 0 : nt_choose(A, B, C, ldc, strC, M, N, K, batch, s);
+  const int v = beta ? 0 : nt_choose(A, B, C, ldc, strC, M, N, K, batch, s, tune);
   kNt[v].launch(A, B, C, ldc, strC, M, N, K, batch, alpha, beta, s);
   check_launch("gemm_nt_bf16x3");
 }

@@ -26,6 +26,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
 
 namespace ned {
 
@@ -136,17 +139,6 @@ __global__ void kern_pi_combine(const float *Pf, int a, int P, uint16_t *hi, uin
   lo[o] = f2bf(v - bf2f(h));
 }
 
-// E[p0 + p] = sum_s part[p S + s]   (2 n2 values per item)
-template <typename T>
-__global__ void kern_sum_splits(const T *part, int S, int P, long long per, T *E) {
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)P * per) return;
-  const long long p = idx / per, e = idx % per;
-  T acc = 0;
-  for (int q = 0; q < S; q++) acc += part[(p * S + q) * per + e];
-  E[idx] = acc;
-}
-
 __global__ void kern_d2f(const double *x, float *y, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = (float)x[i];
@@ -193,36 +185,59 @@ __global__ void kern_items(const z2 *Gret, const z2 *Gles, const z2 *Gtv, int T_
   }
 }
 
+// Sigma of the items p0 .. p0+P-1 of slice T_ into the contour storage: each
+// item's value is the sum of its S partial products (part[(p S + s) 2 n2 +
+// plane n2 + e]).  "<" item t writes Sigma^<(t,T) = -Sigma^<(T,t)^H and
+// Sigma^R(T,t) = Sigma^>(T,t) - Sigma^<(T,t) when its ">" partner (item
+// t + T_ + 1) is in the same chunk, -Sigma^<(T,t) otherwise, to which the
+// partner's chunk, issued later on the same stream, then adds Sigma^>(T,t)
+// (the same rounding: a - b is a + (-b)); tv items write Sigma^rceil(T,tau).
 template <typename T>
-__global__ void kern_scatter(const T *E, int T_, int n, int np, z2 *Sret, z2 *Sles, z2 *Stv) {
+__global__ void kern_sum_scatter(const T *part, int S, int p0, int P, int T_, int n, int np, z2 *Sret, z2 *Sles,
+                                 z2 *Stv) {
   const int n2 = n * n;
-  const int nles = T_ + 1;
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long total = (long long)(nles + np) * n2;
-  if (idx >= total) return;
-  int g = (int)(idx / n2), e = (int)(idx % n2), i = e / n, k = e % n;
-  const size_t rowT = dtri_s(T_);
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)P * n2) return;
+  const int p = (int)(idx / n2), e = (int)(idx % n2), i = e / n, k = e % n;
+  const int g = p0 + p, nles = T_ + 1;
+  const size_t per = 2 * (size_t)n2, rowT = dtri_s(T_);
+  auto item = [&](int q) {
+    T re = 0, im = 0;
+    for (int s = 0; s < S; s++) {
+      re += part[((size_t)q * S + s) * per + e];
+      im += part[((size_t)q * S + s) * per + n2 + e];
+    }
+    return z2{(double)re, (double)im};
+  };
   if (g < nles) {
-    int t = g;
-    const T *El = E + (size_t)t * 2 * n2, *Eg = E + (size_t)(t + nles) * 2 * n2;
-    z2 lt = {(double)El[e], (double)El[n2 + e]};
-    z2 gt = {(double)Eg[e], (double)Eg[n2 + e]};
-    Sret[(rowT + t) * n2 + e] = {gt.x - lt.x, gt.y - lt.y};
-    // Sigma^<(t,T)_ik = -conj(Sigma^<(T,t)_ki)
-    z2 lki = {(double)El[k * n + i], (double)El[n2 + k * n + i]};
-    Sles[(rowT + t) * n2 + e] = {-lki.x, lki.y};
+    const int t = g;
+    const z2 lt = item(p);
+    // Sigma^<(t,T)_ki = -conj(Sigma^<(T,t)_ik)
+    Sles[(rowT + t) * n2 + k * n + i] = {-lt.x, lt.y};
+    const int pg = g + nles - p0;  // the ">" partner's place in this chunk
+    if (pg < P) {
+      const z2 gt = item(pg);
+      Sret[(rowT + t) * n2 + e] = {gt.x - lt.x, gt.y - lt.y};
+    } else {
+      Sret[(rowT + t) * n2 + e] = {-lt.x, -lt.y};
+    }
+  } else if (g < 2 * nles) {
+    const int t = g - nles;
+    if (t >= p0) return;  // the "<" partner is in this chunk and wrote Sigma^R
+    const z2 gt = item(p), r = Sret[(rowT + t) * n2 + e];
+    Sret[(rowT + t) * n2 + e] = {gt.x + r.x, gt.y + r.y};
   } else {
-    int tau = g - nles;
-    const T *Et = E + (size_t)(2 * nles + tau) * 2 * n2;
-    Stv[((size_t)T_ * np + tau) * n2 + e] = {(double)Et[e], (double)Et[n2 + e]};
+    const int tau = g - 2 * nles;
+    Stv[((size_t)T_ * np + tau) * n2 + e] = item(p);
   }
 }
 
 template <typename T>
 struct Bufs {
-  T *A, *B, *Bt, *W, *X, *H, *Pi, *E, *Ep;
+  T *A, *B, *Bt, *W, *X, *H, *Pi, *Ep;
   uint16_t *S1h, *S1l;  // split-bf16 operands (bf16x3 path)
   float *Pf;            // [Wr; Wi][Zr; Zi]^T per item (bf16x3 path)
+  bool tune;            // let the split-bf16 GEMMs time their tile variants
 };
 
 // aux-index splits of the E reduction (K = naux nao is long, M = N = nao small)
@@ -236,7 +251,7 @@ inline bool fast_bf16(const Ctx &c, Prec prec) { return prec == PREC_BF16X3 && c
 
 // One chunk of P items starting at item p0.
 template <typename T>
-void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq, const T *U) {
+void run_chunk(Ctx &c, Prec prec, int T_, int p0, int P, const Bufs<T> &bf, const T *Vq, const T *U) {
   const int n = c.nao, a = c.naux;
   const long long n2 = (long long)n * n, an2 = (long long)a * n2;
   hipStream_t s = c.stream;
@@ -255,7 +270,7 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     c64_z(c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (const float *)bf.Bt, P, a, G, bf.S1h, bf.S1l, s);
     // Pf_p = [Wt_r; Wt_i] [Zt_r; Zt_i]^T (K index (k,i) in both)
     gemm_nt_bf16x3(NtOp{wth, wtl, n2, 2 * an2}, NtOp{bf.S1h, bf.S1l, n2, 2 * an2}, bf.Pf, 2LL * a, 4LL * a * a, 2 * a,
-                   2 * a, NN, P, 1.0f, 0, s);
+                   2 * a, NN, P, 1.0f, 0, s, bf.tune);
     long long tot = (long long)P * 2 * a * a;
     hipLaunchKernelGGL(kern_pi_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Pf, a, P, bf.S1h,
                        bf.S1l, ka, sym ? sg.ns : NN);
@@ -267,10 +282,10 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
       float *Y = (float *)bf.X;
       c.mark(0);
       gemm_nt_bf16x3(NtOp{bf.S1h, bf.S1l, ka, 0}, NtOp{c.Ush.as<uint16_t>(), c.Usl.as<uint16_t>(), sg.ks, 0}, Y,
-                     sg.ldh, 0, M, sg.nsp, (int)sg.ks, 1, 1.0f, 0, s);
+                     sg.ldh, 0, M, sg.nsp, (int)sg.ks, 1, 1.0f, 0, s, bf.tune);
       gemm_nt_bf16x3(NtOp{bf.S1h + sg.ks, bf.S1l + sg.ks, ka, 0},
                      NtOp{c.Uxh.as<uint16_t>(), c.Uxl.as<uint16_t>(), sg.ka, 0}, Y + sg.nsp, sg.ldh, 0, M, sg.nap,
-                     (int)sg.ka, 1, 1.0f, 0, s);
+                     (int)sg.ka, 1, 1.0f, 0, s, bf.tune);
       c.mark(1);
       if (c.prof) c.prof_flops += 2.0 * M * ((double)sg.ns * (sg.ns + a) + (double)sg.na * sg.na);
     } else {
@@ -278,7 +293,7 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
       // Ht^a = ([C | Pi] Baug^T) with Baug rows ordered (j,k): H^a transposed
       c.mark(0);
       gemm_nt_bf16x3(NtOp{bf.S1h, bf.S1l, ka, 0}, NtOp{c.Uah.as<uint16_t>(), c.Ual.as<uint16_t>(), ka, 0}, H, n2, 0,
-                     M, NN, (int)ka, 1, 1.0f, 0, s);
+                     M, NN, (int)ka, 1, 1.0f, 0, s, bf.tune);
       c.mark(1);
       if (c.prof) c.prof_flops += 2.0 * M * (double)NN * (double)ka;
     }
@@ -287,10 +302,10 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
       c64_e(nullptr, nullptr, P, a, S, (float *)bf.Ep, s, (const float *)bf.X, sg.ldh, sg.nsp,
             c.sym_pairs.as<uint16_t>(), wth, wtl);
     else c64_e(W, H, P, a, S, (float *)bf.Ep, s);
-    tot = (long long)P * 2 * n2;
-    hipLaunchKernelGGL(kern_sum_splits<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, P, 2 * n2,
-                       bf.E + (size_t)p0 * 2 * n2);
-    check_launch("sigma E sum");
+    tot = (long long)P * n2;
+    hipLaunchKernelGGL(kern_sum_scatter<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, p0, P,
+                       T_, n, c.np, c.S.ret.as<z2>(), c.S.les.as<z2>(), c.S.tv.as<z2>());
+    check_launch("sigma sum scatter");
     return;
   }
   // W_p = Vstack A_p : (a n x n)(n x n), Vstack[(a,i)][l] = B[a][i][l]
@@ -320,7 +335,7 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     split_bf16((const float *)bf.X, n2, bf.S1h, bf.S1l, n2, (long long)M, NN, s);
     // Pf_p = [Wr; Wi] [Zr; Zi]^T: M = N = 2a, K = n^2, one launch for the chunk
     gemm_nt_bf16x3(NtOp{wh, wl, n2, 2 * an2}, NtOp{bf.S1h, bf.S1l, n2, 2 * an2}, bf.Pf, 2LL * a, 4LL * a * a, 2 * a,
-                   2 * a, NN, P, 1.0f, 0, s);
+                   2 * a, NN, P, 1.0f, 0, s, bf.tune);
   } else {
     // Pi_p[a][b] = sum_(ik) W^a_ik Z^b_ik
     GemmDesc d;
@@ -350,7 +365,7 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     split_bf16((const float *)bf.X, n2, bf.S1h, bf.S1l, ka, (long long)M, NN, s);
     c.mark(0);
     gemm_nt_bf16x3(NtOp{bf.S1h, bf.S1l, ka, 0}, NtOp{c.Uah.as<uint16_t>(), c.Ual.as<uint16_t>(), ka, 0},
-                   (float *)bf.H, n2, 0, M, NN, (int)ka, 1, 1.0f, 0, s);
+                   (float *)bf.H, n2, 0, M, NN, (int)ka, 1, 1.0f, 0, s, bf.tune);
     c.mark(1);
     if (c.prof) c.prof_flops += 2.0 * M * (double)NN * (double)ka;
   } else {
@@ -390,61 +405,65 @@ void run_chunk(Ctx &c, Prec prec, int p0, int P, const Bufs<T> &bf, const T *Vq,
     d.B.bdiv = S; d.B.bs_hi = 2 * an2; d.B.bs_lo = (long long)as * n2;
     d.C.ptr = bf.Ep; d.C.rs = n; d.C.cs = 1; d.C.ps = n2; d.C.bs_lo = 2 * n2;
     gemm(d, prec, s);
-    long long tot = (long long)P * 2 * n2;
-    hipLaunchKernelGGL(kern_sum_splits<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, P, 2 * n2,
-                       bf.E + (size_t)p0 * 2 * n2);
-    check_launch("sigma E sum");
+    long long tot = (long long)P * n2;
+    hipLaunchKernelGGL(kern_sum_scatter<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, p0, P,
+                       T_, n, c.np, c.S.ret.as<z2>(), c.S.les.as<z2>(), c.S.tv.as<z2>());
+    check_launch("sigma sum scatter");
   }
 }
 
+// The scratch of one chunk of P items, carved from one buffer at base (sizes
+// only when base is null): items A, B (and B^T on the nao = 64 path); W
+// (not kept on the split path: E reads W^a from its bf16 split in H); X (the
+// generic path's Z and C; on the split path the two exchange outputs); H;
+// the E partial sums; and on the bf16x3 path the split operand S1 and the Pi
+// product Pf, on the others Pi.  Every piece starts on a 256-byte boundary.
 template <typename T>
-void sigma_typed(Ctx &c, int tstp, Prec prec) {
-  const int n = c.nao, a = c.naux, np = c.np;
-  const long long n2 = (long long)n * n, an2 = (long long)a * n2;
-  const int Ptot = 2 * (tstp + 1) + np;
-  const bool fast = fast_bf16(c, prec);
-  const bool f64w = fast && c.fast64;
-  const int S = e_splits(a);
-  const bool sym = f64w && c.symx;
+size_t sigma_carve(const Ctx &c, Prec prec, int P, char *base, Bufs<T> *bf) {
+  const int n = c.nao, a = c.naux;
+  const size_t n2 = (size_t)n * n, an2 = (size_t)a * n2, Pz = (size_t)P;
+  const bool fast = fast_bf16(c, prec), f64w = fast && c.fast64, sym = f64w && c.symx;
   const SymGeom sg = sym_geom(n, a);
-  const long long ka = sym ? sg.kx : n2 + a;
-  // device scratch per item: items, W, X (not on the nao = 64 path unless the
-  // exchange is split: then it holds the split outputs), H, Pi (generic path),
-  // E partials, and on the bf16x3 path the split operand S1 and the Pi product Pf
-  size_t per_item = (size_t)((f64w ? 2 : 3) * 2 * an2 + (f64w ? 6 : 4) * n2 + (long long)S * 2 * n2) * sizeof(T);
-  if (fast) per_item += (size_t)2 * a * ka * 2 * sizeof(uint16_t) + (size_t)4 * a * a * sizeof(float);
-  else per_item += (size_t)2 * a * a * sizeof(T);
-  if (sym) per_item += (size_t)2 * a * sg.ldh * sizeof(float);
-  const size_t budget = (size_t)6 << 30;
-  int P = c.sigma_chunk > 0 ? c.sigma_chunk : (int)std::max<size_t>(1, budget / std::max<size_t>(per_item, 1));
-  if (c.sigma_chunk <= 0 && P >= 16) {
-    // a multiple of 8 items: the exchange GEMM of a full chunk then has
-    // naux*P/8 tiles of 256 x 256 (at nao = 64), whole rounds of 256 CUs;
-    // rounded up when that stays within 5% of the scratch budget
-    const int up = (P + 7) / 8 * 8;
-    P = (double)up * per_item <= 1.05 * budget ? up : P / 8 * 8;
-  }
-  P = std::min(P, Ptot);
-  c.sg_items_A.ensure((size_t)P * 2 * n2 * sizeof(T));
-  c.sg_items_B.ensure((size_t)P * 2 * n2 * sizeof(T));
-  c.sg_W.ensure((size_t)P * 2 * an2 * sizeof(T));
-  if (f64w) c.sg_items_Bt.ensure((size_t)P * 2 * n2 * sizeof(T));
-  else c.sg_X.ensure((size_t)P * 2 * an2 * sizeof(T));
-  if (sym) c.sg_X.ensure((size_t)P * 2 * a * sg.ldh * sizeof(float));
-  c.sg_H.ensure((size_t)P * 2 * an2 * sizeof(T));
-  c.sg_E.ensure((size_t)Ptot * 2 * n2 * sizeof(T));
-  c.sg_Ep.ensure((size_t)P * S * 2 * n2 * sizeof(T));
+  const size_t ka = sym ? (size_t)sg.kx : n2 + a;
+  const int S = e_splits(a);
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> char * {
+    char *q = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return bytes ? q : nullptr;
+  };
+  Bufs<T> b{};
+  b.A = (T *)take(Pz * 2 * n2 * sizeof(T));
+  b.B = (T *)take(Pz * 2 * n2 * sizeof(T));
+  b.Bt = f64w ? (T *)take(Pz * 2 * n2 * sizeof(T)) : nullptr;
+  b.W = sym ? nullptr : (T *)take(Pz * 2 * an2 * sizeof(T));
+  if (sym) b.X = (T *)take(Pz * 2 * a * (size_t)sg.ldh * sizeof(float));
+  else if (!f64w) b.X = (T *)take(Pz * 2 * an2 * sizeof(T));
+  b.H = (T *)take(Pz * 2 * an2 * sizeof(T));
+  b.Ep = (T *)take(Pz * S * 2 * n2 * sizeof(T));
   if (fast) {
-    c.sg_S1h.ensure((size_t)P * 2 * a * ka * sizeof(uint16_t));
-    c.sg_S1l.ensure((size_t)P * 2 * a * ka * sizeof(uint16_t));
-    c.sg_Pf.ensure((size_t)P * 4 * a * a * sizeof(float));
+    b.S1h = (uint16_t *)take(Pz * 2 * a * ka * sizeof(uint16_t));
+    b.S1l = (uint16_t *)take(Pz * 2 * a * ka * sizeof(uint16_t));
+    b.Pf = (float *)take(Pz * 4 * a * a * sizeof(float));
   } else {
-    c.sg_Pi.ensure((size_t)P * 2 * a * a * sizeof(T));
+    b.Pi = (T *)take(Pz * 2 * a * a * sizeof(T));
   }
-  Bufs<T> bf{c.sg_items_A.as<T>(), c.sg_items_B.as<T>(), f64w ? c.sg_items_Bt.as<T>() : nullptr, c.sg_W.as<T>(),
-             c.sg_X.as<T>(), c.sg_H.as<T>(),
-             c.sg_Pi.as<T>(), c.sg_E.as<T>(), c.sg_Ep.as<T>(), c.sg_S1h.as<uint16_t>(), c.sg_S1l.as<uint16_t>(),
-             c.sg_Pf.as<float>()};
+  b.tune = true;
+  if (bf) *bf = b;
+  return off;
+}
+
+// Sigma of slice tstp in chunks of P items, with the scratch at base (sized
+// by sigma_carve for P)
+template <typename T>
+void sigma_chunks(Ctx &c, int tstp, Prec prec, int P, char *base, bool tune) {
+  const int n = c.nao, np = c.np;
+  const long long n2 = (long long)n * n;
+  const int Ptot = 2 * (tstp + 1) + np;
+  const bool f64w = fast_bf16(c, prec) && c.fast64;
+  Bufs<T> bf;
+  sigma_carve<T>(c, prec, P, base, &bf);
+  bf.tune = tune;
   const T *Vq = sizeof(T) == 8 ? (const T *)c.Bq.p : (const T *)c.Bq32.p;
   const T *U = sizeof(T) == 8 ? (const T *)c.U64.p : (const T *)c.U32.p;
   for (int p0 = 0; p0 < Ptot; p0 += P) {
@@ -454,12 +473,26 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
                        c.G.les.as<z2>(), c.G.tv.as<z2>(), tstp, n, np, c.sig, p0, Pc, bf.A, bf.B, f64w ? 1 : 0,
                        bf.Bt);
     check_launch("sigma items");
-    run_chunk<T>(c, prec, p0, Pc, bf, Vq, U);
+    run_chunk<T>(c, prec, tstp, p0, Pc, bf, Vq, U);
   }
-  long long tot = (long long)(tstp + 1 + np) * n2;
-  hipLaunchKernelGGL(kern_scatter<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, bf.E, tstp, n, np,
-                     c.S.ret.as<z2>(), c.S.les.as<z2>(), c.S.tv.as<z2>());
-  check_launch("sigma scatter");
+}
+
+template <typename T>
+void sigma_typed(Ctx &c, int tstp, Prec prec) {
+  const int Ptot = 2 * (tstp + 1) + c.np;
+  const size_t per_item = sigma_carve<T>(c, prec, 1, nullptr, nullptr);
+  const size_t budget = (size_t)6 << 30;
+  int P = c.sigma_chunk > 0 ? c.sigma_chunk : (int)std::max<size_t>(1, budget / per_item);
+  if (c.sigma_chunk <= 0 && P >= 16) {
+    // a multiple of 8 items: the exchange GEMM of a full chunk then has
+    // naux*P/8 tiles of 256 x 256 (at nao = 64), whole rounds of 256 CUs;
+    // rounded up when that stays within 5% of the scratch budget
+    const int up = (P + 7) / 8 * 8;
+    P = (double)up * per_item <= 1.05 * budget ? up : P / 8 * 8;
+  }
+  P = std::min(P, Ptot);
+  c.sg_arena.ensure(sigma_carve<T>(c, prec, P, nullptr, nullptr));
+  sigma_chunks<T>(c, tstp, prec, P, c.sg_arena.as<char>(), true);
 }
 
 }  // namespace
@@ -489,7 +522,16 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
   check_launch("coulomb U");
   c.uaug = n2 % 128 == 0 && (2 * a) % 128 == 0 && a % 64 == 0;
   c.fast64 = c.uaug && n == 64;
-  if (c.uaug) {
+  // at nao = 64 the exchange runs split by the symmetry of U' (NED_SYMX=0
+  // keeps the full contraction)
+  const char *sx = std::getenv("NED_SYMX");
+  c.symx = c.fast64 && !(sx && std::atoi(sx) == 0);
+  // the full contraction's operand, not read by the split exchange
+  if (c.symx) {
+    c.Uah.release();
+    c.Ual.release();
+  }
+  if (c.uaug && !c.symx) {
     const size_t ka = n2 + a;
     c.Uah.ensure(n2 * ka * sizeof(uint16_t));
     c.Ual.ensure(n2 * ka * sizeof(uint16_t));
@@ -504,10 +546,6 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
     split_bf16(c.Bq32.as<float>(), (long long)n2, c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (long long)n2, a, (int)n2,
                c.stream);
   }
-  // at nao = 64 the exchange runs split by the symmetry of U' (NED_SYMX=0
-  // keeps the full contraction)
-  const char *sx = std::getenv("NED_SYMX");
-  c.symx = c.fast64 && !(sx && std::atoi(sx) == 0);
   if (c.symx) {
     const SymGeom sg = sym_geom(n, a);
     // kern_c64_c zeroes 32 padding columns after each block (naux % 64 == 0)

@@ -164,3 +164,79 @@ def test_sigma_parity_nao64_naux256(orc):
         scale = max(np.max(np.abs(b)), 1e-30)
         err = np.max(np.abs(a - b))
         assert err <= TOL["bf16x3"] * scale, (name, err, scale)
+
+
+@pytest.mark.parametrize("naux", [64, 256])
+def test_sigma_chunking_invariant_nao64(orc, monkeypatch, naux):
+    """The split exchange at nao = 64 under item chunks of 3 and 7 as well as
+    the whole slice (10 items): chunks starting at p0 > 0, a smaller last
+    chunk, the W planes kept in H across a chunk, and the Sigma^R of a ">"
+    item whose "<" partner fell into an earlier chunk.  With one tile variant
+    pinned the result is bitwise the same for every chunking, and within the
+    bf16x3 bound of the oracle."""
+    import os
+    monkeypatch.setenv("NED_SYMX", "1")
+    monkeypatch.setenv("NED_NT_VARIANT", "11")  # 128 x 128: fits every chunk's M = 2 naux P
+    rng = np.random.default_rng(640 + naux)
+    nao, nt, ntau = 64, 2, 3
+    V, B = sym_V(rng, nao, naux)
+    G = random_green(rng, nt, ntau, nao, scale=0.2)
+    S_ref = GreenFunc(nt, ntau, nao)
+    orc.sigma_solve(V, nt, S_ref, G, nthreads=min(os.cpu_count() or 1, 16))
+    outs = []
+    for chunk in (0, 3, 7):
+        print(f"[sigma chunking] nao=64 naux={naux} chunk={chunk}", flush=True)
+        monkeypatch.setenv("NED_SIGMA_CHUNK", str(chunk))
+        outs.append(_sigma_gpu(V, B, G, nt, "bf16x3"))
+    for o in outs[1:]:
+        for comp in ("ret", "les", "tv"):
+            assert np.array_equal(getattr(o, comp), getattr(outs[0], comp)), comp
+    S = outs[0]
+    for name, a, b in (("ret", S.ret_row(nt), S_ref.ret_row(nt)),
+                       ("les", S.les_col(nt), S_ref.les_col(nt)),
+                       ("tv", S.tv[nt], S_ref.tv[nt])):
+        scale = max(np.max(np.abs(b)), 1e-30)
+        assert np.max(np.abs(a - b)) <= TOL["bf16x3"] * scale, name
+
+
+def test_sigma_parity_nao64_naux256_f32(orc):
+    """FP32 Sigma at the benchmark's shape (nao = 64, naux = 256), one slice
+    against the oracle."""
+    import os
+    rng = np.random.default_rng(257)
+    nao, naux, nt, ntau = 64, 256, 2, 3
+    V, B = sym_V(rng, nao, naux)
+    G = random_green(rng, nt, ntau, nao, scale=0.2)
+    S_ref = GreenFunc(nt, ntau, nao)
+    orc.sigma_solve(V, nt, S_ref, G, nthreads=min(os.cpu_count() or 1, 16))
+    print(f"[sigma parity] nao={nao} naux={naux} prec=f32", flush=True)
+    S = _sigma_gpu(V, B, G, nt, "f32")
+    for name, a, b in (("ret", S.ret_row(nt), S_ref.ret_row(nt)),
+                       ("les", S.les_col(nt), S_ref.les_col(nt)),
+                       ("tv", S.tv[nt], S_ref.tv[nt])):
+        scale = max(np.max(np.abs(b)), 1e-30)
+        err = np.max(np.abs(a - b))
+        assert err <= TOL["f32"] * scale, (name, err, scale)
+
+
+def test_nt_autotune_ragged_rows(orc):
+    """The split-bf16 GEMM's tile choice is cached per shape class: a row count
+    that only 128-row tiles divide (M = 2 naux * 17 = 2176) must not reuse a
+    256-row choice made for one in the same octave (M = 2048).  nao = 16,
+    naux = 64 (the fused split-bf16 path), slices of 16 and 17 items (ntau+1 =
+    14 and 15 at tstp = 0) in one process, each against the oracle."""
+    rng = np.random.default_rng(17)
+    nao, naux, nt = 16, 64, 6
+    V, B = sym_V(rng, nao, naux)
+    for ntau in (13, 14):
+        G = random_green(rng, nt, ntau, nao, scale=0.5)
+        S_ref = GreenFunc(nt, ntau, nao)
+        orc.sigma_solve(V, 0, S_ref, G)
+        print(f"[autotune rows] items={2 + ntau + 1}", flush=True)
+        S = _sigma_gpu(V, B, G, 0, "bf16x3")
+        for name, a, b in (("ret", S.ret_row(0), S_ref.ret_row(0)),
+                           ("les", S.les_col(0), S_ref.les_col(0)),
+                           ("tv", S.tv[0], S_ref.tv[0])):
+            scale = max(np.max(np.abs(b)), 1e-30)
+            err = np.max(np.abs(a - b))
+            assert err <= TOL["bf16x3"] * scale, (ntau, name, err, scale)
