@@ -28,6 +28,8 @@
 #ifndef NEDYSON_H
 #define NEDYSON_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -92,6 +94,22 @@ ned_status ned_hmf_read(ned_ctx *ctx, int first, int count, void *host);
  * same slices of G.  prec selects the arithmetic of the contractions; Sigma is
  * stored in FP64 in every case. */
 ned_status ned_sigma_gf2_decomp(ned_ctx *ctx, int tstp, int prec);
+
+/* ned_sigma_gf2_decomp with no device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * context (G, Sigma, the interaction) is set up beforehand as usual; this call
+ * then allocates nothing: its chunk scratch is a static device array of
+ * ned_sigma_noalloc_scratch_bytes() (256 MiB) per GPU, held until the process
+ * ends, and its split-bf16 GEMMs use the tile variant chosen earlier for the
+ * shape (or a fixed default) without timing runs.  Time points are processed
+ * in chunks that fit that array; a shape whose single time point needs more
+ * returns NED_ERR_ARG.  Every call on one GPU shares that GPU's array, so calls
+ * that overlap on one GPU (other threads, other contexts' streams) race:
+ * serialise them.  Same results as ned_sigma_gf2_decomp up to the chunking
+ * (bitwise in FP64). */
+ned_status ned_sigma_gf2_decomp_noalloc(ned_ctx *ctx, int tstp, int prec);
+size_t ned_sigma_noalloc_scratch_bytes(void);
 
 /* G.get_dm(tstp, rho); hmf[tstp] = h0; molHFSolverDecomp::solve_HF(tstp, hmf, rho)
  * (src/greens.h:56-65, src/molHFSolver.cpp:81-110).  h0 is real nao x nao. */

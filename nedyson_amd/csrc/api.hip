@@ -117,6 +117,8 @@ ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
     NED_HIP(hipGetDeviceCount(&ndev));
     ned::require(device >= 0 && device < ndev, "no such HIP device");
     NED_HIP(hipSetDevice(device));
+    // address of this device's copy of the no-allocation Sigma scratch
+    ned::sigma_scratch_init(device);
     ctx = new ned_ctx();
     ned::Ctx &c = ctx->c;
     c.nao = d->nao; c.naux = d->naux; c.nt = d->nt; c.ntau = d->ntau; c.k = d->k; c.sig = d->sig;
@@ -256,6 +258,18 @@ ned_status ned_sigma_gf2_decomp(ned_ctx *ctx, int tstp, int prec) {
     ned::touch_time(ctx->c, tstp);
   });
 }
+
+ned_status ned_sigma_gf2_decomp_noalloc(ned_ctx *ctx, int tstp, int prec) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::require(prec >= 0 && prec <= 2, "prec must be NED_PREC_F64, _F32 or _BF16X3");
+    state(ctx->c.have_coulomb, "interaction not set (ned_set_coulomb)");
+    ned::sigma_gf2_noalloc(ctx->c, tstp, (ned::Prec)prec);
+    ned::touch_time(ctx->c, tstp);
+  });
+}
+
+size_t ned_sigma_noalloc_scratch_bytes(void) { return ned::sigma_scratch_bytes(); }
 
 ned_status ned_hf_decomp(ned_ctx *ctx, int tstp, const double *h0) {
   if (!ctx || !h0) { g_err = "null argument"; return NED_ERR_ARG; }

@@ -138,6 +138,12 @@ void require(bool cond, const char *msg);
 
 // Sigma: molGF2SolverDecomp::solve (src/mol2bSolver_decomp_impl.h:412-429)
 void sigma_gf2(Ctx &c, int tstp, Prec prec);
+// the same with no device allocation: the chunk scratch is a static device
+// array of sigma_scratch_bytes() per GPU; sigma_scratch_init(device), with
+// that device current, looks up its address once per device
+void sigma_gf2_noalloc(Ctx &c, int tstp, Prec prec);
+void sigma_scratch_init(int device);
+size_t sigma_scratch_bytes();
 void build_coulomb(Ctx &c, const double *Bq_host);
 void g0_from_h0(Ctx &c, int tmax, const double *fock, double mu, double beta, double dt);
 void tti_extend(Ctx &c, int which, int tsrc);

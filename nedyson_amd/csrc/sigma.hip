@@ -495,7 +495,65 @@ void sigma_typed(Ctx &c, int tstp, Prec prec) {
   sigma_chunks<T>(c, tstp, prec, P, c.sg_arena.as<char>(), true);
 }
 
+// ---- the same computation with no device allocation: the chunk scratch is
+// this file's static device array (kScratchBytes per GPU, held until the
+// process ends), shared by every call on that GPU
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+std::map<int, char *> g_scratch_ptr;  // device -> address of that device's copy of scratch
+
 }  // namespace
+
+void sigma_scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr.count(device)) return;
+  int cur = -1;
+  NED_HIP(hipGetDevice(&cur));
+  require(cur == device, "sigma scratch: the device must be current");
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = (char *)ptr;
+}
+
+size_t sigma_scratch_bytes() { return kScratchBytes; }
+
+template <typename T>
+static void sigma_noalloc_typed(Ctx &c, int tstp, Prec prec) {
+  char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(c.device);
+    if (it == g_scratch_ptr.end()) throw std::runtime_error("state: sigma scratch not initialised for this device");
+    base = it->second;
+  }
+  const int Ptot = 2 * (tstp + 1) + c.np;
+  const size_t per_item = sigma_carve<T>(c, prec, 1, nullptr, nullptr);
+  if (per_item > kScratchBytes)
+    throw std::invalid_argument("sigma (no allocation): one time point needs " + std::to_string(per_item) +
+                                " bytes of scratch, more than the " + std::to_string(kScratchBytes) + " available");
+  int P = std::min<long long>(Ptot, (long long)(kScratchBytes / per_item));
+  while (P > 1 && sigma_carve<T>(c, prec, P, nullptr, nullptr) > kScratchBytes) P--;
+  // no event timing (hipEventCreate) and no tile autotuning on this path
+  const bool prof = c.prof;
+  c.prof = false;
+  try {
+    sigma_chunks<T>(c, tstp, prec, P, base, false);
+  } catch (...) {
+    c.prof = prof;
+    throw;
+  }
+  c.prof = prof;
+}
+
+void sigma_gf2_noalloc(Ctx &c, int tstp, Prec prec) {
+  require(c.have_coulomb, "sigma: Coulomb tensor not set");
+  require(tstp >= 0 && tstp <= c.nt, "sigma: tstp out of range");
+  if (prec == PREC_F64) sigma_noalloc_typed<double>(c, tstp, prec);
+  else sigma_noalloc_typed<float>(c, tstp, prec);
+}
 
 void build_coulomb(Ctx &c, const double *Bq_host) {
   const int n = c.nao, a = c.naux;
