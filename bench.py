@@ -121,6 +121,116 @@ def cpu_baseline(p, T, corr, log):
             "value_1thread": 1.0 / t_step1, "sec_per_step": t_stepN, "gflops_1thread": rate / 1e9}
 
 
+def precision_trajectory(ned, p, nsteps, every, corr, log, precs=("f32", "bf16x3")):
+    """The whole run from the equilibrium GF2 state (ned_matsubara_gf2) through
+    the quench, stepped in lockstep in FP64 and in each mixed precision for
+    nsteps time steps (bootstrap included, each in its own precision).  After
+    every step the new slice T of G (retarded row, lesser column, mixing row)
+    is compared with the FP64 run's, and the particle number and total energy
+    E_kin + E_pot of each run are recorded; every `every` steps (and at the
+    end) a checkpoint is logged.  Returns the summary for the benchmark line."""
+    t0 = time.perf_counter()
+    ctxs = {}
+    with ned.Context(p.nao, p.naux, p.k, p.ntau, p.k) as c:
+        c.set_coulomb(p.B)
+        try:
+            mu, fock, it, err = c.matsubara_gf2(p.h1, p.beta, nelec=float(p.nelec), mu=p.mu, tol=1e-7)
+        except ned.NedError as e:
+            raise RuntimeError(f"equilibrium GF2 solve failed: {e}")
+        Gmat = c.read_blocks(ned.NED_G, ned.NED_MAT, 0, p.ntau + 1)
+    log(f"[traj] GF2 equilibrium (sigma_B={p.b_sd}): mu {mu:.6f}, {it} iterations, last update {err:.1e} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    runs = ("f64",) + tuple(precs)
+    boot = {}
+    for pr in runs:
+        c = ned.Context(p.nao, p.naux, nsteps, p.ntau, p.k)
+        c.set_coulomb(p.B)
+        c.write_blocks(ned.NED_G, ned.NED_MAT, 0, Gmat)
+        c.set_matsubara(p.beta)
+        c.g0_from_h0(p.k, fock, mu, p.beta, p.dt)
+        try:
+            boot[pr] = c.do_boot(fock, mu, p.beta, p.dt, pr, 60, 1e-11)
+        except ned.NedError as e:
+            if e.code != 5:
+                raise
+            boot[pr] = (60, float("nan"))
+            log(f"[traj] {pr} bootstrap stopped at its iteration limit: {e}")
+        ctxs[pr] = c
+    n, np1 = p.nao, p.ntau + 1
+
+    def slice_of(c, T):
+        s = T * (T + 1) // 2
+        return (c.read_blocks(ned.NED_G, ned.NED_RET, s, T + 1), c.read_blocks(ned.NED_G, ned.NED_LES, s, T + 1),
+                c.read_blocks(ned.NED_G, ned.NED_TV, T * np1, np1))
+
+    obs = {pr: [] for pr in runs}
+    err_by_T = {pr: [] for pr in precs}
+    step_ms = {pr: [] for pr in runs}
+    iters = {pr: [] for pr in runs}
+    checkpoints = []
+    run_max = {pr: 0.0 for pr in precs}
+    diverged = None
+    for T in range(p.k + 1, nsteps + 1):
+        sl = {}
+        for pr in runs:
+            c = ctxs[pr]
+            c.sync()
+            ts = time.perf_counter()
+            try:
+                itn, e = c.do_tstp(T, p.h0, mu, p.beta, p.dt, pr, corr, 0.0)
+            except ned.NedError as ex:
+                diverged = {"prec": pr, "T": T, "error": str(ex)}
+                log(f"[traj] {pr} stopped at T={T}: {ex}")
+                break
+            c.sync()
+            step_ms[pr].append(1e3 * (time.perf_counter() - ts))
+            iters[pr].append(itn)
+            en = c.energy(T, p.h0, p.beta, p.dt)
+            obs[pr].append((T, en[2], en[0] + en[1], e))
+            sl[pr] = slice_of(c, T)
+        if diverged:  # keep the steps that every run completed
+            n_ok = min(len(obs[pr]) for pr in runs)
+            for pr in runs:
+                del obs[pr][n_ok:]
+            break
+        for pr in precs:
+            d = max(float(np.max(np.abs(a - b))) for a, b in zip(sl[pr], sl["f64"]))
+            err_by_T[pr].append(d)
+            run_max[pr] = max(run_max[pr], d)
+        if (T - p.k) % every == 0 or T == nsteps:
+            row = {"T": T}
+            N0, E0 = obs["f64"][0][1], obs["f64"][0][2]
+            row["f64"] = {"N_drift": obs["f64"][-1][1] - N0, "E_drift": obs["f64"][-1][2] - E0,
+                          "corr_update": obs["f64"][-1][3], "ms_per_step": step_ms["f64"][-1]}
+            for pr in precs:
+                row[pr] = {"G_err_at_T": err_by_T[pr][-1], "G_err_max_so_far": run_max[pr],
+                           "N_diff_vs_f64": obs[pr][-1][1] - obs["f64"][-1][1],
+                           "E_diff_vs_f64": obs[pr][-1][2] - obs["f64"][-1][2],
+                           "N_drift": obs[pr][-1][1] - obs[pr][0][1], "E_drift": obs[pr][-1][2] - obs[pr][0][2],
+                           "corr_update": obs[pr][-1][3], "ms_per_step": step_ms[pr][-1]}
+            checkpoints.append(row)
+            log(f"[traj] T={T}: " + "  ".join(
+                f"{pr} dG={row[pr]['G_err_at_T']:.2e} (max {row[pr]['G_err_max_so_far']:.2e}) "
+                f"dN={row[pr]['N_diff_vs_f64']:.1e} dE={row[pr]['E_diff_vs_f64']:.1e}" for pr in precs) +
+                f"  f64 N drift {row['f64']['N_drift']:.1e} E drift {row['f64']['E_drift']:.1e} "
+                f"{row['f64']['ms_per_step']:.0f} ms/step")
+    for c in ctxs.values():
+        c.close()
+    if not obs["f64"]:
+        raise RuntimeError(f"precision check: no step completed ({diverged})")
+    out = {"sigma_B": p.b_sd, "steps": obs["f64"][-1][0], "diverged": diverged, "corrector_passes": corr, "initial": "GF2 equilibrium + quench",
+           "mu": mu, "bootstrap_iters": {pr: boot[pr][0] for pr in runs},
+           "max_abs_G_err_vs_f64": {pr: run_max[pr] for pr in precs},
+           "max_abs_N_diff_vs_f64": {pr: max(abs(a[1] - b[1]) for a, b in zip(obs[pr], obs["f64"])) for pr in precs},
+           "max_abs_E_diff_vs_f64": {pr: max(abs(a[2] - b[2]) for a, b in zip(obs[pr], obs["f64"])) for pr in precs},
+           "max_abs_N_drift": {pr: max(abs(a[1] - obs[pr][0][1]) for a in obs[pr]) for pr in runs},
+           "max_abs_E_drift": {pr: max(abs(a[2] - obs[pr][0][2]) for a in obs[pr]) for pr in runs},
+           "ms_per_step_avg": {pr: float(np.mean(step_ms[pr])) for pr in runs},
+           "checkpoints": checkpoints}
+    log(f"[traj] done in {time.perf_counter() - t0:.1f} s")
+    return out
+
+
 def window_context(ned, p, T0, nt):
     """A context holding the benchmark history t <= T0: the free G of the
     equilibrium Fock matrix with Sigma(T0) continued time-translation
@@ -199,6 +309,12 @@ def main():
     ap.add_argument("--sweep", default=None,
                     help="comma-separated history lengths T0: print per-step times at each and the fitted "
                          "whole-run average instead of the benchmark line")
+    ap.add_argument("--traj-steps", type=int, default=0,
+                    help="also run the precision check: the configuration's own system (its sigma_B) from the GF2 "
+                         "equilibrium through the quench for this many steps in FP64, FP32 and bf16x3 in lockstep")
+    ap.add_argument("--traj-every", type=int, default=50, help="checkpoint interval of the precision check")
+    ap.add_argument("--traj-b-sd", type=float, default=None, help="sigma_B of the precision check's system")
+    ap.add_argument("--traj-only", action="store_true", help="print only the precision check's JSON line")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -224,6 +340,19 @@ def main():
 
     import nedyson_amd as ned
     from nedyson_amd import problem
+
+    traj = None
+    if args.traj_steps > 0:
+        ts = time.perf_counter()
+        pt = problem.config(args.config, nt=args.traj_steps,
+                            **({"b_sd": args.traj_b_sd} if args.traj_b_sd is not None else {}))
+        log(f"[traj] problem c{args.config}: nao={pt.nao} naux={pt.naux} sigma_B={pt.b_sd} "
+            f"({time.perf_counter() - ts:.1f} s)")
+        traj = precision_trajectory(ned, pt, args.traj_steps, args.traj_every, args.corr, log)
+        if args.traj_only:
+            print(json.dumps({"metric": "max-abs G error vs FP64 along a propagated run", "config": args.config,
+                              "nao": pt.nao, "naux": pt.naux, "nL": pt.ntau + 1, "dt": pt.dt, **traj}), flush=True)
+            return
 
     t_setup = time.perf_counter()
     over = {}
@@ -347,6 +476,8 @@ def main():
                                f"per exchange, avg {ugemm_ms:.3f} ms over {prof['ugemm_n']} exchanges (each the "
                                f"symmetric and antisymmetric launch at nao=64)"},
     }
+    if traj is not None:
+        line["precision_run"] = traj
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, T0 + W + 1, args.corr, log)
     print(json.dumps(line), flush=True)

@@ -86,6 +86,10 @@ struct Ctx {
   // under (m,q) -> (q,m): the pair table (m << 8 | q) and the two right operands
   bool symx = false;
   DevBuf sym_pairs, Ush, Usl, Uxh, Uxl;
+  // the same split on the FP64 / FP32 GEMM path (nao >= 8): its right operands
+  // in FP64 and FP32 (both parts, row-major K x N blocks)
+  bool symg = false;
+  DevBuf Usg64, Usg32;
   bool have_coulomb = false;
   // Sigma scratch: the buffers of one chunk of time points, carved from one
   // allocation

@@ -300,9 +300,126 @@ void launch(K kern, const GemmDesc &d, hipStream_t s) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- large real
+// C = alpha A B (+ C) for one large real product in FP64 or FP32 MFMA: A (M x K)
+// row-major with unit K stride, B (K x N) row-major with unit N stride.  128 x
+// 128 tiles of four waves (64 x 64 each, 4 x 4 MFMA blocks of 16 x 16), K in
+// slices of 16 with two LDS stages: the next slice is fetched into registers
+// (two-element loads) while the current one is consumed, one barrier per
+// slice.  Tiles are dealt in groups of 8 row tiles so that workgroups running
+// together share the B tiles in L2.  K, N, the leading dimensions and the
+// operand addresses must be even (element pairs never straddle an edge).
+template <typename T>
+__global__ void __launch_bounds__(NT) kern_gemm_big(const T *__restrict__ A, long long lda, const T *__restrict__ B,
+                                                    long long ldb, T *C, long long ldc, int M, int N, int K, T alpha,
+                                                    int beta) {
+  constexpr int TM = 128, TN = 128, BK = 16;
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  using V = typename std::conditional<sizeof(T) == 8, v4d, v4f>::type;
+  __shared__ T As[2][TM][BK + 2];
+  __shared__ T Bs[2][BK][TN + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tilesM = (M + TM - 1) / TM, tilesN = (N + TN - 1) / TN;
+  const int per_group = 8 * tilesN, grp = blockIdx.x / per_group, first = grp * 8;
+  const int gsz = min(8, tilesM - first), in = blockIdx.x % per_group;
+  const int m0 = (first + in % gsz) * TM, n0 = (in / gsz) * TN;
+  V acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = V{0, 0, 0, 0};
+  V2 ra[4], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      const int e = tid + NT * it;
+      const int r = e >> 3, kp = (e & 7) * 2;
+      const bool ok = m0 + r < M && k0 + kp < K;
+      ra[it] = ok ? *reinterpret_cast<const V2 *>(A + (long long)(m0 + r) * lda + k0 + kp) : V2{0, 0};
+      const int kk = e >> 6, np = (e & 63) * 2;
+      const bool okb = k0 + kk < K && n0 + np < N;
+      rb[it] = okb ? *reinterpret_cast<const V2 *>(B + (long long)(k0 + kk) * ldb + n0 + np) : V2{0, 0};
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      const int e = tid + NT * it;
+      const int r = e >> 3, kp = (e & 7) * 2;
+      As[buf][r][kp] = ra[it].x;
+      As[buf][r][kp + 1] = ra[it].y;
+      const int kk = e >> 6, np = (e & 63) * 2;
+      *reinterpret_cast<V2 *>(&Bs[buf][kk][np]) = rb[it];
+    }
+  };
+  const int nsl = (K + BK - 1) / BK;
+  fetch(0);
+  stage(0);
+  __syncthreads();
+  for (int s = 0; s < nsl; s++) {
+    const int cur = s & 1;
+    if (s + 1 < nsl) fetch((s + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[cur][wm * 64 + i * 16 + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[cur][ks + (lane >> 4)][wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if constexpr (sizeof(T) == 8) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+          else acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (s + 1 < nsl) stage(cur ^ 1);
+    __syncthreads();
+  }
+  // register q of a lane: f64 D[4q + lane/16][lane%16], f32 D[4(lane/16) + q][lane%16]
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int rin = sizeof(T) == 8 ? 4 * q + (lane >> 4) : 4 * (lane >> 4) + q;
+        const int row = m0 + wm * 64 + i * 16 + rin, col = n0 + wn * 64 + j * 16 + (lane & 15);
+        if (row < M && col < N) {
+          T *o = C + (long long)row * ldc + col;
+          const T v = alpha * acc[i][j][q];
+          *o = beta ? v + *o : v;
+        }
+      }
+}
+
+// the large-product kernel applies: one real product, unit K stride in A and
+// unit N stride in B, even sizes and addresses, big enough to fill the chip
+bool big_ok(const GemmDesc &d, size_t es) {
+  const auto even = [es](const void *p) { return ((uintptr_t)p % (2 * es)) == 0; };
+  return !d.ca && !d.cb && d.batch == 1 && d.A.ks_lo == 1 && d.A.kdiv == 0 && d.B.ms == 1 && d.B.kdiv == 0 &&
+         d.C.cs == 1 && d.K % 2 == 0 && d.N % 2 == 0 && d.A.ms % 2 == 0 && d.B.ks_lo % 2 == 0 &&
+         (long long)d.M * d.N >= 128LL * 128 * 64 && d.K >= 64 && even(d.A.ptr) && even(d.B.ptr);
+}
+
+template <typename T>
+void launch_big(const GemmDesc &d, hipStream_t s) {
+  const int tiles = ((d.M + 127) / 128) * ((d.N + 127) / 128);
+  hipLaunchKernelGGL(kern_gemm_big<T>, dim3(tiles), dim3(NT), 0, s, (const T *)d.A.ptr, d.A.ms, (const T *)d.B.ptr,
+                     d.B.ks_lo, (T *)d.C.ptr, d.C.rs, d.M, d.N, d.K, (T)d.alpha, d.beta);
+}
+
 void gemm(const GemmDesc &d, Prec prec, hipStream_t s) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   if (d.K <= 0) throw std::runtime_error("gemm: K must be positive");
+  if (prec != PREC_BF16X3 && big_ok(d, prec == PREC_F64 ? 8 : 4) && !std::getenv("NED_NO_BIG_GEMM")) {
+    if (prec == PREC_F64) launch_big<double>(d, s);
+    else launch_big<float>(d, s);
+    check_launch("gemm (large)");
+    return;
+  }
   const int key = (d.ca ? 2 : 0) + (d.cb ? 1 : 0);
   switch (prec) {
     case PREC_F64:

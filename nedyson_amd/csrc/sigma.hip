@@ -118,6 +118,88 @@ __global__ void kern_build_Usym(const double *U, const double *Bq, const uint16_
   lo[idx] = f2bf(f - bf2f(h));
 }
 
+// The split exchange of the FP64 / FP32 path (and of bf16x3 at shapes without
+// the split-bf16 GEMMs).  Pair indices follow the table of kern_sym_pairs:
+// pairs k <= j row-major over the upper triangle, then pairs k < j.
+__device__ __forceinline__ int pair_sym(int n, int k, int j) { return k * n - k * (k - 1) / 2 + (j - k); }
+__device__ __forceinline__ int pair_anti(int n, int k, int j) { return k * (n - 1) - k * (k - 1) / 2 + (j - k - 1); }
+
+// Right operand, (ns + a + na) x ... as two row-major blocks: first the
+// (ns + a) x ns block of the symmetric part, K rows (m,q) pairs m <= q then the
+// Pi columns b, N columns the output pairs (k,j), k <= j:
+//   -(U'[(m,q)][(k,j)] + U'[(m,q)][(j,k)]) / 2,   2 B[b][k][j];
+// then the na x na block of the antisymmetric part over pairs m < q, k < j:
+//   -(U'[(m,q)][(k,j)] - U'[(m,q)][(j,k)]) / 2.
+template <typename T>
+__global__ void kern_build_Usym_gen(const double *U, const double *Bq, const uint16_t *pairs, int n, int a, T *out) {
+  const long long ns = n * (n + 1) / 2, na = n * (n - 1) / 2, n2 = (long long)n * n;
+  const long long nsym = (ns + a) * ns;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nsym + na * na) return;
+  double v;
+  if (idx < nsym) {
+    const long long c = idx / ns, o = idx % ns;
+    const int kj = pairs[o], k = kj >> 8, j = kj & 255;
+    if (c < ns) {
+      const int mq = pairs[c], m = mq >> 8, q = mq & 255;
+      const double *Ur = U + (size_t)(m * n + q) * n2;
+      v = -0.5 * (Ur[k * n + j] + Ur[j * n + k]);
+    } else {
+      v = 2.0 * Bq[(size_t)(c - ns) * n2 + k * n + j];
+    }
+  } else {
+    const long long c = (idx - nsym) / na, o = (idx - nsym) % na;
+    const int kj = pairs[ns + o], k = kj >> 8, j = kj & 255;
+    const int mq = pairs[ns + c], m = mq >> 8, q = mq & 255;
+    const double *Ur = U + (size_t)(m * n + q) * n2;
+    v = -0.5 * (Ur[k * n + j] - Ur[j * n + k]);
+  }
+  out[idx] = (T)v;
+}
+
+// rows r = (item, plane, x) of C^x (n x n each, X) and Pi (planar per item)
+// -> XF rows [X_mq + X_qm (m <= q; X_mm) | Pi[plane][x][b] | X_mq - X_qm (m < q)]
+template <typename T>
+__global__ void kern_fold_pairs(const T *X, const T *Pi, const uint16_t *pairs, int n, int a, long long rows, T *XF) {
+  const int ns = n * (n + 1) / 2, na = n * (n - 1) / 2;
+  const long long kx = ns + a + na, n2 = (long long)n * n;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * kx) return;
+  const long long r = idx / kx;
+  const int c = (int)(idx % kx);
+  const T *xr = X + r * n2;
+  T v;
+  if (c < ns) {
+    const int mq = pairs[c], m = mq >> 8, q = mq & 255;
+    v = m == q ? xr[m * n + m] : xr[m * n + q] + xr[q * n + m];
+  } else if (c < ns + a) {
+    const long long p = r / (2 * a), pl = (r / a) % 2, x = r % a;
+    v = Pi[(p * 2 + pl) * a * a + x * a + (c - ns)];
+  } else {
+    const int mq = pairs[ns + (c - ns - a)], m = mq >> 8, q = mq & 255;
+    v = xr[m * n + q] - xr[q * n + m];
+  }
+  XF[idx] = v;
+}
+
+// H rows (n x n, row-major (k,j)) from Y rows [Cs (ns) | Ca (na)]:
+// H_kj = Cs + Ca, H_jk = Cs - Ca (k < j), H_kk = Cs
+template <typename T>
+__global__ void kern_unfold_pairs(const T *Y, int n, long long rows, T *H) {
+  const int ns = n * (n + 1) / 2, na = n * (n - 1) / 2;
+  const long long n2 = (long long)n * n;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * n2) return;
+  const long long r = idx / n2;
+  const int e = (int)(idx % n2), k = e / n, j = e % n;
+  const T *y = Y + r * (ns + na);
+  T v;
+  if (k == j) v = y[pair_sym(n, k, k)];
+  else if (k < j) v = y[pair_sym(n, k, j)] + y[ns + pair_anti(n, k, j)];
+  else v = y[pair_sym(n, j, k)] - y[ns + pair_anti(n, j, k)];
+  H[idx] = v;
+}
+
 // Pi of every item from the real product Pf = [Wr; Wi] [Zr; Zi]^T (2a x 2a):
 // Pi_r = Pf_rr - Pf_ii, Pi_i = Pf_ri + Pf_ir, split into bf16 and written to
 // rows (item, plane, a), columns col0 + b of the exchange GEMM's operand A
@@ -235,6 +317,7 @@ __global__ void kern_sum_scatter(const T *part, int S, int p0, int P, int T_, in
 template <typename T>
 struct Bufs {
   T *A, *B, *Bt, *W, *X, *H, *Pi, *Ep;
+  T *XF;                // folded exchange operand (split exchange, generic path)
   uint16_t *S1h, *S1l;  // split-bf16 operands (bf16x3 path)
   float *Pf;            // [Wr; Wi][Zr; Zi]^T per item (bf16x3 path)
   bool tune;            // let the split-bf16 GEMMs time their tile variants
@@ -368,6 +451,40 @@ void run_chunk(Ctx &c, Prec prec, int T_, int p0, int P, const Bufs<T> &bf, cons
                    (float *)bf.H, n2, 0, M, NN, (int)ka, 1, 1.0f, 0, s, bf.tune);
     c.mark(1);
     if (c.prof) c.prof_flops += 2.0 * M * (double)NN * (double)ka;
+  } else if (c.symg) {
+    // the exchange split by the pair symmetry of U' (kern_build_Usym_gen):
+    // fold C and Pi into the rows [xs | Pi | xa], two GEMMs into Y = [Cs | Ca]
+    // (in X, C being folded), then H^a unfolded from Y
+    const int ns = n * (n + 1) / 2, na = n * (n - 1) / 2;
+    const long long kx = ns + a + na, ly = ns + na;
+    const long long tot = (long long)M * kx;
+    hipLaunchKernelGGL(kern_fold_pairs<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.X, bf.Pi,
+                       c.sym_pairs.as<uint16_t>(), n, a, (long long)M, bf.XF);
+    check_launch("sigma fold pairs");
+    const T *Usg = sizeof(T) == 8 ? (const T *)c.Usg64.p : (const T *)c.Usg32.p;
+    c.mark(0);
+    {
+      GemmDesc d;
+      d.M = M; d.N = ns; d.K = ns + a; d.batch = 1;
+      d.A.ptr = bf.XF; d.A.ms = kx; d.A.ks_lo = 1;
+      d.B.ptr = Usg; d.B.ms = 1; d.B.ks_lo = ns;
+      d.C.ptr = bf.X; d.C.rs = ly; d.C.cs = 1;
+      gemm(d, prec, s);
+    }
+    {
+      GemmDesc d;
+      d.M = M; d.N = na; d.K = na; d.batch = 1;
+      d.A.ptr = bf.XF + ns + a; d.A.ms = kx; d.A.ks_lo = 1;
+      d.B.ptr = Usg + (size_t)(ns + a) * ns; d.B.ms = 1; d.B.ks_lo = na;
+      d.C.ptr = bf.X + ns; d.C.rs = ly; d.C.cs = 1;
+      gemm(d, prec, s);
+    }
+    c.mark(1);
+    if (c.prof) c.prof_flops += 2.0 * M * ((double)ns * (ns + a) + (double)na * na);
+    const long long toth = (long long)M * n2;
+    hipLaunchKernelGGL(kern_unfold_pairs<T>, dim3((unsigned)((toth + 255) / 256)), dim3(256), 0, s, bf.X, n,
+                       (long long)M, bf.H);
+    check_launch("sigma unfold pairs");
   } else {
     // H = -C U'   (rows (p, plane, a) of X, all at once; the n^4 naux GEMM)
     {
@@ -447,6 +564,7 @@ size_t sigma_carve(const Ctx &c, Prec prec, int P, char *base, Bufs<T> *bf) {
     b.Pf = (float *)take(Pz * 4 * a * a * sizeof(float));
   } else {
     b.Pi = (T *)take(Pz * 2 * a * a * sizeof(T));
+    if (c.symg) b.XF = (T *)take(Pz * 2 * a * (n2 + a) * sizeof(T));  // ns + a + na = n^2 + a columns
   }
   b.tune = true;
   if (bf) *bf = b;
@@ -604,15 +722,33 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
     split_bf16(c.Bq32.as<float>(), (long long)n2, c.Vh.as<uint16_t>(), c.Vl.as<uint16_t>(), (long long)n2, a, (int)n2,
                c.stream);
   }
-  if (c.symx) {
-    const SymGeom sg = sym_geom(n, a);
-    // kern_c64_c zeroes 32 padding columns after each block (naux % 64 == 0)
-    require(sg.ks - sg.ns - a == 32 && sg.ka - sg.na == 32, "split exchange: unexpected padding");
-    const int npair = sg.ns + sg.na;
+  // the split on the FP64 / FP32 path (and bf16x3 without the split-bf16
+  // GEMMs) from nao = 8 on
+  c.symg = !(sx && std::atoi(sx) == 0) && n >= 8;
+  if (c.symx || c.symg) {
+    const int npair = n * (n + 1) / 2 + n * (n - 1) / 2;
     c.sym_pairs.ensure((size_t)npair * sizeof(uint16_t));
     hipLaunchKernelGGL(kern_sym_pairs, dim3((npair + 255) / 256), dim3(256), 0, c.stream, n,
                        c.sym_pairs.as<uint16_t>());
     check_launch("coulomb pairs");
+  }
+  if (c.symg) {
+    const long long ns = n * (n + 1) / 2, na = n * (n - 1) / 2, tot = (ns + a) * ns + na * na;
+    c.Usg64.ensure((size_t)tot * sizeof(double));
+    c.Usg32.ensure((size_t)tot * sizeof(float));
+    hipLaunchKernelGGL(kern_build_Usym_gen<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream,
+                       c.U64.as<double>(), c.Bq.as<double>(), c.sym_pairs.as<uint16_t>(), n, a, c.Usg64.as<double>());
+    hipLaunchKernelGGL(kern_build_Usym_gen<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream,
+                       c.U64.as<double>(), c.Bq.as<double>(), c.sym_pairs.as<uint16_t>(), n, a, c.Usg32.as<float>());
+    check_launch("coulomb split U (FP64 / FP32)");
+  } else {
+    c.Usg64.release();
+    c.Usg32.release();
+  }
+  if (c.symx) {
+    const SymGeom sg = sym_geom(n, a);
+    // kern_c64_c zeroes 32 padding columns after each block (naux % 64 == 0)
+    require(sg.ks - sg.ns - a == 32 && sg.ka - sg.na == 32, "split exchange: unexpected padding");
     c.Ush.ensure((size_t)sg.nsp * sg.ks * sizeof(uint16_t));
     c.Usl.ensure((size_t)sg.nsp * sg.ks * sizeof(uint16_t));
     c.Uxh.ensure((size_t)sg.nap * sg.ka * sizeof(uint16_t));

@@ -176,6 +176,7 @@ class Problem:
         rng = np.random.default_rng(seed)
         self.nao, self.naux, self.nelec, self.beta = nao, naux, nelec, beta
         self.ntau, self.nt, self.dt, self.k, self.seed = ntau, nt, dt, k, seed
+        self.b_sd, self.quench, self.quench_norm = b_sd, quench, quench_norm
         self.h1 = make_h1(rng, nao, spacing)
         self.B = make_B(rng, nao, naux, b_sd)
         self.V = np.ascontiguousarray(np.transpose(self.B, (1, 2, 0)))  # Vija(i,j,Q)
