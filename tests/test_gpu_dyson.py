@@ -179,23 +179,6 @@ def test_full_run_parity_f64(orc):
         assert _relerr(getattr(ds, comp), getattr(S, comp)) < 1e-10, comp
 
 
-@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16x3", 1e-4)])
-def test_mixed_precision_run_close_to_f64(prec, tol):
-    """Sigma contractions in FP32 / bf16x3, Dyson in FP64: G stays within tol
-    of the all-FP64 run (max-abs over all stored blocks)."""
-    p = _small_problem(nt=24)
-    outs = {}
-    for pr in ("f64", prec):
-        with _ctx_from(p, p.initial_green()) as c:
-            c.do_boot(p.fock, p.mu, p.beta, p.dt, pr, 50, 1e-12)
-            for T in range(p.k + 1, p.nt + 1):
-                c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, pr, 3)
-            outs[pr] = c.download(ned.NED_G)
-    for comp in ("ret", "les", "tv"):
-        err = np.max(np.abs(getattr(outs[prec], comp) - getattr(outs["f64"], comp)))
-        assert err < tol, (comp, err)
-
-
 def test_g0_from_h0_gpu(orc):
     """ned_g0_from_h0 == oracle G0_from_h0 for t <= k, and == the closed-form
     free propagator for every later t."""
