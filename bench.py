@@ -43,6 +43,13 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0 / 3}
 # window stays smooth (corrector update ~1e-3, particle number constant to
 # 1e-7).  The work per step does not depend on the value.
 STABLE_B_SD = {2: 0.01}
+# Coulomb-factor spread of the propagated precision check (the whole run from
+# the GF2 equilibrium through the quench).  At the configuration's 0.1 the
+# equal-time retarded self-energy reaches |Sigma^R(0,0)_ij| = 2.8e3, so
+# dt |Sigma| = 28 at dt = 0.01: the FP64 corrector update grows from 3e4 to
+# 1e100 in four passes of the first step after the bootstrap (0.07: N leaves
+# 32 by 0.24 and the run overflows at step 41).  0.05 propagates.
+TRAJ_B_SD = {2: 0.05}
 
 
 def gf2_equilibrium(ned, p, log):
@@ -96,7 +103,21 @@ def cpu_baseline(p, T, corr, log):
     les_flops = 12 * n ** 4 * a + 36 * n ** 3 * a + 12 * n ** 2 * a ** 2
     rate = les_flops / times["les"]
     t_sigma = (T + 1) * (times["les"] + times["ret"]) + (p.ntau + 1) * times["tv"]
-    t_dyson = dyson_flops(n, T, p.ntau + 1) / rate
+    # one oracle Dyson step (dyson_step.h:329-382 restated) at the same nao and
+    # nodes with a history of Td steps, single-threaded, scaled to T by the
+    # step's flop count 16 T^2 n^3 + 16 T Ntau n^3 + 4 Ntau^2 n^3
+    Td = int(os.environ.get("NED_CPU_DYSON_T", 8))
+    Gd, Sd = rand_green(Td, p.ntau), rand_green(Td, p.ntau)
+    for arr in (Sd.ret, Sd.les, Sd.tv):
+        arr *= 0.01
+    Gd.mat[:] = p.Gmat
+    hmf = np.repeat(p.fock[None].astype(complex), Td + 1, axis=0)
+    D = O.Dyson(Td, p.ntau, n, p.k)
+    t0 = time.perf_counter()
+    D.dyson_step(Td, Gd, Sd, hmf, p.mu, p.beta, p.dt)
+    t_dyson_Td = time.perf_counter() - t0
+    t_dyson = t_dyson_Td * dyson_flops(n, T, p.ntau + 1) / dyson_flops(n, Td, p.ntau + 1)
+    log(f"[cpu] Dyson step at T={Td}, 1 thread: {t_dyson_Td:.2f} s; scaled to T={T}: {t_dyson:.1f} s")
     t_step1 = corr * (t_sigma + t_dyson)
 
     # OpenMP over the time points of a slice with about one point per thread
@@ -110,18 +131,29 @@ def cpu_baseline(p, T, corr, log):
     speedup = serial / t_omp
     log(f"[cpu] Sigma slice T={Ts}, ntau+1={ntau_s + 1} ({2 * Ts + 2 + ntau_s + 1} points), {threads} threads: "
         f"{t_omp:.2f} s (x{speedup:.2f} over 1 thread)")
-    t_stepN = t_step1 / speedup
+    # Sigma spread over the threads as measured; the oracle's Dyson step runs on one
+    t_stepN = corr * (t_sigma / speedup + t_dyson)
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": 1.0 / t_stepN, "unit": "timesteps/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "nproc": os.cpu_count(),
             "sample": f"CPU oracle (reference decomposition restated, C++): one lesser, one retarded and one mixing "
                       f"Sigma time point at nao={n}, naux={a} on 1 thread ({sum(times.values()):.1f} s), then a "
                       f"slice of {2 * Ts + 2 + ntau_s + 1} time points with OpenMP over points on {threads} threads "
-                      f"({t_omp:.1f} s, x{speedup:.2f}); scaled to a timestep at T={T} with {corr} corrector passes "
-                      f"as (T+1)(t_les+t_ret)+Ntau*t_tv plus the Dyson flop model at the measured "
-                      f"{rate / 1e9:.1f} GFLOP/s per thread",
-            "value_1thread": 1.0 / t_step1, "sec_per_step": t_stepN, "gflops_1thread": rate / 1e9}
+                      f"({t_omp:.1f} s, x{speedup:.2f}); one Dyson step at T={Td} on 1 thread ({t_dyson_Td:.1f} s); "
+                      f"scaled to a timestep at T={T} with {corr} corrector passes as (T+1)(t_les+t_ret)+Ntau*t_tv "
+                      f"(Sigma, over the threads) plus the Dyson step scaled by its flop count (T^2 history terms)",
+            "value_1thread": 1.0 / t_step1, "sec_per_step": t_stepN, "gflops_1thread": rate / 1e9,
+            "dyson_s_at_T": t_dyson}
 
 
-def precision_trajectory(ned, p, nsteps, every, corr, log, precs=("f32", "bf16x3")):
+def precision_trajectory(ned, p, nsteps, every, corr, log, precs=("f32", "bf16x3"), probe_passes=0):
     """The whole run from the equilibrium GF2 state (ned_matsubara_gf2) through
     the quench, stepped in lockstep in FP64 and in each mixed precision for
     nsteps time steps (bootstrap included, each in its own precision).  After
@@ -157,6 +189,25 @@ def precision_trajectory(ned, p, nsteps, every, corr, log, precs=("f32", "bf16x3
             log(f"[traj] {pr} bootstrap stopped at its iteration limit: {e}")
         ctxs[pr] = c
     n, np1 = p.nao, p.ntau + 1
+    if probe_passes:
+        # the corrector passes of the first step after the bootstrap, one by
+        # one, with their update norms (FP64)
+        c, T, passes = ctxs["f64"], p.k + 1, []
+        c.extrapolate(T)
+        for q in range(probe_passes):
+            c.hf(T, p.h0)
+            c.sigma(T, "f64")
+            try:
+                d = c.dyson_step(T, mu, p.beta, p.dt)
+            except ned.NedError as ex:
+                log(f"[probe] T={T} pass {q + 1}: {ex}")
+                passes.append(None)
+                break
+            passes.append([float(x) for x in d])
+            log(f"[probe] T={T} pass {q + 1}: update {d.sum():.3e} (ret {d[0]:.2e} tv {d[1]:.2e} les {d[2]:.2e})")
+        for cc in ctxs.values():
+            cc.close()
+        return {"sigma_B": p.b_sd, "dt": p.dt, "mu": mu, "probe_T": T, "pass_updates": passes}
 
     def slice_of(c, T):
         s = T * (T + 1) // 2
@@ -309,11 +360,18 @@ def main():
     ap.add_argument("--sweep", default=None,
                     help="comma-separated history lengths T0: print per-step times at each and the fitted "
                          "whole-run average instead of the benchmark line")
-    ap.add_argument("--traj-steps", type=int, default=0,
+    ap.add_argument("--traj-steps", type=int, default=100,
                     help="also run the precision check: the configuration's own system (its sigma_B) from the GF2 "
                          "equilibrium through the quench for this many steps in FP64, FP32 and bf16x3 in lockstep")
     ap.add_argument("--traj-every", type=int, default=50, help="checkpoint interval of the precision check")
     ap.add_argument("--traj-b-sd", type=float, default=None, help="sigma_B of the precision check's system")
+    ap.add_argument("--traj-dt", type=float, default=None, help="time step of the precision check's run")
+    ap.add_argument("--traj-quench-norm", type=float, default=None,
+                    help="quench norm of the precision check's run (0: the equilibrium propagated unperturbed)")
+    ap.add_argument("--traj-f64-only", action="store_true", help="precision check: the FP64 run alone")
+    ap.add_argument("--probe-passes", type=int, default=0,
+                    help="instead of the precision check, log the FP64 corrector update of each of this many "
+                         "passes of the first step after the bootstrap")
     ap.add_argument("--traj-only", action="store_true", help="print only the precision check's JSON line")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (the reference kernel for "
@@ -344,11 +402,18 @@ def main():
     traj = None
     if args.traj_steps > 0:
         ts = time.perf_counter()
-        pt = problem.config(args.config, nt=args.traj_steps,
-                            **({"b_sd": args.traj_b_sd} if args.traj_b_sd is not None else {}))
+        bsd = args.traj_b_sd if args.traj_b_sd is not None else TRAJ_B_SD.get(args.config)
+        over = {"b_sd": bsd} if bsd is not None else {}
+        if args.traj_dt is not None:
+            over["dt"] = args.traj_dt
+        if args.traj_quench_norm is not None:
+            over["quench_norm"] = args.traj_quench_norm
+        pt = problem.config(args.config, nt=args.traj_steps, **over)
         log(f"[traj] problem c{args.config}: nao={pt.nao} naux={pt.naux} sigma_B={pt.b_sd} "
             f"({time.perf_counter() - ts:.1f} s)")
-        traj = precision_trajectory(ned, pt, args.traj_steps, args.traj_every, args.corr, log)
+        traj = precision_trajectory(ned, pt, args.traj_steps, args.traj_every, args.corr, log,
+                                    precs=() if args.traj_f64_only else ("f32", "bf16x3"),
+                                    probe_passes=args.probe_passes)
         if args.traj_only:
             print(json.dumps({"metric": "max-abs G error vs FP64 along a propagated run", "config": args.config,
                               "nao": pt.nao, "naux": pt.naux, "nL": pt.ntau + 1, "dt": pt.dt, **traj}), flush=True)
@@ -454,12 +519,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": f"{args.prec} Sigma contractions, f64 Dyson",
-        "data": f"synthetic (seeded h1, B[Q,i,j]; {args.initial.upper()} equilibrium; history = free G of its Fock "
-                "matrix)",
+        "data": f"synthetic (seeded h1, B[Q,i,j] ~ N(0, {p.b_sd}); {args.initial.upper()} equilibrium; timed window: "
+                "history = free G of its Fock matrix; precision_run: whole run from the GF2 equilibrium of the "
+                "same generators at its own sigma_B)",
         "config": {"workload": f"GF2 KBE timestep at T={T0 + W + 1}..{nt}, nao={n}, naux={a}, "
                                f"ntau+1={p.ntau + 1}, dt={p.dt}, k={p.k}, {args.corr} corrector passes",
                    "nao": n, "naux": a, "nL": p.ntau + 1, "T": T0 + W + 1, "nt_run": p.nt,
-                   "corrector_passes": args.corr, "sigma_prec": args.prec},
+                   "corrector_passes": args.corr, "sigma_prec": args.prec, "window_sigma_B": p.b_sd,
+                   "precision_run_sigma_B": traj["sigma_B"] if traj else None,
+                   "precision_run_steps": traj["steps"] if traj else None,
+                   "speedup_basis": "FP64 run uses the same split-exchange algorithm (precision-only ratio)"},
         "f64_value": f64_value,
         "speedup_vs_f64": (value / f64_value) if f64_value else None,
         "max_abs_G_err_vs_f64": max_err,
