@@ -95,6 +95,15 @@ ned_status ned_hmf_read(ned_ctx *ctx, int first, int count, void *host);
  * stored in FP64 in every case. */
 ned_status ned_sigma_gf2_decomp(ned_ctx *ctx, int tstp, int prec);
 
+/* molGF2Solver::solve(tstp, Sigma, G) (src/mol2bSolver_impl.h:5-120, 122;
+ * class at src/mol2bSolver.h:178-209): the same slices as
+ * ned_sigma_gf2_decomp from the full, non-decomposed interaction
+ * U_ijkl = sum_Q B[Q][i][j] B[Q][k][l] and the exchange-combined
+ * 2 U_{kj,qm} - U_{qj,km} (the reference's Uijkl_exch), both built on the
+ * device from the context's Cholesky factor on first use (n^4 each):
+ * 24 nao^5 flop per time point in four batched products. */
+ned_status ned_sigma_gf2_full(ned_ctx *ctx, int tstp, int prec);
+
 /* ned_sigma_gf2_decomp with no device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The

@@ -103,6 +103,7 @@ class GREEN {
   int which_;
   int nt_, ntau_, nao_, sig_;
   friend class molGF2SolverDecomp;
+  friend class molGF2Solver;
   friend class dyson;
 };
 
@@ -192,6 +193,26 @@ class molGF2SolverDecomp {
   void solve_HF(int tstp, Hmf &hmf, const double *h0) const {
     detail::same(hmf.owner(), sys_, "hmf");
     check(ned_hf_decomp(sys_->ctx(), tstp, h0));
+  }
+
+ private:
+  System *sys_;
+};
+
+// molGF2Solver (src/mol2bSolver.h:178-209): the full, non-decomposed second-Born
+// self-energy, contracted through U_ijkl = sum_Q V_ijQ V_klQ (built on the
+// device from the System's Cholesky factor, which must be set: construct a
+// molGF2SolverDecomp first, or call ned_set_coulomb).
+class molGF2Solver {
+ public:
+  explicit molGF2Solver(System &sys) : sys_(&sys) {}
+  // solve(tstp, Sigma, G) (src/mol2bSolver_impl.h:5-120, 122)
+  void solve(int tstp, GREEN &Sigma, const GREEN &G, int prec = NED_PREC_F64) const {
+    detail::same(Sigma.owner(), sys_, "Sigma");
+    detail::same(G.owner(), sys_, "G");
+    if (Sigma.which_ != NED_SIGMA || G.which_ != NED_G)
+      throw Error(NED_ERR_ARG, "nedyson_amd: solve(tstp, Sigma, G) takes the System's Sigma and G");
+    check(ned_sigma_gf2_full(sys_->ctx(), tstp, prec));
   }
 
  private:

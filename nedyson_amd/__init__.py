@@ -29,7 +29,8 @@ _STATUS = {0: "NED_OK", 1: "NED_ERR_ARG", 2: "NED_ERR_NOMEM", 3: "NED_ERR_DEVICE
 
 # every symbol declared in include/nedyson.h
 EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "ned_ctx_get_dims", "ned_set_coulomb", "ned_green_write",
-           "ned_green_read", "ned_hmf_write", "ned_hmf_read", "ned_sigma_gf2_decomp", "ned_sigma_gf2_decomp_noalloc",
+           "ned_green_read", "ned_hmf_write", "ned_hmf_read", "ned_sigma_gf2_decomp", "ned_sigma_gf2_full",
+           "ned_sigma_gf2_decomp_noalloc",
            "ned_sigma_noalloc_scratch_bytes", "ned_hf_decomp",
            "ned_set_matsubara", "ned_matsubara_gf2", "ned_dyson_step", "ned_extrapolate", "ned_dyson_start", "ned_do_boot", "ned_do_tstp",
            "ned_energy", "ned_g0_from_h0", "ned_tti_extend", "ned_profile", "ned_profile_read", "ned_gll_nodes",
@@ -79,6 +80,7 @@ def lib():
     L.ned_hmf_read.argtypes = [_vp, C.c_int, C.c_int, _vp]
     L.ned_sigma_gf2_decomp.argtypes = [_vp, C.c_int, C.c_int]
     L.ned_sigma_gf2_decomp_noalloc.argtypes = [_vp, C.c_int, C.c_int]
+    L.ned_sigma_gf2_full.argtypes = [_vp, C.c_int, C.c_int]
     L.ned_sigma_noalloc_scratch_bytes.restype = C.c_size_t
     L.ned_hf_decomp.argtypes = [_vp, C.c_int, _dp]
     L.ned_set_matsubara.argtypes = [_vp, C.c_double]
@@ -240,6 +242,11 @@ class Context:
     # ---- hot path
     def sigma(self, tstp, prec="f64"):
         _check(lib().ned_sigma_gf2_decomp(self.h, tstp, prec_code(prec)))
+
+    def sigma_full(self, tstp, prec="f64"):
+        """molGF2Solver::solve: the same slice of Sigma from the full
+        (non-decomposed) four-index interaction."""
+        _check(lib().ned_sigma_gf2_full(self.h, tstp, prec_code(prec)))
 
     def sigma_noalloc(self, tstp, prec="f64"):
         """ned_sigma_gf2_decomp_noalloc: the same slice of Sigma with no device

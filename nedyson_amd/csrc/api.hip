@@ -271,6 +271,16 @@ ned_status ned_sigma_gf2_decomp_noalloc(ned_ctx *ctx, int tstp, int prec) {
 
 size_t ned_sigma_noalloc_scratch_bytes(void) { return ned::sigma_scratch_bytes(); }
 
+ned_status ned_sigma_gf2_full(ned_ctx *ctx, int tstp, int prec) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  return guard(ctx, [&] {
+    ned::require(prec >= 0 && prec <= 2, "prec must be NED_PREC_F64, _F32 or _BF16X3");
+    state(ctx->c.have_coulomb, "interaction not set (ned_set_coulomb)");
+    ned::sigma_gf2_full(ctx->c, tstp, (ned::Prec)prec);
+    ned::touch_time(ctx->c, tstp);
+  });
+}
+
 ned_status ned_hf_decomp(ned_ctx *ctx, int tstp, const double *h0) {
   if (!ctx || !h0) { g_err = "null argument"; return NED_ERR_ARG; }
   return guard(ctx, [&] {

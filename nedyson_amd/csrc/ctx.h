@@ -90,6 +90,9 @@ struct Ctx {
   // in FP64 and FP32 (both parts, row-major K x N blocks)
   bool symg = false;
   DevBuf Usg64, Usg32;
+  // the full (non-decomposed) Sigma: U4[(i,l)][(n,p)] = U_{il,np} and
+  // Wx[(k,q,m)][j] = 2 U_{kj,qm} - U_{qj,km} (built on first use)
+  DevBuf U4_64, U4_32, Wx64, Wx32;
   bool have_coulomb = false;
   // Sigma scratch: the buffers of one chunk of time points, carved from one
   // allocation
@@ -146,6 +149,9 @@ void sigma_gf2(Ctx &c, int tstp, Prec prec);
 // array of sigma_scratch_bytes() per GPU; sigma_scratch_init(device), with
 // that device current, looks up its address once per device
 void sigma_gf2_noalloc(Ctx &c, int tstp, Prec prec);
+// the same slice by the full (non-decomposed) solver, molGF2Solver::solve
+// (src/mol2bSolver_impl.h:5-120)
+void sigma_gf2_full(Ctx &c, int tstp, Prec prec);
 void sigma_scratch_init(int device);
 size_t sigma_scratch_bytes();
 void build_coulomb(Ctx &c, const double *Bq_host);

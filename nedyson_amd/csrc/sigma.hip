@@ -683,6 +683,7 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
         if (std::fabs(x - y) > 1e-12 * (1.0 + std::fabs(x)))
           throw std::invalid_argument("Coulomb factor B[Q,i,j] must be symmetric in (i,j)");
       }
+  for (DevBuf *b : {&c.U4_64, &c.U4_32, &c.Wx64, &c.Wx32}) b->release();  // the full solver's, rebuilt on use
   c.Bq.ensure(a * n2 * sizeof(double));
   c.Bq32.ensure(a * n2 * sizeof(float));
   c.U64.ensure(n2 * n2 * sizeof(double));
@@ -773,6 +774,119 @@ void sigma_gf2(Ctx &c, int tstp, Prec prec) {
   if (prec == PREC_F64) sigma_typed<double>(c, tstp, prec);
   else sigma_typed<float>(c, tstp, prec);
   c.mark(3);
+}
+
+// ---- the full (non-decomposed) second-Born self-energy, molGF2Solver::solve
+// (src/mol2bSolver_impl.h:5-120): the same functional F[A,B] of every item,
+// contracted through the four-index U instead of the Cholesky vectors,
+//   T1[i,k,n,p] = sum_l A_lk U_{il,np}        (per item and i: n x n^2 x n)
+//   T2[i,k,n,q] = sum_p T1[i,k,n,p] A_pq      (n^3 x n x n)
+//   T3[i,k,q,m] = sum_n T2[i,k,n,q] B_mn      (per (i,k): n x n x n)
+//   Sigma_ij    = sum_{kqm} T3[i,k,q,m] Wx[(k,q,m)][j],  Wx = 2 U_{kj,qm} - U_{qj,km}
+// (the reference's Uexch, src/mol2bSolver.h:44-48), 24 n^5 flop per item.
+
+namespace {
+// The two four-index operands of the full solver from U' as built by
+// kern_build_U, U'[(a,b)][(c,d)] = sum_Q B[Q][c][a] B[Q][b][d] = U_{ca,bd}:
+//   U4[(i,l)][(n,p)]  = U_{il,np} = U'[(l,n)][(i,p)]
+//   Wx[(k,q,m)][j]    = 2 U_{kj,qm} - U_{qj,km} = 2 U'[(j,q)][(k,m)] - U'[(j,k)][(q,m)]
+template <typename T>
+__global__ void kern_build_full_ops(const double *U, int n, T *U4, T *Wx) {
+  const long long n2 = (long long)n * n, n4 = n2 * n2;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n4) return;
+  {
+    const int p = (int)(idx % n), nn = (int)((idx / n) % n), l = (int)((idx / n2) % n), i = (int)(idx / (n2 * n));
+    U4[idx] = (T)U[(l * n + nn) * n2 + i * n + p];
+  }
+  const int j = (int)(idx % n);
+  const long long c = idx / n;
+  const int m = (int)(c % n), q = (int)((c / n) % n), k = (int)(c / n2);
+  Wx[idx] = (T)(2.0 * U[(j * n + q) * n2 + k * n + m] - U[(j * n + k) * n2 + q * n + m]);
+}
+
+template <typename T>
+void sigma_full_typed(Ctx &c, int tstp, Prec prec) {
+  const int n = c.nao, np = c.np;
+  const long long n2 = (long long)n * n, n3 = n2 * n, n4 = n2 * n2;
+  const int Ptot = 2 * (tstp + 1) + np;
+  const int S = n;  // K splits of the last product (K = n^3)
+  // per item: A, B, T1 (also T3), T2 (planar complex) and the S partials of Sigma
+  const size_t per_item = ((size_t)4 * n2 + 4 * n4 + (size_t)S * 2 * n2) * sizeof(T) + 4 * 256;
+  const size_t budget = (size_t)6 << 30;
+  int P = c.sigma_chunk > 0 ? c.sigma_chunk : (int)std::max<size_t>(1, budget / per_item);
+  P = std::min(P, Ptot);
+  const size_t bA = (size_t)P * 2 * n2 * sizeof(T), bT = (size_t)P * 2 * n4 * sizeof(T),
+               bE = (size_t)P * S * 2 * n2 * sizeof(T);
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  c.sg_arena.ensure(2 * up(bA) + 2 * up(bT) + up(bE));
+  char *base = c.sg_arena.as<char>();
+  T *A = (T *)base, *B = (T *)(base + up(bA)), *T1 = (T *)(base + 2 * up(bA)),
+    *T2 = (T *)(base + 2 * up(bA) + up(bT)), *E = (T *)(base + 2 * up(bA) + 2 * up(bT));
+  DevBuf &Wb = sizeof(T) == 8 ? c.Wx64 : c.Wx32;
+  DevBuf &Ub = sizeof(T) == 8 ? c.U4_64 : c.U4_32;
+  if (!Wb.p || !Ub.p) {
+    Wb.ensure((size_t)n4 * sizeof(T));
+    Ub.ensure((size_t)n4 * sizeof(T));
+    hipLaunchKernelGGL(kern_build_full_ops<T>, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, c.stream,
+                       c.U64.as<double>(), n, Ub.as<T>(), Wb.as<T>());
+    check_launch("sigma full operands");
+  }
+  const T *U = Ub.as<T>();
+  hipStream_t s = c.stream;
+  for (int p0 = 0; p0 < Ptot; p0 += P) {
+    const int Pc = std::min(P, Ptot - p0);
+    const long long Pn4 = (long long)P * n4;
+    long long tot = (long long)Pc * n2;
+    hipLaunchKernelGGL(kern_items<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, c.G.ret.as<z2>(),
+                       c.G.les.as<z2>(), c.G.tv.as<z2>(), tstp, n, np, c.sig, p0, Pc, A, B, 0, (T *)nullptr);
+    check_launch("sigma full items");
+    {  // T1[p][i][k][(n,p')] = sum_l A_lk U[(i,l)][(n,p')]
+      GemmDesc d;
+      d.M = n; d.N = (int)n2; d.K = n; d.batch = Pc * n; d.ca = true; d.cb = false;
+      d.A.ptr = A; d.A.ms = 1; d.A.ks_lo = n; d.A.ps = n2; d.A.bdiv = n; d.A.bs_hi = 2 * n2; d.A.bs_lo = 0;
+      d.B.ptr = U; d.B.ms = 1; d.B.ks_lo = n2; d.B.bdiv = n; d.B.bs_hi = 0; d.B.bs_lo = n3;
+      d.C.ptr = T1; d.C.rs = n2; d.C.cs = 1; d.C.ps = Pn4; d.C.bdiv = n; d.C.bs_hi = n4; d.C.bs_lo = n3;
+      gemm(d, prec, s);
+    }
+    {  // T2[p][(i,k,n)][q] = sum_p' T1[p][(i,k,n)][p'] A_p'q
+      GemmDesc d;
+      d.M = (int)n3; d.N = n; d.K = n; d.batch = Pc; d.ca = true; d.cb = true;
+      d.A.ptr = T1; d.A.ms = n; d.A.ks_lo = 1; d.A.ps = Pn4; d.A.bs_lo = n4;
+      d.B.ptr = A; d.B.ms = 1; d.B.ks_lo = n; d.B.ps = n2; d.B.bs_lo = 2 * n2;
+      d.C.ptr = T2; d.C.rs = n; d.C.cs = 1; d.C.ps = Pn4; d.C.bs_lo = n4;
+      gemm(d, prec, s);
+    }
+    {  // T3[p][(i,k)][q][m] = sum_n T2[p][(i,k)][n][q] B_mn   (into T1)
+      GemmDesc d;
+      d.M = n; d.N = n; d.K = n; d.batch = Pc * (int)n2; d.ca = true; d.cb = true;
+      d.A.ptr = T2; d.A.ms = 1; d.A.ks_lo = n; d.A.ps = Pn4; d.A.bdiv = (int)n2; d.A.bs_hi = n4; d.A.bs_lo = n2;
+      d.B.ptr = B; d.B.ms = n; d.B.ks_lo = 1; d.B.ps = n2; d.B.bdiv = (int)n2; d.B.bs_hi = 2 * n2; d.B.bs_lo = 0;
+      d.C.ptr = T1; d.C.rs = n; d.C.cs = 1; d.C.ps = Pn4; d.C.bdiv = (int)n2; d.C.bs_hi = n4; d.C.bs_lo = n2;
+      gemm(d, prec, s);
+    }
+    {  // partial Sigma_ij over K range s: sum_c T3[p][i][c] Wx[c][j], c in [s n^2, (s+1) n^2)
+      GemmDesc d;
+      d.M = n; d.N = n; d.K = (int)(n3 / S); d.batch = Pc * S; d.ca = true; d.cb = false;
+      d.A.ptr = T1; d.A.ms = n3; d.A.ks_lo = 1; d.A.ps = Pn4; d.A.bdiv = S; d.A.bs_hi = n4; d.A.bs_lo = n3 / S;
+      d.B.ptr = Wb.p; d.B.ms = 1; d.B.ks_lo = n; d.B.bdiv = S; d.B.bs_hi = 0; d.B.bs_lo = (n3 / S) * n;
+      d.C.ptr = E; d.C.rs = n; d.C.cs = 1; d.C.ps = n2; d.C.bdiv = S; d.C.bs_hi = (long long)S * 2 * n2;
+      d.C.bs_lo = 2 * n2;
+      gemm(d, prec, s);
+    }
+    tot = (long long)Pc * n2;
+    hipLaunchKernelGGL(kern_sum_scatter<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, E, S, p0, Pc,
+                       tstp, n, np, c.S.ret.as<z2>(), c.S.les.as<z2>(), c.S.tv.as<z2>());
+    check_launch("sigma full scatter");
+  }
+}
+}  // namespace
+
+void sigma_gf2_full(Ctx &c, int tstp, Prec prec) {
+  require(c.have_coulomb, "sigma: Coulomb tensor not set");
+  require(tstp >= 0 && tstp <= c.nt, "sigma: tstp out of range");
+  if (prec == PREC_F64) sigma_full_typed<double>(c, tstp, prec);
+  else sigma_full_typed<float>(c, tstp, prec);
 }
 
 }  // namespace ned

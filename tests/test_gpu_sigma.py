@@ -240,3 +240,58 @@ def test_nt_autotune_ragged_rows(orc):
             scale = max(np.max(np.abs(b)), 1e-30)
             err = np.max(np.abs(a - b))
             assert err <= TOL["bf16x3"] * scale, (ntau, name, err, scale)
+
+
+@pytest.mark.parametrize("nao,naux", [(2, 3), (4, 12), (8, 32), (13, 20), (16, 64), (32, 64)])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3"])
+def test_sigma_full_solver_parity(orc, nao, naux, prec):
+    """The full, non-decomposed solver (ned_sigma_gf2_full, molGF2Solver)
+    against the oracle's decomposed one: the reference's own cross-check that
+    both give the same self-energy (tests/mol2bSolver-test.cpp:229-237)."""
+    print(f"[sigma full] nao={nao} naux={naux} prec={prec}", flush=True)
+    rng = np.random.default_rng(300 * nao + naux)
+    V, B = sym_V(rng, nao, naux)
+    nt, ntau = 6, 7
+    G = random_green(rng, nt, ntau, nao, scale=0.5)
+    with ned.Context(nao, naux, nt, ntau, 1) as c:
+        c.set_coulomb(B)
+        c.upload(ned.NED_G, G)
+        for tstp in (0, 3, nt):
+            S_ref = GreenFunc(nt, ntau, nao)
+            orc.sigma_solve(V, tstp, S_ref, G)
+            c.sigma_full(tstp, prec)
+            S = c.download(ned.NED_SIGMA)
+            for name, a, b in (("ret", S.ret_row(tstp), S_ref.ret_row(tstp)),
+                               ("les", S.les_col(tstp), S_ref.les_col(tstp)),
+                               ("tv", S.tv[tstp], S_ref.tv[tstp])):
+                scale = max(np.max(np.abs(b)), 1e-30)
+                err = np.max(np.abs(a - b))
+                assert err <= TOL[prec] * scale, (name, tstp, err, scale)
+
+
+def test_sigma_full_solver_chunking_and_reupload(orc, monkeypatch):
+    """The full solver in chunks of 3 items equals the whole slice bitwise
+    (FP64); after a second ned_set_coulomb its four-index operands are rebuilt
+    from the new factor."""
+    rng = np.random.default_rng(21)
+    nao, naux, nt, ntau = 8, 20, 5, 6
+    V, B = sym_V(rng, nao, naux)
+    V2, B2 = sym_V(rng, nao, naux)
+    G = random_green(rng, nt, ntau, nao, scale=0.5)
+    outs = []
+    for chunk in (0, 3):
+        monkeypatch.setenv("NED_SIGMA_CHUNK", str(chunk))
+        with ned.Context(nao, naux, nt, ntau, 1) as c:
+            c.set_coulomb(B)
+            c.upload(ned.NED_G, G)
+            c.sigma_full(nt, "f64")
+            outs.append(c.download(ned.NED_SIGMA))
+            if chunk == 3:
+                c.set_coulomb(B2)
+                c.sigma_full(nt, "f64")
+                S2 = c.download(ned.NED_SIGMA)
+    for comp in ("ret", "les", "tv"):
+        assert np.array_equal(getattr(outs[0], comp), getattr(outs[1], comp)), comp
+    S_ref = GreenFunc(nt, ntau, nao)
+    orc.sigma_solve(V2, nt, S_ref, G)
+    assert np.max(np.abs(S2.ret_row(nt) - S_ref.ret_row(nt))) < 1e-12 * np.max(np.abs(S_ref.ret_row(nt)))
