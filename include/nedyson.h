@@ -176,6 +176,15 @@ ned_status ned_do_boot(ned_ctx *ctx, const double *fock, double mu, double beta,
 ned_status ned_do_tstp(ned_ctx *ctx, int tstp, const double *h0, double mu, double beta, double dt, int prec,
                        int corr_max, double corr_tol, int *iters_out, double *err_out);
 
+/* Graph mode of ned_do_tstp (no counterpart in the reference): with enable
+ * nonzero, the corrector passes of a step after the first are one HIP graph,
+ * captured from the second pass on the context's stream (the HF kernels, the
+ * Sigma chain and the Dyson step with its side streams forked and joined) and
+ * replayed for every later pass of that step; the first pass runs as
+ * launched.  Same results as without it.  Also set by NED_GRAPH=1 in the
+ * environment at ned_ctx_create. */
+ned_status ned_set_graph(ned_ctx *ctx, int enable);
+
 /* energy_conv(tstp, Sigma, G, beta, dt) (src/dyson_integrals.h:24-70) and the
  * kinetic part of do_energy (src/molNEsim_decomp_impl.h:132-144):
  * out[0] = Re tr(rho (hmf + h0)), out[1] = 2 * energy_conv, out[2] = 2 tr rho. */

@@ -37,7 +37,8 @@ EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "n
            "ned_integrator", "ned_mlp_create", "ned_mlp_destroy", "ned_mlp_nparams", "ned_mlp_get_params",
            "ned_mlp_set_params", "ned_mlp_loss_grad", "ned_mlp_adam_step", "ned_mlp_train", "ned_mlp_predict",
            "ned_set_predictor", "ned_predictor_residual", "ned_predictor_stats", "ned_batch_create",
-           "ned_batch_destroy", "ned_batch_do_tstp", "ned_gemm_variant_count", "ned_gemm_variant_name"]
+           "ned_batch_destroy", "ned_batch_do_tstp", "ned_gemm_variant_count", "ned_gemm_variant_name",
+           "ned_set_graph"]
 
 
 class NedError(RuntimeError):
@@ -91,6 +92,7 @@ def lib():
     L.ned_dyson_start.argtypes = [_vp, C.c_double, C.c_double, C.c_double, _dp]
     L.ned_do_boot.argtypes = [_vp, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
                               C.POINTER(C.c_int), _dp]
+    L.ned_set_graph.argtypes = [_vp, C.c_int]
     L.ned_do_tstp.argtypes = [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
                               C.POINTER(C.c_int), _dp]
     L.ned_energy.argtypes = [_vp, C.c_int, _dp, C.c_double, C.c_double, _dp]
@@ -297,6 +299,11 @@ class Context:
         _check(lib().ned_do_boot(self.h, _dptr(fock), mu, beta, dt, prec_code(prec), maxiter, tol, C.byref(it),
                                  _dptr(e)))
         return it.value, float(e[0])
+
+    def set_graph(self, enable=True):
+        """ned_set_graph: replay the corrector passes after the first of every
+        step from one captured HIP graph."""
+        _check(lib().ned_set_graph(self.h, int(bool(enable))))
 
     def do_tstp(self, tstp, h0, mu, beta, dt, prec="f64", corr_max=3, corr_tol=0.0):
         h0 = np.ascontiguousarray(h0, dtype=np.float64)

@@ -153,6 +153,7 @@ ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
     // schedule of the Dyson recurrences (tests and tuning)
     if (const char *e = std::getenv("NED_RPANEL")) c.rpanel = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("NED_PANEL_BAND_MAX")) c.panel_band_max = std::atoi(e);
+    if (const char *e = std::getenv("NED_GRAPH")) c.graph_mode = std::atoi(e) != 0;
     NED_HIP(hipStreamSynchronize(c.stream));
   });
   if (st != NED_OK) {
@@ -407,6 +408,12 @@ ned_status ned_tti_extend(ned_ctx *ctx, int which, int tsrc) {
     ned::tti_extend(ctx->c, which, tsrc);
     if (which == NED_SIGMA) ned::invalidate_minv(ctx->c);  // Sigma^R(t,t) enters the cached M_t
   });
+}
+
+ned_status ned_set_graph(ned_ctx *ctx, int enable) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  ctx->c.graph_mode = enable != 0;
+  return NED_OK;
 }
 
 int ned_gemm_variant_count(void) { return ned::nt_variant_count(); }

@@ -131,8 +131,11 @@ struct Ctx {
   bool prof = false;
   double prof_flops = 0;  // algorithmic flops of the timed exchange-GEMM launches
   std::vector<hipEvent_t> ev[6];  // U-GEMM start/end, Sigma start/end, Dyson start/end
+  // corrector passes after the first replayed from one captured HIP graph per
+  // step (ned_set_graph); capturing: a capture is open on the stream
+  bool graph_mode = false, capturing = false;
   void mark(int slot) {
-    if (!prof) return;
+    if (!prof || capturing) return;
     hipEvent_t e;
     NED_HIP(hipEventCreate(&e));
     NED_HIP(hipEventRecord(e, stream));

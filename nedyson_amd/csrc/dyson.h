@@ -22,6 +22,13 @@ void set_matsubara(Ctx &c, double beta);
 void dyson_step(Ctx &c, int tstp, double mu, double beta, double dt, double *err3);
 void extrapolate(Ctx &c, int tstp);
 void hf_decomp(Ctx &c, int tstp, const double *h0_host);
+// hf_decomp in two parts: the h0 upload, and the kernels alone (no host
+// copies or waits, for use inside a captured graph)
+void hf_upload_h0(Ctx &c, const double *h0_host);
+void hf_kernels(Ctx &c, int tstp);
+// the update norms of the last dyson_step (its err3), read back on the
+// context's stream
+void dyson_read_err(Ctx &c, double *err3);
 double dyson_start(Ctx &c, double mu, double beta, double dt);
 void energy(Ctx &c, int tstp, const double *h0_host, double beta, double dt, double *out3);
 // Equilibrium GF2 (corr) or HF Matsubara function on the GLL nodes; writes

@@ -1163,6 +1163,18 @@ Pt make_pt(Ctx &c, int T, double mu, double beta, double dt) {
 
 inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+__global__ void kern_zero_words(unsigned *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// clears a small counter or error buffer with a kernel rather than a memset,
+// so that the sequence stays valid inside a captured HIP graph
+void zero_async(void *p, size_t bytes, hipStream_t s) {
+  const int n = (int)((bytes + 3) / 4);
+  hipLaunchKernelGGL(kern_zero_words, dim3(nblk(n)), dim3(256), 0, s, (unsigned *)p, n);
+}
+
 // (on stream st, c.stream if null, counting zero pivots in *info, the
 // context's counter if null)
 void gesv(Ctx &c, z2 *A, long long sA, z2 *B, long long sB, int N, int R, int nsys, hipStream_t st = nullptr,
@@ -1170,7 +1182,7 @@ void gesv(Ctx &c, z2 *A, long long sA, z2 *B, long long sB, int N, int R, int ns
   require(N <= 320, "linear system larger than 320");
   if (!st) st = c.stream;
   if (!info) info = c.dy_info.as<int>();
-  NED_HIP(hipMemsetAsync(info, 0, sizeof(int), st));
+  zero_async(info, sizeof(int), st);
   if (N <= LDS_N && R <= LDS_N) {
     hipLaunchKernelGGL(kern_gesv_lds, dim3(nsys), dim3(256), 0, st, A, sA, B, sB, N, R, info);
   } else if (nsys == 1) {
@@ -1281,7 +1293,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   update_minv(c, T, mu, dt, w);
   c.dy_err.ensure(4 * sizeof(double));
   double *err = c.dy_err.as<double>();
-  NED_HIP(hipMemsetAsync(err, 0, 4 * sizeof(double), s));
+  zero_async(err, 4 * sizeof(double), s);
   // scratch sized for the last step (nt), so that it is allocated once and
   // no allocation (an implicit device synchronisation) falls between the
   // launches of a step
@@ -1418,12 +1430,14 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     check_launch("les");
   }
   c.mark(5);
-  if (err3) {
-    double h[4];
-    NED_HIP(hipMemcpyAsync(h, err, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-    NED_HIP(hipStreamSynchronize(s));
-    err3[0] = h[0]; err3[1] = h[1]; err3[2] = h[2];
-  }
+  if (err3) dyson_read_err(c, err3);
+}
+
+void dyson_read_err(Ctx &c, double *err3) {
+  double h[4];
+  NED_HIP(hipMemcpyAsync(h, c.dy_err.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  NED_HIP(hipStreamSynchronize(c.stream));
+  err3[0] = h[0]; err3[1] = h[1]; err3[2] = h[2];
 }
 
 void extrapolate(Ctx &c, int T) {
@@ -1440,19 +1454,29 @@ void extrapolate(Ctx &c, int T) {
 }
 
 void hf_decomp(Ctx &c, int T, const double *h0_host) {
-  require(c.have_coulomb, "hf: Coulomb tensor not set");
-  require(T >= 0 && T <= c.nt, "hf: tstp out of range");
+  hf_upload_h0(c, h0_host);
+  hf_kernels(c, T);
+  NED_HIP(hipStreamSynchronize(c.stream));  // h0_host may be released by the caller
+  touch_time(c, T);
+}
+
+void hf_upload_h0(Ctx &c, const double *h0_host) {
   const size_t es = c.es;
   c.dy_h0.ensure(es * sizeof(double));
   c.dy_xq.ensure((size_t)c.naux * sizeof(z2));
   NED_HIP(hipMemcpyAsync(c.dy_h0.p, h0_host, es * sizeof(double), hipMemcpyHostToDevice, c.stream));
+}
+
+// the mean field of step T from the h0 uploaded last (kernels only)
+void hf_kernels(Ctx &c, int T) {
+  require(c.have_coulomb, "hf: Coulomb tensor not set");
+  require(T >= 0 && T <= c.nt, "hf: tstp out of range");
+  const size_t es = c.es;
   Pt p = make_pt(c, T, 0, 0, 0);
   hipLaunchKernelGGL(kern_hf_x, dim3(c.naux), dim3(256), 0, c.stream, p, c.Bq.as<double>(), c.dy_xq.as<z2>());
   hipLaunchKernelGGL(kern_hf_h, dim3(nblk((long long)es)), dim3(256), 0, c.stream, p, c.dy_h0.as<double>(),
                      c.Bq.as<double>(), c.U64.as<double>(), c.dy_xq.as<z2>(), c.naux, c.hmf.as<z2>());
   check_launch("hf");
-  NED_HIP(hipStreamSynchronize(c.stream));  // h0_host may be released by the caller
-  touch_time(c, T);
 }
 
 double dyson_start(Ctx &c, double mu, double beta, double dt) {
@@ -1466,7 +1490,7 @@ double dyson_start(Ctx &c, double mu, double beta, double dt) {
   hipStream_t s = c.stream;
   c.dy_err.ensure(4 * sizeof(double));
   double *err = c.dy_err.as<double>();
-  NED_HIP(hipMemsetAsync(err, 0, 4 * sizeof(double), s));
+  zero_async(err, 4 * sizeof(double), s);
   hipLaunchKernelGGL(kern_boot_tv0, dim3(np), dim3(256), 0, s, p, err);
   c.dy_M.ensure((size_t)kn * kn * sizeof(z2));
   c.dy_bootI.ensure((size_t)kn * kn * sizeof(z2));

@@ -132,16 +132,58 @@ void do_tstp(Ctx &c, int T, const double *h0, double mu, double beta, double dt,
     c.pred_E.ensure(2 * NPT * es * sizeof(z2));
     c.pred_ring.ensure(7 * es * NF * sizeof(float));
   }
+  // graph mode: the first pass runs as launched (it allocates scratch, tunes
+  // tiles and fills the caches of the step); the second is captured on the
+  // context's stream -- with the Dyson step's side streams forked from and
+  // joined back into it -- and every later pass replays that graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  struct GraphGuard {
+    hipGraph_t &g;
+    hipGraphExec_t &x;
+    ~GraphGuard() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } gguard{graph, exec};
+  auto graphed_pass = [&](double *e) {
+    if (!exec) {
+      hf_upload_h0(c, h0);
+      NED_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      c.capturing = true;
+      try {
+        hf_kernels(c, T);
+        sigma_gf2(c, T, prec);
+        dyson_step(c, T, mu, beta, dt, nullptr);
+      } catch (...) {
+        c.capturing = false;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      c.capturing = false;
+      NED_HIP(hipStreamEndCapture(c.stream, &graph));
+      NED_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    touch_time(c, T);
+    NED_HIP(hipGraphLaunch(exec, c.stream));
+    dyson_read_err(c, e);
+  };
   auto corrector = [&](double &err, int &it, double &first) -> bool {
     double e[3] = {0, 0, 0};
     it = 0;
     err = 0;
     first = -1;
     for (; it < corr_max;) {
-      hf_decomp(c, T, h0);
-      sigma_gf2(c, T, prec);
-      touch_time(c, T);
-      dyson_step(c, T, mu, beta, dt, e);
+      if (c.graph_mode && it >= 1) {
+        graphed_pass(e);
+      } else {
+        hf_decomp(c, T, h0);
+        sigma_gf2(c, T, prec);
+        touch_time(c, T);
+        dyson_step(c, T, mu, beta, dt, e);
+      }
       err = e[0] + e[1] + e[2];
       if (first < 0) first = err;
       it++;
