@@ -298,6 +298,60 @@ def window_context(ned, p, T0, nt):
     return c
 
 
+def batch_bench(args, log):
+    """The batched configuration: 16 systems (problem.batch_systems), each
+    from its GF2 equilibrium, timed on a window of steps after a history of T0
+    steps (free G of its Fock matrix, Sigma continued time-translation
+    invariantly), FP32 Sigma and FP64 Dyson, 3 corrector passes: the grouped
+    batch (one launch per link of the Dyson recurrences for all systems), the
+    batch of one host thread and stream per system, and the systems one after
+    another."""
+    import nedyson_amd as ned
+    from nedyson_amd import problem
+    T0 = args.T0 if args.T0 is not None else 750
+    W, K = args.warmup, args.steps
+    nt = T0 + W + K
+    # as for the nao=64 window: the synthetic history is no solution of the
+    # interacting equations, and at 0.05 the nao=48 system's corrector
+    # overflows on it; the work per step does not depend on the values
+    bsd = args.b_sd if args.b_sd is not None else 0.01
+    ps = problem.batch_systems(b_sd=bsd, nt=nt)
+    for p in ps:
+        gf2_equilibrium(ned, p, log)
+    log(f"[batch] {len(ps)} systems, nao {[p.nao for p in ps]}, history T0={T0}")
+    rates = {}
+    for mode in ("grouped", "threads", "sequential"):
+        ctxs = [window_context(ned, p, T0, nt) for p in ps]
+        b = ned.Batch(ctxs, grouped=(mode == "grouped")) if mode != "sequential" else None
+        times = []
+        for i, T in enumerate(range(T0 + 1, nt + 1)):
+            for c in ctxs:
+                c.sync()
+            t0 = time.perf_counter()
+            if b is not None:
+                it, err = b.do_tstp(T, [p.h0 for p in ps], [p.mu for p in ps], [p.beta for p in ps],
+                                    [p.dt for p in ps], "f32", args.corr)
+            else:
+                for c, p in zip(ctxs, ps):
+                    c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, "f32", args.corr)
+            for c in ctxs:
+                c.sync()
+            if i >= W:
+                times.append(time.perf_counter() - t0)
+        if b is not None:
+            b.close()
+        for c in ctxs:
+            c.close()
+        rates[mode] = len(ps) * len(times) / sum(times)
+        log(f"[batch] {mode}: {1e3 * np.mean(times):.1f} ms per batch step, {rates[mode]:.2f} system-steps/s")
+    print(json.dumps({"metric": "batched KBE system-steps/s (16 systems of uneven nao)", "value": rates["grouped"],
+                      "unit": "system-steps/s", "threads_value": rates["threads"],
+                      "sequential_value": rates["sequential"], "steps": K, "warmup": W,
+                      "config": {"workload": f"16 systems nao {[p.nao for p in ps]}, naux 4 nao, ntau+1=64, "
+                                             f"dt=0.02, T={T0 + W + 1}..{nt}, {args.corr} corrector passes, "
+                                             f"FP32 Sigma / FP64 Dyson", "sigma_B": bsd, "T0": T0}}), flush=True)
+
+
 def sweep(args, p, log):
     """Per-step time at several history lengths T, and the whole-run average
     over T = k+1..nt from a quadratic fit t(T) = a + b T + c T^2 (the Sigma
@@ -384,6 +438,10 @@ def main():
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
+
+    if args.config == 3:
+        batch_bench(args, log)
+        return
 
     if args.copy_bytes:
         import torch

@@ -263,15 +263,24 @@ ned_status ned_predictor_stats(ned_ctx *ctx, long long *used, long long *fallbac
  * Independent systems stepped together.  No counterpart in the reference
  * (which propagates one system per process): a batch holds n existing
  * contexts, possibly of different nao, naux, nt and ntau, on the same or on
- * different devices; ned_batch_do_tstp runs ned_do_tstp on all of them at
- * once, one host thread per system, each system's kernels on its own stream
- * so that the GPU overlaps them.  The contexts stay owned by the caller and
- * must outlive the batch.  h0[i] is system i's real nao_i x nao_i Hamiltonian;
- * iters, err and status (arrays of n, each may be NULL) receive the per-system
- * results; the return value is the first failing system's status. */
+ * different devices, stepped at the same tstp.  Grouped (the default, all
+ * contexts on one device): each corrector pass issues every system's HF and
+ * Sigma on its own stream, then one Dyson step for all of them whose
+ * retarded and lesser recurrences -- the dependency chains and bulk panel
+ * updates that are most of its launches -- run as one launch per link and per
+ * panel for the whole group (the system in the grid's third dimension); the
+ * initial guess is the polynomial one, and with corr_tol > 0 a converged
+ * system leaves the group while the others go on.  Otherwise (contexts on
+ * several devices, or ned_batch_set_grouped(b, 0)) ned_do_tstp runs on each,
+ * one host thread and stream per system.  The contexts stay owned by the
+ * caller and must outlive the batch.  h0[i] is system i's real nao_i x nao_i
+ * Hamiltonian; iters, err and status (arrays of n, each may be NULL) receive
+ * the per-system results; the return value is the first failing system's
+ * status. */
 typedef struct ned_batch ned_batch;
 ned_status ned_batch_create(ned_batch **out, ned_ctx *const *ctxs, int n);
 ned_status ned_batch_destroy(ned_batch *b);
+ned_status ned_batch_set_grouped(ned_batch *b, int grouped);
 ned_status ned_batch_do_tstp(ned_batch *b, int tstp, const double *const *h0, const double *mu, const double *beta,
                              const double *dt, int prec, int corr_max, double corr_tol, int *iters, double *err,
                              int *status);

@@ -38,7 +38,7 @@ EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "n
            "ned_mlp_set_params", "ned_mlp_loss_grad", "ned_mlp_adam_step", "ned_mlp_train", "ned_mlp_predict",
            "ned_set_predictor", "ned_predictor_residual", "ned_predictor_stats", "ned_batch_create",
            "ned_batch_destroy", "ned_batch_do_tstp", "ned_gemm_variant_count", "ned_gemm_variant_name",
-           "ned_set_graph"]
+           "ned_set_graph", "ned_batch_set_grouped"]
 
 
 class NedError(RuntimeError):
@@ -116,6 +116,7 @@ def lib():
     L.ned_predictor_residual.argtypes = [_vp, C.c_int, _fp]
     L.ned_predictor_stats.argtypes = [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.ned_batch_create.argtypes = [C.POINTER(_vp), C.POINTER(_vp), C.c_int]
+    L.ned_batch_set_grouped.argtypes = [_vp, C.c_int]
     L.ned_batch_destroy.argtypes = [_vp]
     L.ned_batch_do_tstp.argtypes = [_vp, C.c_int, C.POINTER(_dp), _dp, _dp, _dp, C.c_int, C.c_int, C.c_double,
                                     C.POINTER(C.c_int), _dp, C.POINTER(C.c_int)]
@@ -435,14 +436,16 @@ class Mlp:
 
 
 class Batch:
-    """Independent systems stepped together (ned_batch_*): each Context keeps
-    its own stream and the timesteps of all of them run concurrently."""
+    """Independent systems stepped together (ned_batch_*): grouped (the Dyson
+    recurrences of all systems as one launch per link) by default, or one host
+    thread and stream per system with grouped=False."""
 
-    def __init__(self, contexts):
+    def __init__(self, contexts, grouped=True):
         self.contexts = list(contexts)
         arr = (_vp * len(self.contexts))(*[c.h for c in self.contexts])
         self.h = _vp()
         _check(lib().ned_batch_create(C.byref(self.h), arr, len(self.contexts)))
+        _check(lib().ned_batch_set_grouped(self.h, int(bool(grouped))))
 
     def close(self):
         if getattr(self, "h", None):

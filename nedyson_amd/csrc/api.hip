@@ -84,9 +84,13 @@ struct ned_mlp {
   std::unique_ptr<ned::Mlp> m;
 };
 
-// independent systems stepped together; each member keeps its own stream
+// independent systems stepped together: grouped (one launch per link of the
+// Dyson recurrences for all members, ned::DysonGroup) when the members share a
+// device, else one host thread and stream per member
 struct ned_batch {
   std::vector<ned_ctx *> ctx;
+  bool grouped = true;
+  std::unique_ptr<ned::DysonGroup> group;
 };
 
 template <class F>
@@ -584,11 +588,45 @@ ned_status ned_batch_destroy(ned_batch *b) {
   return NED_OK;
 }
 
+ned_status ned_batch_set_grouped(ned_batch *b, int grouped) {
+  if (!b) { g_err = "null batch"; return NED_ERR_ARG; }
+  b->grouped = grouped != 0;
+  return NED_OK;
+}
+
 ned_status ned_batch_do_tstp(ned_batch *b, int tstp, const double *const *h0, const double *mu, const double *beta,
                              const double *dt, int prec, int corr_max, double corr_tol, int *iters, double *err,
                              int *status) {
   if (!b || !h0 || !mu || !beta || !dt) { g_err = "null argument"; return NED_ERR_ARG; }
   const int n = (int)b->ctx.size();
+  bool same_device = true;
+  for (ned_ctx *c : b->ctx) same_device = same_device && c->c.device == b->ctx[0]->c.device;
+  if (b->grouped && same_device) {
+    std::vector<int> stv(n, 0);
+    ned_status r = guard(b->ctx[0], [&] {
+      ned::require(prec >= 0 && prec <= 2, "bad prec");
+      ned::require(corr_max >= 1, "corr_max must be >= 1");
+      std::vector<ned::Ctx *> cs;
+      for (ned_ctx *c : b->ctx) {
+        state(c->c.have_coulomb, "interaction not set (ned_set_coulomb)");
+        state(c->c.have_mix, "ned_set_matsubara not called");
+        cs.push_back(&c->c);
+      }
+      if (!b->group) b->group.reset(new ned::DysonGroup(b->ctx[0]->c.device));
+      ned::batch_do_tstp(*b->group, cs, tstp, h0, mu, beta, dt, (ned::Prec)prec, corr_max, corr_tol, iters, err,
+                         stv.data());
+    });
+    if (status)
+      for (int i = 0; i < n; i++) status[i] = r != NED_OK ? r : stv[i];
+    if (r != NED_OK) return r;
+    for (int i = 0; i < n; i++)
+      if (stv[i] != NED_OK) {
+        g_err = "system " + std::to_string(i) +
+                (stv[i] == NED_ERR_ARG ? ": tstp must be in (k, nt]" : ": non-finite update in corrector");
+        return (ned_status)stv[i];
+      }
+    return NED_OK;
+  }
   std::vector<ned_status> st(n, NED_OK);
   std::vector<std::string> msg(n);
   std::vector<std::thread> th;

@@ -20,6 +20,30 @@ WTab make_wtab(int k);
 
 void set_matsubara(Ctx &c, double beta);
 void dyson_step(Ctx &c, int tstp, double mu, double beta, double dt, double *err3);
+
+// Streams, events and the device copy of the members' operands for stepping
+// many systems' Dyson recurrences with one launch per link (all on one
+// device).
+struct DysonGroup {
+  int device;
+  hipStream_t s = nullptr, sd = nullptr;  // the grouped chain and its bulk updates
+  hipEvent_t evm = nullptr, evs[2] = {nullptr, nullptr};
+  std::vector<hipEvent_t> evr;  // one per member: its stream -> the group's
+  DevBuf args;                  // the members' ChainArg
+  void *host_args = nullptr;    // pinned staging of the same
+  size_t host_cap = 0;
+  explicit DysonGroup(int device);
+  ~DysonGroup();
+  DysonGroup(const DysonGroup &) = delete;
+  DysonGroup &operator=(const DysonGroup &) = delete;
+};
+// dyson_step for every context of cs at the same tstp (each with its own mu,
+// beta, dt; err3 holds 3 per context): the per-system phases on each
+// context's stream, and the retarded and lesser recurrences of all systems
+// together, one launch per link and per panel update, with the system in
+// blockIdx.z.  The same results as dyson_step on each.
+void dyson_step_group(DysonGroup &g, const std::vector<Ctx *> &cs, int tstp, const double *mu, const double *beta,
+                      const double *dt, double *err3);
 void extrapolate(Ctx &c, int tstp);
 void hf_decomp(Ctx &c, int tstp, const double *h0_host);
 // hf_decomp in two parts: the h0 upload, and the kernels alone (no host

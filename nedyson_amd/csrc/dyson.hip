@@ -79,6 +79,18 @@ struct Pt {
   __device__ const z2 *h(int t) const { return hmf + (size_t)t * es(); }
 };
 
+// One system's operands of the dependency chains and bulk updates of the
+// Dyson recurrences, for the grouped launches that step many systems at once:
+// the two accumulators (QA the chain's, Q the bulk updates'), the lesser
+// column X, the cached inverses and the retarded chain's band norms.
+struct ChainArg {
+  Pt p;
+  z2 *QA, *Q, *X;
+  const z2 *Minv;
+  double *d2;
+  int nb;  // 16-row / 16-column bands of a block
+};
+
 // ------------------------------------------------------------ batched solve
 // A_s X = B_s for every system s (N x N, R right-hand sides), Gaussian
 // elimination with partial pivoting, one workgroup per system, in place in
@@ -423,15 +435,15 @@ __global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, const WTab *__restri
 // workgroup adds only the newest one; the register budget (at most 224 a
 // wave) lets a chain wave share a SIMD with two bulk panel waves (at most 136
 // each); and the waves raise their issue priority.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *QA, const z2 *QB, const z2 *Minv, double *d2) {
+__device__ __forceinline__ void ret_step_band(const Pt &p, const WTab *__restrict__ wtab, int nn, z2 *QA,
+                                              const z2 *QB, const z2 *Minv, double *d2, int b, int by, int nb) {
   __builtin_amdgcn_s_setprio(3);
   const WTab &w = *wtab;
   __shared__ z2 band[16][BO_NMAX + 1];
   __shared__ double red[4];
   // bands on x: the finishing workgroups (m = nn + 1) are the first
   // dispatched, ahead of the updates of later blocks
-  const int m = nn + 1 + blockIdx.y, b = blockIdx.x, nb = gridDim.x, T = p.T, n = p.n;
+  const int m = nn + 1 + by, T = p.T, n = p.n;
   if (m > T) return;
   const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
   const bool fin = m == nn + 1, bdt = m - nn <= w.k + 1;
@@ -478,6 +490,20 @@ kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *QA, const z2
   if (threadIdx.x == 0) d2[(size_t)m * nb + b] = dd;
 }
 
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *QA, const z2 *QB, const z2 *Minv, double *d2) {
+  ret_step_band(p, wtab, nn, QA, QB, Minv, d2, blockIdx.x, blockIdx.y, gridDim.x);
+}
+
+// the same link for every system of a group (system = blockIdx.z; bands
+// beyond a system's own count exit)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_ret_step_band_grp(const ChainArg *args, const WTab *__restrict__ wtab, int nn) {
+  const ChainArg &a = args[blockIdx.z];
+  if ((int)blockIdx.x >= a.nb) return;
+  ret_step_band(a.p, wtab, nn, a.QA, a.Q, a.Minv, a.d2, blockIdx.x, blockIdx.y, a.nb);
+}
+
 // err += sum over m in [m0, m1] of sqrt(sum_b d2[m][b])  (one workgroup)
 __global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *err) {
   __shared__ double red[256];
@@ -499,11 +525,11 @@ __global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *
 
 // contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
 // the history sums stay in registers across the whole panel
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
+__device__ __forceinline__ void ret_panel(const Pt &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
+                                          int bx) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
-  const int m = m0 + blockIdx.x, T = p.T, n = p.n;
+  const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * n * n;
   BlkAcc acc;
@@ -514,16 +540,27 @@ kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *
   blk_store(acc, n, Qm);
 }
 
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
+  ret_panel(p, wtab, q0, q1, m0, Q, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_ret_panel_grp(const ChainArg *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
+  const ChainArg &a = args[blockIdx.y];
+  ret_panel(a.p, wtab, q0, q1, m0, a.Q, blockIdx.x);
+}
+
 // kern_ret_panel split by rows, for the late panels whose (T - q1) blocks
 // alone leave most of the chip idle: workgroup (m, b) owns rows
 // 16b..16b+15 of block m, its waves the four column tiles, four times the
 // workgroups at a quarter of the work each
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_ret_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
+__device__ __forceinline__ void ret_panel_band(const Pt &p, const WTab *__restrict__ wtab, int q0, int q1, int m0,
+                                               z2 *Q, int bx, int by) {
   const WTab &w = *wtab;
-  const int m = m0 + blockIdx.x, T = p.T, n = p.n;
+  const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
-  const int r0 = 16 * blockIdx.y, c0 = 16 * (threadIdx.x >> 6);
+  const int r0 = 16 * by, c0 = 16 * (threadIdx.x >> 6);
   z2 *Qm = Q + (size_t)m * n * n;
   TileAcc acc;
   tile_load(acc, n, Qm, r0, c0);
@@ -536,6 +573,18 @@ kern_ret_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0,
       frag_mac(acc, fa, fb, n, z2{greg(w, m, nn), 0.0}, h * BO_NMAX / 8);
     }
   tile_store(acc, n, Qm, r0, c0);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_ret_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
+  ret_panel_band(p, wtab, q0, q1, m0, Q, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_ret_panel_band_grp(const ChainArg *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
+  const ChainArg &a = args[blockIdx.z];
+  if ((int)blockIdx.y >= a.nb) return;
+  ret_panel_band(a.p, wtab, q0, q1, m0, a.Q, blockIdx.x, blockIdx.y);
 }
 
 // ------------------------------------------------------------ mixing (tv)
@@ -757,12 +806,12 @@ __global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restri
 // X_m = Minv_m Q'_m keeps the columns of Q'_m apart.  Built as
 // kern_ret_step_band (the finishing terms -i/dt bd_l X_{m-l} added ahead by
 // link m-l).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *QA, const z2 *QB, z2 *X, const z2 *Minv) {
+__device__ __forceinline__ void les_step_band(const Pt &p, const WTab *__restrict__ wtab, int l, z2 *QA,
+                                              const z2 *QB, z2 *X, const z2 *Minv, int b, int by) {
   __builtin_amdgcn_s_setprio(3);
   const WTab &w = *wtab;
   __shared__ z2 band[BO_NMAX][17];
-  const int m = l + 1 + blockIdx.y, b = blockIdx.x, T = p.T, n = p.n;  // finishing bands first
+  const int m = l + 1 + by, T = p.T, n = p.n;  // finishing bands first
   if (m > T) return;
   const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
   const bool fin = m == l + 1 && m <= T - 1, bdt = m - l <= w.k + 1 && m <= T - 1;
@@ -797,12 +846,24 @@ kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *QA, const z2 
   tile_store(x, n, X + (size_t)m * es, r0, c0);
 }
 
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *QA, const z2 *QB, z2 *X, const z2 *Minv) {
+  les_step_band(p, wtab, l, QA, QB, X, Minv, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_les_step_band_grp(const ChainArg *args, const WTab *__restrict__ wtab, int l) {
+  const ChainArg &a = args[blockIdx.z];
+  if ((int)blockIdx.x >= a.nb) return;
+  les_step_band(a.p, wtab, l, a.QA, a.Q, a.X, a.Minv, blockIdx.x, blockIdx.y);
+}
+
 // contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
+__device__ __forceinline__ void les_panel(const Pt &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
+                                          const z2 *X, int bx) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
-  const int m = m0 + blockIdx.x, T = p.T, n = p.n;
+  const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * p.es();
   BlkAcc acc;
@@ -813,14 +874,25 @@ kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *
   blk_store(acc, n, Qm);
 }
 
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
+  les_panel(p, wtab, q0, q1, m0, Q, X, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_les_panel_grp(const ChainArg *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
+  const ChainArg &a = args[blockIdx.y];
+  les_panel(a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x);
+}
+
 // kern_les_panel split by columns (workgroup (m, b) owns columns
 // 16b..16b+15 of block m), for the late panels, as kern_ret_panel_band
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_les_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
+__device__ __forceinline__ void les_panel_band(const Pt &p, const WTab *__restrict__ wtab, int q0, int q1, int m0,
+                                               z2 *Q, const z2 *X, int bx, int by) {
   const WTab &w = *wtab;
-  const int m = m0 + blockIdx.x, T = p.T, n = p.n;
+  const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
-  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * blockIdx.y;
+  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * by;
   const size_t es = p.es();
   z2 *Qm = Q + (size_t)m * es;
   TileAcc acc;
@@ -834,6 +906,18 @@ kern_les_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0,
       frag_mac(acc, fa, fb, n, z2{p.dt * greg(w, m, l), 0.0}, h * BO_NMAX / 8);
     }
   tile_store(acc, n, Qm, r0, c0);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_les_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
+  les_panel_band(p, wtab, q0, q1, m0, Q, X, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_les_panel_band_grp(const ChainArg *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
+  const ChainArg &a = args[blockIdx.z];
+  if ((int)blockIdx.y >= a.nb) return;
+  les_panel_band(a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x, blockIdx.y);
 }
 
 // diagonal step, src/dyson_step.h:283-310
@@ -1274,13 +1358,48 @@ void set_matsubara(Ctx &c, double beta) {
   c.have_mix = true;
 }
 
-// The context's stream waits for everything issued so far on the side stream.
-static void join_side(Ctx &c) {
-  NED_HIP(hipEventRecord(c.ev_main, c.side));
-  NED_HIP(hipStreamWaitEvent(c.stream, c.ev_main, 0));
+namespace {
+
+// The blocked right-looking recurrence with look-ahead, shared by the
+// retarded and lesser phases, for one system or a group.  Panel p holds the
+// links nn in [q0, q1).  Each link finishes block nn+1 and adds block nn to
+// the later blocks of its own panel and of the next one (chain(nn, q1x): small
+// launches on stream s, the dependency chain; accumulator QA).  The bulk
+// update -- panel p applied to every block beyond the next panel,
+// panel(q0, q1, q1x, band) -- runs on the side stream sd into a second
+// accumulator Q while the chain goes on, and the chain of panel p+2 waits for
+// it (events evs).  A finishing link adds both sums.  Ends with s waiting for
+// sd.
+template <class Chain, class Panel>
+void recurrence(int k, int T, int RP, int band_max, hipStream_t s, hipStream_t sd, hipEvent_t evm, hipEvent_t *evs,
+                Chain chain, Panel panel) {
+  int pi = 0;
+  for (int q0 = k + 1; q0 < T; q0 += RP, pi++) {
+    const int q1 = std::min(q0 + RP, T), q1x = std::min(q1 + RP, T);
+    if (pi >= 2) NED_HIP(hipStreamWaitEvent(s, evs[pi & 1], 0));
+    for (int nn = q0; nn < q1; nn++) chain(nn, q1x);
+    if (q1x < T) {
+      NED_HIP(hipEventRecord(evm, s));
+      NED_HIP(hipStreamWaitEvent(sd, evm, 0));
+      panel(q0, q1, q1x, T - q1x <= band_max);
+      NED_HIP(hipEventRecord(evs[pi & 1], sd));
+    }
+  }
+  NED_HIP(hipEventRecord(evm, sd));
+  NED_HIP(hipStreamWaitEvent(s, evm, 0));
 }
 
-void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) {
+// blocks per panel; at least k, so that the chain links of a panel reach
+// every block whose finishing terms they fold in (up to k+1 blocks ahead)
+inline int panel_blocks(const Ctx &c) { return std::max(c.rpanel, c.k); }
+inline int bands(const Ctx &c) { return (c.nao + 15) / 16; }  // 16-row / 16-column bands of a block
+
+// ---- phase 1: cached inverses, scratch, the mixing row on the third stream
+// (src/dyson_step.h:130-169: it reads only Sigma and the G^rceil and G^M
+// history, not the retarded row, so it runs beside the retarded recurrence
+// with its own scratch), and the start-up of the retarded row
+// (src/dyson_step.h:28-65) with the history sums' initial values
+void dyson_begin(Ctx &c, int T, double mu, double beta, double dt) {
   require(c.have_mix && c.mix_beta == beta, "dyson_step: call ned_set_matsubara(beta) first");
   require(T > c.k && T <= c.nt, "dyson_step: tstp must be in (k, nt]");
   const WTab w = make_wtab(c.k);
@@ -1305,11 +1424,11 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   c.dy_Y.ensure((size_t)np * es * sizeof(z2));
   c.dy_M.ensure((size_t)kn * kn * sizeof(z2));
   c.dy_Qs.ensure((size_t)kn * n * sizeof(z2));
-  z2 *Q = c.dy_Q.as<z2>(), *X = c.dy_X.as<z2>(), *M = c.dy_M.as<z2>(), *Qs = c.dy_Qs.as<z2>();
-  z2 *QA = c.dy_QA.as<z2>();
-  // ---- right mixing (src/dyson_step.h:130-169) on the third stream: it
-  // reads only Sigma and the G^rceil and G^M history, not the retarded row,
-  // so it runs beside the retarded recurrence (its own scratch throughout)
+  c.dy_d2.ensure((size_t)(c.nt + 1) * bands(c) * sizeof(double));
+  c.dy_retinv.ensure((size_t)kn * kn * sizeof(z2));
+  c.dy_lesinv.ensure((size_t)kn * kn * sizeof(z2));
+  c.dy_Xs.ensure((size_t)kn * n * sizeof(z2));
+  z2 *Q = c.dy_Q.as<z2>(), *M = c.dy_M.as<z2>(), *Qs = c.dy_Qs.as<z2>(), *QA = c.dy_QA.as<z2>();
   {
     const int nch = chunks_for(np, T + np);
     c.dy_tvpart.ensure((size_t)std::max(nch * np, 1) * es * sizeof(z2));
@@ -1329,19 +1448,7 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
     check_launch("tv");
     NED_HIP(hipEventRecord(c.ev_tv, s2));
   }
-
-  // blocks per panel; at least k, so that the chain links of a panel reach
-  // every block whose finishing terms they fold in (up to k+1 blocks ahead)
-  const int RP = std::max(c.rpanel, k);
-  const int nb = (n + 15) / 16;  // 16-row / 16-column bands of a block
-  c.dy_d2.ensure((size_t)(c.nt + 1) * nb * sizeof(double));
-  double *d2 = c.dy_d2.as<double>();
-  const z2 *Minv = c.Minv.as<z2>();
-
-  // ---- retarded (src/dyson_step.h:7-127)
-  c.dy_retinv.ensure((size_t)kn * kn * sizeof(z2));
-  c.dy_lesinv.ensure((size_t)kn * kn * sizeof(z2));
-  c.dy_Xs.ensure((size_t)kn * n * sizeof(z2));
+  // ---- retarded start-up (src/dyson_step.h:7-127)
   z2 *Xs = c.dy_Xs.as<z2>();
   if (c.retinv_T != T) {
     hipLaunchKernelGGL(kern_ret_start, dim3(nblk((long long)kn * kn)), dim3(256), 0, s, p, wd, 1, M,
@@ -1353,84 +1460,214 @@ void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) 
   hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_retinv.as<z2>(), Qs, kn, n, Xs);
   check_launch("ret start");
   hipLaunchKernelGGL(kern_ret_start_store, dim3(k), dim3(256), 0, s, p, Xs, err + 0);
-  hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, QA, Minv, err + 0);
-  // Blocked right-looking recurrence with look-ahead.  Panel p holds the
-  // links nn in [q0, q1).  Each link finishes block nn+1 and adds block nn to
-  // the later blocks of its own panel and of the next one (small launches on
-  // the context's stream, the dependency chain; accumulator QA).  The bulk
-  // update -- panel p applied to every block beyond the next panel -- runs on
-  // the side stream into a second accumulator Q while the chain goes on, and
-  // the chain of panel p+2 waits for it.  A finishing link adds both sums.
-  int pi = 0;
-  for (int q0 = k + 1; q0 < T; q0 += RP, pi++) {
-    const int q1 = std::min(q0 + RP, T), q1x = std::min(q1 + RP, T);
-    if (pi >= 2) NED_HIP(hipStreamWaitEvent(s, c.ev_side[pi & 1], 0));
-    for (int nn = q0; nn < q1; nn++)
-      hipLaunchKernelGGL(kern_ret_step_band, dim3(nb, q1x - nn), dim3(256), 0, s, p, wd, nn, QA, Q, Minv, d2);
-    if (q1x < T) {
-      NED_HIP(hipEventRecord(c.ev_main, s));
-      NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
-      if (T - q1x <= c.panel_band_max)
-        hipLaunchKernelGGL(kern_ret_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
-                           q1x + 1, Q);
-      else
-        hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1,
-                           q1x + 1, Q);
-      NED_HIP(hipEventRecord(c.ev_side[pi & 1], c.side));
-    }
-  }
-  join_side(c);
-  if (T > k + 1) hipLaunchKernelGGL(kern_band_err, dim3(1), dim3(256), 0, s, d2, k + 2, T, nb, err + 0);
+  hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, QA, c.Minv.as<z2>(), err + 0);
+}
+
+// ---- phase 2 (one system): the retarded recurrence
+void dyson_ret_one(Ctx &c, int T, double mu, double beta, double dt) {
+  const WTab *wd = dev_wtab(c);
+  Pt p = make_pt(c, T, mu, beta, dt);
+  const int nb = bands(c);
+  z2 *Q = c.dy_Q.as<z2>(), *QA = c.dy_QA.as<z2>();
+  const z2 *Minv = c.Minv.as<z2>();
+  double *d2 = c.dy_d2.as<double>();
+  recurrence(c.k, T, panel_blocks(c), c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
+             [&](int nn, int q1x) {
+               hipLaunchKernelGGL(kern_ret_step_band, dim3(nb, q1x - nn), dim3(256), 0, c.stream, p, wd, nn, QA, Q,
+                                  Minv, d2);
+             },
+             [&](int q0, int q1, int q1x, bool band) {
+               if (band)
+                 hipLaunchKernelGGL(kern_ret_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
+                                    q1x + 1, Q);
+               else
+                 hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q);
+             });
   check_launch("ret history");
+}
 
-  // ---- right mixing: joined back before the lesser column, which reads it
+// ---- phase 3: the retarded update norm, the mixing row joined back, and
+// the lesser column's history sums and start-up (src/dyson_step.h:171-251)
+void dyson_mid(Ctx &c, int T, double mu, double beta, double dt) {
+  const WTab *wd = dev_wtab(c);
+  const int n = c.nao, k = c.k, kn = k * n, np = c.np, nb = bands(c);
+  const size_t es = c.es;
+  Pt p = make_pt(c, T, mu, beta, dt);
+  hipStream_t s = c.stream;
+  double *err = c.dy_err.as<double>();
+  z2 *Q = c.dy_Q.as<z2>(), *X = c.dy_X.as<z2>(), *M = c.dy_M.as<z2>(), *Qs = c.dy_Qs.as<z2>();
+  if (T > k + 1) hipLaunchKernelGGL(kern_band_err, dim3(1), dim3(256), 0, s, c.dy_d2.as<double>(), k + 2, T, nb,
+                                    err + 0);
   NED_HIP(hipStreamWaitEvent(s, c.ev_tv, 0));
-
-  // ---- lesser (src/dyson_step.h:171-326)
-  {
-    hipLaunchKernelGGL(kern_les_Y, dim3(np), dim3(256), 0, s, p, c.mass.as<double>(), c.dy_Y.as<z2>());
-    int nch = chunks_for(T + 1, np + T + 1);
-    require((long long)nch * (T + 1) <= 8192LL + c.nt + 1, "dyson_step: partial-sum buffer too small");
-    hipLaunchKernelGGL(kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, wd, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
-    long long nel = (long long)(T + 1) * es;
-    hipLaunchKernelGGL(kern_sum_parts, dim3(nblk(nel)), dim3(256), 0, s, c.dy_part.as<z2>(), nch, nel, Q);
-    const int build = c.lesinv_valid ? 0 : 1;
-    hipLaunchKernelGGL(kern_les_x0, dim3(1), dim3(256), 0, s, p, X, err + 2);
-    hipLaunchKernelGGL(kern_les_start, dim3(nblk((long long)kn * (build ? kn : n))), dim3(256), 0, s, p, wd, build, Q,
-                       X, M, c.dy_lesinv.as<z2>(), Qs);
-    if (build) {
-      gesv(c, M, 0, c.dy_lesinv.as<z2>(), 0, kn, kn, 1);
-      c.lesinv_valid = true;
-    }
-    hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_lesinv.as<z2>(), Qs, kn, n,
-                       X + es);
-    hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, QA, X, Minv);
-    // the same look-ahead schedule as the retarded recurrence
-    int pi = 0;
-    for (int q0 = k + 1; q0 < T; q0 += RP, pi++) {
-      const int q1 = std::min(q0 + RP, T), q1x = std::min(q1 + RP, T);
-      if (pi >= 2) NED_HIP(hipStreamWaitEvent(s, c.ev_side[pi & 1], 0));
-      for (int l = q0; l < q1; l++)
-        hipLaunchKernelGGL(kern_les_step_band, dim3(nb, q1x - l), dim3(256), 0, s, p, wd, l, QA, Q, X, Minv);
-      if (q1x < T) {
-        NED_HIP(hipEventRecord(c.ev_main, s));
-        NED_HIP(hipStreamWaitEvent(c.side, c.ev_main, 0));
-        if (T - q1x <= c.panel_band_max)
-          hipLaunchKernelGGL(kern_les_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
-                             q1x + 1, Q, X);
-        else
-          hipLaunchKernelGGL(kern_les_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0,
-                             q1, q1x + 1, Q, X);
-        NED_HIP(hipEventRecord(c.ev_side[pi & 1], c.side));
-      }
-    }
-    join_side(c);
-    hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, Q, QA, X, c.dy_scr.as<z2>());
-    hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, X, err + 2);
-    check_launch("les");
+  hipLaunchKernelGGL(kern_les_Y, dim3(np), dim3(256), 0, s, p, c.mass.as<double>(), c.dy_Y.as<z2>());
+  int nch = chunks_for(T + 1, np + T + 1);
+  require((long long)nch * (T + 1) <= 8192LL + c.nt + 1, "dyson_step: partial-sum buffer too small");
+  hipLaunchKernelGGL(kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, wd, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
+  long long nel = (long long)(T + 1) * es;
+  hipLaunchKernelGGL(kern_sum_parts, dim3(nblk(nel)), dim3(256), 0, s, c.dy_part.as<z2>(), nch, nel, Q);
+  const int build = c.lesinv_valid ? 0 : 1;
+  hipLaunchKernelGGL(kern_les_x0, dim3(1), dim3(256), 0, s, p, X, err + 2);
+  hipLaunchKernelGGL(kern_les_start, dim3(nblk((long long)kn * (build ? kn : n))), dim3(256), 0, s, p, wd, build, Q,
+                     X, M, c.dy_lesinv.as<z2>(), Qs);
+  if (build) {
+    gesv(c, M, 0, c.dy_lesinv.as<z2>(), 0, kn, kn, 1);
+    c.lesinv_valid = true;
   }
+  hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_lesinv.as<z2>(), Qs, kn, n,
+                     X + es);
+  hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, c.dy_QA.as<z2>(), X, c.Minv.as<z2>());
+  check_launch("les start");
+}
+
+// ---- phase 4 (one system): the lesser recurrence, the same schedule
+void dyson_les_one(Ctx &c, int T, double mu, double beta, double dt) {
+  const WTab *wd = dev_wtab(c);
+  Pt p = make_pt(c, T, mu, beta, dt);
+  const int nb = bands(c);
+  z2 *Q = c.dy_Q.as<z2>(), *QA = c.dy_QA.as<z2>(), *X = c.dy_X.as<z2>();
+  const z2 *Minv = c.Minv.as<z2>();
+  recurrence(c.k, T, panel_blocks(c), c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
+             [&](int l, int q1x) {
+               hipLaunchKernelGGL(kern_les_step_band, dim3(nb, q1x - l), dim3(256), 0, c.stream, p, wd, l, QA, Q, X,
+                                  Minv);
+             },
+             [&](int q0, int q1, int q1x, bool band) {
+               if (band)
+                 hipLaunchKernelGGL(kern_les_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
+                                    q1x + 1, Q, X);
+               else
+                 hipLaunchKernelGGL(kern_les_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q,
+                                    X);
+             });
+  check_launch("les history");
+}
+
+// ---- phase 5: the Hermitian diagonal step (src/dyson_step.h:283-310) and the
+// lesser column's store with its update norm
+void dyson_end(Ctx &c, int T, double mu, double beta, double dt, double *err3) {
+  const WTab *wd = dev_wtab(c);
+  Pt p = make_pt(c, T, mu, beta, dt);
+  hipStream_t s = c.stream;
+  hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, c.dy_Q.as<z2>(), c.dy_QA.as<z2>(),
+                     c.dy_X.as<z2>(), c.dy_scr.as<z2>());
+  hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, c.dy_X.as<z2>(), c.dy_err.as<double>() + 2);
+  check_launch("les");
   c.mark(5);
   if (err3) dyson_read_err(c, err3);
+}
+
+}  // namespace
+
+void dyson_step(Ctx &c, int T, double mu, double beta, double dt, double *err3) {
+  dyson_begin(c, T, mu, beta, dt);
+  dyson_ret_one(c, T, mu, beta, dt);
+  dyson_mid(c, T, mu, beta, dt);
+  dyson_les_one(c, T, mu, beta, dt);
+  dyson_end(c, T, mu, beta, dt, err3);
+}
+
+DysonGroup::DysonGroup(int device_) : device(device_) {
+  NED_HIP(hipSetDevice(device));
+  NED_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  NED_HIP(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
+  NED_HIP(hipEventCreateWithFlags(&evm, hipEventDisableTiming));
+  for (hipEvent_t &e : evs) NED_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+DysonGroup::~DysonGroup() {
+  (void)hipSetDevice(device);
+  (void)hipStreamSynchronize(s);
+  (void)hipStreamSynchronize(sd);
+  for (hipEvent_t e : evr) (void)hipEventDestroy(e);
+  for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+  if (evm) (void)hipEventDestroy(evm);
+  if (s) (void)hipStreamDestroy(s);
+  if (sd) (void)hipStreamDestroy(sd);
+  if (host_args) (void)hipHostFree(host_args);
+}
+
+// The group's stream waits for every member's stream (and then the members
+// wait for the group's) through one event per member.
+static void group_join_in(DysonGroup &g, const std::vector<Ctx *> &cs) {
+  for (size_t i = 0; i < cs.size(); i++) {
+    NED_HIP(hipEventRecord(g.evr[i], cs[i]->stream));
+    NED_HIP(hipStreamWaitEvent(g.s, g.evr[i], 0));
+  }
+}
+static void group_join_out(DysonGroup &g, const std::vector<Ctx *> &cs) {
+  NED_HIP(hipEventRecord(g.evm, g.s));
+  for (Ctx *c : cs) NED_HIP(hipStreamWaitEvent(c->stream, g.evm, 0));
+}
+
+void dyson_step_group(DysonGroup &g, const std::vector<Ctx *> &cs, int T, const double *mu, const double *beta,
+                      const double *dt, double *err3) {
+  const int S = (int)cs.size();
+  require(S > 0, "dyson group: no systems");
+  const Ctx &c0 = *cs[0];
+  int nbmax = 0;
+  for (Ctx *c : cs) {
+    require(c->device == g.device, "dyson group: every system must be on the group's device");
+    require(c->k == c0.k && c->rpanel == c0.rpanel && c->panel_band_max == c0.panel_band_max,
+            "dyson group: the systems must share k and the recurrence schedule");
+    nbmax = std::max(nbmax, bands(*c));
+  }
+  while ((int)g.evr.size() < S) {
+    hipEvent_t e;
+    NED_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    g.evr.push_back(e);
+  }
+  for (int i = 0; i < S; i++) dyson_begin(*cs[i], T, mu[i], beta[i], dt[i]);
+  // the members' chain operands, copied from pinned host memory (the previous
+  // copy has completed: every pass ends with a read of the update norms)
+  const size_t bytes = (size_t)S * sizeof(ChainArg);
+  if (g.host_cap < bytes) {
+    if (g.host_args) NED_HIP(hipHostFree(g.host_args));
+    g.host_args = nullptr;
+    NED_HIP(hipHostMalloc(&g.host_args, bytes));
+    g.host_cap = bytes;
+  }
+  g.args.ensure(bytes);
+  ChainArg *ha = reinterpret_cast<ChainArg *>(g.host_args);
+  for (int i = 0; i < S; i++) {
+    Ctx &c = *cs[i];
+    ha[i] = ChainArg{make_pt(c, T, mu[i], beta[i], dt[i]), c.dy_QA.as<z2>(), c.dy_Q.as<z2>(), c.dy_X.as<z2>(),
+                     c.Minv.as<z2>(), c.dy_d2.as<double>(), bands(c)};
+  }
+  group_join_in(g, cs);
+  NED_HIP(hipMemcpyAsync(g.args.p, g.host_args, bytes, hipMemcpyHostToDevice, g.s));
+  const ChainArg *args = g.args.as<ChainArg>();
+  const WTab *wd = dev_wtab(*cs[0]);
+  recurrence(c0.k, T, panel_blocks(c0), c0.panel_band_max, g.s, g.sd, g.evm, g.evs,
+             [&](int nn, int q1x) {
+               hipLaunchKernelGGL(kern_ret_step_band_grp, dim3(nbmax, q1x - nn, S), dim3(256), 0, g.s, args, wd, nn);
+             },
+             [&](int q0, int q1, int q1x, bool band) {
+               if (band)
+                 hipLaunchKernelGGL(kern_ret_panel_band_grp, dim3(T - q1x, nbmax, S), dim3(256), 0, g.sd, args, wd,
+                                    q0, q1, q1x + 1);
+               else
+                 hipLaunchKernelGGL(kern_ret_panel_grp, dim3(T - q1x, S), dim3(256), 0, g.sd, args, wd, q0, q1,
+                                    q1x + 1);
+             });
+  check_launch("group ret history");
+  group_join_out(g, cs);
+  for (int i = 0; i < S; i++) dyson_mid(*cs[i], T, mu[i], beta[i], dt[i]);
+  group_join_in(g, cs);
+  recurrence(c0.k, T, panel_blocks(c0), c0.panel_band_max, g.s, g.sd, g.evm, g.evs,
+             [&](int l, int q1x) {
+               hipLaunchKernelGGL(kern_les_step_band_grp, dim3(nbmax, q1x - l, S), dim3(256), 0, g.s, args, wd, l);
+             },
+             [&](int q0, int q1, int q1x, bool band) {
+               if (band)
+                 hipLaunchKernelGGL(kern_les_panel_band_grp, dim3(T - q1x, nbmax, S), dim3(256), 0, g.sd, args, wd,
+                                    q0, q1, q1x + 1);
+               else
+                 hipLaunchKernelGGL(kern_les_panel_grp, dim3(T - q1x, S), dim3(256), 0, g.sd, args, wd, q0, q1,
+                                    q1x + 1);
+             });
+  check_launch("group les history");
+  group_join_out(g, cs);
+  for (int i = 0; i < S; i++) dyson_end(*cs[i], T, mu[i], beta[i], dt[i], err3 ? err3 + 3 * i : nullptr);
 }
 
 void dyson_read_err(Ctx &c, double *err3) {

@@ -242,4 +242,56 @@ void do_tstp(Ctx &c, int T, const double *h0, double mu, double beta, double dt,
   if (err_out) *err_out = err;
 }
 
+void batch_do_tstp(DysonGroup &g, const std::vector<Ctx *> &cs, int T, const double *const *h0, const double *mu,
+                   const double *beta, const double *dt, Prec prec, int corr_max, double corr_tol, int *iters,
+                   double *err, int *status) {
+  const int S = (int)cs.size();
+  std::vector<int> live, its(S, 0);
+  std::vector<double> errs(S, 0.0);
+  for (int i = 0; i < S; i++) {
+    Ctx &c = *cs[i];
+    if (status) status[i] = 0;
+    if (!(T > c.k && T <= c.nt)) {  // this system cannot take the step
+      if (status) status[i] = 1;    // NED_ERR_ARG
+      continue;
+    }
+    extrapolate(c, T);
+    hf_upload_h0(c, h0[i]);
+    live.push_back(i);
+  }
+  for (int pass = 0; pass < corr_max && !live.empty(); pass++) {
+    std::vector<Ctx *> group;
+    std::vector<double> m, b, d, e(3 * live.size());
+    for (int i : live) {
+      Ctx &c = *cs[i];
+      hf_kernels(c, T);
+      sigma_gf2(c, T, prec);
+      touch_time(c, T);
+      group.push_back(&c);
+      m.push_back(mu[i]);
+      b.push_back(beta[i]);
+      d.push_back(dt[i]);
+    }
+    dyson_step_group(g, group, T, m.data(), b.data(), d.data(), e.data());
+    std::vector<int> next;
+    for (size_t q = 0; q < live.size(); q++) {
+      const int i = live[q];
+      errs[i] = e[3 * q] + e[3 * q + 1] + e[3 * q + 2];
+      its[i]++;
+      if (!std::isfinite(errs[i])) {
+        if (status) status[i] = 4;  // NED_ERR_NONFINITE
+        continue;
+      }
+      if (corr_tol > 0 && errs[i] < corr_tol) continue;
+      next.push_back(i);
+    }
+    live.swap(next);
+  }
+  for (Ctx *c : cs) NED_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < S; i++) {
+    if (iters) iters[i] = its[i];
+    if (err) err[i] = errs[i];
+  }
+}
+
 }  // namespace ned

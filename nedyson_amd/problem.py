@@ -198,6 +198,28 @@ class Problem:
         return free_gf(self.fock, self.mu, self.beta, self.dt, self.nt, self.ntau, self.k, self.Gmat)
 
 
+BATCH_NAOS = (8, 12, 16, 24, 32, 48)
+
+
+def batch_systems(nsys=16, seed=3000, **over):
+    """The batched configuration: nsys independent systems, nao drawn
+    uniformly from BATCH_NAOS (seeded), naux = 4 nao, nao/2 electrons,
+    beta = 10, 64 GLL nodes, dt = 0.02, nt = 1500, k = 5, the same generators
+    (level spacing 0.3), each with its own random quench of norm drawn
+    uniformly from [0.1, 0.3].  over overrides any Problem argument (b_sd, nt,
+    ...) for every system."""
+    rng = np.random.default_rng(seed)
+    naos = rng.choice(BATCH_NAOS, size=nsys)
+    norms = rng.uniform(0.1, 0.3, size=nsys)
+    out = []
+    for i, (n, qn) in enumerate(zip(naos, norms)):
+        p = dict(nao=int(n), naux=4 * int(n), nelec=int(n) // 2, beta=10.0, ntau=63, nt=1500, dt=0.02, k=5,
+                 seed=seed + 1 + i, spacing=0.3, b_sd=0.1, quench="random", quench_norm=float(qn))
+        p.update(over)
+        out.append(Problem(**p))
+    return out
+
+
 def config(idx, **over):
     """The benchmark configurations c0..c2 (spin-restricted GF2, k=5)."""
     if idx == 0:

@@ -84,3 +84,96 @@ def test_batch_reports_the_failing_system():
     b.close()
     for c in ctxs + [short]:
         c.close()
+
+
+def test_batch_threads_mode_matches_grouped():
+    """The two batch modes (grouped Dyson launches, and one host thread and
+    stream per system) give the same G."""
+    nt = 16
+    ps = _systems(nt)
+    outs = []
+    for grouped in (True, False):
+        ctxs = [_booted(p) for p in ps]
+        b = ned.Batch(ctxs, grouped=grouped)
+        for T in range(ps[0].k + 1, nt + 1):
+            b.do_tstp(T, [p.h0 for p in ps], [p.mu for p in ps], [p.beta for p in ps], [p.dt for p in ps], "f32", 3)
+        outs.append([c.download(ned.NED_G) for c in ctxs])
+        b.close()
+        for c in ctxs:
+            c.close()
+    for a, b_ in zip(*outs):
+        for comp in ("ret", "les", "tv"):
+            assert np.array_equal(getattr(a, comp), getattr(b_, comp)), comp
+
+
+def _equilibrium(p):
+    with ned.Context(p.nao, p.naux, p.k, p.ntau, p.k) as c:
+        c.set_coulomb(p.B)
+        mu, F, it, err = c.matsubara_gf2(p.h1, p.beta, nelec=float(p.nelec), mu=p.mu, tol=1e-9)
+        return mu, F, c.read_blocks(ned.NED_G, ned.NED_MAT, 0, p.ntau + 1)
+
+
+def _equilibrium_ctx(p, nt, mu, F, GM):
+    c = ned.Context(p.nao, p.naux, nt, p.ntau, p.k)
+    c.set_coulomb(p.B)
+    c.write_blocks(ned.NED_G, ned.NED_MAT, 0, GM)
+    c.set_matsubara(p.beta)
+    c.g0_from_h0(p.k, F, mu, p.beta, p.dt)
+    try:
+        c.do_boot(F, mu, p.beta, p.dt, "f32", 60, 1e-9)
+    except ned.NedError as e:  # the update stalled above tol: keep the last iterate
+        assert e.code == 5, e
+    return c
+
+
+@pytest.mark.timeout(900)
+def test_batch_configuration_16_uneven_systems(orc):
+    """The batched configuration's systems (problem.batch_systems: 16 systems,
+    nao drawn from {8, 12, 16, 24, 32, 48}, naux = 4 nao, 64 GLL nodes,
+    dt = 0.02, each with its own quench), B entries with sigma 0.05, each from
+    its GF2 equilibrium through 12 steps of FP32 Sigma and FP64 Dyson: the
+    grouped batch equals the systems stepped alone, bit for bit, and the
+    nao = 8 system equals the oracle's FP64 run from the same equilibrium
+    within the FP32 bound."""
+    from nedyson_amd.green import GreenFunc
+    nt = 17
+    ps = problem.batch_systems(b_sd=0.05, nt=nt)
+    print(f"[batch16] nao {[p.nao for p in ps]}", flush=True)
+    alone = []
+    eq = []
+    for p in ps:
+        mu, F, GM = _equilibrium(p)
+        eq.append((mu, F, GM))
+        c = _equilibrium_ctx(p, nt, mu, F, GM)
+        for T in range(p.k + 1, nt + 1):
+            c.do_tstp(T, p.h0, mu, p.beta, p.dt, "f32", 3)
+        alone.append(c.download(ned.NED_G))
+        c.close()
+    print("[batch16] alone done", flush=True)
+    ctxs = [_equilibrium_ctx(p, nt, *e) for p, e in zip(ps, eq)]
+    b = ned.Batch(ctxs)
+    for T in range(ps[0].k + 1, nt + 1):
+        it, err = b.do_tstp(T, [p.h0 for p in ps], [m for m, _, _ in eq], [p.beta for p in ps],
+                            [p.dt for p in ps], "f32", 3)
+        assert np.all(np.isfinite(err))
+    for p, c, ref in zip(ps, ctxs, alone):
+        d = c.download(ned.NED_G)
+        for comp in ("ret", "les", "tv"):
+            assert np.array_equal(getattr(d, comp), getattr(ref, comp)), \
+                (p.nao, comp, np.max(np.abs(getattr(d, comp) - getattr(ref, comp))))
+    b.close()
+    for c in ctxs:
+        c.close()
+    i8 = [p.nao for p in ps].index(8)
+    p, (mu, F, GM) = ps[i8], eq[i8]
+    G = problem.free_gf(F, mu, p.beta, p.dt, nt, p.ntau, p.k, GM)
+    S = GreenFunc(nt, p.ntau, p.nao)
+    hmf = np.zeros((nt + 1, p.nao, p.nao), dtype=complex)
+    D = orc.Dyson(nt, p.ntau, p.nao, p.k)
+    D.do_boot(p.V, G, S, hmf, F, mu, p.beta, p.dt, 60, 1e-12)
+    for T in range(p.k + 1, nt + 1):
+        D.do_tstp(p.V, T, G, S, hmf, p.h0, mu, p.beta, p.dt, 3)
+    for comp in ("ret", "les", "tv"):
+        err = np.max(np.abs(getattr(alone[i8], comp) - getattr(G, comp)))
+        print(f"[batch16] nao=8 vs oracle {comp}: {err:.2e}", flush=True)
+        assert err < 1e-6, comp
