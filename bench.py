@@ -319,8 +319,8 @@ def batch_bench(args, log):
     for p in ps:
         gf2_equilibrium(ned, p, log)
     log(f"[batch] {len(ps)} systems, nao {[p.nao for p in ps]}, history T0={T0}")
-    rates = {}
-    for mode in ("grouped", "threads", "sequential"):
+    rates = {"grouped": None, "threads": None, "sequential": None}
+    for mode in args.batch_modes.split(","):
         ctxs = [window_context(ned, p, T0, nt) for p in ps]
         b = ned.Batch(ctxs, grouped=(mode == "grouped")) if mode != "sequential" else None
         times = []
@@ -414,6 +414,8 @@ def main():
     ap.add_argument("--sweep", default=None,
                     help="comma-separated history lengths T0: print per-step times at each and the fitted "
                          "whole-run average instead of the benchmark line")
+    ap.add_argument("--batch-modes", default="grouped,threads,sequential",
+                    help="batched configuration (--config 3): the modes to time, comma-separated")
     ap.add_argument("--traj-steps", type=int, default=100,
                     help="also run the precision check: the configuration's own system (its sigma_B) from the GF2 "
                          "equilibrium through the quench for this many steps in FP64, FP32 and bf16x3 in lockstep")
