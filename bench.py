@@ -50,6 +50,9 @@ STABLE_B_SD = {2: 0.01}
 # 1e100 in four passes of the first step after the bootstrap (0.07: N leaves
 # 32 by 0.24 and the run overflows at step 41).  0.05 propagates.
 TRAJ_B_SD = {2: 0.05}
+# Coulomb-factor spread of the learned-predictor runs (nao = 48, dt = 0.02),
+# for the same reason: the propagated run must stay a solution.
+PRED_B_SD = 0.05
 
 
 def gf2_equilibrium(ned, p, log):
@@ -352,6 +355,92 @@ def batch_bench(args, log):
                                              f"FP32 Sigma / FP64 Dyson", "sigma_B": bsd, "T0": T0}}), flush=True)
 
 
+def predictor_bench(args, log):
+    """The learned-predictor configuration (problem.predictor_system: nao=48,
+    naux=192, beta=20, 96 GLL nodes, dt=0.02, quench norms 0.05..0.4), at
+    reduced counts: `pred_train` training systems and `pred_test` test
+    systems (the last ones of the 64), each propagated from its GF2
+    equilibrium to step `pred_nt`.  The training runs iterate the FP64
+    corrector to 1e-10 and record the polynomial extrapolation's error at the
+    predictor points; the network (width 512) trains on the GPU; each test
+    system then runs three times from the same equilibrium, the corrector
+    iterated to 1e-8 -- polynomial guess, learned guess with FP32 inference,
+    learned guess with BF16 inference -- and the corrector passes and wall
+    time over the steps where the network acts are compared."""
+    import nedyson_amd as ned
+    from nedyson_amd import predictor, problem
+    nt, bsd = args.pred_nt, args.b_sd if args.b_sd is not None else PRED_B_SD
+    t_all = time.perf_counter()
+
+    def system(i):
+        p = problem.predictor_system(i, nt=nt, b_sd=bsd)
+        gf2_equilibrium(ned, p, log)
+        return p
+
+    def booted(p):
+        c = ned.Context(p.nao, p.naux, nt, p.ntau, p.k)
+        c.set_coulomb(p.B)
+        c.write_blocks(ned.NED_G, ned.NED_MAT, 0, p.Gmat)
+        c.set_matsubara(p.beta)
+        c.g0_from_h0(p.k, p.fock, p.mu, p.beta, p.dt)
+        c.do_boot(p.fock, p.mu, p.beta, p.dt, "f64", 60, 1e-10)
+        return c
+
+    X, Y = [], []
+    for i in range(args.pred_train):
+        p = system(i)
+        t0 = time.perf_counter()
+        with booted(p) as c:
+            r = []
+            for T in range(p.k + 1, nt + 1):
+                c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, "f64", 20, 1e-10)
+                if T >= p.k + 33:
+                    r.append(c.predictor_residual(T))
+        x, y, _ = predictor.dataset(r)
+        X.append(x)
+        Y.append(y)
+        log(f"[pred] training system {i}: quench norm {p.quench_norm:.3f}, {len(r)} steps recorded "
+            f"({time.perf_counter() - t0:.1f} s)")
+    X, Y = predictor.subsample(np.concatenate(X), np.concatenate(Y), args.pred_rows)
+    t0 = time.perf_counter()
+    net, losses = predictor.train(X, Y, width=512, epochs=args.pred_epochs, batch=256, lr=1e-3, seed=0)
+    t_train = time.perf_counter() - t0
+    log(f"[pred] trained on {X.shape[0]} rows: loss {losses[0]:.3e} -> {losses[-1]:.3e} ({t_train:.1f} s)")
+    modes = {"polynomial": None, "learned_fp32": False, "learned_bf16": True}
+    tot = {m: [0, 0.0, 0] for m in modes}  # passes, seconds, fallbacks
+    t1 = None
+    for j in range(args.pred_test):
+        i = problem.PREDICTOR_NSYS - args.pred_test + j
+        p = system(i)
+        t1 = predictor.first_learned_step(p.k)
+        for m, bf16 in modes.items():
+            with booted(p) as c:
+                if bf16 is not None:
+                    net.set_bf16(bf16)
+                    c.set_predictor(net)
+                steps = [s for s in predictor.run_to_tolerance(c, p, p.k + 1, nt) if s[0] >= t1]
+                fb = c.predictor_stats()[1] if bf16 is not None else 0
+                c.set_predictor(None)
+            n_pass, secs = sum(s[1] for s in steps), sum(s[2] for s in steps)
+            tot[m][0] += n_pass
+            tot[m][1] += secs
+            tot[m][2] += fb
+            log(f"[pred] test system {i} ({m}): {n_pass} passes over {len(steps)} steps, {secs:.2f} s, "
+                f"{fb} fallbacks; per step {[s[1] for s in steps]}")
+    nsteps = args.pred_test * (nt - t1 + 1)
+    out = {m: {"passes_per_step": v[0] / nsteps, "ms_per_step": 1e3 * v[1] / nsteps, "fallbacks": v[2]}
+           for m, v in tot.items()}
+    print(json.dumps({"metric": "corrector passes and time to residual 1e-8, learned vs polynomial guess",
+                      "value": out["polynomial"]["ms_per_step"] / out["learned_bf16"]["ms_per_step"],
+                      "unit": "x (time to 1e-8, polynomial / learned BF16)", "modes": out,
+                      "train_seconds": t_train, "total_seconds": time.perf_counter() - t_all,
+                      "config": {"workload": f"nao=48 naux=192 beta=20 ntau+1=96 dt=0.02, {args.pred_train} "
+                                             f"training and {args.pred_test} test systems, steps {t1}..{nt} "
+                                             f"compared, FP64 Sigma, corrector to 1e-8",
+                                 "sigma_B": bsd, "rows": int(X.shape[0]), "epochs": args.pred_epochs}}),
+          flush=True)
+
+
 def sweep(args, p, log):
     """Per-step time at several history lengths T, and the whole-run average
     over T = k+1..nt from a quadratic fit t(T) = a + b T + c T^2 (the Sigma
@@ -416,6 +505,11 @@ def main():
                          "whole-run average instead of the benchmark line")
     ap.add_argument("--batch-modes", default="grouped,threads,sequential",
                     help="batched configuration (--config 3): the modes to time, comma-separated")
+    ap.add_argument("--pred-train", type=int, default=6, help="learned predictor (--config 4): training systems")
+    ap.add_argument("--pred-test", type=int, default=2, help="learned predictor: test systems")
+    ap.add_argument("--pred-nt", type=int, default=100, help="learned predictor: steps per system")
+    ap.add_argument("--pred-epochs", type=int, default=40, help="learned predictor: training epochs")
+    ap.add_argument("--pred-rows", type=int, default=200000, help="learned predictor: training rows drawn")
     ap.add_argument("--traj-steps", type=int, default=100,
                     help="also run the precision check: the configuration's own system (its sigma_B) from the GF2 "
                          "equilibrium through the quench for this many steps in FP64, FP32 and bf16x3 in lockstep")
@@ -443,6 +537,9 @@ def main():
 
     if args.config == 3:
         batch_bench(args, log)
+        return
+    if args.config == 4:
+        predictor_bench(args, log)
         return
 
     if args.copy_bytes:

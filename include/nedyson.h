@@ -247,6 +247,12 @@ ned_status ned_mlp_adam_step(ned_mlp *m, float lr);
 ned_status ned_mlp_train(ned_mlp *m, const float *X, const float *T, int rows, int epochs, int batch, float lr,
                          unsigned long long seed, float *loss_per_epoch);
 ned_status ned_mlp_predict(ned_mlp *m, const float *X, int rows, float *Y);
+/* inference (ned_mlp_predict and the timestep's guess) with the weights and
+ * each layer's input rounded to bf16 and the products on the bf16 matrix
+ * cores, FP32 accumulation, bias and tanh (on != 0), or all in FP32 (0, the
+ * default); training is always FP32.  BF16 needs n_in % 64 == 0 and width,
+ * n_out % 128 == 0.  No counterpart in the reference. */
+ned_status ned_mlp_set_bf16(ned_mlp *m, int on);
 /* initial guess of ned_do_tstp: NULL = polynomial only, else a network with
  * n_in = 768, n_out = 128 on the context's device.  The context keeps the
  * pointer: unset it (or destroy the context) before destroying the network,

@@ -36,7 +36,7 @@ EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "n
            "ned_energy", "ned_g0_from_h0", "ned_tti_extend", "ned_profile", "ned_profile_read", "ned_gll_nodes",
            "ned_integrator", "ned_mlp_create", "ned_mlp_destroy", "ned_mlp_nparams", "ned_mlp_get_params",
            "ned_mlp_set_params", "ned_mlp_loss_grad", "ned_mlp_adam_step", "ned_mlp_train", "ned_mlp_predict",
-           "ned_set_predictor", "ned_predictor_residual", "ned_predictor_stats", "ned_batch_create",
+           "ned_mlp_set_bf16", "ned_set_predictor", "ned_predictor_residual", "ned_predictor_stats", "ned_batch_create",
            "ned_batch_destroy", "ned_batch_do_tstp", "ned_gemm_variant_count", "ned_gemm_variant_name",
            "ned_set_graph", "ned_batch_set_grouped"]
 
@@ -112,6 +112,7 @@ def lib():
     L.ned_mlp_adam_step.argtypes = [_vp, C.c_float]
     L.ned_mlp_train.argtypes = [_vp, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _fp]
     L.ned_mlp_predict.argtypes = [_vp, _fp, C.c_int, _fp]
+    L.ned_mlp_set_bf16.argtypes = [_vp, C.c_int]
     L.ned_set_predictor.argtypes = [_vp, _vp]
     L.ned_predictor_residual.argtypes = [_vp, C.c_int, _fp]
     L.ned_predictor_stats.argtypes = [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
@@ -433,6 +434,11 @@ class Mlp:
         Y = np.zeros((X.shape[0], self.n_out), dtype=np.float32)
         _check(lib().ned_mlp_predict(self.h, _fptr(X), X.shape[0], _fptr(Y)))
         return Y
+
+    def set_bf16(self, on=True):
+        """Inference (predict and the timestep's guess) with bf16 weights and
+        activations on the bf16 matrix cores, FP32 accumulation; training stays FP32."""
+        _check(lib().ned_mlp_set_bf16(self.h, 1 if on else 0))
 
 
 class Batch:

@@ -500,6 +500,7 @@ ned_status ned_mlp_set_params(ned_mlp *m, const float *in) {
   if (!in) { g_err = "null argument"; return NED_ERR_ARG; }
   return mlp_guard(m, [&](ned::Mlp &x) {
     NED_HIP(hipMemcpy(x.params.p, in, x.nparam * sizeof(float), hipMemcpyHostToDevice));
+    x.version++;
     x.reset_adam();
   });
 }
@@ -542,9 +543,18 @@ ned_status ned_mlp_predict(ned_mlp *m, const float *X, int rows, float *Y) {
     dX.ensure((size_t)rows * x.n_in * sizeof(float));
     dY.ensure((size_t)rows * x.n_out * sizeof(float));
     NED_HIP(hipMemcpy(dX.p, X, (size_t)rows * x.n_in * sizeof(float), hipMemcpyHostToDevice));
-    x.forward(dX.as<float>(), rows, dY.as<float>(), x.stream);
+    x.infer(dX.as<float>(), rows, dY.as<float>(), x.stream);
     NED_HIP(hipMemcpyAsync(Y, dY.p, (size_t)rows * x.n_out * sizeof(float), hipMemcpyDeviceToHost, x.stream));
     NED_HIP(hipStreamSynchronize(x.stream));
+  });
+}
+
+ned_status ned_mlp_set_bf16(ned_mlp *m, int on) {
+  return mlp_guard(m, [&](ned::Mlp &x) {
+    if (on)
+      ned::require(x.n_in % 64 == 0 && x.w % 128 == 0 && x.n_out % 128 == 0,
+                   "BF16 inference needs n_in a multiple of 64, width and n_out of 128");
+    x.bf16_infer = on != 0;
   });
 }
 

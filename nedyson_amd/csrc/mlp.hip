@@ -1,8 +1,10 @@
 // The learned next-slice predictor: a three-layer perceptron
 //   x (n_in) -> tanh(W1 x + b1) -> tanh(W2 h1 + b2) -> W3 h2 + b3 (n_out)
-// trained and evaluated on the GPU.  Every matrix product goes through this
-// library's own MFMA GEMM (gemm(), FP32); the rest are the element-wise,
-// reduction and Adam kernels below.  Rows of a batch are independent.
+// trained and evaluated on the GPU.  Training's matrix products go through
+// this library's own MFMA GEMM (gemm(), FP32); inference can instead run with
+// bf16 weights and activations on the bf16 matrix cores (forward_bf16, FP32
+// accumulation).  The rest are the element-wise, reduction and Adam kernels
+// below.  Rows of a batch are independent.
 //
 // The network has no counterpart in the reference: it replaces the
 // polynomial predictor dyson::Extrapolate (src/dyson.cpp:47-117) as the
@@ -82,6 +84,30 @@ __global__ void kern_adam(float *p, const float *g, float *m, float *v, long lon
   m[i] = mi;
   v[i] = vi;
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// hi[r][c] = bf16(x[r][c]) (round to nearest even) for r < rows, 0 for the
+// padding rows up to rows_pad
+__global__ void kern_to_bf16(const float *x, long long rows, long long rows_pad, int cols, uint16_t *hi) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows_pad * cols) return;
+  hi[i] = i / cols < rows ? f2bf(x[i]) : (uint16_t)0;
+}
+
+// the next layer's bf16 input from a layer's product o: bf16(tanh(o + b)) for
+// r < rows, 0 for the padding rows up to rows_pad
+__global__ void kern_bias_tanh_bf16(const float *o, const float *b, long long rows, long long rows_pad, int cols,
+                                    uint16_t *out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows_pad * cols) return;
+  out[i] = i / cols < rows ? f2bf(tanhf(o[i] + b[i % cols])) : (uint16_t)0;
+}
+
+// y[r][c] = o[r][c] + b[c] for r < rows
+__global__ void kern_bias_rows(const float *o, const float *b, long long rows, int cols, float *y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * cols) return;
+  y[i] = o[i] + b[i % cols];
 }
 
 __global__ void kern_gather_rows(const float *src, const int *idx, int rows, int cols, float *dst) {
@@ -190,6 +216,56 @@ void Mlp::forward(const float *X, int rows, float *Y, hipStream_t s) {
   check_launch("mlp forward");
 }
 
+void Mlp::forward_bf16(const float *X, int rows, float *Y, hipStream_t s) {
+  require(n_in % 64 == 0 && w % 128 == 0 && n_out % 128 == 0,
+          "mlp: BF16 inference needs n_in a multiple of 64, width and n_out of 128");
+  const int rp = (rows + 127) / 128 * 128;
+  const long long nw = (long long)w * n_in + (long long)w * w + (long long)n_out * w;
+  if (rp > cap_pad) {
+    abf.ensure((size_t)rp * std::max(n_in, w) * sizeof(uint16_t));
+    pad_out.ensure((size_t)rp * std::max(w, n_out) * sizeof(float));
+    cap_pad = rp;
+  }
+  const size_t nz = std::max<size_t>((size_t)nw, (size_t)cap_pad * std::max(n_in, w));
+  if (zero.bytes < nz * sizeof(uint16_t)) {  // a fill kernel (rows = 0: every entry padding)
+    zero.ensure(nz * sizeof(uint16_t));
+    hipLaunchKernelGGL(kern_to_bf16, dim3(nb((long long)nz)), dim3(256), 0, s, nullptr, 0LL, (long long)nz, 1,
+                       zero.as<uint16_t>());
+  }
+  float *P = params.as<float>();
+  if (bf16_version != version) {  // the weights as bf16, once per parameter update
+    wbf.ensure((size_t)nw * sizeof(uint16_t));
+    uint16_t *W = wbf.as<uint16_t>();
+    const long long n1 = (long long)w * n_in, n2 = (long long)w * w, n3 = (long long)n_out * w;
+    hipLaunchKernelGGL(kern_to_bf16, dim3(nb(n1)), dim3(256), 0, s, P + off[0], (long long)w, (long long)w, n_in, W);
+    hipLaunchKernelGGL(kern_to_bf16, dim3(nb(n2)), dim3(256), 0, s, P + off[2], (long long)w, (long long)w, w, W + n1);
+    hipLaunchKernelGGL(kern_to_bf16, dim3(nb(n3)), dim3(256), 0, s, P + off[4], (long long)n_out, (long long)n_out, w,
+                       W + n1 + n2);
+    bf16_version = version;
+  }
+  const uint16_t *W1 = wbf.as<uint16_t>(), *W2 = W1 + (long long)w * n_in, *W3 = W2 + (long long)w * w;
+  const uint16_t *Z = zero.as<uint16_t>();
+  uint16_t *A = abf.as<uint16_t>();
+  float *O = pad_out.as<float>();
+  // O (rp x out) = A Wl^T: the split-bf16 product with zero low planes, which is
+  // exactly bf16 x bf16 with FP32 accumulation
+  auto mm = [&](int k, const uint16_t *Wl, int out) {
+    gemm_nt_bf16x3(NtOp{A, Z, k, 0}, NtOp{Wl, Z, k, 0}, O, out, 0, rp, out, k, 1, 1.0f, 0, s, false);
+  };
+  hipLaunchKernelGGL(kern_to_bf16, dim3(nb((long long)rp * n_in)), dim3(256), 0, s, X, (long long)rows, (long long)rp,
+                     n_in, A);
+  mm(n_in, W1, w);
+  hipLaunchKernelGGL(kern_bias_tanh_bf16, dim3(nb((long long)rp * w)), dim3(256), 0, s, O, P + off[1], (long long)rows,
+                     (long long)rp, w, A);
+  mm(w, W2, w);
+  hipLaunchKernelGGL(kern_bias_tanh_bf16, dim3(nb((long long)rp * w)), dim3(256), 0, s, O, P + off[3], (long long)rows,
+                     (long long)rp, w, A);
+  mm(w, W3, n_out);
+  hipLaunchKernelGGL(kern_bias_rows, dim3(nb((long long)rows * n_out)), dim3(256), 0, s, O, P + off[5], (long long)rows,
+                     n_out, Y);
+  check_launch("mlp forward (bf16)");
+}
+
 double Mlp::loss_grad(const float *X, const float *T, int rows, hipStream_t s) {
   ensure_rows(rows);
   float *P = params.as<float>(), *Gd = grads.as<float>();
@@ -221,6 +297,7 @@ double Mlp::loss_grad(const float *X, const float *T, int rows, hipStream_t s) {
 
 void Mlp::adam(float lr, hipStream_t s) {
   adam_t++;
+  version++;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - std::pow(b1, (float)adam_t), c2 = 1.0f - std::pow(b2, (float)adam_t);
   hipLaunchKernelGGL(kern_adam, dim3(nb(nparam)), dim3(256), 0, s, params.as<float>(), grads.as<float>(),

@@ -211,7 +211,7 @@ void do_tstp(Ctx &c, int T, const double *h0, double mu, double beta, double dt,
     c.pred_sig.ensure(es * NG * sizeof(float));
     hipLaunchKernelGGL(kern_pred_features, dim3(nb((long long)es)), dim3(256), 0, c.stream, sl, (int)es,
                        c.pred_X.as<float>(), c.pred_sig.as<float>());
-    m.forward(c.pred_X.as<float>(), (int)es, c.pred_Y.as<float>(), c.stream);
+    m.infer(c.pred_X.as<float>(), (int)es, c.pred_Y.as<float>(), c.stream);
     hipLaunchKernelGGL(kern_pred_apply, dim3(nb((long long)2 * NPT * es)), dim3(256), 0, c.stream, c.G.ret.as<z2>(),
                        c.G.les.as<z2>(), T, c.k, c.nao, c.pred_Y.as<float>(), c.pred_sig.as<float>());
     check_launch("predictor apply");

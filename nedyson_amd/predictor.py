@@ -11,12 +11,21 @@ Trajectories come from the method itself: run ned_do_tstp with the
 polynomial predictor and read each step's errors with
 Context.predictor_residual(T).
 """
+import time
+
 import numpy as np
 
 from . import PRED_INPUTS, PRED_OUTPUTS, Mlp
 
 HIST = 6
 GROUPS = 4  # (component, zone) groups of 32 features, each with its own scale
+
+
+def first_learned_step(k):
+    """The first step whose guess can use the network: the extrapolation
+    errors are recorded from step k + 33 on (the 16 quench columns and the 16
+    next to the diagonal must not overlap) and the network needs six."""
+    return k + 33 + HIST
 
 
 def dataset(residuals):
@@ -49,6 +58,29 @@ def collect(ctx, p, t_from, t_to, prec="f64", corr=3, corr_tol=0.0):
         ctx.do_tstp(T, p.h0, p.mu, p.beta, p.dt, prec, corr, corr_tol)
         if T >= p.k + 33:
             out.append(ctx.predictor_residual(T))
+    return out
+
+
+def subsample(X, Y, rows, seed=0):
+    """At most `rows` rows of (X, Y), drawn without replacement (seeded)."""
+    if X.shape[0] <= rows:
+        return X, Y
+    sel = np.sort(np.random.default_rng(seed).choice(X.shape[0], rows, replace=False))
+    return X[sel], Y[sel]
+
+
+def run_to_tolerance(ctx, p, t_from, t_to, prec="f64", corr_tol=1e-8, corr_max=20):
+    """do_tstp for t_from..t_to with the corrector iterated until its update
+    norm is below corr_tol (at most corr_max passes).  Returns per step
+    (T, passes, seconds); the wall time includes the guess (polynomial, and
+    the network where the context has one) and every pass."""
+    out = []
+    for T in range(t_from, t_to + 1):
+        t0 = time.perf_counter()
+        it, err = ctx.do_tstp(T, p.h0, p.mu, p.beta, p.dt, prec, corr_max, corr_tol)
+        out.append((T, it, time.perf_counter() - t0))
+        if not err < corr_tol:
+            raise RuntimeError(f"step {T}: corrector update {err:.3e} after {it} passes")
     return out
 
 

@@ -220,6 +220,24 @@ def batch_systems(nsys=16, seed=3000, **over):
     return out
 
 
+PREDICTOR_NSYS = 64  # systems 0..47 train the network, 48..63 test it
+
+
+def predictor_system(i, seed=4000, **over):
+    """System i of the learned-predictor configuration: 64 independent nao = 48
+    systems (naux = 192, 24 electrons, beta = 20, 96 GLL nodes, dt = 0.02,
+    nt = 4000, k = 5, the same generators), each with its own random quench of
+    norm drawn uniformly from [0.05, 0.4]; systems 0..47 train the network,
+    48..63 test it.  over overrides any Problem argument."""
+    if not 0 <= i < PREDICTOR_NSYS:
+        raise ValueError(i)
+    norms = np.random.default_rng(seed).uniform(0.05, 0.4, size=PREDICTOR_NSYS)
+    p = dict(nao=48, naux=192, nelec=24, beta=20.0, ntau=95, nt=4000, dt=0.02, k=5, seed=seed + 1 + i,
+             spacing=0.3, b_sd=0.1, quench="random", quench_norm=float(norms[i]))
+    p.update(over)
+    return Problem(**p)
+
+
 def config(idx, **over):
     """The benchmark configurations c0..c2 (spin-restricted GF2, k=5)."""
     if idx == 0:
