@@ -285,11 +285,26 @@ def precision_trajectory(ned, p, nsteps, every, corr, log, precs=("f32", "bf16x3
     return out
 
 
-def window_context(ned, p, T0, nt):
+STORAGE = {"whole": 0, "f32": 1, "ring": 2, "f32ring": 3}  # ned_ctx_create_ex flags
+
+
+def storage_bytes(p, nt, storage):
+    """Device bytes of the context's time histories for a storage mode: the
+    G^R / G^< triangles (or their ring of k + 3 rows), the Sigma^R / Sigma^<
+    triangles in FP64 or FP32, and G^rceil / Sigma^rceil."""
+    es, ntri = p.nao * p.nao, (nt + 1) * (nt + 2) // 2
+    f = STORAGE[storage]
+    g = (min(nt + 1, p.k + 3) * (nt + 1) if f & 2 else ntri) * es * 16
+    sg = ntri * es * (8 if f & 1 else 16)
+    tv = (nt + 1) * (p.ntau + 1) * es * 16
+    return 2 * g + 2 * sg + 2 * tv
+
+
+def window_context(ned, p, T0, nt, storage="whole"):
     """A context holding the benchmark history t <= T0: the free G of the
     equilibrium Fock matrix with Sigma(T0) continued time-translation
     invariantly."""
-    c = ned.Context(p.nao, p.naux, nt, p.ntau, p.k)
+    c = ned.Context(p.nao, p.naux, nt, p.ntau, p.k, storage=STORAGE[storage])
     c.set_coulomb(p.B)
     c.write_blocks(ned.NED_G, ned.NED_MAT, 0, p.Gmat)
     c.set_matsubara(p.beta)
@@ -450,7 +465,7 @@ def sweep(args, p, log):
     rows = []
     for T0 in Ts:
         W, K = args.warmup, args.steps
-        c = window_context(ned, p, T0, T0 + W + K)
+        c = window_context(ned, p, T0, T0 + W + K, args.storage)
         times, split = [], []
         for i, T in enumerate(range(T0 + 1, T0 + W + K + 1)):
             if i >= W:
@@ -465,11 +480,12 @@ def sweep(args, p, log):
         pr = c.profile_read()
         c.close()
         ms = 1e3 * float(np.mean(times))
+        gb = storage_bytes(p, T0 + W + K, args.storage) / 1e9
         rows.append({"T": T0 + W + 1, "ms_per_step": ms, "sigma_ms_per_eval": pr["sigma_ms"] / max(pr["sigma_n"], 1),
-                     "dyson_ms_per_step": pr["dyson_ms"] / max(pr["dyson_n"], 1)})
-        log(f"[sweep] T={T0 + W + 1}: {ms:.1f} ms per step")
+                     "dyson_ms_per_step": pr["dyson_ms"] / max(pr["dyson_n"], 1), "history_GB": gb})
+        log(f"[sweep] T={T0 + W + 1}: {ms:.1f} ms per step ({args.storage} storage, histories {gb:.1f} GB)")
     out = {"metric": "per-step time vs history length", "prec": args.prec, "nao": p.nao, "naux": p.naux,
-           "nt_run": p.nt, "points": rows}
+           "nt_run": p.nt, "storage": args.storage, "points": rows}
     if len(rows) >= 3:
         T = np.array([r["T"] for r in rows], dtype=float)
         y = np.array([r["ms_per_step"] for r in rows])
@@ -505,6 +521,8 @@ def main():
                          "whole-run average instead of the benchmark line")
     ap.add_argument("--batch-modes", default="grouped,threads,sequential",
                     help="batched configuration (--config 3): the modes to time, comma-separated")
+    ap.add_argument("--storage", default="whole", choices=sorted(STORAGE),
+                    help="--sweep: storage of the histories (FP32 Sigma triangles, ring of G rows, or both)")
     ap.add_argument("--pred-train", type=int, default=6, help="learned predictor (--config 4): training systems")
     ap.add_argument("--pred-test", type=int, default=2, help="learned predictor: test systems")
     ap.add_argument("--pred-nt", type=int, default=100, help="learned predictor: steps per system")

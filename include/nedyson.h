@@ -72,6 +72,28 @@ ned_status ned_sync(ned_ctx *ctx);
 /* the sizes the context was created with (GREEN::nt(), ntau(), size1()) */
 ned_status ned_ctx_get_dims(const ned_ctx *ctx, ned_dims *out);
 
+/* Storage of the time histories, chosen at creation (no counterpart in the
+ * reference, whose green_func holds every component whole in FP64,
+ * src/greens.cpp:177-195).  For runs whose FP64 triangles exceed the GPU's
+ * memory (nao = 64, nt = 2000: four 131 GB triangles):
+ *  NED_STORE_SIGMA_F32: the Sigma^R / Sigma^< triangles hold complex FP32;
+ *    the Dyson step and the energy widen every element to FP64 as they read
+ *    it (Sigma^rceil, Sigma^M and all of G stay FP64).  The Sigma
+ *    contractions round their result once more, to FP32.
+ *  NED_STORE_G_RING: G^R / G^< keep only their last k + 3 time rows (a
+ *    timestep reads rows T - k - 1 .. T, src/dyson.cpp:64-116 and
+ *    src/dyson_step.h; Sigma at T reads row T alone).  ned_green_read of a
+ *    row no longer held, and any call that needs one, fail with NED_ERR_ARG;
+ *    ned_green_write of rows of G^R / G^< makes its last row the newest.
+ *    ned_tti_extend of G is not available.
+ * Results equal those of whole FP64 storage: bit for bit for the ring; for
+ * FP32 Sigma within the FP32 rounding of Sigma. */
+enum { NED_STORE_SIGMA_F32 = 1, NED_STORE_G_RING = 2 };
+ned_status ned_ctx_create_ex(ned_ctx **out, const ned_dims *dims, int device, unsigned flags);
+/* the storage flags, and the first and last time row of G^R / G^< held
+ * (0 and nt for whole storage) */
+ned_status ned_ctx_get_storage(const ned_ctx *ctx, unsigned *flags, int *g_first, int *g_last);
+
 /* Replaces molGF2SolverDecomp(Vija, Viaj) (src/mol2bSolver.h:101-111):
  * B_Qij is naux x nao x nao, B[Q][i][j] = Vija(i,j,Q); must be symmetric in
  * (i,j).  At nao = 64 that symmetry also splits the exchange contraction into

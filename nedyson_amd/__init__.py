@@ -28,7 +28,7 @@ _STATUS = {0: "NED_OK", 1: "NED_ERR_ARG", 2: "NED_ERR_NOMEM", 3: "NED_ERR_DEVICE
            5: "NED_ERR_NOCONV", 6: "NED_ERR_STATE"}
 
 # every symbol declared in include/nedyson.h
-EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_destroy", "ned_sync", "ned_ctx_get_dims", "ned_set_coulomb", "ned_green_write",
+EXPORTS = ["ned_last_error", "ned_ctx_create", "ned_ctx_create_ex", "ned_ctx_get_storage", "ned_ctx_destroy", "ned_sync", "ned_ctx_get_dims", "ned_set_coulomb", "ned_green_write",
            "ned_green_read", "ned_hmf_write", "ned_hmf_read", "ned_sigma_gf2_decomp", "ned_sigma_gf2_full",
            "ned_sigma_gf2_decomp_noalloc",
            "ned_sigma_noalloc_scratch_bytes", "ned_hf_decomp",
@@ -72,6 +72,8 @@ def lib():
     L = C.CDLL(LIBPATH)
     L.ned_last_error.restype = C.c_char_p
     L.ned_ctx_create.argtypes = [C.POINTER(_vp), C.POINTER(_Dims), C.c_int]
+    L.ned_ctx_create_ex.argtypes = [C.POINTER(_vp), C.POINTER(_Dims), C.c_int, C.c_uint]
+    L.ned_ctx_get_storage.argtypes = [_vp, C.POINTER(C.c_uint), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ned_ctx_destroy.argtypes = [_vp]
     L.ned_sync.argtypes = [_vp]
     L.ned_set_coulomb.argtypes = [_vp, _dp]
@@ -174,15 +176,27 @@ def integrator(k):
     return dict(zip(["poly_interp", "poly_diff", "poly_integ", "bd", "start", "Omega", "ex"], arrs))
 
 
-class Context:
-    """One system resident on one GPU (see include/nedyson.h)."""
+NED_STORE_SIGMA_F32 = 1
+NED_STORE_G_RING = 2
 
-    def __init__(self, nao, naux, nt, ntau, k, sig=-1, device=0):
+
+class Context:
+    """One system resident on one GPU (see include/nedyson.h).  storage:
+    NED_STORE_SIGMA_F32 and / or NED_STORE_G_RING (ned_ctx_create_ex), 0 for
+    whole FP64 histories."""
+
+    def __init__(self, nao, naux, nt, ntau, k, sig=-1, device=0, storage=0):
         self.nao, self.naux, self.nt, self.ntau, self.k, self.sig = nao, naux, nt, ntau, k, sig
         self.np = ntau + 1
         self.h = _vp()
         d = _Dims(nao, naux, nt, ntau, k, sig)
-        _check(lib().ned_ctx_create(C.byref(self.h), C.byref(d), device))
+        _check(lib().ned_ctx_create_ex(C.byref(self.h), C.byref(d), device, storage))
+
+    def storage(self):
+        """(flags, first G row held, last G row held)"""
+        f, a, b = C.c_uint(0), C.c_int(0), C.c_int(0)
+        _check(lib().ned_ctx_get_storage(self.h, C.byref(f), C.byref(a), C.byref(b)))
+        return f.value, a.value, b.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -64,6 +64,65 @@ long long nblocks_of(const ned::Ctx &c, int comp) {
   return c.np;
 }
 
+// host <-> device copy of the packed blocks [first, first + nb) of one
+// component (complex FP64 on the host): Sigma^R / Sigma^< of a context with
+// FP32 Sigma storage are rounded on the way in and widened on the way out;
+// G^R / G^< of a context with ring storage are copied row by row to and from
+// their ring slots (a read needs the rows held; a write makes its last row
+// the newest, and every row it writes must stay held)
+void triangle_copy(ned::Ctx &c, int which, int comp, ned::DevBuf &b, long long first, long long nb, void *host,
+                   bool to_device) {
+  const size_t es = c.es, bs = es * sizeof(ned::z2);
+  const bool tri = comp == NED_RET || comp == NED_LES;
+  if (nb == 0) return;
+  if (tri && which == NED_SIGMA && c.s32) {
+    std::vector<float> f((size_t)nb * es * 2);
+    double *h = static_cast<double *>(host);
+    char *d = (char *)b.p + (size_t)first * es * ned::s_elem(c);
+    if (to_device) {
+      for (size_t i = 0; i < f.size(); i++) f[i] = (float)h[i];
+      NED_HIP(hipMemcpyAsync(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, c.stream));
+      NED_HIP(hipStreamSynchronize(c.stream));
+    } else {
+      NED_HIP(hipMemcpyAsync(f.data(), d, f.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+      NED_HIP(hipStreamSynchronize(c.stream));
+      for (size_t i = 0; i < f.size(); i++) h[i] = (double)f[i];
+    }
+    return;
+  }
+  if (tri && which == NED_G && c.gring) {
+    auto row_of = [](long long blk) {
+      long long t = (long long)((std::sqrt(8.0 * (double)blk + 1.0) - 1.0) / 2.0);
+      while ((t + 1) * (t + 2) / 2 <= blk) t++;
+      while (t * (t + 1) / 2 > blk) t--;
+      return (int)t;
+    };
+    const int t0 = row_of(first), t1 = row_of(first + nb - 1);
+    if (to_device) {
+      ned::g_wrote(c, t1);
+      ned::g_need(c, t0, t1, "green_write");
+    } else {
+      ned::g_need(c, t0, t1, "green_read");
+    }
+    for (long long blk = first; blk < first + nb;) {
+      const int t = row_of(blk);
+      const long long s0 = blk - (long long)t * (t + 1) / 2;
+      const long long cnt = std::min<long long>(first + nb - blk, t + 1 - s0);
+      char *d = (char *)b.p + (ned::g_row(c, t) + (size_t)s0) * bs;
+      char *h = (char *)host + (size_t)(blk - first) * bs;
+      if (to_device) NED_HIP(hipMemcpyAsync(d, h, cnt * bs, hipMemcpyHostToDevice, c.stream));
+      else NED_HIP(hipMemcpyAsync(h, d, cnt * bs, hipMemcpyDeviceToHost, c.stream));
+      blk += cnt;
+    }
+    NED_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
+  char *d = (char *)b.p + (size_t)first * bs;
+  if (to_device) NED_HIP(hipMemcpyAsync(d, host, nb * bs, hipMemcpyHostToDevice, c.stream));
+  else NED_HIP(hipMemcpyAsync(host, d, nb * bs, hipMemcpyDeviceToHost, c.stream));
+  NED_HIP(hipStreamSynchronize(c.stream));
+}
+
 void state(bool ok, const char *msg) {
   if (!ok) throw std::runtime_error(std::string("state: ") + msg);
 }
@@ -106,8 +165,11 @@ extern "C" {
 
 const char *ned_last_error(void) { return g_err.c_str(); }
 
-ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
+ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) { return ned_ctx_create_ex(out, d, device, 0); }
+
+ned_status ned_ctx_create_ex(ned_ctx **out, const ned_dims *d, int device, unsigned flags) {
   if (!out || !d) { g_err = "null argument"; return NED_ERR_ARG; }
+  if (flags & ~(unsigned)(NED_STORE_SIGMA_F32 | NED_STORE_G_RING)) { g_err = "unknown storage flags"; return NED_ERR_ARG; }
   *out = nullptr;
   ned_ctx *ctx = nullptr;
   ned_status st = guard(nullptr, [&] {
@@ -136,10 +198,15 @@ ned_status ned_ctx_create(ned_ctx **out, const ned_dims *d, int device) {
     NED_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
     NED_HIP(hipEventCreateWithFlags(&c.ev_tv, hipEventDisableTiming));
     for (hipEvent_t &e : c.ev_side) NED_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c.s32 = (flags & NED_STORE_SIGMA_F32) != 0;
+    // the ring holds rows T - k - 1 .. T of a timestep, and one more
+    c.gring = (flags & NED_STORE_G_RING) ? std::min(c.nt + 1, c.k + 3) : 0;
     const size_t es = c.es, ntri = (size_t)(c.nt + 1) * (c.nt + 2) / 2;
+    const size_t gtri = c.gring ? (size_t)c.gring * (c.nt + 1) : ntri;
     for (ned::Green *g : {&c.G, &c.S}) {
-      g->ret.ensure(ntri * es * sizeof(ned::z2));
-      g->les.ensure(ntri * es * sizeof(ned::z2));
+      const size_t nblk = g == &c.G ? gtri : ntri, el = g == &c.S ? ned::s_elem(c) : sizeof(ned::z2);
+      g->ret.ensure(nblk * es * el);
+      g->les.ensure(nblk * es * el);
       g->tv.ensure((size_t)(c.nt + 1) * c.np * es * sizeof(ned::z2));
       g->mat.ensure((size_t)c.np * es * sizeof(ned::z2));
       NED_HIP(hipMemsetAsync(g->ret.p, 0, g->ret.bytes, c.stream));
@@ -191,6 +258,15 @@ ned_status ned_ctx_get_dims(const ned_ctx *ctx, ned_dims *out) {
   return NED_OK;
 }
 
+ned_status ned_ctx_get_storage(const ned_ctx *ctx, unsigned *flags, int *g_first, int *g_last) {
+  if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
+  const ned::Ctx &c = ctx->c;
+  if (flags) *flags = (c.s32 ? NED_STORE_SIGMA_F32 : 0u) | (c.gring ? NED_STORE_G_RING : 0u);
+  if (g_first) *g_first = c.gring ? std::max(0, c.g_top - c.gring + 1) : 0;
+  if (g_last) *g_last = c.gring ? c.g_top : c.nt;
+  return NED_OK;
+}
+
 ned_status ned_sync(ned_ctx *ctx) {
   if (!ctx) { g_err = "null context"; return NED_ERR_ARG; }
   return guard(ctx, [&] { NED_HIP(hipStreamSynchronize(ctx->c.stream)); });
@@ -208,9 +284,7 @@ ned_status ned_green_write(ned_ctx *ctx, int which, int comp, long long first, l
     ned::require(which == NED_G || which == NED_SIGMA, "which must be NED_G or NED_SIGMA");
     ned::DevBuf &b = component(c, which, comp);
     ned::require(first >= 0 && nb >= 0 && first + nb <= nblocks_of(c, comp), "block range out of bounds");
-    const size_t bs = c.es * sizeof(ned::z2);
-    NED_HIP(hipMemcpyAsync((char *)b.p + first * bs, host, nb * bs, hipMemcpyHostToDevice, c.stream));
-    NED_HIP(hipStreamSynchronize(c.stream));
+    triangle_copy(c, which, comp, b, first, nb, const_cast<void *>(host), true);
     if (which == NED_SIGMA || (which == NED_G && comp == NED_MAT)) {
       ned::invalidate_minv(c);
       if (comp == NED_MAT) c.have_mix = false;
@@ -225,9 +299,7 @@ ned_status ned_green_read(ned_ctx *ctx, int which, int comp, long long first, lo
     ned::require(which == NED_G || which == NED_SIGMA, "which must be NED_G or NED_SIGMA");
     ned::DevBuf &b = component(c, which, comp);
     ned::require(first >= 0 && nb >= 0 && first + nb <= nblocks_of(c, comp), "block range out of bounds");
-    const size_t bs = c.es * sizeof(ned::z2);
-    NED_HIP(hipMemcpyAsync(host, (char *)b.p + first * bs, nb * bs, hipMemcpyDeviceToHost, c.stream));
-    NED_HIP(hipStreamSynchronize(c.stream));
+    triangle_copy(c, which, comp, b, first, nb, host, false);
   });
 }
 

@@ -46,10 +46,11 @@ struct BlkLds {
 };
 
 // op: 0 = as stored, 1 = adjoint (conjugate transpose), 2 = transpose;
-// ld = row stride of the stored matrix
-__device__ __forceinline__ z2 blk_get(const z2 *M, int ld, int op, int r, int c) {
-  if (op == 0) return M[r * ld + c];
-  z2 v = M[c * ld + r];
+// ld = row stride of the stored matrix (complex FP64 or FP32 elements)
+template <typename E>
+__device__ __forceinline__ z2 blk_get(const E *M, int ld, int op, int r, int c) {
+  if (op == 0) return ldz(M + r * ld + c);
+  z2 v = ldz(M + c * ld + r);
   return op == 1 ? zconj(v) : v;
 }
 
@@ -61,8 +62,8 @@ __device__ __forceinline__ z2 blk_get(const z2 *M, int ld, int op, int r, int c)
 // current slice; 1 (2) reads X (Y) only when it is staged -- for an operand
 // that every workgroup shares and that therefore sits in L2, this saves the
 // prefetch registers (and buys occupancy) at little cost.
-template <int LATE = 0>
-__device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 *X, int opX, const z2 *Y, int opY,
+template <int LATE = 0, typename EX, typename EY>
+__device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const EX *X, int opX, const EY *Y, int opY,
                                         z2 coef, int ldX = 0, int ldY = 0) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (ldX == 0) ldX = n;
@@ -135,7 +136,8 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const z2 
 }
 
 // acc += coef * op(X)   (elementwise, no product)
-__device__ __forceinline__ void blk_axpy(BlkAcc &acc, int n, const z2 *X, int opX, z2 coef) {
+template <typename E>
+__device__ __forceinline__ void blk_axpy(BlkAcc &acc, int n, const E *X, int opX, z2 coef) {
 #pragma unroll
   for (int q = 0; q < 4; q++)
 #pragma unroll
@@ -263,8 +265,8 @@ struct FragT {
 };
 using Frag = FragT<>;
 
-template <int NS>
-__device__ __forceinline__ void frag_left(FragT<NS> &f, int n, const z2 *L, int opL, int r0, int s0 = 0) {
+template <int NS, typename E>
+__device__ __forceinline__ void frag_left(FragT<NS> &f, int n, const E *L, int opL, int r0, int s0 = 0) {
   const int lane = threadIdx.x & 63, r = r0 + (lane & 15);
 #pragma unroll
   for (int s = 0; s < NS; s++) {
@@ -273,8 +275,8 @@ __device__ __forceinline__ void frag_left(FragT<NS> &f, int n, const z2 *L, int 
   }
 }
 
-template <int NS>
-__device__ __forceinline__ void frag_right(FragT<NS> &f, int n, const z2 *R, int opR, int c0, int s0 = 0) {
+template <int NS, typename E>
+__device__ __forceinline__ void frag_right(FragT<NS> &f, int n, const E *R, int opR, int c0, int s0 = 0) {
   const int lane = threadIdx.x & 63, c = c0 + (lane & 15);
 #pragma unroll
   for (int s = 0; s < NS; s++) {

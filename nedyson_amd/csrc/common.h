@@ -36,6 +36,16 @@ __host__ __device__ inline z2 zsub(z2 a, z2 b) { return {a.x - b.x, a.y - b.y}; 
 __host__ __device__ inline z2 zconj(z2 a) { return {a.x, -a.y}; }
 __host__ __device__ inline z2 zscale(z2 a, double s) { return {a.x * s, a.y * s}; }
 
+// one element of a complex history in FP64 (z2) or FP32 (float2) storage:
+// ldz widens to FP64, stz rounds to the storage type (to nearest)
+__device__ __forceinline__ z2 ldz(const z2 *p) { return *p; }
+__device__ __forceinline__ z2 ldz(const float2 *p) {
+  const float2 v = *p;
+  return z2{(double)v.x, (double)v.y};
+}
+__device__ __forceinline__ void stz(z2 *p, z2 v) { *p = v; }
+__device__ __forceinline__ void stz(float2 *p, z2 v) { *p = float2{(float)v.x, (float)v.y}; }
+
 // bf16 round-to-nearest-even of an FP32 value (finite inputs), and back
 __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);

@@ -7,6 +7,8 @@
 #include "common.h"
 #include <vector>
 #include <memory>
+#include <cstdio>
+#include <stdexcept>
 
 namespace ned {
 
@@ -71,6 +73,15 @@ struct Ctx {
   hipStream_t side = nullptr, side2 = nullptr;
   hipEvent_t ev_main = nullptr, ev_side[2] = {nullptr, nullptr}, ev_fork = nullptr, ev_tv = nullptr;
   Green G, S;
+  // storage of the time histories (fixed at creation):
+  // s32: the Sigma^R / Sigma^< triangles hold complex FP32 (float2) elements
+  //   (Sigma^rceil and Sigma^M stay FP64); every reader widens to FP64;
+  // gring > 0: G^R / G^< keep only the last gring time rows, row t in slot
+  //   t mod gring of nt + 1 blocks (the timestep reads rows T - k - 1 .. T);
+  //   g_top is the newest row written, and rows g_top - gring + 1 .. g_top
+  //   are the ones held
+  bool s32 = false;
+  int gring = 0, g_top = -1;
   DevBuf hmf;      // (nt+1) x es complex
   DevBuf Bq;       // naux x nao x nao double, B[Q][i][j] = V(i,j,Q)
   DevBuf Bq32;     // float copy
@@ -145,6 +156,24 @@ struct Ctx {
 
 size_t tri(int t);
 void require(bool cond, const char *msg);
+// block offset of time row t of the G^R / G^< storage (row t of G^R holds
+// G^R(t, 0..t), of G^< holds G^<(0..t, t)): t(t+1)/2, or its ring slot
+inline size_t g_row(const Ctx &c, int t) { return c.gring ? (size_t)(t % c.gring) * (size_t)(c.nt + 1) : tri(t); }
+// with ring storage of G, rows t0..t1 must be held (rows g_top - gring + 1 ..
+// g_top); a no-op for the whole triangle
+inline void g_need(const Ctx &c, int t0, int t1, const char *what) {
+  if (!c.gring) return;
+  t0 = t0 < 0 ? 0 : t0;
+  if (t1 > c.g_top || t0 <= c.g_top - c.gring) {
+    char msg[256];
+    snprintf(msg, sizeof msg, "%s: needs G rows %d..%d, but the ring storage holds rows %d..%d", what, t0, t1,
+             c.g_top - c.gring + 1 < 0 ? 0 : c.g_top - c.gring + 1, c.g_top);
+    throw std::invalid_argument(msg);
+  }
+}
+inline void g_wrote(Ctx &c, int t) { c.g_top = t > c.g_top ? t : c.g_top; }
+// bytes of one stored Sigma^R / Sigma^< element
+inline size_t s_elem(const Ctx &c) { return c.s32 ? 2 * sizeof(float) : 2 * sizeof(double); }
 
 // Sigma: molGF2SolverDecomp::solve (src/mol2bSolver_decomp_impl.h:412-429)
 void sigma_gf2(Ctx &c, int tstp, Prec prec);

@@ -62,34 +62,64 @@ __device__ __forceinline__ double greg(const WTab &w, int n, int j) {
   return om[n - j];
 }
 
-struct Pt {
+// One system's operands of a Dyson step.  SE is the element type of the
+// stored Sigma^R / Sigma^< triangles: complex FP64 (z2), or complex FP32
+// (float2) for a context that keeps its self-energy history in FP32; every
+// read of them goes through ldz(), which widens to FP64.  The G^R / G^<
+// triangles are either whole (gring = 0, row i at block i(i+1)/2) or a ring
+// of the last gring rows (row i in slot i mod gring, growlen blocks a slot).
+template <typename SE>
+struct PtT {
   z2 *Gret, *Gles, *Gtv;
   const z2 *Gmat;
-  const z2 *Sret, *Sles, *Stv;
+  const SE *Sret, *Sles;
+  const z2 *Stv;
   const z2 *hmf;
   int n, np, T, sig;
+  int gring;
+  long long growlen;
   double mu, beta, dt;
   __device__ size_t es() const { return (size_t)n * n; }
-  __device__ z2 *gr(int i, int j) const { return Gret + (dtri(i) + j) * es(); }
-  __device__ z2 *gl(int i, int j) const { return Gles + (dtri(j) + i) * es(); }
+  __device__ size_t grow(int i) const { return gring ? (size_t)(i % gring) * (size_t)growlen : dtri(i); }
+  __device__ z2 *gr(int i, int j) const { return Gret + (grow(i) + j) * es(); }
+  __device__ z2 *gl(int i, int j) const { return Gles + (grow(j) + i) * es(); }
   __device__ z2 *gtv(int i, int m) const { return Gtv + ((size_t)i * np + m) * es(); }
-  __device__ const z2 *sr(int i, int j) const { return Sret + (dtri(i) + j) * es(); }
-  __device__ const z2 *sl(int i, int j) const { return Sles + (dtri(j) + i) * es(); }
+  __device__ const SE *sr(int i, int j) const { return Sret + (dtri(i) + j) * es(); }
+  __device__ const SE *sl(int i, int j) const { return Sles + (dtri(j) + i) * es(); }
   __device__ const z2 *stv(int i, int m) const { return Stv + ((size_t)i * np + m) * es(); }
   __device__ const z2 *h(int t) const { return hmf + (size_t)t * es(); }
 };
+using Pt = PtT<z2>;
+using Pt32 = PtT<float2>;
+
+// the same operands with the Sigma triangles read as FP32 (the layouts of the
+// two instances are identical: only the pointee type of Sret / Sles differs)
+inline Pt32 as_s32(const Pt &p) {
+  Pt32 q;
+  q.Gret = p.Gret; q.Gles = p.Gles; q.Gtv = p.Gtv; q.Gmat = p.Gmat;
+  q.Sret = reinterpret_cast<const float2 *>(p.Sret);
+  q.Sles = reinterpret_cast<const float2 *>(p.Sles);
+  q.Stv = p.Stv; q.hmf = p.hmf;
+  q.n = p.n; q.np = p.np; q.T = p.T; q.sig = p.sig;
+  q.gring = p.gring; q.growlen = p.growlen;
+  q.mu = p.mu; q.beta = p.beta; q.dt = p.dt;
+  return q;
+}
 
 // One system's operands of the dependency chains and bulk updates of the
 // Dyson recurrences, for the grouped launches that step many systems at once:
 // the two accumulators (QA the chain's, Q the bulk updates'), the lesser
 // column X, the cached inverses and the retarded chain's band norms.
-struct ChainArg {
-  Pt p;
+template <typename SE>
+struct ChainArgT {
+  PtT<SE> p;
   z2 *QA, *Q, *X;
   const z2 *Minv;
   double *d2;
   int nb;  // 16-row / 16-column bands of a block
 };
+using ChainArg = ChainArgT<z2>;
+
 
 // ------------------------------------------------------------ batched solve
 // A_s X = B_s for every system s (N x N, R right-hand sides), Gaussian
@@ -280,13 +310,14 @@ __global__ void kern_gj_scale(z2 *B, int N, int R, const z2 *pinv) {
 }
 
 // M_j for the listed j into A (and identity into B), for inversion
-__global__ void kern_build_mj(Pt p, const int *js, int cnt, double bd0, double om0, z2 *A, z2 *B) {
+template <typename SE>
+__global__ void kern_build_mj(PtT<SE> p, const int *js, int cnt, double bd0, double om0, z2 *A, z2 *B) {
   const size_t es = p.es();
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)cnt * es) return;
   int s = (int)(idx / es), e = (int)(idx % es), a = e / p.n, b = e % p.n;
   int j = js[s];
-  z2 hv = p.h(j)[e], sv = p.sr(j, j)[e];
+  z2 hv = p.h(j)[e], sv = ldz(p.sr(j, j) + (e));
   z2 v = {-hv.x - p.dt * om0 * sv.x, -hv.y - p.dt * om0 * sv.y};
   if (a == b) { v.x += p.mu; v.y += bd0 / p.dt; }
   A[(size_t)s * es + e] = v;
@@ -305,7 +336,8 @@ __global__ void kern_scatter_blocks(const z2 *src, const int *js, int cnt, size_
 // k x k block start-up system of src/dyson_step.h:28-65: build_M writes M
 // (kn x kn, depends on times T-k..T-1 only) and sets B to the identity;
 // otherwise the kernel writes Q (kn x n) and sets G^R(T,T) = -i.
-__global__ void __launch_bounds__(256) kern_ret_start(Pt p, const WTab *__restrict__ wtab, int build_M, z2 *M, z2 *Q) {
+template <typename SE>
+__global__ void __launch_bounds__(256) kern_ret_start(PtT<SE> p, const WTab *__restrict__ wtab, int build_M, z2 *M, z2 *Q) {
   const WTab &w = *wtab;
   const int n = p.n, k = w.k, kn = k * n, T = p.T, k1 = k + 1;
   const long long gid = (long long)blockIdx.x * 256 + threadIdx.x, gsz = (long long)gridDim.x * 256;
@@ -323,10 +355,10 @@ __global__ void __launch_bounds__(256) kern_ret_start(Pt p, const WTab *__restri
       }
       double wt = p.dt * greg(w, nn, l);
       if (l <= nn) {
-        z2 s = p.sr(T - l, T - nn)[b * n + a];
+        z2 s = ldz(p.sr(T - l, T - nn) + (b * n + a));
         v.x -= wt * s.x; v.y -= wt * s.y;
       } else {
-        z2 s = p.sr(T - nn, T - l)[a * n + b];
+        z2 s = ldz(p.sr(T - nn, T - l) + (a * n + b));
         v.x += wt * s.x; v.y -= wt * s.y;
       }
       M[e] = v;
@@ -345,7 +377,7 @@ __global__ void __launch_bounds__(256) kern_ret_start(Pt p, const WTab *__restri
     const z2 g = (a == b) ? z2{0.0, -1.0} : z2{0.0, 0.0};
     double pd = w.pdiff[nn * k1 + 0] / p.dt;
     z2 v = {pd * g.y, -pd * g.x};  // -i * pd * g
-    const z2 s = zmul(z2{0.0, -1.0}, p.sr(T, T - nn)[b * n + a]);
+    const z2 s = zmul(z2{0.0, -1.0}, ldz(p.sr(T, T - nn) + (b * n + a)));
     double wt = p.dt * greg(w, nn, 0);
     v.x += wt * s.x; v.y += wt * s.y;
     Q[e] = v;
@@ -378,7 +410,8 @@ __global__ void __launch_bounds__(256) kern_ret_start_store(Pt p, const z2 *X, d
 }
 
 // complete G^R(T,T-m) from its finished history sum (in acc)
-__device__ void ret_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s, int m, z2 *Qm, const z2 *Minv,
+template <typename SE>
+__device__ void ret_finalize(const PtT<SE> &p, const WTab &w, BlkAcc &acc, BlkLds &s, int m, z2 *Qm, const z2 *Minv,
                              double *red, double *err) {
   const int n = p.n, T = p.T;
   blk_scale(acc, z2{p.dt, 0.0});
@@ -397,7 +430,8 @@ __device__ void ret_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s,
 
 // history sums from the start-up blocks l = 0..k, for m = k+1..T
 // (QA: the dependency chain's own accumulator, zeroed here)
-__global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA,
+template <typename SE>
+__global__ void __launch_bounds__(256) kern_ret_qinit(PtT<SE> p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA,
                                                       const z2 *Minv, double *err) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
@@ -435,7 +469,8 @@ __global__ void __launch_bounds__(256) kern_ret_qinit(Pt p, const WTab *__restri
 // workgroup adds only the newest one; the register budget (at most 224 a
 // wave) lets a chain wave share a SIMD with two bulk panel waves (at most 136
 // each); and the waves raise their issue priority.
-__device__ __forceinline__ void ret_step_band(const Pt &p, const WTab *__restrict__ wtab, int nn, z2 *QA,
+template <typename SE>
+__device__ __forceinline__ void ret_step_band(const PtT<SE> &p, const WTab *__restrict__ wtab, int nn, z2 *QA,
                                               const z2 *QB, const z2 *Minv, double *d2, int b, int by, int nb) {
   __builtin_amdgcn_s_setprio(3);
   const WTab &w = *wtab;
@@ -490,16 +525,18 @@ __device__ __forceinline__ void ret_step_band(const Pt &p, const WTab *__restric
   if (threadIdx.x == 0) d2[(size_t)m * nb + b] = dd;
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_ret_step_band(Pt p, const WTab *__restrict__ wtab, int nn, z2 *QA, const z2 *QB, const z2 *Minv, double *d2) {
+kern_ret_step_band(PtT<SE> p, const WTab *__restrict__ wtab, int nn, z2 *QA, const z2 *QB, const z2 *Minv, double *d2) {
   ret_step_band(p, wtab, nn, QA, QB, Minv, d2, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // the same link for every system of a group (system = blockIdx.z; bands
 // beyond a system's own count exit)
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_ret_step_band_grp(const ChainArg *args, const WTab *__restrict__ wtab, int nn) {
-  const ChainArg &a = args[blockIdx.z];
+kern_ret_step_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int nn) {
+  const ChainArgT<SE> &a = args[blockIdx.z];
   if ((int)blockIdx.x >= a.nb) return;
   ret_step_band(a.p, wtab, nn, a.QA, a.Q, a.Minv, a.d2, blockIdx.x, blockIdx.y, a.nb);
 }
@@ -525,7 +562,8 @@ __global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *
 
 // contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
 // the history sums stay in registers across the whole panel
-__device__ __forceinline__ void ret_panel(const Pt &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
+template <typename SE>
+__device__ __forceinline__ void ret_panel(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
                                           int bx) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
@@ -540,14 +578,16 @@ __device__ __forceinline__ void ret_panel(const Pt &p, const WTab *__restrict__ 
   blk_store(acc, n, Qm);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_ret_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
+kern_ret_panel(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
   ret_panel(p, wtab, q0, q1, m0, Q, blockIdx.x);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_ret_panel_grp(const ChainArg *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
-  const ChainArg &a = args[blockIdx.y];
+kern_ret_panel_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
+  const ChainArgT<SE> &a = args[blockIdx.y];
   ret_panel(a.p, wtab, q0, q1, m0, a.Q, blockIdx.x);
 }
 
@@ -555,7 +595,8 @@ kern_ret_panel_grp(const ChainArg *args, const WTab *__restrict__ wtab, int q0, 
 // alone leave most of the chip idle: workgroup (m, b) owns rows
 // 16b..16b+15 of block m, its waves the four column tiles, four times the
 // workgroups at a quarter of the work each
-__device__ __forceinline__ void ret_panel_band(const Pt &p, const WTab *__restrict__ wtab, int q0, int q1, int m0,
+template <typename SE>
+__device__ __forceinline__ void ret_panel_band(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0,
                                                z2 *Q, int bx, int by) {
   const WTab &w = *wtab;
   const int m = m0 + bx, T = p.T, n = p.n;
@@ -575,14 +616,16 @@ __device__ __forceinline__ void ret_panel_band(const Pt &p, const WTab *__restri
   tile_store(acc, n, Qm, r0, c0);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_ret_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
+kern_ret_panel_band(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
   ret_panel_band(p, wtab, q0, q1, m0, Q, blockIdx.x, blockIdx.y);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_ret_panel_band_grp(const ChainArg *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
-  const ChainArg &a = args[blockIdx.z];
+kern_ret_panel_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
+  const ChainArgT<SE> &a = args[blockIdx.z];
   if ((int)blockIdx.y >= a.nb) return;
   ret_panel_band(a.p, wtab, q0, q1, m0, a.Q, blockIdx.x, blockIdx.y);
 }
@@ -591,7 +634,8 @@ kern_ret_panel_band_grp(const ChainArg *args, const WTab *__restrict__ wtab, int
 // src/dyson_step.h:130-169 with Ctv_tstp/CTV1/Conv.mixing
 // (src/dyson_integrals.h:80-116, 254-324): partial sums over chunks of the
 // time history, one workgroup per (node m, chunk s).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_tv_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *mixK, int nchunk, z2 *part) {
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_tv_rhs(PtT<SE> p, const WTab *__restrict__ wtab, const z2 *mixK, int nchunk, z2 *part) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
@@ -611,13 +655,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   blk_store(acc, n, part + ((size_t)ch * np + m) * p.es());
 }
 
-__global__ void kern_tv_matrix(Pt p, const WTab *__restrict__ wtab, z2 *A, z2 *B) {
+template <typename SE>
+__global__ void kern_tv_matrix(PtT<SE> p, const WTab *__restrict__ wtab, z2 *A, z2 *B) {
   const WTab &w = *wtab;
   const int n = p.n;
   const size_t es = p.es();
   for (int e = threadIdx.x; e < (int)es; e += blockDim.x) {
     int a = e / n, b = e % n;
-    z2 hv = p.h(p.T)[e], sv = p.sr(p.T, p.T)[e];
+    z2 hv = p.h(p.T)[e], sv = ldz(p.sr(p.T, p.T) + (e));
     double om0 = w.Omega[w.k * (w.k + 1)];
     z2 v = {-hv.x - p.dt * om0 * sv.x, -hv.y - p.dt * om0 * sv.y};
     if (a == b) { v.x += p.mu; v.y += w.bd[0] / p.dt; }
@@ -668,7 +713,8 @@ __global__ void kern_les_Y(Pt p, const double *mass, z2 *Y) {
 
 // Q_row = Cles3 + Cles2 (src/dyson_integrals.h:336-380, 393-427), partial
 // sums over chunks of the (tau, j) terms.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_les_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *Y, int nchunk, z2 *part) {
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) kern_les_rhs(PtT<SE> p, const WTab *__restrict__ wtab, const z2 *Y, int nchunk, z2 *part) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int row = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
@@ -721,7 +767,8 @@ __global__ void __launch_bounds__(256) kern_les_x0(Pt p, z2 *X, double *err) {
 
 // the start-up system (build_M) and its right-hand sides from X_0 (written
 // by kern_les_x0), every element independent
-__global__ void __launch_bounds__(256) kern_les_start(Pt p, const WTab *__restrict__ wtab, int build_M, const z2 *Q,
+template <typename SE>
+__global__ void __launch_bounds__(256) kern_les_start(PtT<SE> p, const WTab *__restrict__ wtab, int build_M, const z2 *Q,
                                                       const z2 *X0, z2 *M, z2 *MI, z2 *Qs) {
   const WTab &w = *wtab;
   const int n = p.n, k = w.k, kn = k * n, k1 = k + 1;
@@ -739,10 +786,10 @@ __global__ void __launch_bounds__(256) kern_les_start(Pt p, const WTab *__restri
       }
       double wt = p.dt * greg(w, m, l);
       if (m >= l) {
-        z2 s = p.sr(m, l)[a * n + b];
+        z2 s = ldz(p.sr(m, l) + (a * n + b));
         v.x -= wt * s.x; v.y -= wt * s.y;
       } else {
-        z2 s = p.sr(l, m)[b * n + a];
+        z2 s = ldz(p.sr(l, m) + (b * n + a));
         v.x += wt * s.x; v.y -= wt * s.y;
       }
       M[e] = v;
@@ -755,16 +802,17 @@ __global__ void __launch_bounds__(256) kern_les_start(Pt p, const WTab *__restri
     double pd = w.pdiff[m * k1 + 0] / p.dt;
     z2 x0 = X0[a * n + b];
     v.x += pd * x0.y; v.y -= pd * x0.x;   // - i pd x0
-    const z2 *S = p.sr(m, 0);
+    const SE *S = p.sr(m, 0);
     z2 sx = {0.0, 0.0};
-    for (int c = 0; c < n; c++) sx = zadd(sx, zmul(S[a * n + c], X0[c * n + b]));
+    for (int c = 0; c < n; c++) sx = zadd(sx, zmul(ldz(S + a * n + c), X0[c * n + b]));
     double wt = p.dt * greg(w, m, 0);
     v.x += wt * sx.x; v.y += wt * sx.y;
     Qs[e] = v;
   }
 }
 
-__device__ void les_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s, int m, z2 *Qm, z2 *X,
+template <typename SE>
+__device__ void les_finalize(const PtT<SE> &p, const WTab &w, BlkAcc &acc, BlkLds &s, int m, z2 *Qm, z2 *X,
                              const z2 *Minv) {
   const int n = p.n;
   for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, X + (size_t)(m - l) * p.es(), 0, z2{0.0, -w.bd[l] / p.dt});
@@ -778,7 +826,8 @@ __device__ void les_finalize(const Pt &p, const WTab &w, BlkAcc &acc, BlkLds &s,
 }
 
 // (QA: the dependency chain's own accumulator, zeroed here)
-__global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA, z2 *X,
+template <typename SE>
+__global__ void __launch_bounds__(256) kern_les_qinit(PtT<SE> p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA, z2 *X,
                                                       const z2 *Minv) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
@@ -806,7 +855,8 @@ __global__ void __launch_bounds__(256) kern_les_qinit(Pt p, const WTab *__restri
 // X_m = Minv_m Q'_m keeps the columns of Q'_m apart.  Built as
 // kern_ret_step_band (the finishing terms -i/dt bd_l X_{m-l} added ahead by
 // link m-l).
-__device__ __forceinline__ void les_step_band(const Pt &p, const WTab *__restrict__ wtab, int l, z2 *QA,
+template <typename SE>
+__device__ __forceinline__ void les_step_band(const PtT<SE> &p, const WTab *__restrict__ wtab, int l, z2 *QA,
                                               const z2 *QB, z2 *X, const z2 *Minv, int b, int by) {
   __builtin_amdgcn_s_setprio(3);
   const WTab &w = *wtab;
@@ -846,20 +896,23 @@ __device__ __forceinline__ void les_step_band(const Pt &p, const WTab *__restric
   tile_store(x, n, X + (size_t)m * es, r0, c0);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_les_step_band(Pt p, const WTab *__restrict__ wtab, int l, z2 *QA, const z2 *QB, z2 *X, const z2 *Minv) {
+kern_les_step_band(PtT<SE> p, const WTab *__restrict__ wtab, int l, z2 *QA, const z2 *QB, z2 *X, const z2 *Minv) {
   les_step_band(p, wtab, l, QA, QB, X, Minv, blockIdx.x, blockIdx.y);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_les_step_band_grp(const ChainArg *args, const WTab *__restrict__ wtab, int l) {
-  const ChainArg &a = args[blockIdx.z];
+kern_les_step_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int l) {
+  const ChainArgT<SE> &a = args[blockIdx.z];
   if ((int)blockIdx.x >= a.nb) return;
   les_step_band(a.p, wtab, l, a.QA, a.Q, a.X, a.Minv, blockIdx.x, blockIdx.y);
 }
 
 // contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
-__device__ __forceinline__ void les_panel(const Pt &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
+template <typename SE>
+__device__ __forceinline__ void les_panel(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
                                           const z2 *X, int bx) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
@@ -874,20 +927,23 @@ __device__ __forceinline__ void les_panel(const Pt &p, const WTab *__restrict__ 
   blk_store(acc, n, Qm);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_les_panel(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
+kern_les_panel(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
   les_panel(p, wtab, q0, q1, m0, Q, X, blockIdx.x);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_les_panel_grp(const ChainArg *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
-  const ChainArg &a = args[blockIdx.y];
+kern_les_panel_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
+  const ChainArgT<SE> &a = args[blockIdx.y];
   les_panel(a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x);
 }
 
 // kern_les_panel split by columns (workgroup (m, b) owns columns
 // 16b..16b+15 of block m), for the late panels, as kern_ret_panel_band
-__device__ __forceinline__ void les_panel_band(const Pt &p, const WTab *__restrict__ wtab, int q0, int q1, int m0,
+template <typename SE>
+__device__ __forceinline__ void les_panel_band(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0,
                                                z2 *Q, const z2 *X, int bx, int by) {
   const WTab &w = *wtab;
   const int m = m0 + bx, T = p.T, n = p.n;
@@ -908,20 +964,23 @@ __device__ __forceinline__ void les_panel_band(const Pt &p, const WTab *__restri
   tile_store(acc, n, Qm, r0, c0);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_les_panel_band(Pt p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
+kern_les_panel_band(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
   les_panel_band(p, wtab, q0, q1, m0, Q, X, blockIdx.x, blockIdx.y);
 }
 
+template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-kern_les_panel_band_grp(const ChainArg *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
-  const ChainArg &a = args[blockIdx.z];
+kern_les_panel_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
+  const ChainArgT<SE> &a = args[blockIdx.z];
   if ((int)blockIdx.y >= a.nb) return;
   les_panel_band(a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x, blockIdx.y);
 }
 
 // diagonal step, src/dyson_step.h:283-310
-__global__ void __launch_bounds__(256) kern_les_diag(Pt p, const WTab *__restrict__ wtab, z2 *Q, const z2 *QA, z2 *X,
+template <typename SE>
+__global__ void __launch_bounds__(256) kern_les_diag(PtT<SE> p, const WTab *__restrict__ wtab, z2 *Q, const z2 *QA, z2 *X,
                                                      z2 *scr) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
@@ -1050,16 +1109,17 @@ __global__ void kern_hf_h(Pt p, const double *h0, const double *Bq, const double
 
 // ------------------------------------------------------------ energy
 // partial traces of energy_conv's history terms (src/dyson_integrals.h:37-58)
-__global__ void kern_energy(Pt p, const WTab *__restrict__ wtab, int top, const double *mass, double *out) {
+template <typename SE>
+__global__ void kern_energy(PtT<SE> p, const WTab *__restrict__ wtab, int top, const double *mass, double *out) {
   const WTab &w = *wtab;
   __shared__ double rx[256], ry[256];
   const int n = p.n, T = p.T, np = p.np;
   const size_t es = p.es();
   double sx = 0, sy = 0;
   const int i = blockIdx.x;
-  auto trT = [&](const z2 *S, const z2 *G, int e) {  // S_ab G_ba
+  auto trT = [&](const SE *S, const z2 *G, int e) {  // S_ab G_ba
     int a = e / n, b = e % n;
-    return zmul(S[e], G[b * n + a]);
+    return zmul(ldz(S + e), G[b * n + a]);
   };
   if (i <= top) {
     double wt = greg(w, T, i);
@@ -1113,7 +1173,8 @@ __global__ void kern_boot_tv0(Pt p, double *err) {
   if (threadIdx.x == 0) atomicAdd(err, sqrt(d2));
 }
 
-__global__ void kern_boot_matrix(Pt p, const WTab *__restrict__ wtab, z2 *M, z2 *B) {
+template <typename SE>
+__global__ void kern_boot_matrix(PtT<SE> p, const WTab *__restrict__ wtab, z2 *M, z2 *B) {
   const WTab &w = *wtab;
   const int n = p.n, k = w.k, kn = k * n, k1 = k + 1;
   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1129,10 +1190,10 @@ __global__ void kern_boot_matrix(Pt p, const WTab *__restrict__ wtab, z2 *M, z2 
   }
   double wt = p.dt * greg(w, nn, l);
   if (nn >= l) {
-    z2 s = p.sr(nn, l)[a * n + b];
+    z2 s = ldz(p.sr(nn, l) + (a * n + b));
     v.x -= wt * s.x; v.y -= wt * s.y;
   } else {
-    z2 s = p.sr(l, nn)[b * n + a];
+    z2 s = ldz(p.sr(l, nn) + (b * n + a));
     v.x += wt * s.x; v.y -= wt * s.y;
   }
   M[e] = v;
@@ -1140,7 +1201,8 @@ __global__ void kern_boot_matrix(Pt p, const WTab *__restrict__ wtab, z2 *M, z2 
 }
 
 // right-hand side of node m, step nn: tv_it_conv + derivative + integral terms
-__global__ void __launch_bounds__(256) kern_boot_rhs(Pt p, const WTab *__restrict__ wtab, const z2 *mixK, z2 *R) {
+template <typename SE>
+__global__ void __launch_bounds__(256) kern_boot_rhs(PtT<SE> p, const WTab *__restrict__ wtab, const z2 *mixK, z2 *R) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
   const int m = blockIdx.x, nn = blockIdx.y + 1, n = p.n, np = p.np, k1 = w.k + 1;
@@ -1235,15 +1297,38 @@ __global__ void kern_mix_k(const double *la, const z2 *Bs, int np, int nq, size_
   K[((size_t)i * np + j) * es + e] = s;
 }
 
+// (for a context with FP32 Sigma storage the Sigma pointers address float2
+// elements; SE_LAUNCH passes the kernels that read them the FP32 view)
 Pt make_pt(Ctx &c, int T, double mu, double beta, double dt) {
   Pt p;
   p.Gret = c.G.ret.as<z2>(); p.Gles = c.G.les.as<z2>(); p.Gtv = c.G.tv.as<z2>(); p.Gmat = c.G.mat.as<z2>();
   p.Sret = c.S.ret.as<z2>(); p.Sles = c.S.les.as<z2>(); p.Stv = c.S.tv.as<z2>();
   p.hmf = c.hmf.as<z2>();
   p.n = c.nao; p.np = c.np; p.T = T; p.sig = c.sig;
+  p.gring = c.gring; p.growlen = c.nt + 1;
   p.mu = mu; p.beta = beta; p.dt = dt;
   return p;
 }
+
+// launches kernel template K, which reads the Sigma triangles, for the
+// context's Sigma storage: K<z2> with p, or K<float2> with its FP32 view
+#define SE_LAUNCH(c, K, grid, block, shm, st, p, ...)                                         \
+  do {                                                                                         \
+    if ((c).s32)                                                                               \
+      hipLaunchKernelGGL(K<float2>, grid, block, shm, st, as_s32(p), __VA_ARGS__);              \
+    else                                                                                       \
+      hipLaunchKernelGGL(K<z2>, grid, block, shm, st, p, __VA_ARGS__);                          \
+  } while (0)
+// the same for a group's launch over a device array of ChainArg (the two
+// instances of ChainArgT have the same layout)
+#define GRP_LAUNCH(s32, K, grid, block, shm, st, args, ...)                                                       \
+  do {                                                                                                             \
+    if (s32)                                                                                                       \
+      hipLaunchKernelGGL(K<float2>, grid, block, shm, st,                                                          \
+                         reinterpret_cast<const ChainArgT<float2> *>(args), __VA_ARGS__);                          \
+    else                                                                                                           \
+      hipLaunchKernelGGL(K<z2>, grid, block, shm, st, args, __VA_ARGS__);                                          \
+  } while (0)
 
 inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
@@ -1299,7 +1384,7 @@ void update_minv(Ctx &c, int T, double mu, double dt, const WTab &w) {
   c.dy_gA.ensure((size_t)cnt * es * sizeof(z2));
   c.dy_gB.ensure((size_t)cnt * es * sizeof(z2));
   Pt p = make_pt(c, T, mu, 0.0, dt);
-  hipLaunchKernelGGL(kern_build_mj, dim3(nblk((long long)cnt * es)), dim3(256), 0, c.stream, p, c.dy_js.as<int>(), cnt,
+  SE_LAUNCH(c, kern_build_mj, dim3(nblk((long long)cnt * es)), dim3(256), 0, c.stream, p, c.dy_js.as<int>(), cnt,
                      w.bd[0], w.Omega[w.k * (w.k + 1)], c.dy_gA.as<z2>(), c.dy_gB.as<z2>());
   check_launch("build mj");
   gesv(c, c.dy_gA.as<z2>(), es, c.dy_gB.as<z2>(), es, c.nao, c.nao, cnt);
@@ -1402,6 +1487,8 @@ inline int bands(const Ctx &c) { return (c.nao + 15) / 16; }  // 16-row / 16-col
 void dyson_begin(Ctx &c, int T, double mu, double beta, double dt) {
   require(c.have_mix && c.mix_beta == beta, "dyson_step: call ned_set_matsubara(beta) first");
   require(T > c.k && T <= c.nt, "dyson_step: tstp must be in (k, nt]");
+  g_need(c, T - c.k - 1, T - 1, "dyson_step");
+  g_wrote(c, T);
   const WTab w = make_wtab(c.k);
   const WTab *wd = dev_wtab(c);
   const int n = c.nao, k = c.k, kn = k * n, np = c.np;
@@ -1439,9 +1526,9 @@ void dyson_begin(Ctx &c, int T, double mu, double beta, double dt) {
     NED_HIP(hipEventRecord(c.ev_fork, s));
     NED_HIP(hipStreamWaitEvent(c.side2, c.ev_fork, 0));
     hipStream_t s2 = c.side2;
-    hipLaunchKernelGGL(kern_tv_matrix, dim3(1), dim3(256), 0, s2, p, wd, c.dy_tvA.as<z2>(), c.dy_tvB.as<z2>());
+    SE_LAUNCH(c, kern_tv_matrix, dim3(1), dim3(256), 0, s2, p, wd, c.dy_tvA.as<z2>(), c.dy_tvB.as<z2>());
     gesv(c, c.dy_tvA.as<z2>(), 0, c.dy_tvB.as<z2>(), 0, n, n, 1, s2, c.dy_tvinfo.as<int>());
-    hipLaunchKernelGGL(kern_tv_rhs, dim3(np, nch), dim3(256), 0, s2, p, wd, c.mixK.as<z2>(), nch,
+    SE_LAUNCH(c, kern_tv_rhs, dim3(np, nch), dim3(256), 0, s2, p, wd, c.mixK.as<z2>(), nch,
                        c.dy_tvpart.as<z2>());
     hipLaunchKernelGGL(kern_tv_fin, dim3(np), dim3(256), 0, s2, p, c.dy_tvpart.as<z2>(), nch, c.dy_tvB.as<z2>(),
                        c.dy_scr.as<z2>(), err + 1);
@@ -1451,16 +1538,16 @@ void dyson_begin(Ctx &c, int T, double mu, double beta, double dt) {
   // ---- retarded start-up (src/dyson_step.h:7-127)
   z2 *Xs = c.dy_Xs.as<z2>();
   if (c.retinv_T != T) {
-    hipLaunchKernelGGL(kern_ret_start, dim3(nblk((long long)kn * kn)), dim3(256), 0, s, p, wd, 1, M,
+    SE_LAUNCH(c, kern_ret_start, dim3(nblk((long long)kn * kn)), dim3(256), 0, s, p, wd, 1, M,
                        c.dy_retinv.as<z2>());
     gesv(c, M, 0, c.dy_retinv.as<z2>(), 0, kn, kn, 1);
     c.retinv_T = T;
   }
-  hipLaunchKernelGGL(kern_ret_start, dim3(nblk((long long)kn * n)), dim3(256), 0, s, p, wd, 0, M, Qs);
+  SE_LAUNCH(c, kern_ret_start, dim3(nblk((long long)kn * n)), dim3(256), 0, s, p, wd, 0, M, Qs);
   hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_retinv.as<z2>(), Qs, kn, n, Xs);
   check_launch("ret start");
   hipLaunchKernelGGL(kern_ret_start_store, dim3(k), dim3(256), 0, s, p, Xs, err + 0);
-  hipLaunchKernelGGL(kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, QA, c.Minv.as<z2>(), err + 0);
+  SE_LAUNCH(c, kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, QA, c.Minv.as<z2>(), err + 0);
 }
 
 // ---- phase 2 (one system): the retarded recurrence
@@ -1473,15 +1560,15 @@ void dyson_ret_one(Ctx &c, int T, double mu, double beta, double dt) {
   double *d2 = c.dy_d2.as<double>();
   recurrence(c.k, T, panel_blocks(c), c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
              [&](int nn, int q1x) {
-               hipLaunchKernelGGL(kern_ret_step_band, dim3(nb, q1x - nn), dim3(256), 0, c.stream, p, wd, nn, QA, Q,
+               SE_LAUNCH(c, kern_ret_step_band, dim3(nb, q1x - nn), dim3(256), 0, c.stream, p, wd, nn, QA, Q,
                                   Minv, d2);
              },
              [&](int q0, int q1, int q1x, bool band) {
                if (band)
-                 hipLaunchKernelGGL(kern_ret_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
+                 SE_LAUNCH(c, kern_ret_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
                                     q1x + 1, Q);
                else
-                 hipLaunchKernelGGL(kern_ret_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q);
+                 SE_LAUNCH(c, kern_ret_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q);
              });
   check_launch("ret history");
 }
@@ -1502,12 +1589,12 @@ void dyson_mid(Ctx &c, int T, double mu, double beta, double dt) {
   hipLaunchKernelGGL(kern_les_Y, dim3(np), dim3(256), 0, s, p, c.mass.as<double>(), c.dy_Y.as<z2>());
   int nch = chunks_for(T + 1, np + T + 1);
   require((long long)nch * (T + 1) <= 8192LL + c.nt + 1, "dyson_step: partial-sum buffer too small");
-  hipLaunchKernelGGL(kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, wd, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
+  SE_LAUNCH(c, kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, wd, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
   long long nel = (long long)(T + 1) * es;
   hipLaunchKernelGGL(kern_sum_parts, dim3(nblk(nel)), dim3(256), 0, s, c.dy_part.as<z2>(), nch, nel, Q);
   const int build = c.lesinv_valid ? 0 : 1;
   hipLaunchKernelGGL(kern_les_x0, dim3(1), dim3(256), 0, s, p, X, err + 2);
-  hipLaunchKernelGGL(kern_les_start, dim3(nblk((long long)kn * (build ? kn : n))), dim3(256), 0, s, p, wd, build, Q,
+  SE_LAUNCH(c, kern_les_start, dim3(nblk((long long)kn * (build ? kn : n))), dim3(256), 0, s, p, wd, build, Q,
                      X, M, c.dy_lesinv.as<z2>(), Qs);
   if (build) {
     gesv(c, M, 0, c.dy_lesinv.as<z2>(), 0, kn, kn, 1);
@@ -1515,7 +1602,7 @@ void dyson_mid(Ctx &c, int T, double mu, double beta, double dt) {
   }
   hipLaunchKernelGGL(kern_zmm, dim3(nblk((long long)kn * n)), dim3(256), 0, s, c.dy_lesinv.as<z2>(), Qs, kn, n,
                      X + es);
-  hipLaunchKernelGGL(kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, c.dy_QA.as<z2>(), X, c.Minv.as<z2>());
+  SE_LAUNCH(c, kern_les_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, c.dy_QA.as<z2>(), X, c.Minv.as<z2>());
   check_launch("les start");
 }
 
@@ -1528,15 +1615,15 @@ void dyson_les_one(Ctx &c, int T, double mu, double beta, double dt) {
   const z2 *Minv = c.Minv.as<z2>();
   recurrence(c.k, T, panel_blocks(c), c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
              [&](int l, int q1x) {
-               hipLaunchKernelGGL(kern_les_step_band, dim3(nb, q1x - l), dim3(256), 0, c.stream, p, wd, l, QA, Q, X,
+               SE_LAUNCH(c, kern_les_step_band, dim3(nb, q1x - l), dim3(256), 0, c.stream, p, wd, l, QA, Q, X,
                                   Minv);
              },
              [&](int q0, int q1, int q1x, bool band) {
                if (band)
-                 hipLaunchKernelGGL(kern_les_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
+                 SE_LAUNCH(c, kern_les_panel_band, dim3(T - q1x, nb), dim3(256), 0, c.side, p, wd, q0, q1,
                                     q1x + 1, Q, X);
                else
-                 hipLaunchKernelGGL(kern_les_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q,
+                 SE_LAUNCH(c, kern_les_panel, dim3(T - q1x), dim3(256), 0, c.side, p, wd, q0, q1, q1x + 1, Q,
                                     X);
              });
   check_launch("les history");
@@ -1548,7 +1635,7 @@ void dyson_end(Ctx &c, int T, double mu, double beta, double dt, double *err3) {
   const WTab *wd = dev_wtab(c);
   Pt p = make_pt(c, T, mu, beta, dt);
   hipStream_t s = c.stream;
-  hipLaunchKernelGGL(kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, c.dy_Q.as<z2>(), c.dy_QA.as<z2>(),
+  SE_LAUNCH(c, kern_les_diag, dim3(1), dim3(256), 0, s, p, wd, c.dy_Q.as<z2>(), c.dy_QA.as<z2>(),
                      c.dy_X.as<z2>(), c.dy_scr.as<z2>());
   hipLaunchKernelGGL(kern_les_store, dim3(T + 1), dim3(256), 0, s, p, c.dy_X.as<z2>(), c.dy_err.as<double>() + 2);
   check_launch("les");
@@ -1609,8 +1696,10 @@ void dyson_step_group(DysonGroup &g, const std::vector<Ctx *> &cs, int T, const 
     require(c->device == g.device, "dyson group: every system must be on the group's device");
     require(c->k == c0.k && c->rpanel == c0.rpanel && c->panel_band_max == c0.panel_band_max,
             "dyson group: the systems must share k and the recurrence schedule");
+    require(c->s32 == c0.s32, "dyson group: the systems must share the Sigma storage precision");
     nbmax = std::max(nbmax, bands(*c));
   }
+  const bool s32 = c0.s32;
   while ((int)g.evr.size() < S) {
     hipEvent_t e;
     NED_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1639,14 +1728,14 @@ void dyson_step_group(DysonGroup &g, const std::vector<Ctx *> &cs, int T, const 
   const WTab *wd = dev_wtab(*cs[0]);
   recurrence(c0.k, T, panel_blocks(c0), c0.panel_band_max, g.s, g.sd, g.evm, g.evs,
              [&](int nn, int q1x) {
-               hipLaunchKernelGGL(kern_ret_step_band_grp, dim3(nbmax, q1x - nn, S), dim3(256), 0, g.s, args, wd, nn);
+               GRP_LAUNCH(s32, kern_ret_step_band_grp, dim3(nbmax, q1x - nn, S), dim3(256), 0, g.s, args, wd, nn);
              },
              [&](int q0, int q1, int q1x, bool band) {
                if (band)
-                 hipLaunchKernelGGL(kern_ret_panel_band_grp, dim3(T - q1x, nbmax, S), dim3(256), 0, g.sd, args, wd,
+                 GRP_LAUNCH(s32, kern_ret_panel_band_grp, dim3(T - q1x, nbmax, S), dim3(256), 0, g.sd, args, wd,
                                     q0, q1, q1x + 1);
                else
-                 hipLaunchKernelGGL(kern_ret_panel_grp, dim3(T - q1x, S), dim3(256), 0, g.sd, args, wd, q0, q1,
+                 GRP_LAUNCH(s32, kern_ret_panel_grp, dim3(T - q1x, S), dim3(256), 0, g.sd, args, wd, q0, q1,
                                     q1x + 1);
              });
   check_launch("group ret history");
@@ -1655,14 +1744,14 @@ void dyson_step_group(DysonGroup &g, const std::vector<Ctx *> &cs, int T, const 
   group_join_in(g, cs);
   recurrence(c0.k, T, panel_blocks(c0), c0.panel_band_max, g.s, g.sd, g.evm, g.evs,
              [&](int l, int q1x) {
-               hipLaunchKernelGGL(kern_les_step_band_grp, dim3(nbmax, q1x - l, S), dim3(256), 0, g.s, args, wd, l);
+               GRP_LAUNCH(s32, kern_les_step_band_grp, dim3(nbmax, q1x - l, S), dim3(256), 0, g.s, args, wd, l);
              },
              [&](int q0, int q1, int q1x, bool band) {
                if (band)
-                 hipLaunchKernelGGL(kern_les_panel_band_grp, dim3(T - q1x, nbmax, S), dim3(256), 0, g.sd, args, wd,
+                 GRP_LAUNCH(s32, kern_les_panel_band_grp, dim3(T - q1x, nbmax, S), dim3(256), 0, g.sd, args, wd,
                                     q0, q1, q1x + 1);
                else
-                 hipLaunchKernelGGL(kern_les_panel_grp, dim3(T - q1x, S), dim3(256), 0, g.sd, args, wd, q0, q1,
+                 GRP_LAUNCH(s32, kern_les_panel_grp, dim3(T - q1x, S), dim3(256), 0, g.sd, args, wd, q0, q1,
                                     q1x + 1);
              });
   check_launch("group les history");
@@ -1679,6 +1768,8 @@ void dyson_read_err(Ctx &c, double *err3) {
 
 void extrapolate(Ctx &c, int T) {
   require(T > c.k && T <= c.nt, "extrapolate: tstp must be in (k, nt]");
+  g_need(c, T - c.k - 1, T - 1, "extrapolate");
+  g_wrote(c, T);
   const WTab w = make_wtab(c.k);
   const WTab *wd = dev_wtab(c);
   Pt p = make_pt(c, T, 0, 0, 0);
@@ -1708,6 +1799,7 @@ void hf_upload_h0(Ctx &c, const double *h0_host) {
 void hf_kernels(Ctx &c, int T) {
   require(c.have_coulomb, "hf: Coulomb tensor not set");
   require(T >= 0 && T <= c.nt, "hf: tstp out of range");
+  g_need(c, T, T, "hf");
   const size_t es = c.es;
   Pt p = make_pt(c, T, 0, 0, 0);
   hipLaunchKernelGGL(kern_hf_x, dim3(c.naux), dim3(256), 0, c.stream, p, c.Bq.as<double>(), c.dy_xq.as<z2>());
@@ -1719,6 +1811,7 @@ void hf_kernels(Ctx &c, int T) {
 double dyson_start(Ctx &c, double mu, double beta, double dt) {
   require(c.have_mix && c.mix_beta == beta, "dyson_start: call ned_set_matsubara(beta) first");
   require(c.nt >= c.k, "dyson_start: nt must be >= k");
+  g_need(c, 0, c.k, "dyson_start");
   const WTab w = make_wtab(c.k);
   const WTab *wd = dev_wtab(c);
   const int n = c.nao, k = c.k, kn = k * n, np = c.np;
@@ -1731,11 +1824,11 @@ double dyson_start(Ctx &c, double mu, double beta, double dt) {
   hipLaunchKernelGGL(kern_boot_tv0, dim3(np), dim3(256), 0, s, p, err);
   c.dy_M.ensure((size_t)kn * kn * sizeof(z2));
   c.dy_bootI.ensure((size_t)kn * kn * sizeof(z2));
-  hipLaunchKernelGGL(kern_boot_matrix, dim3(nblk((long long)kn * kn)), dim3(256), 0, s, p, wd, c.dy_M.as<z2>(),
+  SE_LAUNCH(c, kern_boot_matrix, dim3(nblk((long long)kn * kn)), dim3(256), 0, s, p, wd, c.dy_M.as<z2>(),
                      c.dy_bootI.as<z2>());
   gesv(c, c.dy_M.as<z2>(), 0, c.dy_bootI.as<z2>(), 0, kn, kn, 1);
   c.dy_part.ensure((size_t)k * np * es * sizeof(z2));
-  hipLaunchKernelGGL(kern_boot_rhs, dim3(np, k), dim3(256), 0, s, p, wd, c.mixK.as<z2>(), c.dy_part.as<z2>());
+  SE_LAUNCH(c, kern_boot_rhs, dim3(np, k), dim3(256), 0, s, p, wd, c.mixK.as<z2>(), c.dy_part.as<z2>());
   hipLaunchKernelGGL(kern_boot_apply, dim3(np, k), dim3(256), 0, s, p, k, c.dy_bootI.as<z2>(), c.dy_part.as<z2>(), err);
   int npairs = (k + 1) * (k + 2);  // ret pairs + les pairs
   hipLaunchKernelGGL(kern_boot_retles, dim3(npairs), dim3(256), 0, s, p, k, err);
@@ -1752,16 +1845,17 @@ void energy(Ctx &c, int T, const double *h0_host, double beta, double dt, double
   const WTab w = make_wtab(c.k);
   const WTab *wd = dev_wtab(c);
   const int top = T >= c.k ? T : c.k, np = c.np;
+  g_need(c, T, top, "energy");
   const int nb = top + 1 + np;
   c.dy_en.ensure((size_t)2 * nb * sizeof(double));
   Pt p = make_pt(c, T, 0, beta, dt);
-  hipLaunchKernelGGL(kern_energy, dim3(nb), dim3(256), 0, c.stream, p, wd, top, c.mass.as<double>(), c.dy_en.as<double>());
+  SE_LAUNCH(c, kern_energy, dim3(nb), dim3(256), 0, c.stream, p, wd, top, c.mass.as<double>(), c.dy_en.as<double>());
   check_launch("energy");
   std::vector<double> h(2 * nb);
   std::vector<z2> gl(c.es), hm(c.es);
   NED_HIP(hipMemcpyAsync(h.data(), c.dy_en.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  NED_HIP(hipMemcpyAsync(gl.data(), c.G.les.as<z2>() + (tri(T) + T) * c.es, c.es * sizeof(z2), hipMemcpyDeviceToHost,
-                         c.stream));
+  NED_HIP(hipMemcpyAsync(gl.data(), c.G.les.as<z2>() + (g_row(c, T) + T) * c.es, c.es * sizeof(z2),
+                         hipMemcpyDeviceToHost, c.stream));
   NED_HIP(hipMemcpyAsync(hm.data(), c.hmf.as<z2>() + (size_t)T * c.es, c.es * sizeof(z2), hipMemcpyDeviceToHost,
                          c.stream));
   NED_HIP(hipStreamSynchronize(c.stream));

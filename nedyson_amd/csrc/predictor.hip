@@ -34,28 +34,28 @@ constexpr int NF = 2 * NPT * 2;  // features per pair and step: (comp, point, re
 
 __device__ __forceinline__ int pred_t(int pt, int T, int k) { return pt < NZ ? k + 1 + pt : T - (NPT - 1) + pt; }
 
-__device__ __forceinline__ z2 *pred_block(z2 *Gret, z2 *Gles, int comp, int t, int T, size_t es) {
-  // comp 0: G^R(T, t) (block T(T+1)/2 + t), comp 1: G^<(t, T) (block T(T+1)/2 + t)
-  const size_t blk = (size_t)T * (T + 1) / 2 + t;
-  return (comp == 0 ? Gret : Gles) + blk * es;
+// Rrow, Lrow: time row T of the G^R and G^< storage; comp 0: G^R(T, t)
+// (block t of Rrow), comp 1: G^<(t, T) (block t of Lrow)
+__device__ __forceinline__ z2 *pred_block(z2 *Rrow, z2 *Lrow, int comp, int t, size_t es) {
+  return (comp == 0 ? Rrow : Lrow) + (size_t)t * es;
 }
 
 // E[(comp, pt)] = current slice values at the predictor points
-__global__ void kern_pred_snapshot(z2 *Gret, z2 *Gles, int T, int k, int n, z2 *E) {
+__global__ void kern_pred_snapshot(z2 *Rrow, z2 *Lrow, int T, int k, int n, z2 *E) {
   const size_t es = (size_t)n * n;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)2 * NPT * es) return;
   const int cp = (int)(i / es), e = (int)(i % es), comp = cp / NPT, pt = cp % NPT;
-  E[i] = pred_block(Gret, Gles, comp, pred_t(pt, T, k), T, es)[e];
+  E[i] = pred_block(Rrow, Lrow, comp, pred_t(pt, T, k), es)[e];
 }
 
 // R[e][f] = (G - E) at the points, f = (comp * NPT + pt) * 2 + re/im
-__global__ void kern_pred_resid(z2 *Gret, z2 *Gles, int T, int k, int n, const z2 *E, float *R) {
+__global__ void kern_pred_resid(z2 *Rrow, z2 *Lrow, int T, int k, int n, const z2 *E, float *R) {
   const size_t es = (size_t)n * n;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)2 * NPT * es) return;
   const int cp = (int)(i / es), e = (int)(i % es), comp = cp / NPT, pt = cp % NPT;
-  const z2 g = pred_block(Gret, Gles, comp, pred_t(pt, T, k), T, es)[e];
+  const z2 g = pred_block(Rrow, Lrow, comp, pred_t(pt, T, k), es)[e];
   const z2 d = zsub(g, E[i]);
   R[(size_t)e * NF + (size_t)cp * 2] = (float)d.x;
   R[(size_t)e * NF + (size_t)cp * 2 + 1] = (float)d.y;
@@ -91,12 +91,12 @@ __global__ void kern_pred_features(Slots sl, int n2, float *X, float *sig) {
 }
 
 // G += sigma * Y at the points
-__global__ void kern_pred_apply(z2 *Gret, z2 *Gles, int T, int k, int n, const float *Y, const float *sig) {
+__global__ void kern_pred_apply(z2 *Rrow, z2 *Lrow, int T, int k, int n, const float *Y, const float *sig) {
   const size_t es = (size_t)n * n;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)2 * NPT * es) return;
   const int cp = (int)(i / es), e = (int)(i % es), comp = cp / NPT, pt = cp % NPT;
-  z2 *g = pred_block(Gret, Gles, comp, pred_t(pt, T, k), T, es) + e;
+  z2 *g = pred_block(Rrow, Lrow, comp, pred_t(pt, T, k), es) + e;
   const int f = cp * 2;
   const float s = sig[(size_t)e * NG + f / GF];
   g->x += (double)(s * Y[(size_t)e * NF + f]);
@@ -193,9 +193,9 @@ void do_tstp(Ctx &c, int T, const double *h0, double mu, double beta, double dt,
     return true;
   };
   extrapolate(c, T);
+  z2 *Rrow = c.G.ret.as<z2>() + g_row(c, T) * es, *Lrow = c.G.les.as<z2>() + g_row(c, T) * es;
   if (track) {
-    hipLaunchKernelGGL(kern_pred_snapshot, dim3(nb((long long)2 * NPT * es)), dim3(256), 0, c.stream, c.G.ret.as<z2>(),
-                       c.G.les.as<z2>(), T, c.k, c.nao, c.pred_E.as<z2>());
+    hipLaunchKernelGGL(kern_pred_snapshot, dim3(nb((long long)2 * NPT * es)), dim3(256), 0, c.stream, Rrow, Lrow, T, c.k, c.nao, c.pred_E.as<z2>());
     check_launch("predictor snapshot");
   }
   bool use_mlp = track && c.pred_mlp != nullptr;
@@ -212,8 +212,7 @@ void do_tstp(Ctx &c, int T, const double *h0, double mu, double beta, double dt,
     hipLaunchKernelGGL(kern_pred_features, dim3(nb((long long)es)), dim3(256), 0, c.stream, sl, (int)es,
                        c.pred_X.as<float>(), c.pred_sig.as<float>());
     m.infer(c.pred_X.as<float>(), (int)es, c.pred_Y.as<float>(), c.stream);
-    hipLaunchKernelGGL(kern_pred_apply, dim3(nb((long long)2 * NPT * es)), dim3(256), 0, c.stream, c.G.ret.as<z2>(),
-                       c.G.les.as<z2>(), T, c.k, c.nao, c.pred_Y.as<float>(), c.pred_sig.as<float>());
+    hipLaunchKernelGGL(kern_pred_apply, dim3(nb((long long)2 * NPT * es)), dim3(256), 0, c.stream, Rrow, Lrow, T, c.k, c.nao, c.pred_Y.as<float>(), c.pred_sig.as<float>());
     check_launch("predictor apply");
     c.pred_used++;
   }
@@ -232,8 +231,7 @@ void do_tstp(Ctx &c, int T, const double *h0, double mu, double beta, double dt,
   if (!ok) throw std::runtime_error("non-finite update in corrector");
   if (track) {
     const int slot = T % 7;
-    hipLaunchKernelGGL(kern_pred_resid, dim3(nb((long long)2 * NPT * es)), dim3(256), 0, c.stream, c.G.ret.as<z2>(),
-                       c.G.les.as<z2>(), T, c.k, c.nao, c.pred_E.as<z2>(),
+    hipLaunchKernelGGL(kern_pred_resid, dim3(nb((long long)2 * NPT * es)), dim3(256), 0, c.stream, Rrow, Lrow, T, c.k, c.nao, c.pred_E.as<z2>(),
                        c.pred_ring.as<float>() + (size_t)slot * es * NF);
     check_launch("predictor residual");
     c.pred_ring_T[slot] = T;

@@ -45,8 +45,27 @@ __global__ void kern_fill_tti(const z2 *R, const z2 *L, int n, int tmax, z2 *ret
   les[idx] = L[(size_t)(t - s) * es + e];
 }
 
+// the same for rows t0..tmax of a ring of G rows (row t in slot t mod gring,
+// rowlen blocks a slot): one thread per element of (row, column s <= tmax)
+__global__ void kern_fill_tti_ring(const z2 *R, const z2 *L, int n, int t0, int tmax, int gring, long long rowlen,
+                                   z2 *ret, z2 *les) {
+  const size_t es = (size_t)n * n;
+  const long long per = (long long)(tmax + 1) * (long long)es;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)(tmax - t0 + 1) * per) return;
+  const int t = t0 + (int)(idx / per);
+  const long long r = idx % per;
+  const int s = (int)(r / es), e = (int)(r % es);
+  if (s > t) return;
+  const size_t o = ((size_t)(t % gring) * (size_t)rowlen + s) * es + e;
+  ret[o] = R[(size_t)(t - s) * es + e];
+  les[o] = L[(size_t)(t - s) * es + e];
+}
+
 // copy slice tsrc to every t <= tmax (tv), and the triangles by difference
-__global__ void kern_tti_extend(z2 *ret, z2 *les, z2 *tv, int n, int np, int tsrc, int tmax) {
+// (SE: the triangles' element type, complex FP64 or FP32)
+template <typename SE>
+__global__ void kern_tti_extend(SE *ret, SE *les, z2 *tv, int n, int np, int tsrc, int tmax) {
   const size_t es = (size_t)n * n;
   const long long nb = (long long)(tmax + 1) * (tmax + 2) / 2;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,25 +139,39 @@ void g0_from_h0(Ctx &c, int tmax, const double *fock, double mu, double beta, do
   hipLaunchKernelGGL(kern_vdv, dim3(nb((long long)(tmax + 1) * es)), dim3(256), 0, s, dV.as<double>(), dc.as<z2>(), n,
                      tmax + 1, z2{0.0, 1.0}, dL.as<z2>());
   NED_HIP(hipStreamSynchronize(s));
-  long long ntri = (long long)(tmax + 1) * (tmax + 2) / 2;
-  hipLaunchKernelGGL(kern_fill_tti, dim3(nb(ntri * (long long)es)), dim3(256), 0, s, dR.as<z2>(), dL.as<z2>(), n, tmax,
-                     c.G.ret.as<z2>(), c.G.les.as<z2>());
+  if (c.gring) {  // only the rows the ring holds
+    const int t0 = std::max(0, tmax - c.gring + 1);
+    const long long tot = (long long)(tmax - t0 + 1) * (tmax + 1) * (long long)es;
+    hipLaunchKernelGGL(kern_fill_tti_ring, dim3(nb(tot)), dim3(256), 0, s, dR.as<z2>(), dL.as<z2>(), n, t0, tmax,
+                       c.gring, (long long)c.nt + 1, c.G.ret.as<z2>(), c.G.les.as<z2>());
+  } else {
+    long long ntri = (long long)(tmax + 1) * (tmax + 2) / 2;
+    hipLaunchKernelGGL(kern_fill_tti, dim3(nb(ntri * (long long)es)), dim3(256), 0, s, dR.as<z2>(), dL.as<z2>(), n,
+                       tmax, c.G.ret.as<z2>(), c.G.les.as<z2>());
+  }
   NED_HIP(hipMemcpyAsync(dc.p, cT.data(), cT.size() * sizeof(z2), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(kern_vdv, dim3(nb((long long)(tmax + 1) * np * es)), dim3(256), 0, s, dV.as<double>(), dc.as<z2>(),
                      n, (tmax + 1) * np, z2{0.0, 1.0}, c.G.tv.as<z2>());
   check_launch("g0_from_h0");
   NED_HIP(hipStreamSynchronize(s));
+  c.g_top = tmax;
   invalidate_minv(c);
 }
 
 void tti_extend(Ctx &c, int which, int tsrc) {
   require(tsrc >= 0 && tsrc <= c.nt, "tti_extend: tsrc out of range");
+  require(which == 1 || !c.gring, "tti_extend: G keeps only its last rows (ring storage)");
   Green &g = which == 1 ? c.S : c.G;
   const size_t es = c.es;
   long long ntri = (long long)(tsrc + 1) * (tsrc + 2) / 2;
   long long tot = ntri * (long long)es + (long long)(tsrc + 1) * c.np * (long long)es;
-  hipLaunchKernelGGL(kern_tti_extend, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, g.ret.as<z2>(),
-                     g.les.as<z2>(), g.tv.as<z2>(), c.nao, c.np, tsrc, tsrc);
+  const dim3 grid((unsigned)((tot + 255) / 256));
+  if (which == 1 && c.s32)
+    hipLaunchKernelGGL(kern_tti_extend<float2>, grid, dim3(256), 0, c.stream, g.ret.as<float2>(), g.les.as<float2>(),
+                       g.tv.as<z2>(), c.nao, c.np, tsrc, tsrc);
+  else
+    hipLaunchKernelGGL(kern_tti_extend<z2>, grid, dim3(256), 0, c.stream, g.ret.as<z2>(), g.les.as<z2>(),
+                       g.tv.as<z2>(), c.nao, c.np, tsrc, tsrc);
   check_launch("tti_extend");
   NED_HIP(hipStreamSynchronize(c.stream));
   invalidate_minv(c);

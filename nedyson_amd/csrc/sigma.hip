@@ -227,16 +227,15 @@ __global__ void kern_d2f(const double *x, float *y, long long n) {
 }
 
 template <typename T>
-__global__ void kern_items(const z2 *Gret, const z2 *Gles, const z2 *Gtv, int T_, int n, int np, int sig, int p0,
-                           int P, T *Ap, T *Bp, int transA, T *Btp) {
+__global__ void kern_items(const z2 *Gret, const z2 *Gles, const z2 *Gtv, int T_, size_t rowT, int n, int np, int sig,
+                           int p0, int P, T *Ap, T *Bp, int transA, T *Btp) {
   const int n2 = n * n;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)P * n2) return;
   int p = (int)(idx / n2), e = (int)(idx % n2), i = e / n, k = e % n;
   int g = p0 + p;
   z2 A, B;
-  const size_t rowT = dtri_s(T_);
-  if (g <= 2 * T_ + 1) {
+  if (g <= 2 * T_ + 1) {  // row T_ of G^R and G^< starts at block rowT
     int t = g <= T_ ? g : g - T_ - 1;
     const z2 *L = Gles + (rowT + t) * n2, *R = Gret + (rowT + t) * n2;
     z2 Lik = L[i * n + k], Lki = L[k * n + i], Rik = R[i * n + k], Rki = R[k * n + i];
@@ -274,8 +273,8 @@ __global__ void kern_items(const z2 *Gret, const z2 *Gles, const z2 *Gtv, int T_
 // t + T_ + 1) is in the same chunk, -Sigma^<(T,t) otherwise, to which the
 // partner's chunk, issued later on the same stream, then adds Sigma^>(T,t)
 // (the same rounding: a - b is a + (-b)); tv items write Sigma^rceil(T,tau).
-template <typename T>
-__global__ void kern_sum_scatter(const T *part, int S, int p0, int P, int T_, int n, int np, z2 *Sret, z2 *Sles,
+template <typename T, typename SE>
+__global__ void kern_sum_scatter(const T *part, int S, int p0, int P, int T_, int n, int np, SE *Sret, SE *Sles,
                                  z2 *Stv) {
   const int n2 = n * n;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,23 +294,36 @@ __global__ void kern_sum_scatter(const T *part, int S, int p0, int P, int T_, in
     const int t = g;
     const z2 lt = item(p);
     // Sigma^<(t,T)_ki = -conj(Sigma^<(T,t)_ik)
-    Sles[(rowT + t) * n2 + k * n + i] = {-lt.x, lt.y};
+    stz(Sles + (rowT + t) * n2 + k * n + i, z2{-lt.x, lt.y});
     const int pg = g + nles - p0;  // the ">" partner's place in this chunk
     if (pg < P) {
       const z2 gt = item(pg);
-      Sret[(rowT + t) * n2 + e] = {gt.x - lt.x, gt.y - lt.y};
+      stz(Sret + (rowT + t) * n2 + e, z2{gt.x - lt.x, gt.y - lt.y});
     } else {
-      Sret[(rowT + t) * n2 + e] = {-lt.x, -lt.y};
+      stz(Sret + (rowT + t) * n2 + e, z2{-lt.x, -lt.y});
     }
   } else if (g < 2 * nles) {
     const int t = g - nles;
     if (t >= p0) return;  // the "<" partner is in this chunk and wrote Sigma^R
-    const z2 gt = item(p), r = Sret[(rowT + t) * n2 + e];
-    Sret[(rowT + t) * n2 + e] = {gt.x + r.x, gt.y + r.y};
+    const z2 gt = item(p), r = ldz(Sret + (rowT + t) * n2 + e);
+    stz(Sret + (rowT + t) * n2 + e, z2{gt.x + r.x, gt.y + r.y});
   } else {
     const int tau = g - 2 * nles;
     Stv[((size_t)T_ * np + tau) * n2 + e] = item(p);
   }
+}
+
+// kern_sum_scatter into the context's Sigma storage (complex FP64, or FP32)
+template <typename T>
+void sum_scatter(Ctx &c, const T *part, int S, int p0, int P, int T_, hipStream_t s) {
+  const long long tot = (long long)P * c.es;
+  const dim3 grid((unsigned)((tot + 255) / 256));
+  if (c.s32)
+    hipLaunchKernelGGL(kern_sum_scatter<T>, grid, dim3(256), 0, s, part, S, p0, P, T_, c.nao, c.np,
+                       c.S.ret.as<float2>(), c.S.les.as<float2>(), c.S.tv.as<z2>());
+  else
+    hipLaunchKernelGGL(kern_sum_scatter<T>, grid, dim3(256), 0, s, part, S, p0, P, T_, c.nao, c.np,
+                       c.S.ret.as<z2>(), c.S.les.as<z2>(), c.S.tv.as<z2>());
 }
 
 template <typename T>
@@ -386,8 +398,7 @@ void run_chunk(Ctx &c, Prec prec, int T_, int p0, int P, const Bufs<T> &bf, cons
             c.sym_pairs.as<uint16_t>(), wth, wtl);
     else c64_e(W, H, P, a, S, (float *)bf.Ep, s);
     tot = (long long)P * n2;
-    hipLaunchKernelGGL(kern_sum_scatter<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, p0, P,
-                       T_, n, c.np, c.S.ret.as<z2>(), c.S.les.as<z2>(), c.S.tv.as<z2>());
+    sum_scatter(c, bf.Ep, S, p0, P, T_, s);
     check_launch("sigma sum scatter");
     return;
   }
@@ -523,8 +534,7 @@ void run_chunk(Ctx &c, Prec prec, int T_, int p0, int P, const Bufs<T> &bf, cons
     d.C.ptr = bf.Ep; d.C.rs = n; d.C.cs = 1; d.C.ps = n2; d.C.bs_lo = 2 * n2;
     gemm(d, prec, s);
     long long tot = (long long)P * n2;
-    hipLaunchKernelGGL(kern_sum_scatter<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Ep, S, p0, P,
-                       T_, n, c.np, c.S.ret.as<z2>(), c.S.les.as<z2>(), c.S.tv.as<z2>());
+    sum_scatter(c, bf.Ep, S, p0, P, T_, s);
     check_launch("sigma sum scatter");
   }
 }
@@ -588,7 +598,8 @@ void sigma_chunks(Ctx &c, int tstp, Prec prec, int P, char *base, bool tune) {
     int Pc = std::min(P, Ptot - p0);
     long long tot = (long long)Pc * n2;
     hipLaunchKernelGGL(kern_items<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, c.G.ret.as<z2>(),
-                       c.G.les.as<z2>(), c.G.tv.as<z2>(), tstp, n, np, c.sig, p0, Pc, bf.A, bf.B, f64w ? 1 : 0,
+                       c.G.les.as<z2>(), c.G.tv.as<z2>(), tstp, g_row(c, tstp), n, np, c.sig, p0, Pc, bf.A, bf.B,
+                       f64w ? 1 : 0,
                        bf.Bt);
     check_launch("sigma items");
     run_chunk<T>(c, prec, tstp, p0, Pc, bf, Vq, U);
@@ -669,6 +680,7 @@ static void sigma_noalloc_typed(Ctx &c, int tstp, Prec prec) {
 void sigma_gf2_noalloc(Ctx &c, int tstp, Prec prec) {
   require(c.have_coulomb, "sigma: Coulomb tensor not set");
   require(tstp >= 0 && tstp <= c.nt, "sigma: tstp out of range");
+  g_need(c, tstp, tstp, "sigma");
   if (prec == PREC_F64) sigma_noalloc_typed<double>(c, tstp, prec);
   else sigma_noalloc_typed<float>(c, tstp, prec);
 }
@@ -770,6 +782,7 @@ void build_coulomb(Ctx &c, const double *Bq_host) {
 void sigma_gf2(Ctx &c, int tstp, Prec prec) {
   require(c.have_coulomb, "sigma: Coulomb tensor not set");
   require(tstp >= 0 && tstp <= c.nt, "sigma: tstp out of range");
+  g_need(c, tstp, tstp, "sigma");
   c.mark(2);
   if (prec == PREC_F64) sigma_typed<double>(c, tstp, prec);
   else sigma_typed<float>(c, tstp, prec);
@@ -839,7 +852,8 @@ void sigma_full_typed(Ctx &c, int tstp, Prec prec) {
     const long long Pn4 = (long long)P * n4;
     long long tot = (long long)Pc * n2;
     hipLaunchKernelGGL(kern_items<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, c.G.ret.as<z2>(),
-                       c.G.les.as<z2>(), c.G.tv.as<z2>(), tstp, n, np, c.sig, p0, Pc, A, B, 0, (T *)nullptr);
+                       c.G.les.as<z2>(), c.G.tv.as<z2>(), tstp, g_row(c, tstp), n, np, c.sig, p0, Pc, A, B, 0,
+                       (T *)nullptr);
     check_launch("sigma full items");
     {  // T1[p][i][k][(n,p')] = sum_l A_lk U[(i,l)][(n,p')]
       GemmDesc d;
@@ -875,8 +889,7 @@ void sigma_full_typed(Ctx &c, int tstp, Prec prec) {
       gemm(d, prec, s);
     }
     tot = (long long)Pc * n2;
-    hipLaunchKernelGGL(kern_sum_scatter<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, E, S, p0, Pc,
-                       tstp, n, np, c.S.ret.as<z2>(), c.S.les.as<z2>(), c.S.tv.as<z2>());
+    sum_scatter(c, E, S, p0, Pc, tstp, s);
     check_launch("sigma full scatter");
   }
 }
@@ -885,6 +898,7 @@ void sigma_full_typed(Ctx &c, int tstp, Prec prec) {
 void sigma_gf2_full(Ctx &c, int tstp, Prec prec) {
   require(c.have_coulomb, "sigma: Coulomb tensor not set");
   require(tstp >= 0 && tstp <= c.nt, "sigma: tstp out of range");
+  g_need(c, tstp, tstp, "sigma");
   if (prec == PREC_F64) sigma_full_typed<double>(c, tstp, prec);
   else sigma_full_typed<float>(c, tstp, prec);
 }
