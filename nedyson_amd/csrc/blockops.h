@@ -54,6 +54,20 @@ __device__ __forceinline__ z2 blk_get(const E *M, int ld, int op, int r, int c) 
   return op == 1 ? zconj(v) : v;
 }
 
+// The same element left in its storage type, for operands loaded ahead of
+// their use: widening (or conjugating) a value right after its load would
+// make the wave wait for that load there, before the work the load is meant
+// to overlap.  widen() and blk_op() finish it where it is consumed.
+template <typename E>
+__device__ __forceinline__ E blk_raw(const E *M, int ld, int op, int r, int c) {
+  return op == 0 ? M[r * ld + c] : M[c * ld + r];
+}
+__device__ __forceinline__ z2 widen(z2 v) { return v; }
+__device__ __forceinline__ z2 widen(float2 v) { return z2{(double)v.x, (double)v.y}; }
+__device__ __forceinline__ z2 blk_op(z2 v, int op) { return op == 1 ? zconj(v) : v; }
+__device__ __forceinline__ z2 zero_of(const z2 *) { return z2{0.0, 0.0}; }
+__device__ __forceinline__ float2 zero_of(const float2 *) { return float2{0.0f, 0.0f}; }
+
 // acc += coef * op(X) * op(Y)   (all threads of the workgroup call this);
 // ldX, ldY are the row strides of X and Y as stored (n for a dense block).
 // The next K-slice is loaded into registers while the current one is
@@ -71,7 +85,8 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const EX 
   const bool wave_on = 16 * w < n;
   constexpr int PER = BO_NMAX * BO_KT / BO_T;  // staged elements per thread and operand
   constexpr int PX = LATE == 1 ? 1 : PER, PY = LATE == 2 ? 1 : PER;
-  z2 px[PX], py[PY];
+  EX px[PX];  // the next K slice, in the operands' storage types
+  EY py[PY];
   // element e of a slice -> (row, k) of X and (k, col) of Y, chosen so that
   // consecutive threads read consecutive addresses of the stored matrix
   // (along k when it is used as stored, along the other index when it is
@@ -85,12 +100,12 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const EX 
   auto getx = [&](int e, int k0) __attribute__((always_inline)) {
     int r, kk;
     xmap(e, r, kk);
-    return (r < n && k0 + kk < n) ? blk_get(X, ldX, opX, r, k0 + kk) : z2{0.0, 0.0};
+    return (r < n && k0 + kk < n) ? blk_raw(X, ldX, opX, r, k0 + kk) : zero_of(X);
   };
   auto gety = [&](int e, int k0) __attribute__((always_inline)) {
     int kk, c;
     ymap(e, kk, c);
-    return (c < n && k0 + kk < n) ? blk_get(Y, ldY, opY, k0 + kk, c) : z2{0.0, 0.0};
+    return (c < n && k0 + kk < n) ? blk_raw(Y, ldY, opY, k0 + kk, c) : zero_of(Y);
   };
   auto fetch = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
@@ -110,8 +125,8 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const EX 
       int r, kk, kk2, c;
       xmap(e, r, kk);
       ymap(e, kk2, c);
-      s.X[r][kk] = zmul(coef, LATE == 1 ? getx(e, k0) : px[LATE == 1 ? 0 : i]);
-      s.Y[c][kk2] = LATE == 2 ? gety(e, k0) : py[LATE == 2 ? 0 : i];
+      s.X[r][kk] = zmul(coef, blk_op(widen(LATE == 1 ? getx(e, k0) : px[LATE == 1 ? 0 : i]), opX));
+      s.Y[c][kk2] = blk_op(widen(LATE == 2 ? gety(e, k0) : py[LATE == 2 ? 0 : i]), opY);
     }
     __syncthreads();
     if (k0 + BO_KT < n) fetch(k0 + BO_KT);
@@ -259,41 +274,46 @@ __device__ __forceinline__ void tile_store(const TileAcc &acc, int n, z2 *X, int
 // and zero outside nao x nao.
 // A fragment may hold only the NS steps from s0 on (half the K range in
 // throughput kernels, which keeps registers for occupancy).
-template <int NS = BO_NMAX / 4>
+// E: the element type the fragment is loaded from; an FP32 fragment is
+// widened where frag_mac uses it, so its loads overlap whatever comes between
+// (FP32 fragments are read as stored, op 0)
+template <int NS = BO_NMAX / 4, typename E = z2>
 struct FragT {
-  z2 v[NS];
+  E v[NS];
 };
 using Frag = FragT<>;
 
 template <int NS, typename E>
-__device__ __forceinline__ void frag_left(FragT<NS> &f, int n, const E *L, int opL, int r0, int s0 = 0) {
+__device__ __forceinline__ void frag_left(FragT<NS, E> &f, int n, const E *L, int opL, int r0, int s0 = 0) {
   const int lane = threadIdx.x & 63, r = r0 + (lane & 15);
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const int kk = 4 * (s0 + s) + (lane >> 4);
-    f.v[s] = (r < n && kk < n) ? blk_get(L, n, opL, r, kk) : z2{0.0, 0.0};
+    if constexpr (sizeof(E) == sizeof(z2)) f.v[s] = (r < n && kk < n) ? blk_get(L, n, opL, r, kk) : z2{0.0, 0.0};
+    else f.v[s] = (r < n && kk < n) ? blk_raw(L, n, 0, r, kk) : zero_of(L);
   }
 }
 
 template <int NS, typename E>
-__device__ __forceinline__ void frag_right(FragT<NS> &f, int n, const E *R, int opR, int c0, int s0 = 0) {
+__device__ __forceinline__ void frag_right(FragT<NS, E> &f, int n, const E *R, int opR, int c0, int s0 = 0) {
   const int lane = threadIdx.x & 63, c = c0 + (lane & 15);
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const int kk = 4 * (s0 + s) + (lane >> 4);
-    f.v[s] = (kk < n && c < n) ? blk_get(R, n, opR, kk, c) : z2{0.0, 0.0};
+    if constexpr (sizeof(E) == sizeof(z2)) f.v[s] = (kk < n && c < n) ? blk_get(R, n, opR, kk, c) : z2{0.0, 0.0};
+    else f.v[s] = (kk < n && c < n) ? blk_raw(R, n, 0, kk, c) : zero_of(R);
   }
 }
 
 // acc += coef * L R over the fragments' steps; coef scales the left operand,
 // as the LDS-staged products do, so both forms round alike
-template <int NS>
-__device__ __forceinline__ void frag_mac(TileAcc &acc, const FragT<NS> &a, const FragT<NS> &b, int n, z2 coef,
+template <int NS, typename EA, typename EB>
+__device__ __forceinline__ void frag_mac(TileAcc &acc, const FragT<NS, EA> &a, const FragT<NS, EB> &b, int n, z2 coef,
                                          int s0 = 0) {
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     if (4 * (s0 + s) < n) {
-      const z2 x = zmul(coef, a.v[s]), y = b.v[s];
+      const z2 x = zmul(coef, widen(a.v[s])), y = widen(b.v[s]);
       acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, acc.re, 0, 0, 0);
       acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, acc.im, 0, 0, 0);
       acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, acc.re, 0, 0, 0);

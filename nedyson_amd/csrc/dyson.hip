@@ -484,7 +484,8 @@ __device__ __forceinline__ void ret_step_band(const PtT<SE> &p, const WTab *__re
   const bool fin = m == nn + 1, bdt = m - nn <= w.k + 1;
   z2 *Qm = QA + (size_t)m * n * n;
   TileAcc acc, gt, qb, old;
-  Frag fa, fb;
+  Frag fa;
+  FragT<BO_NMAX / 4, SE> fb;
   tile_load(acc, n, Qm, r0, c0);
   frag_left(fa, n, p.gr(T, T - nn), 0, r0);
   frag_right(fb, n, p.sr(T - nn, T - m), 0, c0);
@@ -608,7 +609,8 @@ __device__ __forceinline__ void ret_panel_band(const PtT<SE> &p, const WTab *__r
   for (int nn = q0; nn < q1; nn++)
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      FragT<BO_NMAX / 8> fa, fb;
+      FragT<BO_NMAX / 8> fa;
+      FragT<BO_NMAX / 8, SE> fb;
       frag_left(fa, n, p.gr(T, T - nn), 0, r0, h * BO_NMAX / 8);
       frag_right(fb, n, p.sr(T - nn, T - m), 0, c0, h * BO_NMAX / 8);
       frag_mac(acc, fa, fb, n, z2{greg(w, m, nn), 0.0}, h * BO_NMAX / 8);
@@ -868,7 +870,8 @@ __device__ __forceinline__ void les_step_band(const PtT<SE> &p, const WTab *__re
   const size_t es = p.es();
   z2 *Qm = QA + (size_t)m * es;
   TileAcc acc, gt, qb;
-  Frag fa, fb;
+  FragT<BO_NMAX / 4, SE> fa;
+  Frag fb;
   tile_load(acc, n, Qm, r0, c0);
   frag_left(fa, n, p.sr(m, l), 0, r0);
   frag_right(fb, n, X + (size_t)l * es, 0, c0);
@@ -956,7 +959,8 @@ __device__ __forceinline__ void les_panel_band(const PtT<SE> &p, const WTab *__r
   for (int l = q0; l < q1; l++)
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      FragT<BO_NMAX / 8> fa, fb;
+      FragT<BO_NMAX / 8, SE> fa;
+      FragT<BO_NMAX / 8> fb;
       frag_left(fa, n, p.sr(m, l), 0, r0, h * BO_NMAX / 8);
       frag_right(fb, n, X + (size_t)l * es, 0, c0, h * BO_NMAX / 8);
       frag_mac(acc, fa, fb, n, z2{p.dt * greg(w, m, l), 0.0}, h * BO_NMAX / 8);
