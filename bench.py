@@ -506,7 +506,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--T0", type=int, default=None, help="last history step (default nt/2)")
-    ap.add_argument("--prec", default="bf16x3", choices=["f32", "bf16x3"])
+    ap.add_argument("--prec", default="bf16x3", choices=["f32", "bf16x3", "f64"],
+                    help="Sigma arithmetic of the timed run (f64: for --sweep)")
     ap.add_argument("--corr", type=int, default=3, help="corrector passes per step")
     ap.add_argument("--f64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
