@@ -204,3 +204,49 @@ def test_c1_window_T500_parity(orc):
     assert _relerr(ret, G.ret_row(T)) < 1e-10
     assert _relerr(les, G.les_col(T)) < 1e-10
     assert _relerr(tv, G.tv[T]) < 1e-10
+
+
+@pytest.mark.timeout(900)
+def test_c2_window_step_parity(orc):
+    """The nao = 64 configuration's shape (naux = 256, 128 GLL nodes,
+    dt = 0.01; B entries of spread 0.01 as in the benchmark window) at a
+    history of 20 steps (free G of the Fock matrix, Sigma(20) continued
+    time-translation invariantly), then one full time step of 3 corrector
+    passes: FP64 on the GPU against the oracle (1e-10); the same with FP32
+    Sigma storage and the G ring (NED_STORE_SIGMA_F32 | NED_STORE_G_RING,
+    the storage of the whole nt = 2000 run; 1e-7); and bf16x3 Sigma (1e-6)."""
+    T0, K = 20, 1
+    p = problem.config(2, nt=T0 + K, b_sd=0.01)
+    G = problem.free_gf(p.fock, p.mu, p.beta, p.dt, p.nt, p.ntau, T0, p.Gmat)
+    S = GreenFunc(p.nt, p.ntau, p.nao)
+    hmf = np.zeros((p.nt + 1, p.nao, p.nao), dtype=complex)
+    hmf[:T0 + 1] = p.fock
+    print(f"[c2 window] oracle Sigma({T0}) on {NTH} threads", flush=True)
+    orc.sigma_solve(p.V, T0, S, G, nthreads=NTH)
+    for t in range(T0):
+        for tp in range(t + 1):
+            S.retptr(t, tp)[:] = S.retptr(T0, T0 - (t - tp))
+            S.lesptr(tp, t)[:] = S.lesptr(T0 - (t - tp), T0)
+        S.tv[t] = S.tv[T0]
+    D = orc.Dyson(p.nt, p.ntau, p.nao, p.k)
+    T = T0 + K
+    D.do_tstp(p.V, T, G, S, hmf, p.h0, p.mu, p.beta, p.dt, 3, nthreads=NTH)
+    print("[c2 window] oracle step done", flush=True)
+    for storage, prec, tol in ((0, "f64", 1e-10), (ned.NED_STORE_SIGMA_F32 | ned.NED_STORE_G_RING, "f64", 1e-7),
+                               (0, "bf16x3", 1e-6)):
+        with ned.Context(p.nao, p.naux, p.nt, p.ntau, p.k, storage=storage) as c:
+            c.set_coulomb(p.B)
+            c.write_blocks(ned.NED_G, ned.NED_MAT, 0, p.Gmat)
+            c.set_matsubara(p.beta)
+            c.g0_from_h0(T0, p.fock, p.mu, p.beta, p.dt)
+            c.write_hmf(np.repeat(p.fock[None].astype(complex), T0 + 1, axis=0))
+            c.sigma(T0, "f64")
+            c.tti_extend(ned.NED_SIGMA, T0)
+            c.do_tstp(T, p.h0, p.mu, p.beta, p.dt, prec, 3)
+            s = T * (T + 1) // 2
+            ret = c.read_blocks(ned.NED_G, ned.NED_RET, s, T + 1)
+            les = c.read_blocks(ned.NED_G, ned.NED_LES, s, T + 1)
+            tv = c.read_blocks(ned.NED_G, ned.NED_TV, T * (p.ntau + 1), p.ntau + 1)
+        errs = (_relerr(ret, G.ret_row(T)), _relerr(les, G.les_col(T)), _relerr(tv, G.tv[T]))
+        print(f"[c2 window] storage {storage} {prec}: relative errors ret/les/tv {errs}", flush=True)
+        assert max(errs) < tol, (storage, prec, errs)
