@@ -5,6 +5,7 @@
 // tile; each wave owns a 32x32 quadrant = 2x2 MFMA 16x16 blocks.  Operand
 // tiles are staged in LDS; the loader follows whichever operand dimension is
 // unit-stride so that consecutive lanes read consecutive addresses.
+#include <type_traits>
 #include "common.h"
 #include <algorithm>
 #include <cstdlib>
@@ -466,11 +467,13 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int NT_TK = 64;
 
-template <int TM, int TN, int WM, int WN, int TK = NT_TK>
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+template <int TM, int TN, int WM, int WN, int TK = NT_TK, int MF = 16>
 struct NtCfg {
   static constexpr int WAVES = (TM / WM) * (TN / WN);
   static constexpr int THREADS = 64 * WAVES;
-  static constexpr int BI = WM / 16, BJ = WN / 16;  // MFMA blocks per wave
+  static constexpr int BI = WM / MF, BJ = WN / MF;  // MFMA blocks per wave
   static constexpr int SEG = TK / 8;                // 16-byte segments per row and K slice
   static constexpr int LA = TM * TK / 8 / THREADS, LB = TN * TK / 8 / THREADS;  // 16-byte loads per thread
 };
@@ -492,15 +495,28 @@ struct NtCfg {
 // conflicts on gfx950, where the padded 80-byte rows measured a conflict
 // cycle for every five active ones; a staging write (4 rows, all segments)
 // stays one contiguous 256-byte run.
+// MF: the matrix-core block, 16 (v_mfma_f32_16x16x32_bf16: a 16-row or
+// 16-column operand fragment, 32 deep) or 32 (v_mfma_f32_32x32x16_bf16: 32
+// rows or columns, 16 deep, twice the products per fragment read from LDS;
+// lane l holds row / column l % 32 at depth 8 (l / 32) + 0..7, and output
+// element e of lane l is row 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32).
 template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false, bool SWZ = false,
-          int IGLP = -1>
+          int IGLP = -1, int MF = 16>
 __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
     kern_nt_bf16x3(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo, long long lda, long long strA,
                    const uint16_t *__restrict__ Bhi, const uint16_t *__restrict__ Blo, long long ldb, long long strB,
                    float *__restrict__ C, long long ldc, long long strC, int M, int N, int K, float alpha, int beta) {
-  using Cfg = NtCfg<TM, TN, WM, WN, TK>;
+  using Cfg = NtCfg<TM, TN, WM, WN, TK, MF>;
   constexpr int NTH = Cfg::THREADS, BI = Cfg::BI, BJ = Cfg::BJ, LA = Cfg::LA, LB = Cfg::LB, SEG = Cfg::SEG;
   static_assert(!SWZ || TK == 32, "the swizzled layout is for 32-deep slices");
+  static_assert(MF == 16 || (MF == 32 && !SWZ), "32 x 32 blocks read the padded layout");
+  // one MFMA: depth KS, fragment rows / columns MF, lane groups of MF lanes
+  constexpr int KS = MF == 16 ? 32 : 16;
+  using Acc = typename std::conditional<MF == 16, v4f, v16f>::type;
+  auto mfma = [](const v8bf &a, const v8bf &b, const Acc &c) __attribute__((always_inline)) {
+    if constexpr (MF == 16) return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  };
   constexpr int LDK = SWZ ? TK : TK + 8;
   constexpr int WCOLS = TN / WN;
   // element offset of 8-element segment sg of LDS row rw
@@ -531,11 +547,11 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
   const int tm = g * group + r % gm, tn = r / gm;
   const int m0 = tm * TM, n0 = tn * TN;
 
-  v4f acc[BI][BJ];
+  Acc acc[BI][BJ];
 #pragma unroll
   for (int i = 0; i < BI; i++)
 #pragma unroll
-    for (int j = 0; j < BJ; j++) acc[i][j] = v4f{0, 0, 0, 0};
+    for (int j = 0; j < BJ; j++) acc[i][j] = Acc{};
 
   u32x4 ra0[LA], ra1[LA], rb0[LB], rb1[LB];
   auto gload = [&](int k0) __attribute__((always_inline)) {
@@ -571,52 +587,55 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
   auto compute = [&](int st) __attribute__((always_inline)) {
     if constexpr (IGLP >= 0) __builtin_amdgcn_iglp_opt(IGLP);
 #pragma unroll
-    for (int kk = 0; kk < TK; kk += 32) {
-      const int kseg = (kk >> 3) + (lane >> 4);
+    for (int kk = 0; kk < TK; kk += KS) {
+      const int kseg = (kk >> 3) + lane / MF;
       if constexpr (STREAM) {
         v8bf bh[BJ], bl[BJ];
 #pragma unroll
         for (int j = 0; j < BJ; j++) {
-          const int col = wn * WN + j * 16 + (lane & 15);
+          const int col = wn * WN + j * MF + lane % MF;
           bh[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][0][col][lofs(col, kseg)]));
           bl[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][1][col][lofs(col, kseg)]));
         }
 #pragma unroll
         for (int i = 0; i < BI; i++) {
-          const int row = wm * WM + i * 16 + (lane & 15);
+          const int row = wm * WM + i * MF + lane % MF;
           const v8bf a_h = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][0][row][lofs(row, kseg)]));
           const v8bf a_l = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][1][row][lofs(row, kseg)]));
 #pragma unroll
-          for (int j = 0; j < BJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, bh[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < BJ; j++) acc[i][j] = mfma(a_l, bh[j], acc[i][j]);
 #pragma unroll
-          for (int j = 0; j < BJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bl[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < BJ; j++) acc[i][j] = mfma(a_h, bl[j], acc[i][j]);
 #pragma unroll
-          for (int j = 0; j < BJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bh[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < BJ; j++) acc[i][j] = mfma(a_h, bh[j], acc[i][j]);
         }
       } else {
         u32x4 ah[BI], al[BI], bh[BJ], bl[BJ];
 #pragma unroll
         for (int i = 0; i < BI; i++) {
-          const int row = wm * WM + i * 16 + (lane & 15);
+          const int row = wm * WM + i * MF + lane % MF;
           ah[i] = *reinterpret_cast<const u32x4 *>(&sA[st][0][row][lofs(row, kseg)]);
           al[i] = *reinterpret_cast<const u32x4 *>(&sA[st][1][row][lofs(row, kseg)]);
         }
 #pragma unroll
         for (int j = 0; j < BJ; j++) {
-          const int col = wn * WN + j * 16 + (lane & 15);
+          const int col = wn * WN + j * MF + lane % MF;
           bh[j] = *reinterpret_cast<const u32x4 *>(&sB[st][0][col][lofs(col, kseg)]);
           bl[j] = *reinterpret_cast<const u32x4 *>(&sB[st][1][col][lofs(col, kseg)]);
         }
+        // three passes over every (i, j): BI x BJ independent accumulators lie
+        // between two products into the same one (the 32 x 32 MFMA's result
+        // latency is twice the 16 x 16 one's)
 #pragma unroll
-        for (int i = 0; i < BI; i++)
+        for (int pass = 0; pass < 3; pass++)
 #pragma unroll
-          for (int j = 0; j < BJ; j++) {
-            v8bf a_h = __builtin_bit_cast(v8bf, ah[i]), a_l = __builtin_bit_cast(v8bf, al[i]);
-            v8bf b_h = __builtin_bit_cast(v8bf, bh[j]), b_l = __builtin_bit_cast(v8bf, bl[j]);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, b_h, acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_l, acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, b_h, acc[i][j], 0, 0, 0);
-          }
+          for (int i = 0; i < BI; i++)
+#pragma unroll
+            for (int j = 0; j < BJ; j++) {
+              const v8bf a = __builtin_bit_cast(v8bf, pass == 0 ? al[i] : ah[i]);
+              const v8bf b = __builtin_bit_cast(v8bf, pass == 1 ? bl[j] : bh[j]);
+              acc[i][j] = mfma(a, b, acc[i][j]);
+            }
       }
     }
   };
@@ -649,9 +668,9 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
 #pragma unroll
     for (int j = 0; j < BJ; j++)
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int row = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + q;
-        const int col = n0 + wn * WN + j * 16 + (lane & 15);
+      for (int q = 0; q < (MF == 16 ? 4 : 16); q++) {
+        const int row = m0 + wm * WM + i * MF + (MF == 16 ? 4 * (lane >> 4) + q : 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3));
+        const int col = n0 + wn * WN + j * MF + lane % MF;
         float *cp = C + (size_t)row * ldc + col;
         float v = alpha * acc[i][j][q];
         if (beta) v += *cp;
@@ -689,15 +708,15 @@ struct NtVariant {
 };
 
 template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false, bool SWZ = false,
-          int IGLP = -1>
+          int IGLP = -1, int MF = 16>
 void nt_launch(const NtOp &A, const NtOp &B, float *C, long long ldc, long long strC, int M, int N, int K, int batch,
                float alpha, int beta, hipStream_t s) {
   // the grid covers whole tiles only: a ragged M or N would leave rows or
   // columns unwritten
   if (M % TM || N % TN) throw std::invalid_argument("gemm_nt_bf16x3: tile does not divide M or N");
   const int tiles = (M / TM) * (N / TN);
-  hipLaunchKernelGGL((kern_nt_bf16x3<TM, TN, WM, WN, TK, STREAM, DB, SWZ, IGLP>), dim3(tiles, batch),
-                     dim3(NtCfg<TM, TN, WM, WN, TK>::THREADS), 0, s, A.hi, A.lo, A.ld, A.stride, B.hi, B.lo, B.ld,
+  hipLaunchKernelGGL((kern_nt_bf16x3<TM, TN, WM, WN, TK, STREAM, DB, SWZ, IGLP, MF>), dim3(tiles, batch),
+                     dim3(NtCfg<TM, TN, WM, WN, TK, MF>::THREADS), 0, s, A.hi, A.lo, A.ld, A.stride, B.hi, B.lo, B.ld,
                      B.stride, C, ldc, strC, M, N, K, alpha, beta);
 }
 
@@ -722,6 +741,14 @@ const NtVariant kNt[] = {
      "256x256 / 8 waves of 128x64, K32, streamed A, 2 stages, swizzled, MFMA/LDS interleave"},
     {256, 128, nt_launch<256, 128, 64, 64, 64, false, false, false, 0>,
      "256x128 / 8 waves of 64x64, MFMA/LDS interleave"},
+    {256, 256, nt_launch<256, 256, 128, 64, 32, true, true, false, 0, 32>,
+     "256x256 / 8 waves of 128x64, K32, streamed A, 2 stages, MFMA/LDS interleave, 32x32 blocks"},
+    {256, 256, nt_launch<256, 256, 128, 64, 32, false, true, false, 0, 32>,
+     "256x256 / 8 waves of 128x64, K32, 2 stages, MFMA/LDS interleave, 32x32 blocks"},
+    {128, 128, nt_launch<128, 128, 64, 64, 32, true, true, false, 0, 32>,
+     "128x128 / 4 waves of 64x64, K32, streamed A, 2 stages, MFMA/LDS interleave, 32x32 blocks"},
+    {256, 128, nt_launch<256, 128, 64, 64, 32, true, true, false, 0, 32>,
+     "256x128 / 8 waves of 64x64, K32, streamed A, 2 stages, MFMA/LDS interleave, 32x32 blocks"},
 };
 constexpr int kNtCount = sizeof(kNt) / sizeof(kNt[0]);
 
