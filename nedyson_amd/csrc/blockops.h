@@ -22,6 +22,10 @@ typedef double blk_v4d __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ int blk_i(int q) { return 16 * (threadIdx.x >> 6) + 4 * q + ((threadIdx.x & 63) >> 4); }
 __device__ __forceinline__ int blk_j(int c) { return 16 * c + (threadIdx.x & 15); }
 
+__device__ __forceinline__ blk_v4d mfma_f64(double a, double b, blk_v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 struct BlkAcc {
   blk_v4d re[4], im[4];  // [column tile][q]
   __device__ void zero() {
@@ -32,6 +36,52 @@ struct BlkAcc {
   __device__ void set(int q, int c, z2 v) {
     re[c][q] = v.x;
     im[c][q] = v.y;
+  }
+  // column tile c += a b over one 4-deep step (lane's fragment values a, b):
+  // four real products
+  __device__ __forceinline__ void mma(const z2 &a, const z2 &b, int c) {
+    re[c] = mfma_f64(a.x, b.x, re[c]);
+    re[c] = mfma_f64(-a.y, b.y, re[c]);
+    im[c] = mfma_f64(a.x, b.y, im[c]);
+    im[c] = mfma_f64(a.y, b.x, im[c]);
+  }
+};
+
+// The same block accumulated with three real products per complex one
+// (Gauss's 3M form): t1 = sum Re a Re b, t2 = sum Im a Im b,
+// t3 = sum (Re a + Im a)(Re b + Im b), so that Re = t1 - t2 and
+// Im = t3 - t1 - t2.  A quarter fewer matrix-core passes than BlkAcc, for the
+// long history sums, which the FP64 matrix core bounds; the imaginary part's
+// rounding error grows to about eps (|Re a| + |Im a|)(|Re b| + |Im b|) per
+// term, twice the four-product form's at most.
+struct BlkAcc3 {
+  blk_v4d t1[4], t2[4], t3[4];
+  __device__ void zero() {
+#pragma unroll
+    for (int c = 0; c < 4; c++) t1[c] = t2[c] = t3[c] = blk_v4d{0.0, 0.0, 0.0, 0.0};
+  }
+  // start from the value v
+  __device__ void init(const BlkAcc &v) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      t1[c] = v.re[c];
+      t2[c] = blk_v4d{0.0, 0.0, 0.0, 0.0};
+      t3[c] = v.re[c] + v.im[c];
+    }
+  }
+  __device__ BlkAcc value() const {
+    BlkAcc v;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      v.re[c] = t1[c] - t2[c];
+      v.im[c] = t3[c] - t1[c] - t2[c];
+    }
+    return v;
+  }
+  __device__ __forceinline__ void mma(const z2 &a, const z2 &b, int c) {
+    t1[c] = mfma_f64(a.x, b.x, t1[c]);
+    t2[c] = mfma_f64(a.y, b.y, t2[c]);
+    t3[c] = mfma_f64(a.x + a.y, b.x + b.y, t3[c]);
   }
 };
 
@@ -76,8 +126,9 @@ __device__ __forceinline__ float2 zero_of(const float2 *) { return float2{0.0f, 
 // current slice; 1 (2) reads X (Y) only when it is staged -- for an operand
 // that every workgroup shares and that therefore sits in L2, this saves the
 // prefetch registers (and buys occupancy) at little cost.
-template <int LATE = 0, typename EX, typename EY>
-__device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const EX *X, int opX, const EY *Y, int opY,
+// Acc: BlkAcc (four real products per complex one) or BlkAcc3 (three).
+template <int LATE = 0, typename Acc, typename EX, typename EY>
+__device__ __forceinline__ void blk_mac(Acc &acc, BlkLds &s, int n, const EX *X, int opX, const EY *Y, int opY,
                                         z2 coef, int ldX = 0, int ldY = 0) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (ldX == 0) ldX = n;
@@ -139,10 +190,7 @@ __device__ __forceinline__ void blk_mac(BlkAcc &acc, BlkLds &s, int n, const EX 
         for (int c = 0; c < 4; c++) {
           if (16 * c < n) {
             const z2 b = s.Y[16 * c + (lane & 15)][kk];
-            acc.re[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, acc.re[c], 0, 0, 0);
-            acc.re[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, acc.re[c], 0, 0, 0);
-            acc.im[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, acc.im[c], 0, 0, 0);
-            acc.im[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, acc.im[c], 0, 0, 0);
+            acc.mma(a, b, c);
           }
         }
       }
@@ -242,6 +290,39 @@ struct TileAcc {
     re[q] = v.x;
     im[q] = v.y;
   }
+  __device__ __forceinline__ void mma(const z2 &x, const z2 &y) {
+    re = mfma_f64(x.x, y.x, re);
+    im = mfma_f64(x.x, y.y, im);
+    re = mfma_f64(-x.y, y.y, re);
+    im = mfma_f64(x.y, y.x, im);
+  }
+};
+
+// one tile in the three-product form of BlkAcc3
+struct TileAcc3 {
+  blk_v4d t1, t2, t3;
+  __device__ void zero() { t1 = t2 = t3 = blk_v4d{0.0, 0.0, 0.0, 0.0}; }
+  __device__ void init(const TileAcc &v) {
+    t1 = v.re;
+    t2 = blk_v4d{0.0, 0.0, 0.0, 0.0};
+    t3 = v.re + v.im;
+  }
+  __device__ TileAcc value() const {
+    TileAcc v;
+    v.re = t1 - t2;
+    v.im = t3 - t1 - t2;
+    return v;
+  }
+  // element q += z
+  __device__ void add(int q, z2 z) {
+    t1[q] += z.x;
+    t3[q] += z.x + z.y;
+  }
+  __device__ __forceinline__ void mma(const z2 &x, const z2 &y) {
+    t1 = mfma_f64(x.x, y.x, t1);
+    t2 = mfma_f64(x.y, y.y, t2);
+    t3 = mfma_f64(x.x + x.y, y.x + y.y, t3);
+  }
 };
 
 __device__ __forceinline__ int tile_i(int r0, int q) { return r0 + 4 * q + ((threadIdx.x & 63) >> 4); }
@@ -307,18 +388,13 @@ __device__ __forceinline__ void frag_right(FragT<NS, E> &f, int n, const E *R, i
 
 // acc += coef * L R over the fragments' steps; coef scales the left operand,
 // as the LDS-staged products do, so both forms round alike
-template <int NS, typename EA, typename EB>
-__device__ __forceinline__ void frag_mac(TileAcc &acc, const FragT<NS, EA> &a, const FragT<NS, EB> &b, int n, z2 coef,
+// (Acc: TileAcc or TileAcc3)
+template <int NS, typename Acc, typename EA, typename EB>
+__device__ __forceinline__ void frag_mac(Acc &acc, const FragT<NS, EA> &a, const FragT<NS, EB> &b, int n, z2 coef,
                                          int s0 = 0) {
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    if (4 * (s0 + s) < n) {
-      const z2 x = zmul(coef, widen(a.v[s])), y = widen(b.v[s]);
-      acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, acc.re, 0, 0, 0);
-      acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, acc.im, 0, 0, 0);
-      acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, acc.re, 0, 0, 0);
-      acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, acc.im, 0, 0, 0);
-    }
+    if (4 * (s0 + s) < n) acc.mma(zmul(coef, widen(a.v[s])), widen(b.v[s]));
   }
 }
 

@@ -112,6 +112,12 @@ struct Ctx {
   // Dyson recurrences: blocks per panel, and the largest number of later
   // blocks for which a panel's bulk update is split into 16-row/column bands
   int rpanel = 16, panel_band_max = 128;
+  // the dependency chain of a panel as one launch (kern_ret_chain /
+  // kern_les_chain) rather than one launch per link; its hand-over flags
+  // (one per block and band, for each recurrence) and counters (ticket, done,
+  // epoch)
+  bool chain_panel = true;
+  DevBuf dy_flags, dy_ctr;
   // Dyson state
   DevBuf mixK;                // mixing kernel of G^M (np x np blocks)
   DevBuf mass;                // (np x np) GLL mass matrix

@@ -494,7 +494,12 @@ __device__ __forceinline__ void ret_step_band(const PtT<SE> &p, const WTab *__re
     tile_load(qb, n, QB + (size_t)m * n * n, r0, c0);
     tile_load(old, n, p.gr(T, T - m), r0, c0);
   }
-  frag_mac(acc, fa, fb, n, z2{greg(w, m, nn), 0.0});
+  {
+    TileAcc3 a3;
+    a3.init(acc);
+    frag_mac(a3, fa, fb, n, z2{greg(w, m, nn), 0.0});
+    acc = a3.value();
+  }
   if (bdt) {
     const z2 cl = {0.0, -w.bd[m - nn] / (p.dt * p.dt)};
 #pragma unroll
@@ -511,9 +516,10 @@ __device__ __forceinline__ void ret_step_band(const PtT<SE> &p, const WTab *__re
   acc.im = (acc.im + qb.im) * p.dt;
   Frag fq;
   frag_left_band(fq, acc, band, n);
-  TileAcc x;
-  x.zero();
-  frag_mac(x, fq, fm, n, z2{1.0, 0.0});
+  TileAcc3 x3;
+  x3.zero();
+  frag_mac(x3, fq, fm, n, z2{1.0, 0.0});
+  const TileAcc x = x3.value();
   double s2 = 0;
 #pragma unroll
   for (int q = 0; q < 4; q++)
@@ -540,6 +546,142 @@ kern_ret_step_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab,
   const ChainArgT<SE> &a = args[blockIdx.z];
   if ((int)blockIdx.x >= a.nb) return;
   ret_step_band(a.p, wtab, nn, a.QA, a.Q, a.Minv, a.d2, blockIdx.x, blockIdx.y, a.nb);
+}
+
+// ---- a whole panel of the dependency chain in one launch
+// The links of one panel [q0, q1) as one launch instead of one launch each:
+// workgroup (m, b) owns band b of block m for one m in (q0, q1x] and adds the
+// contributions of the blocks nn in [q0, min(m, q1)) one after the other, as
+// they are finished; a block m <= q1 is then finished by it, a later one (in
+// the next panel) leaves its sum in the chain accumulator.  Block q0 comes
+// from an earlier launch; every later one is handed over inside the launch:
+// its finishing workgroup stores the band, makes its stores visible at device
+// scope (L2 write-back, for workgroups on other XCDs) and sets its flag to
+// this phase's epoch, and a reader polls that flag and then invalidates its
+// caches before it reads the band.
+//
+// Workgroups take (m, b) from a ticket drawn when they start, in the order of
+// m, so a flag is only ever awaited from a workgroup that started after the
+// one that sets it: the wait ends.  As a guard against a fault elsewhere, a
+// wait gives up after about 2^21 polls (some 100 ms), sets *fail to 1 and
+// carries on without the block, and so does every later wait of the step
+// (*fail is zeroed with the step's update norms); the host turns it into an
+// error.  ctr[0] is the ticket counter and ctr[1] counts the
+// workgroups that are done: the last one zeroes both for the next launch.
+// The epoch is read from device memory (kern_chain_epoch bumps it once per
+// recurrence), so that a replayed graph waits for this replay's blocks.
+constexpr unsigned CHAIN_SPIN_MAX = 1u << 21;
+
+__global__ void kern_chain_epoch(unsigned *epoch) { *epoch += 1u; }
+
+__device__ __forceinline__ int chain_ticket(unsigned *ctr) {
+  __shared__ int s_ticket;
+  if (threadIdx.x == 0) s_ticket = (int)atomicAdd(ctr, 1u);
+  __syncthreads();
+  return s_ticket;
+}
+
+__device__ __forceinline__ void chain_wait(const unsigned *flag, unsigned epoch, double *fail) {
+  if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      // after one wait has given up, the others of the step do not wait
+      if (++spins == CHAIN_SPIN_MAX || __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0.0) {
+        __hip_atomic_store(fail, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// every wave's stores of the band are complete and written back before the
+// flag is set
+__device__ __forceinline__ void chain_publish(unsigned *flag, unsigned epoch) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void chain_leave(unsigned *ctr) {
+  if (threadIdx.x == 0 && atomicAdd(ctr + 1, 1u) == gridDim.x - 1) {
+    atomicExch(ctr, 0u);
+    atomicExch(ctr + 1, 0u);
+  }
+}
+
+// the retarded panel (rows 16b..16b+15 of block m; the waves take the four
+// column tiles); the terms are those of kern_ret_step_band
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_ret_chain(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int nb, z2 *QA, const z2 *QB,
+               const z2 *Minv, double *d2, unsigned *flags, unsigned *ctr, const unsigned *epoch_p, double *fail) {
+  __builtin_amdgcn_s_setprio(3);
+  const WTab &w = *wtab;
+  __shared__ z2 band[16][BO_NMAX + 1];
+  __shared__ double red[4];
+  const int t = chain_ticket(ctr);
+  const unsigned epoch = *epoch_p;
+  const int m = q0 + 1 + t / nb, b = t % nb, T = p.T, n = p.n;
+  const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
+  const bool fin = m <= q1;
+  z2 *Qm = QA + (size_t)m * n * n;
+  TileAcc3 acc;
+  {
+    TileAcc q;
+    tile_load(q, n, Qm, r0, c0);
+    acc.init(q);
+  }
+  const int nn1 = min(m, q1);
+  for (int nn = q0; nn < nn1; nn++) {
+    FragT<BO_NMAX / 4, SE> fb;
+    frag_right(fb, n, p.sr(T - nn, T - m), 0, c0);  // ahead of the wait: it does not depend on the chain
+    if (nn > q0) chain_wait(flags + (size_t)nn * nb + b, epoch, fail);
+    Frag fa;
+    frag_left(fa, n, p.gr(T, T - nn), 0, r0);
+    frag_mac(acc, fa, fb, n, z2{greg(w, m, nn), 0.0});
+    if (m - nn <= w.k + 1) {
+      TileAcc gt;
+      tile_load(gt, n, p.gr(T, T - nn), r0, c0);
+      const z2 cl = {0.0, -w.bd[m - nn] / (p.dt * p.dt)};
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc.add(q, zmul(cl, gt.get(q)));
+    }
+  }
+  if (!fin) {
+    tile_store(acc.value(), n, Qm, r0, c0);
+    chain_leave(ctr);
+    return;
+  }
+  // finish the band: Q' = dt (QA + QB), X = Q' Minv_{T-m}
+  Frag fm;
+  TileAcc qb, old;
+  frag_right(fm, n, Minv + (size_t)(T - m) * n * n, 0, c0);
+  tile_load(qb, n, QB + (size_t)m * n * n, r0, c0);
+  tile_load(old, n, p.gr(T, T - m), r0, c0);
+  TileAcc a = acc.value();
+  a.re = (a.re + qb.re) * p.dt;
+  a.im = (a.im + qb.im) * p.dt;
+  Frag fq;
+  frag_left_band(fq, a, band, n);
+  TileAcc3 x3;
+  x3.zero();
+  frag_mac(x3, fq, fm, n, z2{1.0, 0.0});
+  const TileAcc x = x3.value();
+  double s2 = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (tile_i(r0, q) < n && tile_j(c0) < n) {
+      const z2 d = zsub(x.get(q), old.get(q));
+      s2 += d.x * d.x + d.y * d.y;
+    }
+  const double dd = wg_sum(s2, red);
+  tile_store(x, n, p.gr(T, T - m), r0, c0);
+  if (threadIdx.x == 0) d2[(size_t)m * nb + b] = dd;
+  chain_publish(flags + (size_t)m * nb + b, epoch);
+  chain_leave(ctr);
 }
 
 // err += sum over m in [m0, m1] of sqrt(sum_b d2[m][b])  (one workgroup)
@@ -571,12 +713,16 @@ __device__ __forceinline__ void ret_panel(const PtT<SE> &p, const WTab *__restri
   const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * n * n;
-  BlkAcc acc;
-  blk_load(acc, n, Qm);
+  BlkAcc3 acc;
+  {
+    BlkAcc q;
+    blk_load(q, n, Qm);
+    acc.init(q);
+  }
   for (int nn = q0; nn < q1; nn++)
     blk_mac<1>(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
   __syncthreads();
-  blk_store(acc, n, Qm);
+  blk_store(acc.value(), n, Qm);
 }
 
 template <typename SE>
@@ -604,8 +750,12 @@ __device__ __forceinline__ void ret_panel_band(const PtT<SE> &p, const WTab *__r
   if (m > T) return;
   const int r0 = 16 * by, c0 = 16 * (threadIdx.x >> 6);
   z2 *Qm = Q + (size_t)m * n * n;
-  TileAcc acc;
-  tile_load(acc, n, Qm, r0, c0);
+  TileAcc3 acc;
+  {
+    TileAcc q;
+    tile_load(q, n, Qm, r0, c0);
+    acc.init(q);
+  }
   for (int nn = q0; nn < q1; nn++)
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -615,7 +765,7 @@ __device__ __forceinline__ void ret_panel_band(const PtT<SE> &p, const WTab *__r
       frag_right(fb, n, p.sr(T - nn, T - m), 0, c0, h * BO_NMAX / 8);
       frag_mac(acc, fa, fb, n, z2{greg(w, m, nn), 0.0}, h * BO_NMAX / 8);
     }
-  tile_store(acc, n, Qm, r0, c0);
+  tile_store(acc.value(), n, Qm, r0, c0);
 }
 
 template <typename SE>
@@ -646,12 +796,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   const int nterms = T + np;
   const int per = (nterms + nchunk - 1) / nchunk;
   const int t0 = ch * per, t1 = min(nterms, t0 + per);
-  BlkAcc acc;
-  acc.zero();
+  BlkAcc3 a3;
+  a3.zero();
   for (int j = t0; j < min(t1, T); j++)
-    blk_mac<1>(acc, s, n, p.sr(T, j), 0, p.gtv(j, m), 0, z2{p.dt * greg(w, T, j), 0.0});
+    blk_mac<1>(a3, s, n, p.sr(T, j), 0, p.gtv(j, m), 0, z2{p.dt * greg(w, T, j), 0.0});
   for (int jp = max(t0, T) - T; jp < t1 - T; jp++)
-    blk_mac<1>(acc, s, n, p.stv(T, jp), 0, mixK + ((size_t)m * np + jp) * p.es(), 0, z2{1.0, 0.0});
+    blk_mac<1>(a3, s, n, p.stv(T, jp), 0, mixK + ((size_t)m * np + jp) * p.es(), 0, z2{1.0, 0.0});
+  BlkAcc acc = a3.value();
   if (ch == 0)
     for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, p.gtv(T - l, m), 0, z2{0.0, -w.bd[l] / p.dt});
   blk_store(acc, n, part + ((size_t)ch * np + m) * p.es());
@@ -723,7 +874,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   const int nterms = np + T + 1;
   const int per = (nterms + nchunk - 1) / nchunk;
   const int t0 = ch * per, t1 = min(nterms, t0 + per);
-  BlkAcc acc;
+  BlkAcc3 acc;
   acc.zero();
   // three runs of terms, each with fixed operand forms (one inlined product
   // per run keeps the live registers to one copy):
@@ -739,7 +890,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     const int j = t - np;
     blk_mac<2>(acc, s, n, p.sl(row, j), 0, p.gr(T, j), 1, z2{p.dt * greg(w, T, j), 0.0});
   }
-  blk_store(acc, n, part + ((size_t)ch * (T + 1) + row) * p.es());
+  blk_store(acc.value(), n, part + ((size_t)ch * (T + 1) + row) * p.es());
 }
 
 __global__ void kern_sum_parts(const z2 *part, int nchunk, long long nel, z2 *out) {
@@ -877,7 +1028,12 @@ __device__ __forceinline__ void les_step_band(const PtT<SE> &p, const WTab *__re
   frag_right(fb, n, X + (size_t)l * es, 0, c0);
   if (bdt) tile_load(gt, n, X + (size_t)l * es, r0, c0);
   if (fin) tile_load(qb, n, QB + (size_t)m * es, r0, c0);
-  frag_mac(acc, fa, fb, n, z2{p.dt * greg(w, m, l), 0.0});
+  {
+    TileAcc3 a3;
+    a3.init(acc);
+    frag_mac(a3, fa, fb, n, z2{p.dt * greg(w, m, l), 0.0});
+    acc = a3.value();
+  }
   if (bdt) {
     const z2 cl = {0.0, -w.bd[m - l] / p.dt};
 #pragma unroll
@@ -893,10 +1049,10 @@ __device__ __forceinline__ void les_step_band(const PtT<SE> &p, const WTab *__re
   acc.im += qb.im;
   Frag fq;
   frag_right_band(fq, acc, band, n);
-  TileAcc x;
-  x.zero();
-  frag_mac(x, fm, fq, n, z2{1.0, 0.0});
-  tile_store(x, n, X + (size_t)m * es, r0, c0);
+  TileAcc3 x3;
+  x3.zero();
+  frag_mac(x3, fm, fq, n, z2{1.0, 0.0});
+  tile_store(x3.value(), n, X + (size_t)m * es, r0, c0);
 }
 
 template <typename SE>
@@ -913,6 +1069,67 @@ kern_les_step_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab,
   les_step_band(a.p, wtab, l, a.QA, a.Q, a.X, a.Minv, blockIdx.x, blockIdx.y);
 }
 
+// the lesser panel in one launch, as kern_ret_chain (columns 16b..16b+15 of
+// block m; the waves take the four row tiles; the terms of
+// kern_les_step_band; block T is left to the diagonal step)
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_les_chain(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int nb, z2 *QA, const z2 *QB, z2 *X,
+               const z2 *Minv, unsigned *flags, unsigned *ctr, const unsigned *epoch_p, double *fail) {
+  __builtin_amdgcn_s_setprio(3);
+  const WTab &w = *wtab;
+  __shared__ z2 band[BO_NMAX][17];
+  const int t = chain_ticket(ctr);
+  const unsigned epoch = *epoch_p;
+  const int m = q0 + 1 + t / nb, b = t % nb, T = p.T, n = p.n;
+  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
+  const bool fin = m <= q1 && m <= T - 1;
+  const size_t es = p.es();
+  z2 *Qm = QA + (size_t)m * es;
+  TileAcc3 acc;
+  {
+    TileAcc q;
+    tile_load(q, n, Qm, r0, c0);
+    acc.init(q);
+  }
+  const int l1 = min(m, q1);
+  for (int l = q0; l < l1; l++) {
+    FragT<BO_NMAX / 4, SE> fa;
+    frag_left(fa, n, p.sr(m, l), 0, r0);  // ahead of the wait
+    if (l > q0) chain_wait(flags + (size_t)l * nb + b, epoch, fail);
+    Frag fb;
+    frag_right(fb, n, X + (size_t)l * es, 0, c0);
+    frag_mac(acc, fa, fb, n, z2{p.dt * greg(w, m, l), 0.0});
+    if (m - l <= w.k + 1 && m <= T - 1) {
+      TileAcc gt;
+      tile_load(gt, n, X + (size_t)l * es, r0, c0);
+      const z2 cl = {0.0, -w.bd[m - l] / p.dt};
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc.add(q, zmul(cl, gt.get(q)));
+    }
+  }
+  if (!fin) {
+    tile_store(acc.value(), n, Qm, r0, c0);
+    chain_leave(ctr);
+    return;
+  }
+  Frag fm;
+  TileAcc qb;
+  frag_left(fm, n, Minv + (size_t)m * es, 0, r0);
+  tile_load(qb, n, QB + (size_t)m * es, r0, c0);
+  TileAcc a = acc.value();
+  a.re += qb.re;
+  a.im += qb.im;
+  Frag fq;
+  frag_right_band(fq, a, band, n);
+  TileAcc3 x3;
+  x3.zero();
+  frag_mac(x3, fm, fq, n, z2{1.0, 0.0});
+  tile_store(x3.value(), n, X + (size_t)m * es, r0, c0);
+  chain_publish(flags + (size_t)m * nb + b, epoch);
+  chain_leave(ctr);
+}
+
 // contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
 template <typename SE>
 __device__ __forceinline__ void les_panel(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
@@ -922,12 +1139,16 @@ __device__ __forceinline__ void les_panel(const PtT<SE> &p, const WTab *__restri
   const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * p.es();
-  BlkAcc acc;
-  blk_load(acc, n, Qm);
+  BlkAcc3 acc;
+  {
+    BlkAcc q;
+    blk_load(q, n, Qm);
+    acc.init(q);
+  }
   for (int l = q0; l < q1; l++)
     blk_mac<2>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
   __syncthreads();
-  blk_store(acc, n, Qm);
+  blk_store(acc.value(), n, Qm);
 }
 
 template <typename SE>
@@ -954,8 +1175,12 @@ __device__ __forceinline__ void les_panel_band(const PtT<SE> &p, const WTab *__r
   const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * by;
   const size_t es = p.es();
   z2 *Qm = Q + (size_t)m * es;
-  TileAcc acc;
-  tile_load(acc, n, Qm, r0, c0);
+  TileAcc3 acc;
+  {
+    TileAcc q;
+    tile_load(q, n, Qm, r0, c0);
+    acc.init(q);
+  }
   for (int l = q0; l < q1; l++)
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -965,7 +1190,7 @@ __device__ __forceinline__ void les_panel_band(const PtT<SE> &p, const WTab *__r
       frag_right(fb, n, X + (size_t)l * es, 0, c0, h * BO_NMAX / 8);
       frag_mac(acc, fa, fb, n, z2{p.dt * greg(w, m, l), 0.0}, h * BO_NMAX / 8);
     }
-  tile_store(acc, n, Qm, r0, c0);
+  tile_store(acc.value(), n, Qm, r0, c0);
 }
 
 template <typename SE>
@@ -1452,8 +1677,9 @@ namespace {
 // The blocked right-looking recurrence with look-ahead, shared by the
 // retarded and lesser phases, for one system or a group.  Panel p holds the
 // links nn in [q0, q1).  Each link finishes block nn+1 and adds block nn to
-// the later blocks of its own panel and of the next one (chain(nn, q1x): small
-// launches on stream s, the dependency chain; accumulator QA).  The bulk
+// the later blocks of its own panel and of the next one (chain(q0, q1, q1x):
+// the panel's links on stream s, the dependency chain, as one launch each or
+// one launch for the panel; accumulator QA).  The bulk
 // update -- panel p applied to every block beyond the next panel,
 // panel(q0, q1, q1x, band) -- runs on the side stream sd into a second
 // accumulator Q while the chain goes on, and the chain of panel p+2 waits for
@@ -1466,7 +1692,7 @@ void recurrence(int k, int T, int RP, int band_max, hipStream_t s, hipStream_t s
   for (int q0 = k + 1; q0 < T; q0 += RP, pi++) {
     const int q1 = std::min(q0 + RP, T), q1x = std::min(q1 + RP, T);
     if (pi >= 2) NED_HIP(hipStreamWaitEvent(s, evs[pi & 1], 0));
-    for (int nn = q0; nn < q1; nn++) chain(nn, q1x);
+    chain(q0, q1, q1x);
     if (q1x < T) {
       NED_HIP(hipEventRecord(evm, s));
       NED_HIP(hipStreamWaitEvent(sd, evm, 0));
@@ -1482,6 +1708,30 @@ void recurrence(int k, int T, int RP, int band_max, hipStream_t s, hipStream_t s
 // every block whose finishing terms they fold in (up to k+1 blocks ahead)
 inline int panel_blocks(const Ctx &c) { return std::max(c.rpanel, c.k); }
 inline int bands(const Ctx &c) { return (c.nao + 15) / 16; }  // 16-row / 16-column bands of a block
+
+// One launch per panel for the dependency chains (kern_ret_chain /
+// kern_les_chain) unless the context asks for one launch per link: the
+// flags (one per block and band, for each recurrence), the ticket counters
+// and the epoch are allocated and zeroed with the step's other scratch
+// (chain_alloc); the epoch is bumped on the device for each recurrence
+void chain_alloc(Ctx &c) {
+  if (!c.chain_panel) return;
+  const size_t nflag = 2 * (size_t)(c.nt + 1) * bands(c);
+  if (c.dy_flags.bytes >= nflag * sizeof(unsigned)) return;
+  c.dy_flags.ensure(nflag * sizeof(unsigned));
+  c.dy_ctr.ensure(4 * sizeof(unsigned));
+  hipLaunchKernelGGL(kern_zero_words, dim3(nblk((long long)nflag)), dim3(256), 0, c.stream, c.dy_flags.as<unsigned>(),
+                     (int)nflag);
+  hipLaunchKernelGGL(kern_zero_words, dim3(1), dim3(256), 0, c.stream, c.dy_ctr.as<unsigned>(), 4);
+  check_launch("dyson chain flags");
+}
+
+bool chain_begin(Ctx &c) {
+  if (!c.chain_panel) return false;
+  hipLaunchKernelGGL(kern_chain_epoch, dim3(1), dim3(1), 0, c.stream, c.dy_ctr.as<unsigned>() + 2);
+  check_launch("dyson chain epoch");
+  return true;
+}
 
 // ---- phase 1: cached inverses, scratch, the mixing row on the third stream
 // (src/dyson_step.h:130-169: it reads only Sigma and the G^rceil and G^M
@@ -1519,6 +1769,7 @@ void dyson_begin(Ctx &c, int T, double mu, double beta, double dt) {
   c.dy_retinv.ensure((size_t)kn * kn * sizeof(z2));
   c.dy_lesinv.ensure((size_t)kn * kn * sizeof(z2));
   c.dy_Xs.ensure((size_t)kn * n * sizeof(z2));
+  chain_alloc(c);
   z2 *Q = c.dy_Q.as<z2>(), *M = c.dy_M.as<z2>(), *Qs = c.dy_Qs.as<z2>(), *QA = c.dy_QA.as<z2>();
   {
     const int nch = chunks_for(np, T + np);
@@ -1562,10 +1813,18 @@ void dyson_ret_one(Ctx &c, int T, double mu, double beta, double dt) {
   z2 *Q = c.dy_Q.as<z2>(), *QA = c.dy_QA.as<z2>();
   const z2 *Minv = c.Minv.as<z2>();
   double *d2 = c.dy_d2.as<double>();
+  const bool persist = chain_begin(c);
   recurrence(c.k, T, panel_blocks(c), c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
-             [&](int nn, int q1x) {
-               SE_LAUNCH(c, kern_ret_step_band, dim3(nb, q1x - nn), dim3(256), 0, c.stream, p, wd, nn, QA, Q,
-                                  Minv, d2);
+             [&](int q0, int q1, int q1x) {
+               if (persist) {
+                 SE_LAUNCH(c, kern_ret_chain, dim3(nb * (q1x - q0)), dim3(256), 0, c.stream, p, wd, q0, q1, nb, QA,
+                           Q, Minv, d2, c.dy_flags.as<unsigned>(), c.dy_ctr.as<unsigned>(),
+                           c.dy_ctr.as<unsigned>() + 2, c.dy_err.as<double>() + 3);
+                 return;
+               }
+               for (int nn = q0; nn < q1; nn++)
+                 SE_LAUNCH(c, kern_ret_step_band, dim3(nb, q1x - nn), dim3(256), 0, c.stream, p, wd, nn, QA, Q,
+                           Minv, d2);
              },
              [&](int q0, int q1, int q1x, bool band) {
                if (band)
@@ -1617,10 +1876,18 @@ void dyson_les_one(Ctx &c, int T, double mu, double beta, double dt) {
   const int nb = bands(c);
   z2 *Q = c.dy_Q.as<z2>(), *QA = c.dy_QA.as<z2>(), *X = c.dy_X.as<z2>();
   const z2 *Minv = c.Minv.as<z2>();
+  const bool persist = chain_begin(c);
   recurrence(c.k, T, panel_blocks(c), c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
-             [&](int l, int q1x) {
-               SE_LAUNCH(c, kern_les_step_band, dim3(nb, q1x - l), dim3(256), 0, c.stream, p, wd, l, QA, Q, X,
-                                  Minv);
+             [&](int q0, int q1, int q1x) {
+               if (persist) {
+                 SE_LAUNCH(c, kern_les_chain, dim3(nb * (q1x - q0)), dim3(256), 0, c.stream, p, wd, q0, q1, nb, QA,
+                           Q, X, Minv, c.dy_flags.as<unsigned>() + (size_t)(c.nt + 1) * nb, c.dy_ctr.as<unsigned>(),
+                           c.dy_ctr.as<unsigned>() + 2, c.dy_err.as<double>() + 3);
+                 return;
+               }
+               for (int l = q0; l < q1; l++)
+                 SE_LAUNCH(c, kern_les_step_band, dim3(nb, q1x - l), dim3(256), 0, c.stream, p, wd, l, QA, Q, X,
+                           Minv);
              },
              [&](int q0, int q1, int q1x, bool band) {
                if (band)
@@ -1731,8 +1998,9 @@ void dyson_step_group(DysonGroup &g, const std::vector<Ctx *> &cs, int T, const 
   const ChainArg *args = g.args.as<ChainArg>();
   const WTab *wd = dev_wtab(*cs[0]);
   recurrence(c0.k, T, panel_blocks(c0), c0.panel_band_max, g.s, g.sd, g.evm, g.evs,
-             [&](int nn, int q1x) {
-               GRP_LAUNCH(s32, kern_ret_step_band_grp, dim3(nbmax, q1x - nn, S), dim3(256), 0, g.s, args, wd, nn);
+             [&](int q0, int q1, int q1x) {
+               for (int nn = q0; nn < q1; nn++)
+                 GRP_LAUNCH(s32, kern_ret_step_band_grp, dim3(nbmax, q1x - nn, S), dim3(256), 0, g.s, args, wd, nn);
              },
              [&](int q0, int q1, int q1x, bool band) {
                if (band)
@@ -1747,8 +2015,9 @@ void dyson_step_group(DysonGroup &g, const std::vector<Ctx *> &cs, int T, const 
   for (int i = 0; i < S; i++) dyson_mid(*cs[i], T, mu[i], beta[i], dt[i]);
   group_join_in(g, cs);
   recurrence(c0.k, T, panel_blocks(c0), c0.panel_band_max, g.s, g.sd, g.evm, g.evs,
-             [&](int l, int q1x) {
-               GRP_LAUNCH(s32, kern_les_step_band_grp, dim3(nbmax, q1x - l, S), dim3(256), 0, g.s, args, wd, l);
+             [&](int q0, int q1, int q1x) {
+               for (int l = q0; l < q1; l++)
+                 GRP_LAUNCH(s32, kern_les_step_band_grp, dim3(nbmax, q1x - l, S), dim3(256), 0, g.s, args, wd, l);
              },
              [&](int q0, int q1, int q1x, bool band) {
                if (band)
@@ -1767,6 +2036,7 @@ void dyson_read_err(Ctx &c, double *err3) {
   double h[4];
   NED_HIP(hipMemcpyAsync(h, c.dy_err.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   NED_HIP(hipStreamSynchronize(c.stream));
+  if (h[3] != 0.0) throw std::runtime_error("dyson_step: a wait of the one-launch-per-panel dependency chain passed its bound");
   err3[0] = h[0]; err3[1] = h[1]; err3[2] = h[2];
 }
 

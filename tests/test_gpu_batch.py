@@ -86,9 +86,12 @@ def test_batch_reports_the_failing_system():
         c.close()
 
 
-def test_batch_threads_mode_matches_grouped():
+def test_batch_threads_mode_matches_grouped(monkeypatch):
     """The two batch modes (grouped Dyson launches, and one host thread and
-    stream per system) give the same G."""
+    stream per system) give the same G.  The grouped recurrences launch one
+    dependency-chain link at a time, so the contexts are made to do the same
+    (NED_CHAIN_PANEL=0) for a bitwise comparison."""
+    monkeypatch.setenv("NED_CHAIN_PANEL", "0")
     nt = 16
     ps = _systems(nt)
     outs = []
@@ -127,15 +130,17 @@ def _equilibrium_ctx(p, nt, mu, F, GM):
 
 
 @pytest.mark.timeout(900)
-def test_batch_configuration_16_uneven_systems(orc):
+def test_batch_configuration_16_uneven_systems(orc, monkeypatch):
     """The batched configuration's systems (problem.batch_systems: 16 systems,
     nao drawn from {8, 12, 16, 24, 32, 48}, naux = 4 nao, 64 GLL nodes,
     dt = 0.02, each with its own quench), B entries with sigma 0.05, each from
     its GF2 equilibrium through 12 steps of FP32 Sigma and FP64 Dyson: the
     grouped batch equals the systems stepped alone, bit for bit, and the
     nao = 8 system equals the oracle's FP64 run from the same equilibrium
-    within the FP32 bound."""
+    within the FP32 bound.  (The systems alone launch one dependency-chain
+    link at a time, as the grouped recurrences do: NED_CHAIN_PANEL=0.)"""
     from nedyson_amd.green import GreenFunc
+    monkeypatch.setenv("NED_CHAIN_PANEL", "0")
     nt = 17
     ps = problem.batch_systems(b_sd=0.05, nt=nt)
     print(f"[batch16] nao {[p.nao for p in ps]}", flush=True)
