@@ -113,10 +113,11 @@ struct Ctx {
   // blocks for which a panel's bulk update is split into 16-row/column bands
   int rpanel = 16, panel_band_max = 128;
   // the dependency chain of a panel as one launch (kern_ret_chain /
-  // kern_les_chain) rather than one launch per link; its hand-over flags
-  // (one per block and band, for each recurrence) and counters (ticket, done,
-  // epoch)
-  bool chain_panel = true;
+  // kern_les_chain) rather than one launch per link (NED_CHAIN_PANEL=1; at
+  // nao = 64, T = 1000 it measured no faster: its links wait on device-scope
+  // hand-overs about as long as a launch takes); its hand-over flags (one per
+  // block and band, for each recurrence) and counters (ticket, done, epoch)
+  bool chain_panel = false;
   DevBuf dy_flags, dy_ctr;
   // Dyson state
   DevBuf mixK;                // mixing kernel of G^M (np x np blocks)

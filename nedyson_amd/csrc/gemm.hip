@@ -523,24 +523,28 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
   auto lofs = [](int rw, int sg) __attribute__((always_inline)) {
     return SWZ ? ((sg ^ ((rw >> 1) & 3)) << 3) : (sg << 3);
   };
-  Ahi += (size_t)blockIdx.y * strA;
-  Alo += (size_t)blockIdx.y * strA;
-  Bhi += (size_t)blockIdx.y * strB;
-  Blo += (size_t)blockIdx.y * strB;
-  C += (size_t)blockIdx.y * strC;
+  // Workgroups are dealt round-robin to the 8 XCDs in launch order, each XCD
+  // with its own L2: give every XCD a contiguous run of the (product, tile)
+  // sequence -- every tile of a product of a batch on one XCD, at the same
+  // time, so that they share its A and B rows in that L2 -- and, within a
+  // product, of the group-ordered tile sequence, so that the 8-row x
+  // few-column window an XCD works on at a time shares its tiles
+  const int tilesN = N / TN, tilesM = M / TM, tiles = tilesM * tilesN;
+  const int lin = blockIdx.y * gridDim.x + blockIdx.x, total = tiles * gridDim.y;
+  const int xcd = lin % 8, perX = total / 8, rem = total % 8;
+  const int unit = xcd * perX + min(xcd, rem) + lin / 8;
+  const int item = unit / tiles, bid = unit % tiles;
+  Ahi += (size_t)item * strA;
+  Alo += (size_t)item * strA;
+  Bhi += (size_t)item * strB;
+  Blo += (size_t)item * strB;
+  C += (size_t)item * strC;
   constexpr int NS = DB ? 2 : 1;
   __shared__ __attribute__((aligned(16))) uint16_t sA[NS][2][TM][LDK];
   __shared__ __attribute__((aligned(16))) uint16_t sB[NS][2][TN][LDK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WCOLS, wn = wave % WCOLS;
-  const int tilesN = N / TN, tilesM = M / TM;
   const int group = 8;
-  // Workgroups are dealt round-robin to the 8 XCDs (bid % 8), each with its
-  // own L2: give every XCD a contiguous run of the group-ordered tile
-  // sequence, so that the 8-row x few-column window it works on at a time
-  // shares its A and B tiles in that XCD's L2
-  const int total = tilesM * tilesN, xcd = blockIdx.x % 8, perX = total / 8, rem = total % 8;
-  const int bid = xcd * perX + min(xcd, rem) + (int)(blockIdx.x / 8);
   const int gsize = group * tilesN;
   const int g = bid / gsize, r = bid % gsize;
   const int gm = min(group, tilesM - g * group);
