@@ -466,9 +466,8 @@ __global__ void __launch_bounds__(256) kern_ret_qinit(PtT<SE> p, const WTab *__r
 // side stream.  Hence: operands go straight into registers (no LDS staging);
 // of the finishing terms -i/dt bd_l G^R(T, T-m+l) in Q' = dt Q - ..., each
 // link adds those of block nn to the blocks up to k+1 ahead, so a finishing
-// workgroup adds only the newest one; the register budget (at most 224 a
-// wave) lets a chain wave share a SIMD with two bulk panel waves (at most 136
-// each); and the waves raise their issue priority.
+// workgroup adds only the newest one; the products take three real ones per
+// complex one (TileAcc3); and the waves raise their issue priority.
 template <typename SE>
 __device__ __forceinline__ void ret_step_band(const PtT<SE> &p, const WTab *__restrict__ wtab, int nn, z2 *QA,
                                               const z2 *QB, const z2 *Minv, double *d2, int b, int by, int nb) {
@@ -704,7 +703,10 @@ __global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *
 }
 
 // contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
-// the history sums stay in registers across the whole panel
+// the history sums stay in registers across the whole panel.  Both operands'
+// next K slices are fetched ahead (two workgroups a CU): with the shared G^R
+// block read only when staged and three workgroups a CU, the panels took 10%
+// longer.
 template <typename SE>
 __device__ __forceinline__ void ret_panel(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
                                           int bx) {
@@ -720,19 +722,19 @@ __device__ __forceinline__ void ret_panel(const PtT<SE> &p, const WTab *__restri
     acc.init(q);
   }
   for (int nn = q0; nn < q1; nn++)
-    blk_mac<1>(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
+    blk_mac<0>(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
   __syncthreads();
   blk_store(acc.value(), n, Qm);
 }
 
 template <typename SE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 kern_ret_panel(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
   ret_panel(p, wtab, q0, q1, m0, Q, blockIdx.x);
 }
 
 template <typename SE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 kern_ret_panel_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
   const ChainArgT<SE> &a = args[blockIdx.y];
   ret_panel(a.p, wtab, q0, q1, m0, a.Q, blockIdx.x);
@@ -1131,6 +1133,7 @@ kern_les_chain(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int nb,
 }
 
 // contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
+// (both operands fetched ahead, as in ret_panel)
 template <typename SE>
 __device__ __forceinline__ void les_panel(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
                                           const z2 *X, int bx) {
@@ -1146,19 +1149,19 @@ __device__ __forceinline__ void les_panel(const PtT<SE> &p, const WTab *__restri
     acc.init(q);
   }
   for (int l = q0; l < q1; l++)
-    blk_mac<2>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+    blk_mac<0>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
   __syncthreads();
   blk_store(acc.value(), n, Qm);
 }
 
 template <typename SE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 kern_les_panel(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
   les_panel(p, wtab, q0, q1, m0, Q, X, blockIdx.x);
 }
 
 template <typename SE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 kern_les_panel_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
   const ChainArgT<SE> &a = args[blockIdx.y];
   les_panel(a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x);

@@ -230,8 +230,11 @@ def test_c2_window_step_parity(orc):
         S.tv[t] = S.tv[T0]
     D = orc.Dyson(p.nt, p.ntau, p.nao, p.k)
     T = T0 + K
-    D.do_tstp(p.V, T, G, S, hmf, p.h0, p.mu, p.beta, p.dt, 3, nthreads=NTH)
-    print("[c2 window] oracle step done", flush=True)
+    # the step's three corrector passes one call each (the predictor before the
+    # first), a progress line after each: a pass takes minutes on the CPU
+    for ps in range(3):
+        D.do_tstp(p.V, T, G, S, hmf, p.h0, p.mu, p.beta, p.dt, 1, nthreads=NTH, extrapolate=ps == 0)
+        print(f"[c2 window] oracle pass {ps + 1} of 3 done", flush=True)
     for storage, prec, tol in ((0, "f64", 1e-10), (ned.NED_STORE_SIGMA_F32 | ned.NED_STORE_G_RING, "f64", 1e-7),
                                (0, "bf16x3", 1e-6)):
         with ned.Context(p.nao, p.naux, p.nt, p.ntau, p.k, storage=storage) as c:
