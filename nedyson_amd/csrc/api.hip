@@ -224,7 +224,8 @@ ned_status ned_ctx_create_ex(ned_ctx **out, const ned_dims *d, int device, unsig
     // schedule of the Dyson recurrences (tests and tuning)
     if (const char *e = std::getenv("NED_RPANEL")) c.rpanel = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("NED_PANEL_BAND_MAX")) c.panel_band_max = std::atoi(e);
-    if (const char *e = std::getenv("NED_CHAIN_PANEL")) c.chain_panel = std::atoi(e) != 0;
+    if (const char *e = std::getenv("NED_CHAIN")) c.chain_mode = std::atoi(e);
+    ned::require(c.chain_mode >= 0 && c.chain_mode <= 2, "NED_CHAIN must be 0, 1 or 2");
     if (const char *e = std::getenv("NED_GRAPH")) c.graph_mode = std::atoi(e) != 0;
     NED_HIP(hipStreamSynchronize(c.stream));
   });

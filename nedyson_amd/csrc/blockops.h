@@ -37,6 +37,10 @@ struct BlkAcc {
     re[c][q] = v.x;
     im[c][q] = v.y;
   }
+  __device__ void add(int q, int c, z2 v) {
+    re[c][q] += v.x;
+    im[c][q] += v.y;
+  }
   // column tile c += a b over one 4-deep step (lane's fragment values a, b):
   // four real products
   __device__ __forceinline__ void mma(const z2 &a, const z2 &b, int c) {
@@ -77,6 +81,11 @@ struct BlkAcc3 {
       v.im[c] = t3[c] - t1[c] - t2[c];
     }
     return v;
+  }
+  // element (q, c) += z
+  __device__ void add(int q, int c, z2 z) {
+    t1[c][q] += z.x;
+    t3[c][q] += z.x + z.y;
   }
   __device__ __forceinline__ void mma(const z2 &a, const z2 &b, int c) {
     t1[c] = mfma_f64(a.x, b.x, t1[c]);
@@ -198,15 +207,15 @@ __device__ __forceinline__ void blk_mac(Acc &acc, BlkLds &s, int n, const EX *X,
   }
 }
 
-// acc += coef * op(X)   (elementwise, no product)
-template <typename E>
-__device__ __forceinline__ void blk_axpy(BlkAcc &acc, int n, const E *X, int opX, z2 coef) {
+// acc += coef * op(X)   (elementwise, no product; Acc: BlkAcc or BlkAcc3)
+template <typename Acc, typename E>
+__device__ __forceinline__ void blk_axpy(Acc &acc, int n, const E *X, int opX, z2 coef) {
 #pragma unroll
   for (int q = 0; q < 4; q++)
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       int i = blk_i(q), j = blk_j(c);
-      if (i < n && j < n) acc.set(q, c, zadd(acc.get(q, c), zmul(coef, blk_get(X, n, opX, i, j))));
+      if (i < n && j < n) acc.add(q, c, zmul(coef, blk_get(X, n, opX, i, j)));
     }
 }
 

@@ -112,12 +112,21 @@ struct Ctx {
   // Dyson recurrences: blocks per panel, and the largest number of later
   // blocks for which a panel's bulk update is split into 16-row/column bands
   int rpanel = 16, panel_band_max = 128;
-  // the dependency chain of a panel as one launch (kern_ret_chain /
-  // kern_les_chain) rather than one launch per link (NED_CHAIN_PANEL=1; at
-  // nao = 64, T = 1000 it measured no faster: its links wait on device-scope
-  // hand-overs about as long as a launch takes); its hand-over flags (one per
-  // block and band, for each recurrence) and counters (ticket, done, epoch)
-  bool chain_panel = false;
+  // the form of the recurrences' dependency chains (NED_CHAIN):
+  //   1 (default): one launch per link, the panel's history sums carried
+  //     times M^-1 so that a link is one product (kern_*_xform, kern_*_link),
+  //     with transfer blocks formed once per time step (dy_tbr / dy_tbl, valid
+  //     for step tb_T[0] / tb_T[1] at mu tb_mu, dt tb_dt and tb_rp blocks a panel);
+  //   0: one launch per link of two products (kern_*_step_band), the form the
+  //     grouped recurrences of many systems use;
+  //   2: one launch per panel (kern_ret_chain / kern_les_chain), with hand-over
+  //     flags (one per block and band, for each recurrence) and counters
+  //     (ticket, done, epoch); at nao = 64, T = 1000 no faster than 0: its
+  //     links wait on device-scope hand-overs about as long as a launch takes
+  int chain_mode = 1;
+  DevBuf dy_tbr, dy_tbl;
+  int tb_T[2] = {-1, -1}, tb_rp[2] = {0, 0};
+  double tb_mu[2] = {0, 0}, tb_dt[2] = {0, 0};
   DevBuf dy_flags, dy_ctr;
   // Dyson state
   DevBuf mixK;                // mixing kernel of G^M (np x np blocks)

@@ -547,6 +547,123 @@ kern_ret_step_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab,
   ret_step_band(a.p, wtab, nn, a.QA, a.Q, a.Minv, a.d2, blockIdx.x, blockIdx.y, a.nb);
 }
 
+// ---- chain links of one product: the history sums carried times M^-1
+// A block is finished as X_m = dt S_m Minv_{T-m}, S_m its history sum.  Within
+// a panel the chain carries R_m = dt S_m Minv_{T-m} instead: with the transfer
+// blocks
+//   B_{t,m} = dt (greg(m,t) Sigma^R(T-t, T-m) + [m-t <= k+1] c_{m-t}) Minv_{T-m}
+// (c_l = -i bd_l / dt^2, the folded finishing terms) a link adds
+// R_m += X_t B_{t,m}, and the finishing workgroup's X_{t+1} = R_{t+1} +
+// X_t B_{t,t+1} is one product, where kern_ret_step_band takes a product, an
+// exchange of the band through LDS and a second product.  The transfer blocks
+// read Sigma^R rows before T and the cached inverses of times before T only,
+// so they are formed once per time step (kern_ret_tblk) for every corrector
+// pass.  At the start of a panel, kern_ret_xform turns the panel's sums into
+// R (R_m = dt (Q_m + QA_m) Minv_{T-m}, in place in QA); the links add to the
+// next panel's blocks in the plain form, as kern_ret_step_band does.
+//
+// Transfer block of link t to block m = t + d (d = 1..RP, m up to the end of
+// t's panel): tb[t RP + d - 1]
+template <typename SE>
+__global__ void __launch_bounds__(256) kern_ret_tblk(PtT<SE> p, const WTab *__restrict__ wtab, int RP,
+                                                     const z2 *Minv, z2 *tb) {
+  const WTab &w = *wtab;
+  __shared__ BlkLds s;
+  const int d = blockIdx.x + 1, t = w.k + 1 + blockIdx.y, m = t + d, T = p.T, n = p.n;
+  const int q0 = w.k + 1 + RP * ((t - w.k - 1) / RP), q1 = min(q0 + RP, T);
+  if (m > q1) return;
+  const z2 *Mi = Minv + (size_t)(T - m) * n * n;
+  BlkAcc3 acc;
+  acc.zero();
+  blk_mac(acc, s, n, p.sr(T - t, T - m), 0, Mi, 0, z2{p.dt * greg(w, m, t), 0.0});
+  if (d <= w.k + 1) blk_axpy(acc, n, Mi, 0, z2{0.0, -w.bd[d] / p.dt});
+  blk_store(acc.value(), n, tb + ((size_t)t * RP + d - 1) * n * n);
+}
+
+// R_m = dt (Q_m + QA_m) Minv_{T-m} for the blocks m of panel (q0, q1], in
+// place in QA; workgroup (b, m) owns rows 16b..16b+15, its waves the column
+// tiles
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_ret_xform(Pt p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
+  const int b = blockIdx.x, m = q0 + 1 + blockIdx.y, T = p.T, n = p.n;
+  const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
+  const size_t es = p.es();
+  z2 *QAm = QA + (size_t)m * es;
+  TileAcc3 x;
+  x.zero();
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    FragT<BO_NMAX / 8> fa, fq;
+    FragT<BO_NMAX / 8> fm;
+    frag_left(fa, n, (const z2 *)QAm, 0, r0, h * BO_NMAX / 8);
+    frag_left(fq, n, Q + (size_t)m * es, 0, r0, h * BO_NMAX / 8);
+    frag_right(fm, n, Minv + (size_t)(T - m) * es, 0, c0, h * BO_NMAX / 8);
+#pragma unroll
+    for (int s = 0; s < BO_NMAX / 8; s++) fa.v[s] = zadd(fa.v[s], fq.v[s]);
+    frag_mac(x, fa, fm, n, z2{p.dt, 0.0}, h * BO_NMAX / 8);
+  }
+  __syncthreads();  // every wave has read the band before any writes its tile
+  tile_store(x.value(), n, QAm, r0, c0);
+}
+
+// link nn: blocks m of its panel (m <= q1) get R_m += X_nn B_{nn,m} and the
+// finishing one (m = nn + 1) is X_m = R_m + X_nn B_{nn,m}; blocks of the next
+// panel get the plain contribution of kern_ret_step_band
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_ret_link(PtT<SE> p, const WTab *__restrict__ wtab, int nn, int q1, int RP, z2 *QA, const z2 *tb, double *d2) {
+  __builtin_amdgcn_s_setprio(3);
+  const WTab &w = *wtab;
+  __shared__ double red[4];
+  const int b = blockIdx.x, nb = gridDim.x, m = nn + 1 + blockIdx.y, T = p.T, n = p.n;
+  const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
+  const size_t es = p.es();
+  z2 *Qm = QA + (size_t)m * es;
+  TileAcc3 acc;
+  {
+    TileAcc q;
+    tile_load(q, n, Qm, r0, c0);
+    acc.init(q);
+  }
+  Frag fa;
+  frag_left(fa, n, p.gr(T, T - nn), 0, r0);
+  if (m <= q1) {
+    Frag fb;
+    frag_right(fb, n, tb + ((size_t)nn * RP + (m - nn - 1)) * es, 0, c0);
+    TileAcc old;
+    const bool fin = m == nn + 1;
+    if (fin) tile_load(old, n, p.gr(T, T - m), r0, c0);
+    frag_mac(acc, fa, fb, n, z2{1.0, 0.0});
+    const TileAcc x = acc.value();
+    if (!fin) {
+      tile_store(x, n, Qm, r0, c0);
+      return;
+    }
+    double s2 = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (tile_i(r0, q) < n && tile_j(c0) < n) {
+        const z2 dd = zsub(x.get(q), old.get(q));
+        s2 += dd.x * dd.x + dd.y * dd.y;
+      }
+    const double dsum = wg_sum(s2, red);
+    tile_store(x, n, p.gr(T, T - m), r0, c0);
+    if (threadIdx.x == 0) d2[(size_t)m * nb + b] = dsum;
+    return;
+  }
+  FragT<BO_NMAX / 4, SE> fb;
+  frag_right(fb, n, p.sr(T - nn, T - m), 0, c0);
+  frag_mac(acc, fa, fb, n, z2{greg(w, m, nn), 0.0});
+  if (m - nn <= w.k + 1) {
+    TileAcc gt;
+    tile_load(gt, n, p.gr(T, T - nn), r0, c0);
+    const z2 cl = {0.0, -w.bd[m - nn] / (p.dt * p.dt)};
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc.add(q, zmul(cl, gt.get(q)));
+  }
+  tile_store(acc.value(), n, Qm, r0, c0);
+}
+
 // ---- a whole panel of the dependency chain in one launch
 // The links of one panel [q0, q1) as one launch instead of one launch each:
 // workgroup (m, b) owns band b of block m for one m in (q0, q1x] and adds the
@@ -1069,6 +1186,91 @@ kern_les_step_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab,
   const ChainArgT<SE> &a = args[blockIdx.z];
   if ((int)blockIdx.x >= a.nb) return;
   les_step_band(a.p, wtab, l, a.QA, a.Q, a.X, a.Minv, blockIdx.x, blockIdx.y);
+}
+
+// ---- lesser chain links of one product, as kern_ret_link: X_m =
+// Minv_m S_m, so the panel carries R_m = Minv_m S_m, with the transfer blocks
+//   B_{m,l} = Minv_m (dt greg(m,l) Sigma^R(m,l) + [m-l <= k+1] c_{m-l}),
+// c_l = -i bd_l / dt, multiplying X_l from the left (R_m += B_{m,l} X_l).
+// Block T is finished by the diagonal step from the plain sums, so it always
+// takes the plain contribution.  Transfer block of link l to m = l + d:
+// tb[l RP + d - 1].
+template <typename SE>
+__global__ void __launch_bounds__(256) kern_les_tblk(PtT<SE> p, const WTab *__restrict__ wtab, int RP,
+                                                     const z2 *Minv, z2 *tb) {
+  const WTab &w = *wtab;
+  __shared__ BlkLds s;
+  const int d = blockIdx.x + 1, l = w.k + 1 + blockIdx.y, m = l + d, T = p.T, n = p.n;
+  const int q0 = w.k + 1 + RP * ((l - w.k - 1) / RP), q1 = min(q0 + RP, T);
+  if (m > q1 || m > T - 1) return;
+  const z2 *Mi = Minv + (size_t)m * n * n;
+  BlkAcc3 acc;
+  acc.zero();
+  blk_mac(acc, s, n, Mi, 0, p.sr(m, l), 0, z2{p.dt * greg(w, m, l), 0.0});
+  if (d <= w.k + 1) blk_axpy(acc, n, Mi, 0, z2{0.0, -w.bd[d] / p.dt});
+  blk_store(acc.value(), n, tb + ((size_t)l * RP + d - 1) * n * n);
+}
+
+// R_m = Minv_m (Q_m + QA_m) for the blocks m of panel (q0, q1] (not T), in
+// place in QA; workgroup (b, m) owns columns 16b..16b+15, its waves the row
+// tiles
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_les_xform(Pt p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
+  const int b = blockIdx.x, m = q0 + 1 + blockIdx.y, n = p.n;
+  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
+  const size_t es = p.es();
+  z2 *QAm = QA + (size_t)m * es;
+  TileAcc3 x;
+  x.zero();
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    FragT<BO_NMAX / 8> fm, fa, fq;
+    frag_left(fm, n, Minv + (size_t)m * es, 0, r0, h * BO_NMAX / 8);
+    frag_right(fa, n, (const z2 *)QAm, 0, c0, h * BO_NMAX / 8);
+    frag_right(fq, n, Q + (size_t)m * es, 0, c0, h * BO_NMAX / 8);
+#pragma unroll
+    for (int s = 0; s < BO_NMAX / 8; s++) fa.v[s] = zadd(fa.v[s], fq.v[s]);
+    frag_mac(x, fm, fa, n, z2{1.0, 0.0}, h * BO_NMAX / 8);
+  }
+  __syncthreads();  // every wave has read the band before any writes its tile
+  tile_store(x.value(), n, QAm, r0, c0);
+}
+
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_les_link(PtT<SE> p, const WTab *__restrict__ wtab, int l, int q1, int RP, z2 *QA, z2 *X, const z2 *tb) {
+  __builtin_amdgcn_s_setprio(3);
+  const WTab &w = *wtab;
+  const int b = blockIdx.x, m = l + 1 + blockIdx.y, T = p.T, n = p.n;
+  const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
+  const size_t es = p.es();
+  z2 *Qm = QA + (size_t)m * es;
+  TileAcc3 acc;
+  {
+    TileAcc q;
+    tile_load(q, n, Qm, r0, c0);
+    acc.init(q);
+  }
+  Frag fb;
+  frag_right(fb, n, X + (size_t)l * es, 0, c0);
+  if (m <= q1 && m <= T - 1) {
+    Frag fa;
+    frag_left(fa, n, tb + ((size_t)l * RP + (m - l - 1)) * es, 0, r0);
+    frag_mac(acc, fa, fb, n, z2{1.0, 0.0});
+    tile_store(acc.value(), n, m == l + 1 ? X + (size_t)m * es : Qm, r0, c0);
+    return;
+  }
+  FragT<BO_NMAX / 4, SE> fa;
+  frag_left(fa, n, p.sr(m, l), 0, r0);
+  frag_mac(acc, fa, fb, n, z2{p.dt * greg(w, m, l), 0.0});
+  if (m - l <= w.k + 1 && m <= T - 1) {
+    TileAcc gt;
+    tile_load(gt, n, X + (size_t)l * es, r0, c0);
+    const z2 cl = {0.0, -w.bd[m - l] / p.dt};
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc.add(q, zmul(cl, gt.get(q)));
+  }
+  tile_store(acc.value(), n, Qm, r0, c0);
 }
 
 // the lesser panel in one launch, as kern_ret_chain (columns 16b..16b+15 of
@@ -1641,12 +1843,16 @@ void touch_time(Ctx &c, int t) {
   if (t >= 0 && t < (int)c.minv_valid.size()) c.minv_valid[t] = 0;
   if (t <= c.k) c.lesinv_valid = false;
   if (t < c.retinv_T) c.retinv_T = -1;
+  // the transfer blocks of step T read times before T only
+  for (int r = 0; r < 2; r++)
+    if (t < c.tb_T[r]) c.tb_T[r] = -1;
 }
 
 void invalidate_minv(Ctx &c) {
   std::fill(c.minv_valid.begin(), c.minv_valid.end(), 0);
   c.retinv_T = -1;
   c.lesinv_valid = false;
+  c.tb_T[0] = c.tb_T[1] = -1;
 }
 
 void set_matsubara(Ctx &c, double beta) {
@@ -1718,7 +1924,7 @@ inline int bands(const Ctx &c) { return (c.nao + 15) / 16; }  // 16-row / 16-col
 // and the epoch are allocated and zeroed with the step's other scratch
 // (chain_alloc); the epoch is bumped on the device for each recurrence
 void chain_alloc(Ctx &c) {
-  if (!c.chain_panel) return;
+  if (c.chain_mode != 2) return;
   const size_t nflag = 2 * (size_t)(c.nt + 1) * bands(c);
   if (c.dy_flags.bytes >= nflag * sizeof(unsigned)) return;
   c.dy_flags.ensure(nflag * sizeof(unsigned));
@@ -1730,7 +1936,7 @@ void chain_alloc(Ctx &c) {
 }
 
 bool chain_begin(Ctx &c) {
-  if (!c.chain_panel) return false;
+  if (c.chain_mode != 2) return false;
   hipLaunchKernelGGL(kern_chain_epoch, dim3(1), dim3(1), 0, c.stream, c.dy_ctr.as<unsigned>() + 2);
   check_launch("dyson chain epoch");
   return true;
@@ -1808,21 +2014,55 @@ void dyson_begin(Ctx &c, int T, double mu, double beta, double dt) {
   SE_LAUNCH(c, kern_ret_qinit, dim3(T - k), dim3(256), 0, s, p, wd, Q, QA, c.Minv.as<z2>(), err + 0);
 }
 
+// The transfer blocks of the one-product links (kern_ret_tblk /
+// kern_les_tblk; r = 0 retarded, 1 lesser): formed on the first call for a
+// time step, kept for its corrector passes (their inputs, Sigma^R rows and
+// inverses of the times before T, do not change between them; touch_time and
+// invalidate_minv drop them when they do)
+z2 *transfer_blocks(Ctx &c, int r, int T, double mu, double dt, const Pt &p, const WTab *wd) {
+  const int RP = panel_blocks(c), nlink = T - c.k - 1;
+  DevBuf &tb = r == 0 ? c.dy_tbr : c.dy_tbl;
+  if (nlink < 1) return tb.as<z2>();
+  if (c.tb_T[r] != T || c.tb_rp[r] != RP || c.tb_mu[r] != mu || c.tb_dt[r] != dt) {
+    tb.ensure((size_t)(c.nt + 1) * RP * c.es * sizeof(z2));
+    if (r == 0)
+      SE_LAUNCH(c, kern_ret_tblk, dim3(RP, nlink), dim3(256), 0, c.stream, p, wd, RP, c.Minv.as<z2>(),
+                tb.as<z2>());
+    else
+      SE_LAUNCH(c, kern_les_tblk, dim3(RP, nlink), dim3(256), 0, c.stream, p, wd, RP, c.Minv.as<z2>(),
+                tb.as<z2>());
+    check_launch("dyson transfer blocks");
+    c.tb_T[r] = T;
+    c.tb_rp[r] = RP;
+    c.tb_mu[r] = mu;
+    c.tb_dt[r] = dt;
+  }
+  return tb.as<z2>();
+}
+
 // ---- phase 2 (one system): the retarded recurrence
 void dyson_ret_one(Ctx &c, int T, double mu, double beta, double dt) {
   const WTab *wd = dev_wtab(c);
   Pt p = make_pt(c, T, mu, beta, dt);
-  const int nb = bands(c);
+  const int nb = bands(c), RP = panel_blocks(c);
   z2 *Q = c.dy_Q.as<z2>(), *QA = c.dy_QA.as<z2>();
   const z2 *Minv = c.Minv.as<z2>();
   double *d2 = c.dy_d2.as<double>();
   const bool persist = chain_begin(c);
-  recurrence(c.k, T, panel_blocks(c), c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
+  const z2 *tb = c.chain_mode == 1 ? transfer_blocks(c, 0, T, mu, dt, p, wd) : nullptr;
+  recurrence(c.k, T, RP, c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
              [&](int q0, int q1, int q1x) {
                if (persist) {
                  SE_LAUNCH(c, kern_ret_chain, dim3(nb * (q1x - q0)), dim3(256), 0, c.stream, p, wd, q0, q1, nb, QA,
                            Q, Minv, d2, c.dy_flags.as<unsigned>(), c.dy_ctr.as<unsigned>(),
                            c.dy_ctr.as<unsigned>() + 2, c.dy_err.as<double>() + 3);
+                 return;
+               }
+               if (tb) {
+                 hipLaunchKernelGGL(kern_ret_xform, dim3(nb, q1 - q0), dim3(256), 0, c.stream, p, q0, QA, Q, Minv);
+                 for (int nn = q0; nn < q1; nn++)
+                   SE_LAUNCH(c, kern_ret_link, dim3(nb, q1x - nn), dim3(256), 0, c.stream, p, wd, nn, q1, RP, QA, tb,
+                             d2);
                  return;
                }
                for (int nn = q0; nn < q1; nn++)
@@ -1876,16 +2116,26 @@ void dyson_mid(Ctx &c, int T, double mu, double beta, double dt) {
 void dyson_les_one(Ctx &c, int T, double mu, double beta, double dt) {
   const WTab *wd = dev_wtab(c);
   Pt p = make_pt(c, T, mu, beta, dt);
-  const int nb = bands(c);
+  const int nb = bands(c), RP = panel_blocks(c);
   z2 *Q = c.dy_Q.as<z2>(), *QA = c.dy_QA.as<z2>(), *X = c.dy_X.as<z2>();
   const z2 *Minv = c.Minv.as<z2>();
   const bool persist = chain_begin(c);
-  recurrence(c.k, T, panel_blocks(c), c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
+  const z2 *tb = c.chain_mode == 1 ? transfer_blocks(c, 1, T, mu, dt, p, wd) : nullptr;
+  recurrence(c.k, T, RP, c.panel_band_max, c.stream, c.side, c.ev_main, c.ev_side,
              [&](int q0, int q1, int q1x) {
                if (persist) {
                  SE_LAUNCH(c, kern_les_chain, dim3(nb * (q1x - q0)), dim3(256), 0, c.stream, p, wd, q0, q1, nb, QA,
                            Q, X, Minv, c.dy_flags.as<unsigned>() + (size_t)(c.nt + 1) * nb, c.dy_ctr.as<unsigned>(),
                            c.dy_ctr.as<unsigned>() + 2, c.dy_err.as<double>() + 3);
+                 return;
+               }
+               if (tb) {
+                 const int qf = std::min(q1, T - 1);  // block T is left to the diagonal step
+                 if (qf > q0)
+                   hipLaunchKernelGGL(kern_les_xform, dim3(nb, qf - q0), dim3(256), 0, c.stream, p, q0, QA, Q, Minv);
+                 for (int l = q0; l < q1; l++)
+                   SE_LAUNCH(c, kern_les_link, dim3(nb, q1x - l), dim3(256), 0, c.stream, p, wd, l, q1, RP, QA, X,
+                             tb);
                  return;
                }
                for (int l = q0; l < q1; l++)

@@ -89,9 +89,9 @@ def test_batch_reports_the_failing_system():
 def test_batch_threads_mode_matches_grouped(monkeypatch):
     """The two batch modes (grouped Dyson launches, and one host thread and
     stream per system) give the same G.  The grouped recurrences launch one
-    dependency-chain link at a time, so the contexts are made to do the same
-    (NED_CHAIN_PANEL=0) for a bitwise comparison."""
-    monkeypatch.setenv("NED_CHAIN_PANEL", "0")
+    dependency-chain link of two products at a time, so the contexts are made
+    to do the same (NED_CHAIN=0) for a bitwise comparison."""
+    monkeypatch.setenv("NED_CHAIN", "0")
     nt = 16
     ps = _systems(nt)
     outs = []
@@ -138,9 +138,10 @@ def test_batch_configuration_16_uneven_systems(orc, monkeypatch):
     grouped batch equals the systems stepped alone, bit for bit, and the
     nao = 8 system equals the oracle's FP64 run from the same equilibrium
     within the FP32 bound.  (The systems alone launch one dependency-chain
-    link at a time, as the grouped recurrences do: NED_CHAIN_PANEL=0.)"""
+    link of two products at a time, as the grouped recurrences do:
+    NED_CHAIN=0.)"""
     from nedyson_amd.green import GreenFunc
-    monkeypatch.setenv("NED_CHAIN_PANEL", "0")
+    monkeypatch.setenv("NED_CHAIN", "0")
     nt = 17
     ps = problem.batch_systems(b_sd=0.05, nt=nt)
     print(f"[batch16] nao {[p.nao for p in ps]}", flush=True)

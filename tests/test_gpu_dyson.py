@@ -334,12 +334,17 @@ def test_dyson_step_schedule_parity(orc, monkeypatch, rpanel, band_max):
 
 
 @pytest.mark.parametrize("nao,nt,rpanel", [(20, 150, 5), (64, 90, 16)])
-def test_dyson_step_chain_launch_modes(orc, monkeypatch, nao, nt, rpanel):
-    """The dependency chain of a panel as one launch (blocks handed between
-    its workgroups through device-scope flags) and as one launch per link:
-    three passes of the same Dyson step in each mode (the flags' epoch
-    advances with every recurrence, so a stale flag never passes a wait),
-    both against the oracle's dyson_step and within rounding of each other."""
+def test_dyson_step_chain_forms(orc, monkeypatch, nao, nt, rpanel):
+    """The three forms of the recurrences' dependency chains (NED_CHAIN): one
+    launch per link of one product, with the panel's history sums carried
+    times M^-1 and transfer blocks formed once per time step (1, the
+    default); one launch per link of two products (0); one launch per panel,
+    blocks handed between its workgroups through device-scope flags (2).
+    Three passes of the same Dyson step in each (the transfer blocks are
+    reused, the flags' epoch advances), each against the oracle's dyson_step
+    and within rounding of the others.  Then, in the default form, a new
+    self-energy history written over the old one at the same time step: the
+    transfer blocks are formed again (against the oracle on the new history)."""
     monkeypatch.setenv("NED_RPANEL", str(rpanel))
     ntau, k, beta, dt, mu = 15, 5, 10.0, 0.01, 0.0
     rng = np.random.default_rng(100 + nao)
@@ -348,14 +353,17 @@ def test_dyson_step_chain_launch_modes(orc, monkeypatch, nao, nt, rpanel):
     x, _ = orc.gll(ntau)
     G = free_green(h, mu, beta, dt, nt, x)
     S = random_green(rng, nt, ntau, nao, scale=0.02)
+    S2 = random_green(rng, nt, ntau, nao, scale=0.02)
     hmf = np.repeat(h[None].astype(complex), nt + 1, axis=0)
     G_ref = G.copy()
     D = orc.Dyson(nt, ntau, nao, k)
     D.dyson_step(nt, G_ref, S, hmf, mu, beta, dt)
+    G_ref2 = G.copy()
+    D.dyson_step(nt, G_ref2, S2, hmf, mu, beta, dt)
     rows = []
-    for mode in ("1", "0"):
-        monkeypatch.setenv("NED_CHAIN_PANEL", mode)
-        print(f"[chain] nao={nao} T={nt} rpanel={rpanel} panel launch={mode}", flush=True)
+    for mode in ("1", "0", "2"):
+        monkeypatch.setenv("NED_CHAIN", mode)
+        print(f"[chain] nao={nao} T={nt} rpanel={rpanel} form {mode}", flush=True)
         with ned.Context(nao, 1, nt, ntau, k) as c:
             c.upload(ned.NED_G, G)
             c.upload(ned.NED_SIGMA, S)
@@ -363,13 +371,21 @@ def test_dyson_step_chain_launch_modes(orc, monkeypatch, nao, nt, rpanel):
             c.set_matsubara(beta)
             errs = [c.dyson_step(nt, mu, beta, dt) for _ in range(3)]
             d = c.download(ned.NED_G)
-        assert all(np.all(np.isfinite(e)) for e in errs)
-        assert _relerr(d.ret_row(nt), G_ref.ret_row(nt)) < 1e-10
-        assert _relerr(d.les_col(nt), G_ref.les_col(nt)) < 1e-10
-        assert _relerr(d.tv[nt], G_ref.tv[nt]) < 1e-10
-        rows.append((d.ret_row(nt), d.les_col(nt)))
-    for a, b in zip(rows[0], rows[1]):
-        assert _relerr(a, b) < 1e-12
+            assert all(np.all(np.isfinite(e)) for e in errs)
+            assert _relerr(d.ret_row(nt), G_ref.ret_row(nt)) < 1e-10
+            assert _relerr(d.les_col(nt), G_ref.les_col(nt)) < 1e-10
+            assert _relerr(d.tv[nt], G_ref.tv[nt]) < 1e-10
+            rows.append((d.ret_row(nt), d.les_col(nt)))
+            if mode == "1":
+                c.upload(ned.NED_SIGMA, S2)
+                c.set_matsubara(beta)  # the upload replaced Sigma^M as well
+                c.dyson_step(nt, mu, beta, dt)
+                d2 = c.download(ned.NED_G)
+                assert _relerr(d2.ret_row(nt), G_ref2.ret_row(nt)) < 1e-10
+                assert _relerr(d2.les_col(nt), G_ref2.les_col(nt)) < 1e-10
+    for other in rows[1:]:
+        for a, b in zip(rows[0], other):
+            assert _relerr(a, b) < 1e-12
 
 
 def test_c0_shape_full_run_parity(orc):
