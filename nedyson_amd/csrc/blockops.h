@@ -95,14 +95,18 @@ struct BlkAcc3 {
 };
 
 // Both operands' K-slices are held with K innermost (X[row][k], Y[col][k])
-// in rows of 17: an MFMA fragment read (16 rows or columns, one k each per
-// lane group) then reads 16 rows at the same column, which the LDS serves
-// without conflicts; a k-major Y (16 consecutive columns of one k per lane
-// group) measured four times slower.
-struct BlkLds {
-  z2 X[BO_NMAX][BO_KT + 1];
-  z2 Y[BO_NMAX][BO_KT + 1];
+// in rows of KT + 1: an MFMA fragment read (16 rows or columns, one k each
+// per lane group) then reads 16 rows at the same column, which the LDS
+// serves without conflicts; a k-major Y (16 consecutive columns of one k per
+// lane group) measured four times slower.  KT = 16, or 32 (twice the LDS, half
+// the barriers) for kernels that run at two workgroups per CU.
+template <int KT>
+struct BlkLdsT {
+  static constexpr int kt = KT;
+  z2 X[BO_NMAX][KT + 1];
+  z2 Y[BO_NMAX][KT + 1];
 };
+using BlkLds = BlkLdsT<BO_KT>;
 
 // op: 0 = as stored, 1 = adjoint (conjugate transpose), 2 = transpose;
 // ld = row stride of the stored matrix (complex FP64 or FP32 elements)
@@ -136,14 +140,16 @@ __device__ __forceinline__ float2 zero_of(const float2 *) { return float2{0.0f, 
 // that every workgroup shares and that therefore sits in L2, this saves the
 // prefetch registers (and buys occupancy) at little cost.
 // Acc: BlkAcc (four real products per complex one) or BlkAcc3 (three).
-template <int LATE = 0, typename Acc, typename EX, typename EY>
-__device__ __forceinline__ void blk_mac(Acc &acc, BlkLds &s, int n, const EX *X, int opX, const EY *Y, int opY,
+// Lds: BlkLdsT<KT>, K slices of KT.
+template <int LATE = 0, typename Acc, typename Lds, typename EX, typename EY>
+__device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n, const EX *X, int opX, const EY *Y, int opY,
                                         z2 coef, int ldX = 0, int ldY = 0) {
+  constexpr int KT = Lds::kt;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (ldX == 0) ldX = n;
   if (ldY == 0) ldY = n;
   const bool wave_on = 16 * w < n;
-  constexpr int PER = BO_NMAX * BO_KT / BO_T;  // staged elements per thread and operand
+  constexpr int PER = BO_NMAX * KT / BO_T;  // staged elements per thread and operand
   constexpr int PX = LATE == 1 ? 1 : PER, PY = LATE == 2 ? 1 : PER;
   EX px[PX];  // the next K slice, in the operands' storage types
   EY py[PY];
@@ -152,10 +158,10 @@ __device__ __forceinline__ void blk_mac(Acc &acc, BlkLds &s, int n, const EX *X,
   // (along k when it is used as stored, along the other index when it is
   // used transposed or adjoint)
   auto xmap = [&](int e, int &r, int &kk) __attribute__((always_inline)) {
-    if (opX == 0) { r = e / BO_KT; kk = e % BO_KT; } else { kk = e / BO_NMAX; r = e % BO_NMAX; }
+    if (opX == 0) { r = e / KT; kk = e % KT; } else { kk = e / BO_NMAX; r = e % BO_NMAX; }
   };
   auto ymap = [&](int e, int &kk, int &c) __attribute__((always_inline)) {
-    if (opY == 0) { kk = e / BO_NMAX; c = e % BO_NMAX; } else { c = e / BO_KT; kk = e % BO_KT; }
+    if (opY == 0) { kk = e / BO_NMAX; c = e % BO_NMAX; } else { c = e / KT; kk = e % KT; }
   };
   auto getx = [&](int e, int k0) __attribute__((always_inline)) {
     int r, kk;
@@ -177,7 +183,7 @@ __device__ __forceinline__ void blk_mac(Acc &acc, BlkLds &s, int n, const EX *X,
   };
   __syncthreads();  // operands written by this workgroup just before the call are visible
   fetch(0);
-  for (int k0 = 0; k0 < n; k0 += BO_KT) {
+  for (int k0 = 0; k0 < n; k0 += KT) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < PER; i++) {
@@ -189,9 +195,9 @@ __device__ __forceinline__ void blk_mac(Acc &acc, BlkLds &s, int n, const EX *X,
       s.Y[c][kk2] = blk_op(widen(LATE == 2 ? gety(e, k0) : py[LATE == 2 ? 0 : i]), opY);
     }
     __syncthreads();
-    if (k0 + BO_KT < n) fetch(k0 + BO_KT);
+    if (k0 + KT < n) fetch(k0 + KT);
     if (wave_on) {
-      const int ksteps = min(BO_KT, n - k0);
+      const int ksteps = min(KT, n - k0);
       for (int ks = 0; ks < ksteps; ks += 4) {
         const int kk = ks + (lane >> 4);
         const z2 a = s.X[16 * w + (lane & 15)][kk];

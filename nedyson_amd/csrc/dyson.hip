@@ -828,7 +828,7 @@ template <typename SE>
 __device__ __forceinline__ void ret_panel(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
                                           int bx) {
   const WTab &w = *wtab;
-  __shared__ BlkLds s;
+  __shared__ BlkLdsT<32> s;  // 32-deep K slices: two workgroups a CU
   const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * n * n;
@@ -1340,7 +1340,7 @@ template <typename SE>
 __device__ __forceinline__ void les_panel(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
                                           const z2 *X, int bx) {
   const WTab &w = *wtab;
-  __shared__ BlkLds s;
+  __shared__ BlkLdsT<32> s;  // 32-deep K slices: two workgroups a CU
   const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * p.es();
