@@ -820,15 +820,17 @@ __global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *
 }
 
 // contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
-// the history sums stay in registers across the whole panel.  Both operands'
-// next K slices are fetched ahead (two workgroups a CU): with the shared G^R
-// block read only when staged and three workgroups a CU, the panels took 10%
-// longer.
-template <typename SE>
+// the history sums stay in registers across the whole panel.  One system
+// (WIDE): both operands' next K slices fetched ahead and 32-deep K slices, two
+// workgroups a CU -- with the shared G^R block read only when staged and three
+// workgroups a CU, the panels took 10% longer at nao = 64.  A group of
+// systems, mostly of small nao, keeps 16-deep slices (a 32-deep one stages
+// twice the zero padding below nao = 32) and the late read, three a CU.
+template <bool WIDE, typename SE>
 __device__ __forceinline__ void ret_panel(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
                                           int bx) {
   const WTab &w = *wtab;
-  __shared__ BlkLdsT<32> s;  // 32-deep K slices: two workgroups a CU
+  __shared__ BlkLdsT<WIDE ? 32 : 16> s;
   const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * n * n;
@@ -839,7 +841,7 @@ __device__ __forceinline__ void ret_panel(const PtT<SE> &p, const WTab *__restri
     acc.init(q);
   }
   for (int nn = q0; nn < q1; nn++)
-    blk_mac<0>(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
+    blk_mac<WIDE ? 0 : 1>(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
   __syncthreads();
   blk_store(acc.value(), n, Qm);
 }
@@ -847,14 +849,14 @@ __device__ __forceinline__ void ret_panel(const PtT<SE> &p, const WTab *__restri
 template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 kern_ret_panel(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
-  ret_panel(p, wtab, q0, q1, m0, Q, blockIdx.x);
+  ret_panel<true>(p, wtab, q0, q1, m0, Q, blockIdx.x);
 }
 
 template <typename SE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kern_ret_panel_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
   const ChainArgT<SE> &a = args[blockIdx.y];
-  ret_panel(a.p, wtab, q0, q1, m0, a.Q, blockIdx.x);
+  ret_panel<false>(a.p, wtab, q0, q1, m0, a.Q, blockIdx.x);
 }
 
 // kern_ret_panel split by rows, for the late panels whose (T - q1) blocks
@@ -1336,11 +1338,11 @@ kern_les_chain(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int nb,
 
 // contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
 // (both operands fetched ahead, as in ret_panel)
-template <typename SE>
+template <bool WIDE, typename SE>
 __device__ __forceinline__ void les_panel(const PtT<SE> &p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q,
                                           const z2 *X, int bx) {
   const WTab &w = *wtab;
-  __shared__ BlkLdsT<32> s;  // 32-deep K slices: two workgroups a CU
+  __shared__ BlkLdsT<WIDE ? 32 : 16> s;
   const int m = m0 + bx, T = p.T, n = p.n;
   if (m > T) return;
   z2 *Qm = Q + (size_t)m * p.es();
@@ -1351,7 +1353,7 @@ __device__ __forceinline__ void les_panel(const PtT<SE> &p, const WTab *__restri
     acc.init(q);
   }
   for (int l = q0; l < q1; l++)
-    blk_mac<0>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+    blk_mac<WIDE ? 0 : 2>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
   __syncthreads();
   blk_store(acc.value(), n, Qm);
 }
@@ -1359,14 +1361,14 @@ __device__ __forceinline__ void les_panel(const PtT<SE> &p, const WTab *__restri
 template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 kern_les_panel(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
-  les_panel(p, wtab, q0, q1, m0, Q, X, blockIdx.x);
+  les_panel<true>(p, wtab, q0, q1, m0, Q, X, blockIdx.x);
 }
 
 template <typename SE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kern_les_panel_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
   const ChainArgT<SE> &a = args[blockIdx.y];
-  les_panel(a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x);
+  les_panel<false>(a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x);
 }
 
 // kern_les_panel split by columns (workgroup (m, b) owns columns
