@@ -136,21 +136,26 @@ __device__ __forceinline__ float2 zero_of(const float2 *) { return float2{0.0f, 
 // The next K-slice is loaded into registers while the current one is
 // multiplied out of LDS.
 // LATE: 0 prefetches both operands' next K slice into registers during the
-// current slice; 1 (2) reads X (Y) only when it is staged -- for an operand
-// that every workgroup shares and that therefore sits in L2, this saves the
-// prefetch registers (and buys occupancy) at little cost.
+// current slice; bit 1 (bit 2) reads X (Y) only when it is staged -- for an
+// operand that every workgroup shares and that therefore sits in L2, this
+// saves the prefetch registers (and buys occupancy) at little cost.
 // Acc: BlkAcc (four real products per complex one) or BlkAcc3 (three).
 // Lds: BlkLdsT<KT>, K slices of KT.
-template <int LATE = 0, typename Acc, typename Lds, typename EX, typename EY>
-__device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n, const EX *X, int opX, const EY *Y, int opY,
+// NF: the block order n when it is known at compile time (0: read from n_).
+// A known order folds away the edge tests and the per-column branches, so
+// that a k-step's five fragment reads are issued together ahead of its
+// twelve products; with them, each column's read is awaited in turn.
+template <int LATE = 0, int NF = 0, typename Acc, typename Lds, typename EX, typename EY>
+__device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n_, const EX *X, int opX, const EY *Y, int opY,
                                         z2 coef, int ldX = 0, int ldY = 0) {
   constexpr int KT = Lds::kt;
+  const int n = NF ? NF : n_;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (ldX == 0) ldX = n;
   if (ldY == 0) ldY = n;
   const bool wave_on = 16 * w < n;
   constexpr int PER = BO_NMAX * KT / BO_T;  // staged elements per thread and operand
-  constexpr int PX = LATE == 1 ? 1 : PER, PY = LATE == 2 ? 1 : PER;
+  constexpr int PX = (LATE & 1) ? 1 : PER, PY = (LATE & 2) ? 1 : PER;
   EX px[PX];  // the next K slice, in the operands' storage types
   EY py[PY];
   // element e of a slice -> (row, k) of X and (k, col) of Y, chosen so that
@@ -177,8 +182,8 @@ __device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n, const EX *X, in
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int e = tid + BO_T * i;
-      if constexpr (LATE != 1) px[i] = getx(e, k0);
-      if constexpr (LATE != 2) py[i] = gety(e, k0);
+      if constexpr (!(LATE & 1)) px[i] = getx(e, k0);
+      if constexpr (!(LATE & 2)) py[i] = gety(e, k0);
     }
   };
   __syncthreads();  // operands written by this workgroup just before the call are visible
@@ -191,8 +196,8 @@ __device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n, const EX *X, in
       int r, kk, kk2, c;
       xmap(e, r, kk);
       ymap(e, kk2, c);
-      s.X[r][kk] = zmul(coef, blk_op(widen(LATE == 1 ? getx(e, k0) : px[LATE == 1 ? 0 : i]), opX));
-      s.Y[c][kk2] = blk_op(widen(LATE == 2 ? gety(e, k0) : py[LATE == 2 ? 0 : i]), opY);
+      s.X[r][kk] = zmul(coef, blk_op(widen((LATE & 1) ? getx(e, k0) : px[(LATE & 1) ? 0 : i]), opX));
+      s.Y[c][kk2] = blk_op(widen((LATE & 2) ? gety(e, k0) : py[(LATE & 2) ? 0 : i]), opY);
     }
     __syncthreads();
     if (k0 + KT < n) fetch(k0 + KT);
