@@ -98,8 +98,8 @@ struct BlkAcc3 {
 // in rows of KT + 1: an MFMA fragment read (16 rows or columns, one k each
 // per lane group) then reads 16 rows at the same column, which the LDS
 // serves without conflicts; a k-major Y (16 consecutive columns of one k per
-// lane group) measured four times slower.  KT = 16, or 32 (twice the LDS, half
-// the barriers) for kernels that run at two workgroups per CU.
+// lane group) measured four times slower.  KT = 16 for blk_mac (one stage);
+// blk_mac2 takes two stages of 8 (BlkLds2<8>).
 template <int KT>
 struct BlkLdsT {
   static constexpr int kt = KT;
@@ -215,6 +215,84 @@ __device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n_, const EX *X, i
         }
       }
     }
+  }
+}
+
+// Two LDS stages of KT-deep slices for blk_mac2.
+template <int KT>
+struct BlkLds2 {
+  static constexpr int kt = KT;
+  z2 X[2][BO_NMAX][KT + 1];
+  z2 Y[2][BO_NMAX][KT + 1];
+};
+
+// blk_mac in two LDS stages: the next slice of both operands is fetched into
+// registers while the current one is multiplied out of its stage, then
+// written into the idle stage; one barrier per slice instead of two, and a
+// whole slice's products between a load and its use.
+template <int NF = 0, typename Acc, typename Lds, typename EX, typename EY>
+__device__ __forceinline__ void blk_mac2(Acc &acc, Lds &s, int n_, const EX *X, int opX, const EY *Y, int opY,
+                                         z2 coef) {
+  constexpr int KT = Lds::kt;
+  const int n = NF ? NF : n_;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const bool wave_on = 16 * w < n;
+  constexpr int PER = BO_NMAX * KT / BO_T;
+  EX px[PER];
+  EY py[PER];
+  auto xmap = [&](int e, int &r, int &kk) __attribute__((always_inline)) {
+    if (opX == 0) { r = e / KT; kk = e % KT; } else { kk = e / BO_NMAX; r = e % BO_NMAX; }
+  };
+  auto ymap = [&](int e, int &kk, int &c) __attribute__((always_inline)) {
+    if (opY == 0) { kk = e / BO_NMAX; c = e % BO_NMAX; } else { c = e / KT; kk = e % KT; }
+  };
+  auto fetch = [&](int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int e = tid + BO_T * i;
+      int r, kk, kk2, c;
+      xmap(e, r, kk);
+      ymap(e, kk2, c);
+      px[i] = (r < n && k0 + kk < n) ? blk_raw(X, n, opX, r, k0 + kk) : zero_of(X);
+      py[i] = (c < n && k0 + kk2 < n) ? blk_raw(Y, n, opY, k0 + kk2, c) : zero_of(Y);
+    }
+  };
+  auto stage = [&](int st) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int e = tid + BO_T * i;
+      int r, kk, kk2, c;
+      xmap(e, r, kk);
+      ymap(e, kk2, c);
+      s.X[st][r][kk] = zmul(coef, blk_op(widen(px[i]), opX));
+      s.Y[st][c][kk2] = blk_op(widen(py[i]), opY);
+    }
+  };
+  __syncthreads();  // operands written just before the call are visible; the last call's stages are free
+  fetch(0);
+  stage(0);
+  __syncthreads();
+  int st = 0;
+  for (int k0 = 0; k0 < n; k0 += KT) {
+    const bool more = k0 + KT < n;
+    if (more) fetch(k0 + KT);
+    if (wave_on) {
+      const int ksteps = min(KT, n - k0);
+      for (int ks = 0; ks < ksteps; ks += 4) {
+        const int kk = ks + (lane >> 4);
+        const z2 a = s.X[st][16 * w + (lane & 15)][kk];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          if (16 * c < n) {
+            const z2 b = s.Y[st][16 * c + (lane & 15)][kk];
+            acc.mma(a, b, c);
+          }
+        }
+      }
+    }
+    if (more) stage(st ^ 1);
+    __syncthreads();
+    st ^= 1;
   }
 }
 

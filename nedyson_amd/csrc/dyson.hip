@@ -820,14 +820,14 @@ __global__ void kern_band_err(const double *d2, int m0, int m1, int nb, double *
 }
 
 // contributions of the finished blocks nn in [q0, q1) to every m in (q1, T]:
-// the history sums stay in registers across the whole panel.  One system
-// (WIDE): both operands' next K slices fetched ahead and 32-deep K slices, two
-// workgroups a CU -- with the shared G^R block read only when staged and three
-// workgroups a CU, the panels took 10% longer at nao = 64.  A group of
-// systems, mostly of small nao, keeps 16-deep slices (a 32-deep one stages
-// twice the zero padding below nao = 32) and the late read, three a CU.
-template <bool WIDE, int NF, typename SE>
-__device__ __forceinline__ void ret_panel(BlkLdsT<WIDE ? 32 : 16> &s, const PtT<SE> &p, const WTab *__restrict__ wtab,
+// the history sums stay in registers across the whole panel.  8-deep K
+// slices in two LDS stages (blk_mac2), both operands fetched a slice ahead,
+// three workgroups a CU -- at nao = 64 the Dyson step took 2.6 ms longer with
+// 32-deep slices in one stage at two a CU, 1.1 ms longer with 16-deep ones in
+// two stages; a group of systems of small nao ran 1% faster than with 16-deep
+// slices in one stage and the shared G^R block read only when staged.
+template <int NF, typename SE>
+__device__ __forceinline__ void ret_panel(BlkLds2<8> &s, const PtT<SE> &p, const WTab *__restrict__ wtab,
                                           int q0, int q1, int m0, z2 *Q, int bx) {
   const WTab &w = *wtab;
   const int m = m0 + bx, T = p.T, n = p.n;
@@ -840,26 +840,26 @@ __device__ __forceinline__ void ret_panel(BlkLdsT<WIDE ? 32 : 16> &s, const PtT<
     acc.init(q);
   }
   for (int nn = q0; nn < q1; nn++)
-    blk_mac<WIDE ? 0 : 1, NF>(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
+    blk_mac2<NF>(acc, s, n, p.gr(T, T - nn), 0, p.sr(T - nn, T - m), 0, z2{greg(w, m, nn), 0.0});
   __syncthreads();
   blk_store(acc.value(), n, Qm);
 }
 
 template <typename SE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kern_ret_panel(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q) {
-  __shared__ BlkLdsT<32> s;
-  if (p.n == BO_NMAX) ret_panel<true, BO_NMAX>(s, p, wtab, q0, q1, m0, Q, blockIdx.x);
-  else ret_panel<true, 0>(s, p, wtab, q0, q1, m0, Q, blockIdx.x);
+  __shared__ BlkLds2<8> s;
+  if (p.n == BO_NMAX) ret_panel<BO_NMAX>(s, p, wtab, q0, q1, m0, Q, blockIdx.x);
+  else ret_panel<0>(s, p, wtab, q0, q1, m0, Q, blockIdx.x);
 }
 
 template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kern_ret_panel_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
   const ChainArgT<SE> &a = args[blockIdx.y];
-  __shared__ BlkLdsT<16> s;
-  if (a.p.n == BO_NMAX) ret_panel<false, BO_NMAX>(s, a.p, wtab, q0, q1, m0, a.Q, blockIdx.x);
-  else ret_panel<false, 0>(s, a.p, wtab, q0, q1, m0, a.Q, blockIdx.x);
+  __shared__ BlkLds2<8> s;
+  if (a.p.n == BO_NMAX) ret_panel<BO_NMAX>(s, a.p, wtab, q0, q1, m0, a.Q, blockIdx.x);
+  else ret_panel<0>(s, a.p, wtab, q0, q1, m0, a.Q, blockIdx.x);
 }
 
 // kern_ret_panel split by rows, for the late panels whose (T - q1) blocks
@@ -911,7 +911,7 @@ kern_ret_panel_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab
 // (src/dyson_integrals.h:80-116, 254-324): partial sums over chunks of the
 // time history, one workgroup per (node m, chunk s).
 template <int NF, typename SE>
-__device__ __forceinline__ void tv_rhs(BlkLds &s, const PtT<SE> &p, const WTab *__restrict__ wtab, const z2 *mixK,
+__device__ __forceinline__ void tv_rhs(BlkLds2<8> &s, const PtT<SE> &p, const WTab *__restrict__ wtab, const z2 *mixK,
                                        int nchunk, z2 *part) {
   const WTab &w = *wtab;
   const int m = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
@@ -923,9 +923,9 @@ __device__ __forceinline__ void tv_rhs(BlkLds &s, const PtT<SE> &p, const WTab *
   BlkAcc3 a3;
   a3.zero();
   for (int j = t0; j < min(t1, T); j++)
-    blk_mac<1, NF>(a3, s, n, p.sr(T, j), 0, p.gtv(j, m), 0, z2{p.dt * greg(w, T, j), 0.0});
+    blk_mac2<NF>(a3, s, n, p.sr(T, j), 0, p.gtv(j, m), 0, z2{p.dt * greg(w, T, j), 0.0});
   for (int jp = max(t0, T) - T; jp < t1 - T; jp++)
-    blk_mac<1, NF>(a3, s, n, p.stv(T, jp), 0, mixK + ((size_t)m * np + jp) * p.es(), 0, z2{1.0, 0.0});
+    blk_mac2<NF>(a3, s, n, p.stv(T, jp), 0, mixK + ((size_t)m * np + jp) * p.es(), 0, z2{1.0, 0.0});
   BlkAcc acc = a3.value();
   if (ch == 0)
     for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, p.gtv(T - l, m), 0, z2{0.0, -w.bd[l] / p.dt});
@@ -935,7 +935,7 @@ __device__ __forceinline__ void tv_rhs(BlkLds &s, const PtT<SE> &p, const WTab *
 template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kern_tv_rhs(PtT<SE> p, const WTab *__restrict__ wtab, const z2 *mixK, int nchunk, z2 *part) {
-  __shared__ BlkLds s;
+  __shared__ BlkLds2<8> s;
   if (p.n == BO_NMAX) tv_rhs<BO_NMAX>(s, p, wtab, mixK, nchunk, part);
   else tv_rhs<0>(s, p, wtab, mixK, nchunk, part);
 }
@@ -997,9 +997,12 @@ __global__ void kern_les_Y(Pt p, const double *mass, z2 *Y) {
 }
 
 // Q_row = Cles3 + Cles2 (src/dyson_integrals.h:336-380, 393-427), partial
-// sums over chunks of the (tau, j) terms.
+// sums over chunks of the (tau, j) terms.  8-deep K slices in two LDS stages
+// (blk_mac2) keep both operands fetched a slice ahead at three workgroups a
+// CU: 3% faster at nao = 64 than 16-deep slices in one stage with the shared
+// G^R block read only when staged.
 template <int NF, typename SE>
-__device__ __forceinline__ void les_rhs(BlkLds &s, const PtT<SE> &p, const WTab *__restrict__ wtab, const z2 *Y,
+__device__ __forceinline__ void les_rhs(BlkLds2<8> &s, const PtT<SE> &p, const WTab *__restrict__ wtab, const z2 *Y,
                                         int nchunk, z2 *part) {
   const WTab &w = *wtab;
   const int row = blockIdx.x, ch = blockIdx.y, n = p.n, T = p.T, np = p.np;
@@ -1013,14 +1016,14 @@ __device__ __forceinline__ void les_rhs(BlkLds &s, const PtT<SE> &p, const WTab 
   // tv terms t < np; then j = t - np: -Sigma^<(j,row)^H X_j for j < row and
   // Sigma^<(row,j) X_j for j >= row, with X_j = dt w(T,j) G^R(T,j)^H
   for (int t = t0; t < min(t1, np); t++)
-    blk_mac<2, NF>(acc, s, n, p.stv(row, t), 0, Y + (size_t)t * p.es(), 0, z2{1.0, 0.0});
+    blk_mac2<NF>(acc, s, n, p.stv(row, t), 0, Y + (size_t)t * p.es(), 0, z2{1.0, 0.0});
   for (int t = max(t0, np); t < min(t1, np + row); t++) {
     const int j = t - np;
-    blk_mac<2, NF>(acc, s, n, p.sl(j, row), 1, p.gr(T, j), 1, z2{-p.dt * greg(w, T, j), 0.0});
+    blk_mac2<NF>(acc, s, n, p.sl(j, row), 1, p.gr(T, j), 1, z2{-p.dt * greg(w, T, j), 0.0});
   }
   for (int t = max(t0, np + row); t < t1; t++) {
     const int j = t - np;
-    blk_mac<2, NF>(acc, s, n, p.sl(row, j), 0, p.gr(T, j), 1, z2{p.dt * greg(w, T, j), 0.0});
+    blk_mac2<NF>(acc, s, n, p.sl(row, j), 0, p.gr(T, j), 1, z2{p.dt * greg(w, T, j), 0.0});
   }
   blk_store(acc.value(), n, part + ((size_t)ch * (T + 1) + row) * p.es());
 }
@@ -1028,7 +1031,7 @@ __device__ __forceinline__ void les_rhs(BlkLds &s, const PtT<SE> &p, const WTab 
 template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kern_les_rhs(PtT<SE> p, const WTab *__restrict__ wtab, const z2 *Y, int nchunk, z2 *part) {
-  __shared__ BlkLds s;
+  __shared__ BlkLds2<8> s;
   if (p.n == BO_NMAX) les_rhs<BO_NMAX>(s, p, wtab, Y, nchunk, part);
   else les_rhs<0>(s, p, wtab, Y, nchunk, part);
 }
@@ -1356,9 +1359,9 @@ kern_les_chain(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int nb,
 }
 
 // contributions of the finished X_l, l in [q0, q1), to every m in (q1, T]
-// (both operands fetched ahead, as in ret_panel)
-template <bool WIDE, int NF, typename SE>
-__device__ __forceinline__ void les_panel(BlkLdsT<WIDE ? 32 : 16> &s, const PtT<SE> &p, const WTab *__restrict__ wtab,
+// (LDS stages and slices chosen as in ret_panel)
+template <int NF, typename SE>
+__device__ __forceinline__ void les_panel(BlkLds2<8> &s, const PtT<SE> &p, const WTab *__restrict__ wtab,
                                           int q0, int q1, int m0, z2 *Q, const z2 *X, int bx) {
   const WTab &w = *wtab;
   const int m = m0 + bx, T = p.T, n = p.n;
@@ -1371,26 +1374,26 @@ __device__ __forceinline__ void les_panel(BlkLdsT<WIDE ? 32 : 16> &s, const PtT<
     acc.init(q);
   }
   for (int l = q0; l < q1; l++)
-    blk_mac<WIDE ? 0 : 2, NF>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+    blk_mac2<NF>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
   __syncthreads();
   blk_store(acc.value(), n, Qm);
 }
 
 template <typename SE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kern_les_panel(PtT<SE> p, const WTab *__restrict__ wtab, int q0, int q1, int m0, z2 *Q, const z2 *X) {
-  __shared__ BlkLdsT<32> s;
-  if (p.n == BO_NMAX) les_panel<true, BO_NMAX>(s, p, wtab, q0, q1, m0, Q, X, blockIdx.x);
-  else les_panel<true, 0>(s, p, wtab, q0, q1, m0, Q, X, blockIdx.x);
+  __shared__ BlkLds2<8> s;
+  if (p.n == BO_NMAX) les_panel<BO_NMAX>(s, p, wtab, q0, q1, m0, Q, X, blockIdx.x);
+  else les_panel<0>(s, p, wtab, q0, q1, m0, Q, X, blockIdx.x);
 }
 
 template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kern_les_panel_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab, int q0, int q1, int m0) {
   const ChainArgT<SE> &a = args[blockIdx.y];
-  __shared__ BlkLdsT<16> s;
-  if (a.p.n == BO_NMAX) les_panel<false, BO_NMAX>(s, a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x);
-  else les_panel<false, 0>(s, a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x);
+  __shared__ BlkLds2<8> s;
+  if (a.p.n == BO_NMAX) les_panel<BO_NMAX>(s, a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x);
+  else les_panel<0>(s, a.p, wtab, q0, q1, m0, a.Q, a.X, blockIdx.x);
 }
 
 // kern_les_panel split by columns (workgroup (m, b) owns columns
