@@ -583,11 +583,11 @@ __global__ void __launch_bounds__(256) kern_ret_tblk(PtT<SE> p, const WTab *__re
 // R_m = dt (Q_m + QA_m) Minv_{T-m} for the blocks m of panel (q0, q1], in
 // place in QA; workgroup (b, m) owns rows 16b..16b+15, its waves the column
 // tiles
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_ret_xform(Pt p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
-  const int b = blockIdx.x, m = q0 + 1 + blockIdx.y, T = p.T, n = p.n;
+template <int NF>
+__device__ __forceinline__ void ret_xform(const Pt &p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
+  const int b = blockIdx.x, m = q0 + 1 + blockIdx.y, T = p.T, n = NF ? NF : p.n;
   const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
-  const size_t es = p.es();
+  const size_t es = (size_t)n * n;
   z2 *QAm = QA + (size_t)m * es;
   TileAcc3 x;
   x.zero();
@@ -606,18 +606,22 @@ kern_ret_xform(Pt p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
   tile_store(x.value(), n, QAm, r0, c0);
 }
 
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_ret_xform(Pt p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
+  if (p.n == BO_NMAX) ret_xform<BO_NMAX>(p, q0, QA, Q, Minv);
+  else ret_xform<0>(p, q0, QA, Q, Minv);
+}
+
 // link nn: blocks m of its panel (m <= q1) get R_m += X_nn B_{nn,m} and the
 // finishing one (m = nn + 1) is X_m = R_m + X_nn B_{nn,m}; blocks of the next
 // panel get the plain contribution of kern_ret_step_band
-template <typename SE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_ret_link(PtT<SE> p, const WTab *__restrict__ wtab, int nn, int q1, int RP, z2 *QA, const z2 *tb, double *d2) {
-  __builtin_amdgcn_s_setprio(3);
+template <int NF, typename SE>
+__device__ __forceinline__ void ret_link(const PtT<SE> &p, const WTab *__restrict__ wtab, int nn, int q1, int RP,
+                                         z2 *QA, const z2 *tb, double *d2, double *red) {
   const WTab &w = *wtab;
-  __shared__ double red[4];
-  const int b = blockIdx.x, nb = gridDim.x, m = nn + 1 + blockIdx.y, T = p.T, n = p.n;
+  const int b = blockIdx.x, nb = gridDim.x, m = nn + 1 + blockIdx.y, T = p.T, n = NF ? NF : p.n;
   const int r0 = 16 * b, c0 = 16 * (threadIdx.x >> 6);
-  const size_t es = p.es();
+  const size_t es = (size_t)n * n;
   z2 *Qm = QA + (size_t)m * es;
   TileAcc3 acc;
   {
@@ -662,6 +666,15 @@ kern_ret_link(PtT<SE> p, const WTab *__restrict__ wtab, int nn, int q1, int RP, 
     for (int q = 0; q < 4; q++) acc.add(q, zmul(cl, gt.get(q)));
   }
   tile_store(acc.value(), n, Qm, r0, c0);
+}
+
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_ret_link(PtT<SE> p, const WTab *__restrict__ wtab, int nn, int q1, int RP, z2 *QA, const z2 *tb, double *d2) {
+  __builtin_amdgcn_s_setprio(3);
+  __shared__ double red[4];
+  if (p.n == BO_NMAX) ret_link<BO_NMAX>(p, wtab, nn, q1, RP, QA, tb, d2, red);
+  else ret_link<0>(p, wtab, nn, q1, RP, QA, tb, d2, red);
 }
 
 // ---- a whole panel of the dependency chain in one launch
@@ -1238,11 +1251,11 @@ __global__ void __launch_bounds__(256) kern_les_tblk(PtT<SE> p, const WTab *__re
 // R_m = Minv_m (Q_m + QA_m) for the blocks m of panel (q0, q1] (not T), in
 // place in QA; workgroup (b, m) owns columns 16b..16b+15, its waves the row
 // tiles
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_les_xform(Pt p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
-  const int b = blockIdx.x, m = q0 + 1 + blockIdx.y, n = p.n;
+template <int NF>
+__device__ __forceinline__ void les_xform(const Pt &p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
+  const int b = blockIdx.x, m = q0 + 1 + blockIdx.y, n = NF ? NF : p.n;
   const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
-  const size_t es = p.es();
+  const size_t es = (size_t)n * n;
   z2 *QAm = QA + (size_t)m * es;
   TileAcc3 x;
   x.zero();
@@ -1260,14 +1273,19 @@ kern_les_xform(Pt p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
   tile_store(x.value(), n, QAm, r0, c0);
 }
 
-template <typename SE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-kern_les_link(PtT<SE> p, const WTab *__restrict__ wtab, int l, int q1, int RP, z2 *QA, z2 *X, const z2 *tb) {
-  __builtin_amdgcn_s_setprio(3);
+kern_les_xform(Pt p, int q0, z2 *QA, const z2 *Q, const z2 *Minv) {
+  if (p.n == BO_NMAX) les_xform<BO_NMAX>(p, q0, QA, Q, Minv);
+  else les_xform<0>(p, q0, QA, Q, Minv);
+}
+
+template <int NF, typename SE>
+__device__ __forceinline__ void les_link(const PtT<SE> &p, const WTab *__restrict__ wtab, int l, int q1, int RP,
+                                         z2 *QA, z2 *X, const z2 *tb) {
   const WTab &w = *wtab;
-  const int b = blockIdx.x, m = l + 1 + blockIdx.y, T = p.T, n = p.n;
+  const int b = blockIdx.x, m = l + 1 + blockIdx.y, T = p.T, n = NF ? NF : p.n;
   const int r0 = 16 * (threadIdx.x >> 6), c0 = 16 * b;
-  const size_t es = p.es();
+  const size_t es = (size_t)n * n;
   z2 *Qm = QA + (size_t)m * es;
   TileAcc3 acc;
   {
@@ -1295,6 +1313,14 @@ kern_les_link(PtT<SE> p, const WTab *__restrict__ wtab, int l, int q1, int RP, z
     for (int q = 0; q < 4; q++) acc.add(q, zmul(cl, gt.get(q)));
   }
   tile_store(acc.value(), n, Qm, r0, c0);
+}
+
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+kern_les_link(PtT<SE> p, const WTab *__restrict__ wtab, int l, int q1, int RP, z2 *QA, z2 *X, const z2 *tb) {
+  __builtin_amdgcn_s_setprio(3);
+  if (p.n == BO_NMAX) les_link<BO_NMAX>(p, wtab, l, q1, RP, QA, X, tb);
+  else les_link<0>(p, wtab, l, q1, RP, QA, X, tb);
 }
 
 // the lesser panel in one launch, as kern_ret_chain (columns 16b..16b+15 of
