@@ -138,7 +138,7 @@ struct Ctx {
   bool have_mix = false;
   DevBuf dy_tvA, dy_tvB, dy_tvpart, dy_tvinfo;  // the mixing row's own scratch
   DevBuf dy_info, dy_js, dy_gA, dy_gB, dy_err, dy_Q, dy_QA, dy_X, dy_M, dy_Qs, dy_part, dy_scr, dy_Y;
-  DevBuf dy_la, dy_lb, dy_mixtmp, dy_h0, dy_xq, dy_bootI, dy_en;
+  DevBuf dy_la, dy_lb, dy_mixtmp, dy_h0, dy_xq, dy_bootI, dy_en, dy_hfpart;
   // inverses of the k x k block start-up systems: the retarded one depends on
   // times T-k..T-1 only (valid for every corrector pass of step retinv_T), the
   // lesser one on times 0..k only (valid until the early history changes)

@@ -1572,25 +1572,39 @@ __global__ void kern_hf_x(Pt p, const double *Bq, z2 *Xq) {
   if (threadIdx.x == 0) Xq[Q] = {rx[0], ry[0]};
 }
 
-// hmf(T)_ij = h0_ij + 2 sum_Q X_Q B[Q][i][j] - sum_(l,k) rho_kl U'[(l,k)][(i,j)]
-__global__ void kern_hf_h(Pt p, const double *h0, const double *Bq, const double *U, const z2 *Xq, int naux, z2 *hmf) {
-  const int n = p.n;
+// hmf(T)_ij = h0_ij + 2 sum_Q X_Q B[Q][i][j] - sum_(l,k) rho_kl U'[(l,k)][(i,j)]:
+// a matrix-vector product over naux + n^2 terms, HBM-bound on U' (n^4
+// doubles).  The terms are split over gridDim.y workgroup rows (partial sums
+// part[s][e]) so that the whole chip streams U', then kern_hf_sum adds them.
+__global__ void kern_hf_h(Pt p, const double *Bq, const double *U, const z2 *Xq, int naux, z2 *part) {
+  const int n = p.n, S = gridDim.y, s = blockIdx.y;
   const size_t es = p.es();
-  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)es) return;
-  z2 v = {h0[e], 0.0};
-  for (int Q = 0; Q < naux; Q++) {
-    double b = Bq[(size_t)Q * es + e];
+  const int terms = naux + n * n, per = (terms + S - 1) / S;
+  const int t0 = s * per, t1 = min(terms, t0 + per);
+  z2 v = {0.0, 0.0};
+  for (int Q = t0; Q < min(t1, naux); Q++) {
+    const double b = Bq[(size_t)Q * es + e];
     v.x += 2 * Xq[Q].x * b; v.y += 2 * Xq[Q].y * b;
   }
   const z2 *gl = p.gl(p.T, p.T);
-  for (int l = 0; l < n; l++)
-    for (int k = 0; k < n; k++) {
-      z2 g = gl[k * n + l];
-      z2 rho = {-p.sig * g.y, p.sig * g.x};
-      double u = U[((size_t)l * n + k) * es + e];
-      v.x -= rho.x * u; v.y -= rho.y * u;
-    }
+  for (int t = max(t0, naux); t < t1; t++) {
+    const int lk = t - naux, l = lk / n, k = lk % n;
+    const z2 g = gl[k * n + l];
+    const z2 rho = {-p.sig * g.y, p.sig * g.x};
+    const double u = U[(size_t)lk * es + e];
+    v.x -= rho.x * u; v.y -= rho.y * u;
+  }
+  part[(size_t)s * es + e] = v;
+}
+
+__global__ void kern_hf_sum(Pt p, const double *h0, const z2 *part, int S, z2 *hmf) {
+  const size_t es = p.es();
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)es) return;
+  z2 v = {h0[e], 0.0};
+  for (int s = 0; s < S; s++) v = zadd(v, part[(size_t)s * es + e]);
   hmf[(size_t)p.T * es + e] = v;
 }
 
@@ -2368,10 +2382,18 @@ void hf_decomp(Ctx &c, int T, const double *h0_host) {
   touch_time(c, T);
 }
 
+// workgroup rows of kern_hf_h: about 1024 workgroups at nao = 64, at most
+// one term each for tiny systems
+int hf_splits(const Ctx &c) {
+  const long long wg = (long long)(c.es + 255) / 256;
+  return (int)std::max(1LL, std::min((long long)(c.naux + c.nao * c.nao), (1024 + wg - 1) / wg));
+}
+
 void hf_upload_h0(Ctx &c, const double *h0_host) {
   const size_t es = c.es;
   c.dy_h0.ensure(es * sizeof(double));
   c.dy_xq.ensure((size_t)c.naux * sizeof(z2));
+  c.dy_hfpart.ensure((size_t)hf_splits(c) * es * sizeof(z2));
   NED_HIP(hipMemcpyAsync(c.dy_h0.p, h0_host, es * sizeof(double), hipMemcpyHostToDevice, c.stream));
 }
 
@@ -2383,8 +2405,12 @@ void hf_kernels(Ctx &c, int T) {
   const size_t es = c.es;
   Pt p = make_pt(c, T, 0, 0, 0);
   hipLaunchKernelGGL(kern_hf_x, dim3(c.naux), dim3(256), 0, c.stream, p, c.Bq.as<double>(), c.dy_xq.as<z2>());
-  hipLaunchKernelGGL(kern_hf_h, dim3(nblk((long long)es)), dim3(256), 0, c.stream, p, c.dy_h0.as<double>(),
-                     c.Bq.as<double>(), c.U64.as<double>(), c.dy_xq.as<z2>(), c.naux, c.hmf.as<z2>());
+  const int S = hf_splits(c);
+  require(c.dy_hfpart.bytes >= (size_t)S * es * sizeof(z2), "hf: partial-sum buffer not allocated (upload h0 first)");
+  hipLaunchKernelGGL(kern_hf_h, dim3(nblk((long long)es), S), dim3(256), 0, c.stream, p, c.Bq.as<double>(),
+                     c.U64.as<double>(), c.dy_xq.as<z2>(), c.naux, c.dy_hfpart.as<z2>());
+  hipLaunchKernelGGL(kern_hf_sum, dim3(nblk((long long)es)), dim3(256), 0, c.stream, p, c.dy_h0.as<double>(),
+                     c.dy_hfpart.as<z2>(), S, c.hmf.as<z2>());
   check_launch("hf");
 }
 
