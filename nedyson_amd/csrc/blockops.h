@@ -142,9 +142,9 @@ __device__ __forceinline__ float2 zero_of(const float2 *) { return float2{0.0f, 
 // Acc: BlkAcc (four real products per complex one) or BlkAcc3 (three).
 // Lds: BlkLdsT<KT>, K slices of KT.
 // NF: the block order n when it is known at compile time (0: read from n_).
-// A known order folds away the edge tests and the per-column branches, so
-// that a k-step's five fragment reads are issued together ahead of its
-// twelve products; with them, each column's read is awaited in turn.
+// A known order folds away the edge tests and the per-column branches: the
+// k-steps unroll, and each column's fragment read is issued among the
+// previous column's products instead of waiting in a branch of its own.
 template <int LATE = 0, int NF = 0, typename Acc, typename Lds, typename EX, typename EY>
 __device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n_, const EX *X, int opX, const EY *Y, int opY,
                                         z2 coef, int ldX = 0, int ldY = 0) {
