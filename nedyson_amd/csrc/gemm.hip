@@ -500,8 +500,15 @@ struct NtCfg {
 // rows or columns, 16 deep, twice the products per fragment read from LDS;
 // lane l holds row / column l % 32 at depth 8 (l / 32) + 0..7, and output
 // element e of lane l is row 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32).
+// DMA (two stages, swizzled): the next slice goes straight from memory into
+// the idle LDS stage (global_load_lds_dwordx4: each wave instruction fills
+// one contiguous 1 KB run, 16 rows of 64 bytes, lane l the 16 bytes at 16 l,
+// so lane l fetches the logical segment that the swizzle stores there), with
+// no register staging; the two stages are separate arrays, so that the
+// compiler's wait for the transfers into one is not put before the reads of
+// the other.
 template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false, bool SWZ = false,
-          int IGLP = -1, int MF = 16>
+          int IGLP = -1, int MF = 16, bool DMA = false>
 __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
     kern_nt_bf16x3(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo, long long lda, long long strA,
                    const uint16_t *__restrict__ Bhi, const uint16_t *__restrict__ Blo, long long ldb, long long strB,
@@ -510,6 +517,7 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
   constexpr int NTH = Cfg::THREADS, BI = Cfg::BI, BJ = Cfg::BJ, LA = Cfg::LA, LB = Cfg::LB, SEG = Cfg::SEG;
   static_assert(!SWZ || TK == 32, "the swizzled layout is for 32-deep slices");
   static_assert(MF == 16 || (MF == 32 && !SWZ), "32 x 32 blocks read the padded layout");
+  static_assert(!DMA || (DB && SWZ), "direct-to-LDS loads fill the swizzled two-stage layout");
   // one MFMA: depth KS, fragment rows / columns MF, lane groups of MF lanes
   constexpr int KS = MF == 16 ? 32 : 16;
   using Acc = typename std::conditional<MF == 16, v4f, v16f>::type;
@@ -539,9 +547,12 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
   Bhi += (size_t)item * strB;
   Blo += (size_t)item * strB;
   C += (size_t)item * strC;
-  constexpr int NS = DB ? 2 : 1;
+  constexpr int NS = (DB && !DMA) ? 2 : 1;
   __shared__ __attribute__((aligned(16))) uint16_t sA[NS][2][TM][LDK];
   __shared__ __attribute__((aligned(16))) uint16_t sB[NS][2][TN][LDK];
+  // the second stage of the DMA form (a placeholder otherwise)
+  __shared__ __attribute__((aligned(16))) uint16_t sA1[DMA ? 2 : 1][DMA ? TM : 1][DMA ? LDK : 1];
+  __shared__ __attribute__((aligned(16))) uint16_t sB1[DMA ? 2 : 1][DMA ? TN : 1][DMA ? LDK : 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WCOLS, wn = wave % WCOLS;
   const int group = 8;
@@ -588,7 +599,7 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
       *reinterpret_cast<u32x4 *>(&sB[st][1][row][lofs(row, seg)]) = rb1[i];
     }
   };
-  auto compute = [&](int st) __attribute__((always_inline)) {
+  auto compute = [&](const uint16_t(*cA)[TM][LDK], const uint16_t(*cB)[TN][LDK]) __attribute__((always_inline)) {
     if constexpr (IGLP >= 0) __builtin_amdgcn_iglp_opt(IGLP);
 #pragma unroll
     for (int kk = 0; kk < TK; kk += KS) {
@@ -598,14 +609,14 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
 #pragma unroll
         for (int j = 0; j < BJ; j++) {
           const int col = wn * WN + j * MF + lane % MF;
-          bh[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][0][col][lofs(col, kseg)]));
-          bl[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sB[st][1][col][lofs(col, kseg)]));
+          bh[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&cB[0][col][lofs(col, kseg)]));
+          bl[j] = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&cB[1][col][lofs(col, kseg)]));
         }
 #pragma unroll
         for (int i = 0; i < BI; i++) {
           const int row = wm * WM + i * MF + lane % MF;
-          const v8bf a_h = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][0][row][lofs(row, kseg)]));
-          const v8bf a_l = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&sA[st][1][row][lofs(row, kseg)]));
+          const v8bf a_h = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&cA[0][row][lofs(row, kseg)]));
+          const v8bf a_l = __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(&cA[1][row][lofs(row, kseg)]));
 #pragma unroll
           for (int j = 0; j < BJ; j++) acc[i][j] = mfma(a_l, bh[j], acc[i][j]);
 #pragma unroll
@@ -618,14 +629,14 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
 #pragma unroll
         for (int i = 0; i < BI; i++) {
           const int row = wm * WM + i * MF + lane % MF;
-          ah[i] = *reinterpret_cast<const u32x4 *>(&sA[st][0][row][lofs(row, kseg)]);
-          al[i] = *reinterpret_cast<const u32x4 *>(&sA[st][1][row][lofs(row, kseg)]);
+          ah[i] = *reinterpret_cast<const u32x4 *>(&cA[0][row][lofs(row, kseg)]);
+          al[i] = *reinterpret_cast<const u32x4 *>(&cA[1][row][lofs(row, kseg)]);
         }
 #pragma unroll
         for (int j = 0; j < BJ; j++) {
           const int col = wn * WN + j * MF + lane % MF;
-          bh[j] = *reinterpret_cast<const u32x4 *>(&sB[st][0][col][lofs(col, kseg)]);
-          bl[j] = *reinterpret_cast<const u32x4 *>(&sB[st][1][col][lofs(col, kseg)]);
+          bh[j] = *reinterpret_cast<const u32x4 *>(&cB[0][col][lofs(col, kseg)]);
+          bl[j] = *reinterpret_cast<const u32x4 *>(&cB[1][col][lofs(col, kseg)]);
         }
         // three passes over every (i, j): BI x BJ independent accumulators lie
         // between two products into the same one (the 32 x 32 MFMA's result
@@ -643,8 +654,42 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
       }
     }
   };
-  gload(0);
-  if constexpr (DB) {
+  if constexpr (DMA) {
+    // 1 KB runs of a slice (16 rows x 64 bytes of one plane): A's planes,
+    // then B's; run WV i + w goes to wave w, so that the operand and plane of
+    // a wave's i-th run are known at compile time
+    constexpr int WV = NTH / 64, RA = 2 * (TM / 16), RUNS = RA + 2 * (TN / 16);
+    static_assert((TM / 16) % WV == 0 && (TN / 16) % WV == 0, "a plane's runs in whole rounds of the waves");
+    const uint16_t *const gAh = Ahi, *const gAl = Alo, *const gBh = Bhi, *const gBl = Blo;
+    auto dma = [&](uint16_t(*dA)[TM][LDK], uint16_t(*dB)[TN][LDK], int k0) __attribute__((always_inline)) {
+#pragma unroll
+      for (int i = 0; i < RUNS / WV; i++) {
+        const int q0 = WV * i;
+        const bool isA = q0 < RA;
+        const int per = (isA ? TM : TN) / 16, qq0 = isA ? q0 : q0 - RA;
+        const int pl = qq0 / per, rb = qq0 % per + wave;
+        const int rw = 16 * rb + (lane >> 2), sg = (lane & 3) ^ ((rw >> 1) & 3);
+        const uint16_t *src = isA ? (pl ? gAl : gAh) + (size_t)(m0 + rw) * lda + k0 + 8 * sg
+                                  : (pl ? gBl : gBh) + (size_t)(n0 + rw) * ldb + k0 + 8 * sg;
+        uint16_t *dst = isA ? &dA[pl][16 * rb][0] : &dB[pl][16 * rb][0];
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                         (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+      }
+    };
+    dma(sA[0], sB[0], 0);
+    __syncthreads();  // waits for this wave's transfers, then for every wave
+    for (int k0 = 0; k0 < K; k0 += 2 * TK) {
+      if (k0 + TK < K) dma(sA1, sB1, k0 + TK);
+      compute(sA[0], sB[0]);
+      __syncthreads();
+      if (k0 + TK < K) {
+        if (k0 + 2 * TK < K) dma(sA[0], sB[0], k0 + 2 * TK);
+        compute(sA1, sB1);
+        __syncthreads();
+      }
+    }
+  } else if constexpr (DB) {
+    gload(0);
     lstore(0);
     if (TK < K) gload(TK);
     __syncthreads();
@@ -654,17 +699,18 @@ __global__ void __launch_bounds__(64 * (TM / WM) * (TN / WN))
         lstore(st ^ 1);
         if (k0 + 2 * TK < K) gload(k0 + 2 * TK);
       }
-      compute(st);
+      compute(sA[st], sB[st]);
       __syncthreads();
       st ^= 1;
     }
   } else {
+    gload(0);
     for (int k0 = 0; k0 < K; k0 += TK) {
       __syncthreads();
       lstore(0);
       __syncthreads();
       if (k0 + TK < K) gload(k0 + TK);
-      compute(0);
+      compute(sA[0], sB[0]);
     }
   }
 #pragma unroll
@@ -712,14 +758,14 @@ struct NtVariant {
 };
 
 template <int TM, int TN, int WM, int WN, int TK = NT_TK, bool STREAM = false, bool DB = false, bool SWZ = false,
-          int IGLP = -1, int MF = 16>
+          int IGLP = -1, int MF = 16, bool DMA = false>
 void nt_launch(const NtOp &A, const NtOp &B, float *C, long long ldc, long long strC, int M, int N, int K, int batch,
                float alpha, int beta, hipStream_t s) {
   // the grid covers whole tiles only: a ragged M or N would leave rows or
   // columns unwritten
   if (M % TM || N % TN) throw std::invalid_argument("gemm_nt_bf16x3: tile does not divide M or N");
   const int tiles = (M / TM) * (N / TN);
-  hipLaunchKernelGGL((kern_nt_bf16x3<TM, TN, WM, WN, TK, STREAM, DB, SWZ, IGLP, MF>), dim3(tiles, batch),
+  hipLaunchKernelGGL((kern_nt_bf16x3<TM, TN, WM, WN, TK, STREAM, DB, SWZ, IGLP, MF, DMA>), dim3(tiles, batch),
                      dim3(NtCfg<TM, TN, WM, WN, TK, MF>::THREADS), 0, s, A.hi, A.lo, A.ld, A.stride, B.hi, B.lo, B.ld,
                      B.stride, C, ldc, strC, M, N, K, alpha, beta);
 }
@@ -753,6 +799,10 @@ const NtVariant kNt[] = {
      "128x128 / 4 waves of 64x64, K32, streamed A, 2 stages, MFMA/LDS interleave, 32x32 blocks"},
     {256, 128, nt_launch<256, 128, 64, 64, 32, true, true, false, 0, 32>,
      "256x128 / 8 waves of 64x64, K32, streamed A, 2 stages, MFMA/LDS interleave, 32x32 blocks"},
+    {256, 256, nt_launch<256, 256, 128, 64, 32, true, true, true, -1, 16, true>,
+     "256x256 / 8 waves of 128x64, K32, streamed A, 2 stages, swizzled, direct-to-LDS loads"},
+    {128, 128, nt_launch<128, 128, 64, 64, 32, true, true, true, -1, 16, true>,
+     "128x128 / 4 waves of 64x64, K32, streamed A, 2 stages, swizzled, direct-to-LDS loads"},
 };
 constexpr int kNtCount = sizeof(kNt) / sizeof(kNt[0]);
 
