@@ -992,21 +992,23 @@ __global__ void __launch_bounds__(256) kern_tv_fin(Pt p, const z2 *part, int nch
 // ------------------------------------------------------------ lesser
 // Y_tau = beta/2 sum_tau' mass[tau][tau'] * sig i G^rceil(T, ntau-tau')^H
 // (the right factor of Cles3_tstp, src/dyson_integrals.h:393-427)
+// (workgroup (tau, chunk of 256 elements); thread f reads element f of each
+// G^rceil block, so that the reads of a warp are contiguous, and writes the
+// transposed element of Y once)
 __global__ void kern_les_Y(Pt p, const double *mass, z2 *Y) {
   const int tau = blockIdx.x, n = p.n, np = p.np;
   const size_t es = p.es();
-  for (int e = threadIdx.x; e < (int)es; e += blockDim.x) {
-    int a = e / n, b = e % n;
-    z2 acc = {0.0, 0.0};
-    for (int t2 = 0; t2 < np; t2++) {
-      double wgt = 0.5 * p.beta * mass[(size_t)tau * np + t2];
-      z2 g = p.gtv(p.T, np - 1 - t2)[b * n + a];  // conj-transpose element
-      // sig * i * conj(g)
-      z2 v = {p.sig * g.y, p.sig * g.x};
-      acc.x += wgt * v.x; acc.y += wgt * v.y;
-    }
-    Y[(size_t)tau * es + e] = acc;
+  const int f = blockIdx.y * blockDim.x + threadIdx.x;  // element (b, a) of G^rceil(T, .)
+  if (f >= (int)es) return;
+  const int b = f / n, a = f % n;
+  z2 acc = {0.0, 0.0};
+  for (int t2 = 0; t2 < np; t2++) {
+    const double wgt = 0.5 * p.beta * mass[(size_t)tau * np + t2];
+    const z2 g = p.gtv(p.T, np - 1 - t2)[f];  // element (a, b) of the adjoint
+    // sig * i * conj(g)
+    acc.x += wgt * (p.sig * g.y); acc.y += wgt * (p.sig * g.x);
   }
+  Y[(size_t)tau * es + (size_t)a * n + b] = acc;
 }
 
 // Q_row = Cles3 + Cles2 (src/dyson_integrals.h:336-380, 393-427), partial
@@ -2159,7 +2161,8 @@ void dyson_mid(Ctx &c, int T, double mu, double beta, double dt) {
   if (T > k + 1) hipLaunchKernelGGL(kern_band_err, dim3(1), dim3(256), 0, s, c.dy_d2.as<double>(), k + 2, T, nb,
                                     err + 0);
   NED_HIP(hipStreamWaitEvent(s, c.ev_tv, 0));
-  hipLaunchKernelGGL(kern_les_Y, dim3(np), dim3(256), 0, s, p, c.mass.as<double>(), c.dy_Y.as<z2>());
+  hipLaunchKernelGGL(kern_les_Y, dim3(np, nblk((long long)es)), dim3(256), 0, s, p, c.mass.as<double>(),
+                     c.dy_Y.as<z2>());
   int nch = chunks_for(T + 1, np + T + 1);
   require((long long)nch * (T + 1) <= 8192LL + c.nt + 1, "dyson_step: partial-sum buffer too small");
   SE_LAUNCH(c, kern_les_rhs, dim3(T + 1, nch), dim3(256), 0, s, p, wd, c.dy_Y.as<z2>(), nch, c.dy_part.as<z2>());
