@@ -411,8 +411,8 @@ __global__ void __launch_bounds__(256) kern_ret_start_store(Pt p, const z2 *X, d
 
 // complete G^R(T,T-m) from its finished history sum (in acc)
 template <typename SE>
-__device__ void ret_finalize(const PtT<SE> &p, const WTab &w, BlkAcc &acc, BlkLds &s, int m, z2 *Qm, const z2 *Minv,
-                             double *red, double *err) {
+__device__ void ret_finalize(const PtT<SE> &p, const WTab &w, BlkAcc &acc, BlkLds2<8> &s, int m, z2 *Qm,
+                             const z2 *Minv, double *red, double *err) {
   const int n = p.n, T = p.T;
   blk_scale(acc, z2{p.dt, 0.0});
   for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, p.gr(T, T - m + l), 0, z2{0.0, -w.bd[l] / p.dt});
@@ -421,7 +421,7 @@ __device__ void ret_finalize(const PtT<SE> &p, const WTab &w, BlkAcc &acc, BlkLd
   __syncthreads();
   BlkAcc x;
   x.zero();
-  blk_mac(x, s, n, Qm, 0, Minv + (size_t)(T - m) * n * n, 0, z2{1.0, 0.0});
+  blk_mac2(x, s, n, Qm, 0, Minv + (size_t)(T - m) * n * n, 0, z2{1.0, 0.0});
   z2 *g = p.gr(T, T - m);
   double d2 = blk_dist2(x, n, g, red);
   blk_store(x, n, g);
@@ -429,19 +429,20 @@ __device__ void ret_finalize(const PtT<SE> &p, const WTab &w, BlkAcc &acc, BlkLd
 }
 
 // history sums from the start-up blocks l = 0..k, for m = k+1..T
-// (QA: the dependency chain's own accumulator, zeroed here)
-template <typename SE>
-__global__ void __launch_bounds__(256) kern_ret_qinit(PtT<SE> p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA,
-                                                      const z2 *Minv, double *err) {
+// (QA: the dependency chain's own accumulator, zeroed here); three real
+// products per complex one and two LDS stages, as the bulk updates
+template <int NF, typename SE>
+__device__ __forceinline__ void ret_qinit(BlkLds2<8> &s, double *red, const PtT<SE> &p, const WTab *__restrict__ wtab,
+                                          z2 *Q, z2 *QA, const z2 *Minv, double *err) {
   const WTab &w = *wtab;
-  __shared__ BlkLds s;
-  __shared__ double red[256];
-  const int m = w.k + 1 + blockIdx.x, T = p.T, n = p.n;
+  const int m = w.k + 1 + blockIdx.x, T = p.T, n = NF ? NF : p.n;
   if (m > T) return;
   for (int e = threadIdx.x; e < n * n; e += 256) QA[(size_t)m * n * n + e] = z2{0.0, 0.0};
-  BlkAcc acc;
-  acc.zero();
-  for (int l = 0; l <= w.k; l++) blk_mac(acc, s, n, p.gr(T, T - l), 0, p.sr(T - l, T - m), 0, z2{greg(w, m, l), 0.0});
+  BlkAcc3 a3;
+  a3.zero();
+  for (int l = 0; l <= w.k; l++)
+    blk_mac2<NF>(a3, s, n, p.gr(T, T - l), 0, p.sr(T - l, T - m), 0, z2{greg(w, m, l), 0.0});
+  BlkAcc acc = a3.value();
   z2 *Qm = Q + (size_t)m * n * n;
   if (m == w.k + 1) {
     ret_finalize(p, w, acc, s, m, Qm, Minv, red, err);
@@ -452,6 +453,15 @@ __global__ void __launch_bounds__(256) kern_ret_qinit(PtT<SE> p, const WTab *__r
   for (int j = m - w.k - 1; j <= w.k; j++)
     blk_axpy(acc, n, p.gr(T, T - j), 0, z2{0.0, -w.bd[m - j] / (p.dt * p.dt)});
   blk_store(acc, n, Qm);
+}
+
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_ret_qinit(PtT<SE> p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA, const z2 *Minv, double *err) {
+  __shared__ BlkLds2<8> s;
+  __shared__ double red[256];
+  if (p.n == BO_NMAX) ret_qinit<BO_NMAX>(s, red, p, wtab, Q, QA, Minv, err);
+  else ret_qinit<0>(s, red, p, wtab, Q, QA, Minv, err);
 }
 
 // One link nn of the retarded recurrence: add block nn's contribution to
@@ -1123,7 +1133,7 @@ __global__ void __launch_bounds__(256) kern_les_start(PtT<SE> p, const WTab *__r
 }
 
 template <typename SE>
-__device__ void les_finalize(const PtT<SE> &p, const WTab &w, BlkAcc &acc, BlkLds &s, int m, z2 *Qm, z2 *X,
+__device__ void les_finalize(const PtT<SE> &p, const WTab &w, BlkAcc &acc, BlkLds2<8> &s, int m, z2 *Qm, z2 *X,
                              const z2 *Minv) {
   const int n = p.n;
   for (int l = 1; l <= w.k + 1; l++) blk_axpy(acc, n, X + (size_t)(m - l) * p.es(), 0, z2{0.0, -w.bd[l] / p.dt});
@@ -1132,24 +1142,29 @@ __device__ void les_finalize(const PtT<SE> &p, const WTab &w, BlkAcc &acc, BlkLd
   __syncthreads();
   BlkAcc x;
   x.zero();
-  blk_mac(x, s, n, Minv + (size_t)m * p.es(), 0, Qm, 0, z2{1.0, 0.0});
+  blk_mac2(x, s, n, Minv + (size_t)m * p.es(), 0, Qm, 0, z2{1.0, 0.0});
   blk_store(x, n, X + (size_t)m * p.es());
 }
 
-// (QA: the dependency chain's own accumulator, zeroed here)
-template <typename SE>
-__global__ void __launch_bounds__(256) kern_les_qinit(PtT<SE> p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA, z2 *X,
-                                                      const z2 *Minv) {
+// (QA: the dependency chain's own accumulator, zeroed here); three real
+// products per complex one and two LDS stages, as the bulk updates
+template <int NF, typename SE>
+__device__ __forceinline__ void les_qinit(BlkLds2<8> &s, const PtT<SE> &p, const WTab *__restrict__ wtab, z2 *Q,
+                                          z2 *QA, z2 *X, const z2 *Minv) {
   const WTab &w = *wtab;
-  __shared__ BlkLds s;
-  const int m = w.k + 1 + blockIdx.x, T = p.T, n = p.n;
+  const int m = w.k + 1 + blockIdx.x, T = p.T, n = NF ? NF : p.n;
   if (m > T) return;
   for (int e = threadIdx.x; e < n * n; e += 256) QA[(size_t)m * p.es() + e] = z2{0.0, 0.0};
   z2 *Qm = Q + (size_t)m * p.es();
-  BlkAcc acc;
-  blk_load(acc, n, Qm);
+  BlkAcc3 a3;
+  {
+    BlkAcc q;
+    blk_load(q, n, Qm);
+    a3.init(q);
+  }
   for (int l = 0; l <= w.k; l++)
-    blk_mac<2>(acc, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+    blk_mac2<NF>(a3, s, n, p.sr(m, l), 0, X + (size_t)l * p.es(), 0, z2{p.dt * greg(w, m, l), 0.0});
+  BlkAcc acc = a3.value();
   if (m == w.k + 1 && m <= T - 1) {
     les_finalize(p, w, acc, s, m, Qm, X, Minv);
     return;
@@ -1159,6 +1174,14 @@ __global__ void __launch_bounds__(256) kern_les_qinit(PtT<SE> p, const WTab *__r
   if (m <= T - 1)
     for (int j = m - w.k - 1; j <= w.k; j++) blk_axpy(acc, n, X + (size_t)j * p.es(), 0, z2{0.0, -w.bd[m - j] / p.dt});
   blk_store(acc, n, Qm);
+}
+
+template <typename SE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+kern_les_qinit(PtT<SE> p, const WTab *__restrict__ wtab, z2 *Q, z2 *QA, z2 *X, const z2 *Minv) {
+  __shared__ BlkLds2<8> s;
+  if (p.n == BO_NMAX) les_qinit<BO_NMAX>(s, p, wtab, Q, QA, X, Minv);
+  else les_qinit<0>(s, p, wtab, Q, QA, X, Minv);
 }
 
 // One link l of the lesser recurrence, split by columns: workgroup (b, m)
