@@ -575,7 +575,7 @@ kern_ret_step_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab,
 // Transfer block of link t to block m = t + d (d = 1..RP, m up to the end of
 // t's panel): tb[t RP + d - 1]
 template <typename SE>
-__global__ void __launch_bounds__(256) kern_ret_tblk(PtT<SE> p, const WTab *__restrict__ wtab, int RP,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) kern_ret_tblk(PtT<SE> p, const WTab *__restrict__ wtab, int RP,
                                                      const z2 *Minv, z2 *tb) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;
@@ -1258,7 +1258,7 @@ kern_les_step_band_grp(const ChainArgT<SE> *args, const WTab *__restrict__ wtab,
 // takes the plain contribution.  Transfer block of link l to m = l + d:
 // tb[l RP + d - 1].
 template <typename SE>
-__global__ void __launch_bounds__(256) kern_les_tblk(PtT<SE> p, const WTab *__restrict__ wtab, int RP,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) kern_les_tblk(PtT<SE> p, const WTab *__restrict__ wtab, int RP,
                                                      const z2 *Minv, z2 *tb) {
   const WTab &w = *wtab;
   __shared__ BlkLds s;

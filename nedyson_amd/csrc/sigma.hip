@@ -221,6 +221,44 @@ __global__ void kern_pi_combine(const float *Pf, int a, int P, uint16_t *hi, uin
   lo[o] = f2bf(v - bf2f(h));
 }
 
+// the same four columns b..b+3 at a time (a, ldh and col0 multiples of 4):
+// float4 reads and 8-byte split stores
+__global__ void kern_pi_combine4(const float *Pf, int a, int P, uint16_t *hi, uint16_t *lo, long long ldh, int col0) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long a2 = (long long)a * a, q2 = a2 / 4;
+  if (idx >= (long long)P * 2 * q2) return;
+  const int p = (int)(idx / (2 * q2));
+  const long long rem = idx % (2 * q2);
+  const int pl = (int)(rem / q2), x = (int)((rem % q2) / (a / 4)), b = 4 * (int)(rem % (a / 4));
+  const float *F = Pf + (size_t)p * 4 * a2;
+  const long long w2 = 2LL * a;
+  const float4 u = *reinterpret_cast<const float4 *>(F + x * w2 + (pl ? a + b : b));
+  const float4 t = *reinterpret_cast<const float4 *>(F + (a + x) * w2 + (pl ? b : a + b));
+  const float v[4] = {pl ? u.x + t.x : u.x - t.x, pl ? u.y + t.y : u.y - t.y, pl ? u.z + t.z : u.z - t.z,
+                      pl ? u.w + t.w : u.w - t.w};
+  uint16_t h[4], l[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    h[q] = f2bf(v[q]);
+    l[q] = f2bf(v[q] - bf2f(h[q]));
+  }
+  const size_t o = ((size_t)p * 2 * a + (size_t)pl * a + x) * ldh + col0 + b;
+  *reinterpret_cast<uint2 *>(hi + o) =
+      uint2{(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
+  *reinterpret_cast<uint2 *>(lo + o) =
+      uint2{(uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16)};
+}
+
+void pi_combine(const float *Pf, int a, int P, uint16_t *hi, uint16_t *lo, long long ldh, int col0, hipStream_t s) {
+  const long long tot = (long long)P * 2 * a * a;
+  if (a % 4 == 0 && ldh % 4 == 0 && col0 % 4 == 0)
+    hipLaunchKernelGGL(kern_pi_combine4, dim3((unsigned)((tot / 4 + 255) / 256)), dim3(256), 0, s, Pf, a, P, hi, lo,
+                       ldh, col0);
+  else
+    hipLaunchKernelGGL(kern_pi_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, Pf, a, P, hi, lo, ldh,
+                       col0);
+}
+
 __global__ void kern_d2f(const double *x, float *y, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = (float)x[i];
@@ -366,9 +404,7 @@ void run_chunk(Ctx &c, Prec prec, int T_, int p0, int P, const Bufs<T> &bf, cons
     // Pf_p = [Wt_r; Wt_i] [Zt_r; Zt_i]^T (K index (k,i) in both)
     gemm_nt_bf16x3(NtOp{wth, wtl, n2, 2 * an2}, NtOp{bf.S1h, bf.S1l, n2, 2 * an2}, bf.Pf, 2LL * a, 4LL * a * a, 2 * a,
                    2 * a, NN, P, 1.0f, 0, s, bf.tune);
-    long long tot = (long long)P * 2 * a * a;
-    hipLaunchKernelGGL(kern_pi_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Pf, a, P, bf.S1h,
-                       bf.S1l, ka, sym ? sg.ns : NN);
+    pi_combine(bf.Pf, a, P, bf.S1h, bf.S1l, ka, sym ? sg.ns : NN, s);
     check_launch("sigma pi combine");
     if (sym) {
       // the exchange in its symmetric and antisymmetric parts (half the flops
@@ -397,7 +433,6 @@ void run_chunk(Ctx &c, Prec prec, int T_, int p0, int P, const Bufs<T> &bf, cons
       c64_e(nullptr, nullptr, P, a, S, (float *)bf.Ep, s, (const float *)bf.X, sg.ldh, sg.nsp,
             c.sym_pairs.as<uint16_t>(), wth, wtl);
     else c64_e(W, H, P, a, S, (float *)bf.Ep, s);
-    tot = (long long)P * n2;
     sum_scatter(c, bf.Ep, S, p0, P, T_, s);
     check_launch("sigma sum scatter");
     return;
@@ -452,9 +487,7 @@ void run_chunk(Ctx &c, Prec prec, int T_, int p0, int P, const Bufs<T> &bf, cons
     // H = [C | Pi] [-U'^T | 2 Vmat^T]^T = -C U' + 2 Pi Vmat: the exchange and the
     // bubble's Pi V in one GEMM, rows (item, plane, a), K = n^2 + a
     const long long ka = n2 + a;
-    long long tot = (long long)P * 2 * a * a;
-    hipLaunchKernelGGL(kern_pi_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, bf.Pf, a, P, bf.S1h,
-                       bf.S1l, ka, NN);
+    pi_combine(bf.Pf, a, P, bf.S1h, bf.S1l, ka, NN, s);
     check_launch("sigma pi combine");
     split_bf16((const float *)bf.X, n2, bf.S1h, bf.S1l, ka, (long long)M, NN, s);
     c.mark(0);
