@@ -133,6 +133,28 @@ __device__ __forceinline__ void zero(f32x4 (&acc)[4][2]) {
 // split the accumulators into an image, natural ([i][k]) or transposed ([k][i])
 __device__ __forceinline__ void stage(Img &st, const f32x4 (&acc)[4][2], bool transposed) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (transposed) {
+    // a lane's four values q are consecutive i of one row k of the image:
+    // one 8-byte write per plane and part instead of four 2-byte ones
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int pl = 0; pl < 2; pl++) {
+        const int i0 = 16 * w + 4 * (lane >> 4), k = 16 * j + (lane & 15);
+        uint16_t h[4], l[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const float v = acc[j][pl][q];
+          h[q] = f2bf(v);
+          l[q] = f2bf(v - bf2f(h[q]));
+        }
+        *reinterpret_cast<uint2 *>(&st.h[pl][k][i0]) =
+            uint2{(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
+        *reinterpret_cast<uint2 *>(&st.l[pl][k][i0]) =
+            uint2{(uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16)};
+      }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < 4; j++)
 #pragma unroll
