@@ -547,8 +547,8 @@ def main():
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print(f"bench.py: the timestep runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(sharding the history over GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     def log(msg):

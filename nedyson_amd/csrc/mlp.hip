@@ -9,7 +9,7 @@
 // The network has no counterpart in the reference: it replaces the
 // polynomial predictor dyson::Extrapolate (src/dyson.cpp:47-117) as the
 // corrector's initial guess on a few time points of each slice (see
-// predictor.hip), and the plain iteration stays the judge of convergence.
+// predictor.hip), and the plain iteration still decides convergence.
 #include "mlp.h"
 #include <algorithm>
 #include <cmath>
