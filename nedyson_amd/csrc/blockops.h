@@ -133,31 +133,22 @@ __device__ __forceinline__ float2 zero_of(const float2 *) { return float2{0.0f, 
 
 // acc += coef * op(X) * op(Y)   (all threads of the workgroup call this);
 // ldX, ldY are the row strides of X and Y as stored (n for a dense block).
-// The next K-slice is loaded into registers while the current one is
-// multiplied out of LDS.
-// LATE: 0 prefetches both operands' next K slice into registers during the
-// current slice; bit 1 (bit 2) reads X (Y) only when it is staged -- for an
-// operand that every workgroup shares and that therefore sits in L2, this
-// saves the prefetch registers (and buys occupancy) at little cost.
+// The next K-slice of both operands is loaded into registers while the
+// current one is multiplied out of LDS (one stage, two barriers a slice: the
+// form of the one-off products; the long sums use blk_mac2).
 // Acc: BlkAcc (four real products per complex one) or BlkAcc3 (three).
 // Lds: BlkLdsT<KT>, K slices of KT.
-// NF: the block order n when it is known at compile time (0: read from n_).
-// A known order folds away the edge tests and the per-column branches: the
-// k-steps unroll, and each column's fragment read is issued among the
-// previous column's products instead of waiting in a branch of its own.
-template <int LATE = 0, int NF = 0, typename Acc, typename Lds, typename EX, typename EY>
-__device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n_, const EX *X, int opX, const EY *Y, int opY,
+template <typename Acc, typename Lds, typename EX, typename EY>
+__device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n, const EX *X, int opX, const EY *Y, int opY,
                                         z2 coef, int ldX = 0, int ldY = 0) {
   constexpr int KT = Lds::kt;
-  const int n = NF ? NF : n_;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (ldX == 0) ldX = n;
   if (ldY == 0) ldY = n;
   const bool wave_on = 16 * w < n;
   constexpr int PER = BO_NMAX * KT / BO_T;  // staged elements per thread and operand
-  constexpr int PX = (LATE & 1) ? 1 : PER, PY = (LATE & 2) ? 1 : PER;
-  EX px[PX];  // the next K slice, in the operands' storage types
-  EY py[PY];
+  EX px[PER];  // the next K slice, in the operands' storage types
+  EY py[PER];
   // element e of a slice -> (row, k) of X and (k, col) of Y, chosen so that
   // consecutive threads read consecutive addresses of the stored matrix
   // (along k when it is used as stored, along the other index when it is
@@ -182,8 +173,8 @@ __device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n_, const EX *X, i
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int e = tid + BO_T * i;
-      if constexpr (!(LATE & 1)) px[i] = getx(e, k0);
-      if constexpr (!(LATE & 2)) py[i] = gety(e, k0);
+      px[i] = getx(e, k0);
+      py[i] = gety(e, k0);
     }
   };
   __syncthreads();  // operands written by this workgroup just before the call are visible
@@ -196,8 +187,8 @@ __device__ __forceinline__ void blk_mac(Acc &acc, Lds &s, int n_, const EX *X, i
       int r, kk, kk2, c;
       xmap(e, r, kk);
       ymap(e, kk2, c);
-      s.X[r][kk] = zmul(coef, blk_op(widen((LATE & 1) ? getx(e, k0) : px[(LATE & 1) ? 0 : i]), opX));
-      s.Y[c][kk2] = blk_op(widen((LATE & 2) ? gety(e, k0) : py[(LATE & 2) ? 0 : i]), opY);
+      s.X[r][kk] = zmul(coef, blk_op(widen(px[i]), opX));
+      s.Y[c][kk2] = blk_op(widen(py[i]), opY);
     }
     __syncthreads();
     if (k0 + KT < n) fetch(k0 + KT);
