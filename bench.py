@@ -669,17 +669,49 @@ def main():
     # taken on this workload's shape
     traffic, traffic_note = None, "no counter run recorded for this shape"
     tfile = os.path.join(ROOT, "profiles", "kern_nt_bf16x3_nao64_T1000_traffic.json")
-    if args.prec == "bf16x3" and os.path.exists(tfile) and p.nao == 64 and p.naux == 256:
+    # (the counter runs used the default chunk of 464 items, M = 237568 rows;
+    # the library reads NED_SIGMA_CHUNK with atoi, so a non-number means default)
+    try:
+        forced_chunk = int(os.environ.get("NED_SIGMA_CHUNK", "0").strip() or 0)
+    except ValueError:
+        forced_chunk = 0
+    if (args.prec == "bf16x3" and os.path.exists(tfile) and p.nao == 64 and p.naux == 256
+            and forced_chunk <= 0):
         t = json.load(open(tfile))
         traffic = t["traffic_bytes_per_launch"]
         traffic_note = (f"bytes per full-chunk exchange ({t['shape']}) from rocprofv3 "
                         f"FETCH_SIZE x {t['fetch_scale']:.3f} + WRITE_SIZE x {t['write_scale']:.3f}, scaled by a 1 GiB "
                         f"copy; algorithmic {t['algorithmic_bytes_per_launch']:.3e} B")
     n_sigma = max(prof["sigma_n"], 1)
+    # the library splits the exchange by the pair symmetry of U' (NED_SYMX=0
+    # keeps the full contraction, as atoi reads it): the split-bf16 GEMMs do so
+    # at nao = 64 only and contract all pairs at the other shapes they take
+    # (nao^2 and 2 naux multiples of 128, naux of 64); every other exchange
+    # splits from nao = 8 on
+    symx_env = os.environ.get("NED_SYMX")
+    try:
+        symx_off = symx_env is not None and int(symx_env.strip() or 0) == 0
+    except ValueError:
+        symx_off = True
+    uaug = (p.nao * p.nao) % 128 == 0 and (2 * p.naux) % 128 == 0 and p.naux % 64 == 0
+    if args.prec == "bf16x3" and uaug:
+        split_exchange = p.nao == 64 and not symx_off
+    else:
+        split_exchange = p.nao >= 8 and not symx_off
+    f64_split = p.nao >= 8 and not symx_off
+    if split_exchange == f64_split:
+        speedup_basis = ("FP64 run uses the same exchange form ("
+                         + ("split by pair symmetry" if f64_split else "full contraction")
+                         + "; precision-only ratio)")
+    else:
+        speedup_basis = ("FP64 run splits the exchange by pair symmetry, the mixed run contracts all pairs"
+                         if f64_split else "mixed run splits the exchange by pair symmetry, the FP64 run contracts all pairs")
     # algorithmic flops of one exchange with M = 2 naux P rows (P time points
     # of the slice): 2 M (ns (ns + naux) + na^2) over the ns = nao(nao+1)/2
-    # and na = nao(nao-1)/2 pairs at nao = 64 (padding not counted), 2 M n^2
-    # (n^2 + naux) for the full contraction elsewhere; averaged over exchanges
+    # and na = nao(nao-1)/2 pairs where the exchange is split (padding not
+    # counted), 2 M n^2 (n^2 + naux) for the split-bf16 full contraction (Pi
+    # folded in), 2 M n^4 for the other full contraction; averaged over
+    # exchanges
     flops_per_launch = prof["ugemm_flops"] / max(prof["ugemm_n"], 1)
     achieved = flops_per_launch / (ugemm_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.prec]
@@ -704,7 +736,7 @@ def main():
                    "corrector_passes": args.corr, "sigma_prec": args.prec, "window_sigma_B": p.b_sd,
                    "precision_run_sigma_B": traj["sigma_B"] if traj else None,
                    "precision_run_steps": traj["steps"] if traj else None,
-                   "speedup_basis": "FP64 run uses the same split-exchange algorithm (precision-only ratio)"},
+                   "speedup_basis": speedup_basis},
         "f64_value": f64_value,
         "speedup_vs_f64": (value / f64_value) if f64_value else None,
         "max_abs_G_err_vs_f64": max_err,
@@ -718,8 +750,9 @@ def main():
                      "frac": achieved / peak, "traffic": traffic,
                      "traffic_note": traffic_note,
                      "kernel": f"Sigma exchange GEMM ({args.prec}), {flops_per_launch:.3e} algorithmic flop "
-                               f"per exchange, avg {ugemm_ms:.3f} ms over {prof['ugemm_n']} exchanges (each the "
-                               f"symmetric and antisymmetric launch at nao=64)"},
+                               f"per exchange, avg {ugemm_ms:.3f} ms over {prof['ugemm_n']} exchanges ("
+                               + ("each the symmetric and antisymmetric launch)"
+                                  if split_exchange else "each one full-contraction launch)")},
     }
     if traj is not None:
         line["precision_run"] = traj
